@@ -1,0 +1,183 @@
+/*
+ * tsr_gpu.h - C ABI of the MI355X (gfx950) occupancy-field + dual-marching-cells
+ * library.  Plain C types only; every handle owns device memory on one HIP
+ * stream.  The entry points replace the C++ class interfaces that TSR's
+ * Scene::reconstruct calls on its hot path (paths relative to
+ * SurfaceReconstruction/SurfaceReconstruction/):
+ *
+ *   tsr_tree_build          <- Tree::Tree()                 Scene/Tree/Tree.cpp:26-37
+ *                              (Nodes ctor Nodes.cpp:186-194, Leaves ctor Leaves.cpp:22-28,
+ *                               DualCells ctor DualCells.cpp:21-31, Samples::reorder Samples.cpp:825-835)
+ *   tsr_occupancy_compute   <- Occupancy::Occupancy(const Tree*)  SurfaceExtraction/Occupancy.cpp:78-127
+ *   tsr_dmc_extract         <- DualMarchingCells::extractMesh     SurfaceExtraction/DualMarchingCells.cpp:41-111
+ *   tsr_*_download          <- the accessors of Nodes/Leaves/DualCells/Occupancy/FlexibleMesh
+ *                              (Nodes.h:66-89, Leaves.h:36-67, DualCells.h:33-35, Occupancy.h:49-90)
+ *
+ * Errors: every call returns a tsr_status; tsr_last_error() gives the message
+ * (thread-local).  The status codes map the reference's exceptions (see the
+ * enum comments).  Calls on one context are serialised on its stream; handles
+ * created on one context must not be used with another.
+ */
+#ifndef TSR_GPU_H
+#define TSR_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum tsr_status {
+    TSR_OK = 0,
+    TSR_ERR_INVALID_ARGUMENT = 1,
+    TSR_ERR_NO_SAMPLES = 2,            /* Scene.cpp:137-151: reconstruct() returns false          */
+    TSR_ERR_SAMPLE_OUTSIDE_ROOT = 3,   /* Nodes.cpp:237-238 findContainer exception                */
+    TSR_ERR_NODE_OVERFLOW = 4,         /* Nodes.cpp:399-407 more than 2^32-2 nodes                  */
+    TSR_ERR_TREE_TOO_DEEP = 5,         /* octree deeper than 21 levels (63-bit location codes)     */
+    TSR_ERR_LINK_OVERFLOW = 6,         /* Samples.h:395-402 more than 2^32-1 camera links          */
+    TSR_ERR_NON_MANIFOLD_EDGE = 7,     /* FlexibleMesh.cpp:1227-1260 addEdgeConflict "BOOM"        */
+    TSR_ERR_HIP = 8,                   /* HIP runtime failure                                      */
+    TSR_ERR_OUT_OF_MEMORY = 9,
+    TSR_ERR_ZERO_SIZE_NODE = 10,       /* Nodes.cpp:303-304 node of size 0                          */
+    TSR_ERR_SAMPLE_NOT_INSERTED = 11,  /* Nodes.cpp:615-616                                          */
+    TSR_ERR_INVALID_SCALE = 12         /* Samples.cpp:280-288 non-positive sample scale             */
+} tsr_status;
+
+/* Parameters of the path, names as in App.cfg:63-73 (defaults = App.cfg values). */
+typedef struct tsr_params {
+    float bandwidth_factor;              /* Occupancy::bandwidthFactor = 2            */
+    float confidence_threshold;          /* Occupancy::confidenceThreshold = 100      */
+    float relative_radius_for_priors;    /* Occupancy::relativeRadiusForPriors = 25   */
+    float relative_sample_cone_length;   /* Occupancy::relativeSampleConeLength = 1   */
+    float max_relative_sampling_distance;/* Samples::maxRelativeSamplingDistance = 2  */
+} tsr_params;
+
+/* Occupancy accumulation modes (tsr_occupancy_compute flags). */
+enum {
+    TSR_ACCUM_KAHAN_SEGMENTED = 0,   /* FP32 compensated, deterministic segmented reduction (default) */
+    TSR_ACCUM_FP32_ATOMIC = 1,       /* plain FP32 atomics (precision study only, nondeterministic)  */
+    TSR_CULL_LEARNED = 0x100         /* skip emptiness cones in nodes the culling MLP marks far     */
+};
+
+typedef struct tsr_context tsr_context;
+typedef struct tsr_scene tsr_scene;
+typedef struct tsr_tree tsr_tree;
+typedef struct tsr_field tsr_field;
+typedef struct tsr_mesh tsr_mesh;
+typedef struct tsr_synthetic tsr_synthetic;
+typedef struct tsr_culler tsr_culler;
+
+/* Float32 SoA view of the samples (Samples.h:300-312) in host memory. */
+typedef struct tsr_samples_view {
+    uint32_t count;
+    uint32_t max_cameras_per_sample;   /* Samples::getMaxCamerasPerSample            */
+    const float *px, *py, *pz;         /* positions                                   */
+    const float *nx, *ny, *nz;         /* unit normals                                */
+    const float *scale;                /* sample scale (> 0)                          */
+    const float *confidence;
+    const uint32_t *parent_cameras;    /* count * max_cameras_per_sample; >= camera count = invalid */
+} tsr_samples_view;
+
+/* Camera centres (Cameras::getPositionWS, Cameras.h:161-176). */
+typedef struct tsr_cameras_view {
+    uint32_t count;
+    const float *x, *y, *z;
+} tsr_cameras_view;
+
+typedef struct tsr_tree_info {
+    uint32_t node_count;     /* Nodes::getCount          */
+    uint32_t leaf_count;     /* Leaves::getCount         */
+    uint32_t cell_count;     /* DualCells::getCellCount  */
+    uint32_t max_depth;
+    float root_min[3];       /* Tree::getRootScope       */
+    float root_size;         /* Tree::getRootNodeSize    */
+} tsr_tree_info;
+
+typedef struct tsr_mesh_info {
+    uint32_t vertex_count;
+    uint32_t triangle_count;
+} tsr_mesh_info;
+
+void tsr_params_default(tsr_params *out);
+const char *tsr_last_error(void);
+
+/* Context: one device, one HIP stream (default priority). */
+tsr_status tsr_context_create(int device, tsr_context **out);
+void tsr_context_destroy(tsr_context *ctx);
+tsr_status tsr_context_synchronize(tsr_context *ctx);
+/* Milliseconds of the last tree build / occupancy / DMC phases (HIP events):
+ * [0] tree, [1] S pass, [2] E pass, [3] priors, [4] occupancy, [5] DMC.  n <= 6. */
+tsr_status tsr_context_phase_times(tsr_context *ctx, float *ms, int n);
+
+/* Scene = device copy of samples + cameras (the Scene singleton's data). */
+tsr_status tsr_scene_upload(tsr_context *ctx, const tsr_samples_view *samples, const tsr_cameras_view *cameras,
+                            tsr_scene **out);
+void tsr_scene_destroy(tsr_scene *scene);
+/* Download the scene's samples in their current (tree) order. Arrays may be NULL. */
+tsr_status tsr_scene_download(tsr_context *ctx, const tsr_scene *scene, float *px, float *py, float *pz, float *nx,
+                              float *ny, float *nz, float *scale, float *confidence, uint32_t *parent_cameras);
+
+/* Tree::Tree(): octree nodes, leaves and dual cells built on the GPU.  The
+ * scene's samples are permuted into node order (Nodes.cpp:594-636). */
+tsr_status tsr_tree_build(tsr_context *ctx, tsr_scene *scene, const tsr_params *params, tsr_tree **out);
+tsr_status tsr_tree_get_info(const tsr_tree *tree, tsr_tree_info *info);
+/* Host copies of the tables (NULL pointers are skipped):
+ *   children/parents/sample_counts/sample_starts: node_count (Nodes.h:143-147)
+ *   node_to_leaf: node_count (Leaves.h:90); leaf_nodes: leaf_count
+ *   leaf_scopes: 4 floats per leaf = min x,y,z, size (Scope, Scope.h:36-38)
+ *   cells: 8 * cell_count leaf indices (DualCells.h:67)
+ *   sample_target: new index of each original sample (Nodes.cpp:625-626)
+ *   node_depth: node_count */
+tsr_status tsr_tree_download(tsr_context *ctx, const tsr_tree *tree, uint32_t *children, uint32_t *parents,
+                             uint32_t *sample_counts, uint32_t *sample_starts, uint32_t *node_to_leaf,
+                             uint32_t *leaf_nodes, float *leaf_scopes, uint32_t *cells, uint32_t *sample_target,
+                             uint32_t *node_depth);
+void tsr_tree_destroy(tsr_tree *tree);
+
+/* Occupancy::Occupancy(tree): S pass, flag propagation, E pass, priors and the
+ * FP64 occupancy ratio.  flags: TSR_ACCUM_* | optional TSR_CULL_LEARNED (then
+ * `culler` must be non-NULL). */
+tsr_status tsr_occupancy_compute(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *scene,
+                                 const tsr_params *params, int flags, const tsr_culler *culler, tsr_field **out);
+/* kernel_sums / cone_lengths: 2 * leaf_count floats, [0..L) = EMPTINESS, [L..2L) = SAMPLENESS
+ * (Occupancy.h:137-138); priors: leaf_count (mPriorsForEmptiness); node_states: node_count;
+ * occupancy: leaf_count doubles (getOccupancy, Occupancy.cpp:570-587); cone_count: M. */
+tsr_status tsr_field_download(tsr_context *ctx, const tsr_field *field, float *kernel_sums, float *cone_lengths,
+                              float *priors, uint32_t *node_states, double *occupancy, double *cone_count);
+/* counts of (link, leaf) kernel contributions of the last passes: [0] = E, [1] = S */
+tsr_status tsr_field_hit_counts(const tsr_field *field, uint64_t *hits2);
+void tsr_field_destroy(tsr_field *field);
+
+/* DualMarchingCells::extractMesh on the GPU (raw crust mesh, before isle
+ * removal / hole filling).  Fails with TSR_ERR_NON_MANIFOLD_EDGE like
+ * FlexibleMesh::addEdgeConflict. */
+tsr_status tsr_dmc_extract(tsr_context *ctx, const tsr_tree *tree, const tsr_field *field, tsr_mesh **out);
+tsr_status tsr_mesh_get_info(const tsr_mesh *mesh, tsr_mesh_info *info);
+/* positions: 3 doubles per vertex; scales: 1 double per vertex; edge_keys: 2
+ * leaf indices per vertex (EdgeVertexIndex); triangles: 3 vertex ids each. */
+tsr_status tsr_mesh_download(tsr_context *ctx, const tsr_mesh *mesh, double *positions, double *scales,
+                             uint32_t *edge_keys, uint32_t *triangles);
+void tsr_mesh_destroy(tsr_mesh *mesh);
+
+/* Learned node culling (no reference counterpart): MLP 16-64-64-64-1, BF16
+ * MFMA with FP32 accumulation.  Weights are a flat FP32 array in the order
+ * W1[64x16], b1[64], W2[64x64], b2[64], W3[64x64], b3[64], W4[1x64], b4[1]. */
+#define TSR_MLP_PARAM_COUNT (64 * 16 + 64 + 64 * 64 + 64 + 64 * 64 + 64 + 64 + 1)
+tsr_status tsr_culler_create(tsr_context *ctx, const float *weights, float threshold, tsr_culler **out);
+tsr_status tsr_culler_set_weights(tsr_context *ctx, tsr_culler *culler, const float *weights, float threshold);
+void tsr_culler_destroy(tsr_culler *culler);
+/* 16 float features per node for a tree + S-pass field. */
+tsr_status tsr_node_features(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *scene, const tsr_field *field,
+                             float *features_host);
+
+/* Seeded synthetic scenes (analytic stand-in for SyntheticScene.cpp): config 0..4,
+ * resolution_scale multiplies the image size. Views stay valid until destroy. */
+tsr_status tsr_synthetic_create(int config, uint64_t seed, double resolution_scale, tsr_synthetic **out);
+tsr_status tsr_synthetic_views(const tsr_synthetic *scene, tsr_samples_view *samples, tsr_cameras_view *cameras);
+void tsr_synthetic_destroy(tsr_synthetic *scene);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TSR_GPU_H */

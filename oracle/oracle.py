@@ -1,0 +1,147 @@
+"""ctypes wrapper of the CPU oracle (oracle/tsr_oracle.cpp) -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this module,
+and only as the checker / CPU baseline; the product (tsr_amd) never uses it.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} missing: run `make -C oracle`")
+        L = ctypes.CDLL(LIB_PATH)
+        f32p = ctypes.POINTER(ctypes.c_float)
+        u32p = ctypes.POINTER(ctypes.c_uint32)
+        L.oracle_create.restype = ctypes.c_void_p
+        L.oracle_create.argtypes = [ctypes.c_uint32] + [f32p] * 8 + [u32p, ctypes.c_uint32, ctypes.c_uint32] + \
+            [f32p] * 3 + [ctypes.POINTER(ctypes.c_double)]
+        L.oracle_destroy.argtypes = [ctypes.c_void_p]
+        L.oracle_error.restype = ctypes.c_char_p
+        L.oracle_error.argtypes = [ctypes.c_void_p]
+        L.oracle_build_tree.argtypes = [ctypes.c_void_p]
+        L.oracle_occupancy.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.oracle_dmc.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
+        L.oracle_count.restype = ctypes.c_uint64
+        L.oracle_count.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
+        L.oracle_data.restype = ctypes.c_void_p
+        L.oracle_data.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
+        L.oracle_mc_case.restype = ctypes.c_char_p
+        L.oracle_mc_case.argtypes = [ctypes.c_uint32]
+        L.oracle_mc_edges.restype = ctypes.c_uint32
+        L.oracle_mc_edges.argtypes = [ctypes.c_uint32]
+        _lib = L
+    return _lib
+
+
+class OracleError(RuntimeError):
+    pass
+
+
+class Oracle:
+    """Serial FP64 restatement of Tree / Occupancy / DualMarchingCells for one scene."""
+
+    def __init__(self, samples, cameras, params=(2.0, 100.0, 25.0, 1.0, 2.0), max_cams=1):
+        L = lib()
+        self._keep = {k: np.ascontiguousarray(v) for k, v in samples.items()}
+        self._keep.update({"c" + k: np.ascontiguousarray(v, dtype=np.float32) for k, v in cameras.items()})
+        s = self._keep
+        f = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))  # noqa: E731
+        prm = np.array(params, np.float64)
+        self._prm = prm
+        self.h = L.oracle_create(len(s["px"]), f(s["px"]), f(s["py"]), f(s["pz"]), f(s["nx"]), f(s["ny"]),
+                                 f(s["nz"]), f(s["scale"]), f(s["confidence"]),
+                                 s["parent_cameras"].ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), max_cams,
+                                 len(s["cx"]), f(s["cx"]), f(s["cy"]), f(s["cz"]),
+                                 prm.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().oracle_destroy(self.h)
+            self.h = None
+
+    def _check(self, rc):
+        if rc != 0:
+            raise OracleError(lib().oracle_error(self.h).decode())
+
+    def build_tree(self):
+        self._check(lib().oracle_build_tree(self.h))
+        return self
+
+    def occupancy(self, threads=1, brute_force=False):
+        self._check(lib().oracle_occupancy(self.h, (int(threads) << 8) | int(bool(brute_force))))
+        return self
+
+    def dmc(self, occupancy=None):
+        p = None
+        if occupancy is not None:
+            self._occ_in = np.ascontiguousarray(occupancy, np.float64)
+            p = self._occ_in.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        self._check(lib().oracle_dmc(self.h, p))
+        return self
+
+    def count(self, what):
+        return int(lib().oracle_count(self.h, what.encode()))
+
+    def array(self, what, dtype, n):
+        ptr = lib().oracle_data(self.h, what.encode())
+        if not ptr or n == 0:
+            return np.zeros(n, dtype)
+        buf = (ctypes.c_char * (np.dtype(dtype).itemsize * n)).from_address(ptr)
+        return np.frombuffer(buf, dtype=dtype, count=n).copy()
+
+    def tree(self):
+        N, Lc, C, S = self.count("nodes"), self.count("leaves"), self.count("cells"), self.count("samples")
+        return {
+            "children": self.array("children", np.uint32, N),
+            "parents": self.array("parents", np.uint32, N),
+            "sample_counts": self.array("sample_counts", np.uint32, N),
+            "sample_starts": self.array("sample_starts", np.uint32, N),
+            "node_to_leaf": self.array("node_to_leaf", np.uint32, N),
+            "leaf_nodes": self.array("leaf_node", np.uint32, Lc),
+            "leaf_scopes": self.array("leaf_scope", np.float32, 4 * Lc),
+            "leaf_depth": self.array("leaf_depth", np.uint32, Lc),
+            "cells": self.array("cells", np.uint32, 8 * C),
+            "sample_target": self.array("sample_target", np.uint32, S),
+            "sample_depth": self.array("sample_depth", np.uint32, S),
+            "root_min": self.array("root_min", np.float32, 3),
+            "root_size": self.array("root_size", np.float32, 1)[0],
+        }
+
+    def field(self):
+        Lc, N = self.count("leaves"), self.count("nodes")
+        return {
+            "kernel_sums": self.array("kernel_sums", np.float64, 2 * Lc),
+            "cone_lengths": self.array("cone_lengths", np.float64, 2 * Lc),
+            "priors": self.array("priors", np.float64, Lc),
+            "node_states": self.array("node_states", np.uint32, N),
+            "occupancy": self.array("occupancy", np.float64, Lc),
+            "cone_count": self.array("cone_count", np.float64, 1)[0],
+        }
+
+    def mesh(self):
+        V, T = self.count("vertices"), self.count("triangles")
+        pos = np.stack([self.array("vpos_x", np.float64, V), self.array("vpos_y", np.float64, V),
+                        self.array("vpos_z", np.float64, V)], axis=1).reshape(-1)
+        keys = np.stack([self.array("vkey0", np.uint32, V), self.array("vkey1", np.uint32, V)], axis=1).reshape(-1)
+        return {"positions": pos, "scales": self.array("vscale", np.float64, V), "edge_keys": keys,
+                "triangles": self.array("triangles", np.uint32, 3 * T), "conflicts": self.count("conflicts")}
+
+    def timings(self):
+        return self.array("timings", np.float64, 6)
+
+
+def mc_case(cfg):
+    return lib().oracle_mc_case(cfg).decode()
+
+
+def mc_edges(cfg):
+    return int(lib().oracle_mc_edges(cfg))

@@ -1,0 +1,8 @@
+"""MI355X (gfx950) occupancy field + dual marching cells for TSR-style surface reconstruction.
+
+The compute path is libtsr_amd.so (HIP kernels behind the C ABI in include/tsr_gpu.h);
+this package is its Python binding.
+"""
+from ._lib import TsrError, default_params, lib  # noqa: F401
+from .api import (Context, DualMarchingCells, Occupancy, Scene, SyntheticScene, Tree,  # noqa: F401
+                  reconstruct)

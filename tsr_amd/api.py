@@ -1,0 +1,263 @@
+"""Python face of the GPU occupancy + DMC library, shaped like TSR's hot path.
+
+Scene.reconstruct() follows Scene::reconstruct (SR/Scene/Scene.cpp:169-197):
+Tree() -> Occupancy(tree) -> DualMarchingCells::extractMesh.  The objects own
+device memory through the C ABI (include/tsr_gpu.h); downloads return numpy arrays.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib as L
+
+
+def _fp(a):
+    return a.ctypes.data_as(L.f32p) if a is not None else None
+
+
+def _dp(a):
+    return a.ctypes.data_as(L.f64p) if a is not None else None
+
+
+def _up(a):
+    return a.ctypes.data_as(L.u32p) if a is not None else None
+
+
+class Context:
+    """One HIP device + one stream (tsr_context)."""
+
+    def __init__(self, device=0):
+        h = L.vp()
+        L.check(L.lib().tsr_context_create(device, ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            L.lib().tsr_context_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def synchronize(self):
+        L.check(L.lib().tsr_context_synchronize(self.h))
+
+    def phase_times(self):
+        """Milliseconds: tree, S pass, E pass, priors, occupancy, DMC (HIP events)."""
+        ms = np.zeros(6, np.float32)
+        L.check(L.lib().tsr_context_phase_times(self.h, _fp(ms), 6))
+        return ms
+
+
+class SyntheticScene:
+    """Seeded analytic scene (config 0..4) generated on the host (tsr_synthetic_*)."""
+
+    def __init__(self, config, seed=0, resolution_scale=1.0):
+        h = L.vp()
+        L.check(L.lib().tsr_synthetic_create(int(config), int(seed), float(resolution_scale), ctypes.byref(h)))
+        self.h = h
+        sv, cv = L.SamplesView(), L.CamerasView()
+        L.check(L.lib().tsr_synthetic_views(h, ctypes.byref(sv), ctypes.byref(cv)))
+        n, k = sv.count, cv.count
+
+        def arr(ptr, count, dt):
+            if count == 0:
+                return np.zeros(0, dt)
+            return np.ctypeslib.as_array(ptr, shape=(count,)).astype(dt, copy=True)
+
+        self.samples = {
+            "px": arr(sv.px, n, np.float32), "py": arr(sv.py, n, np.float32), "pz": arr(sv.pz, n, np.float32),
+            "nx": arr(sv.nx, n, np.float32), "ny": arr(sv.ny, n, np.float32), "nz": arr(sv.nz, n, np.float32),
+            "scale": arr(sv.scale, n, np.float32), "confidence": arr(sv.confidence, n, np.float32),
+            "parent_cameras": arr(sv.parent_cameras, n, np.uint32),
+        }
+        self.cameras = {"x": arr(cv.x, k, np.float32), "y": arr(cv.y, k, np.float32), "z": arr(cv.z, k, np.float32)}
+        L.lib().tsr_synthetic_destroy(h)
+        self.h = None
+
+    @property
+    def count(self):
+        return len(self.samples["px"])
+
+
+def samples_view(samples, max_cams=1):
+    s = {k: np.ascontiguousarray(v) for k, v in samples.items()}
+    v = L.SamplesView()
+    v.count = len(s["px"])
+    v.max_cameras_per_sample = max_cams
+    for k in ("px", "py", "pz", "nx", "ny", "nz", "scale", "confidence"):
+        setattr(v, k, _fp(s[k]))
+    v.parent_cameras = _up(s["parent_cameras"])
+    return v, s
+
+
+def cameras_view(cameras):
+    c = {k: np.ascontiguousarray(v, dtype=np.float32) for k, v in cameras.items()}
+    v = L.CamerasView()
+    v.count = len(c["x"])
+    v.x, v.y, v.z = _fp(c["x"]), _fp(c["y"]), _fp(c["z"])
+    return v, c
+
+
+class Scene:
+    """Device copy of samples + cameras (the data TSR reads through Scene::getSingleton)."""
+
+    def __init__(self, ctx, samples, cameras, max_cams=1):
+        self.ctx = ctx
+        sv, keep1 = samples_view(samples, max_cams)
+        cv, keep2 = cameras_view(cameras)
+        h = L.vp()
+        L.check(L.lib().tsr_scene_upload(ctx.h, ctypes.byref(sv), ctypes.byref(cv), ctypes.byref(h)))
+        self.h = h
+        self.count = sv.count
+        self.max_cams = max_cams
+
+    def close(self):
+        if self.h:
+            L.lib().tsr_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def download(self):
+        n = self.count
+        out = {k: np.zeros(n, np.float32) for k in ("px", "py", "pz", "nx", "ny", "nz", "scale", "confidence")}
+        out["parent_cameras"] = np.zeros(n * self.max_cams, np.uint32)
+        L.check(L.lib().tsr_scene_download(self.ctx.h, self.h, *[_fp(out[k]) for k in
+                                                                  ("px", "py", "pz", "nx", "ny", "nz", "scale",
+                                                                   "confidence")], _up(out["parent_cameras"])))
+        return out
+
+
+class Tree:
+    """Tree::Tree() on the GPU (nodes, leaves, dual cells; permutes the scene's samples)."""
+
+    def __init__(self, ctx, scene, params=None):
+        self.ctx = ctx
+        p = params or L.default_params()
+        h = L.vp()
+        L.check(L.lib().tsr_tree_build(ctx.h, scene.h, ctypes.byref(p), ctypes.byref(h)))
+        self.h = h
+        info = L.TreeInfo()
+        L.check(L.lib().tsr_tree_get_info(h, ctypes.byref(info)))
+        self.node_count, self.leaf_count, self.cell_count = info.node_count, info.leaf_count, info.cell_count
+        self.max_depth = info.max_depth
+        self.root_min = np.array(info.root_min[:], np.float32)
+        self.root_size = np.float32(info.root_size)
+        self.sample_count = scene.count
+
+    def close(self):
+        if self.h:
+            L.lib().tsr_tree_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def download(self):
+        N, Lc, C, S = self.node_count, self.leaf_count, self.cell_count, self.sample_count
+        o = {
+            "children": np.zeros(N, np.uint32), "parents": np.zeros(N, np.uint32),
+            "sample_counts": np.zeros(N, np.uint32), "sample_starts": np.zeros(N, np.uint32),
+            "node_to_leaf": np.zeros(N, np.uint32), "leaf_nodes": np.zeros(Lc, np.uint32),
+            "leaf_scopes": np.zeros(4 * Lc, np.float32), "cells": np.zeros(8 * C, np.uint32),
+            "sample_target": np.zeros(S, np.uint32), "node_depth": np.zeros(N, np.uint32),
+        }
+        L.check(L.lib().tsr_tree_download(self.ctx.h, self.h, _up(o["children"]), _up(o["parents"]),
+                                          _up(o["sample_counts"]), _up(o["sample_starts"]), _up(o["node_to_leaf"]),
+                                          _up(o["leaf_nodes"]), _fp(o["leaf_scopes"]), _up(o["cells"]),
+                                          _up(o["sample_target"]), _up(o["node_depth"])))
+        return o
+
+
+class Occupancy:
+    """Occupancy::Occupancy(tree) on the GPU."""
+
+    def __init__(self, ctx, tree, scene, params=None, flags=L.TSR_ACCUM_KAHAN_SEGMENTED, culler=None):
+        self.ctx = ctx
+        p = params or L.default_params()
+        h = L.vp()
+        L.check(L.lib().tsr_occupancy_compute(ctx.h, tree.h, scene.h, ctypes.byref(p), int(flags),
+                                              culler.h if culler is not None else None, ctypes.byref(h)))
+        self.h = h
+        self.leaf_count = tree.leaf_count
+        self.node_count = tree.node_count
+
+    def close(self):
+        if self.h:
+            L.lib().tsr_field_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def hit_counts(self):
+        h = np.zeros(2, np.uint64)
+        L.check(L.lib().tsr_field_hit_counts(self.h, h.ctypes.data_as(L.u64p)))
+        return h
+
+    def download(self):
+        Lc, N = self.leaf_count, self.node_count
+        o = {"kernel_sums": np.zeros(2 * Lc, np.float32), "cone_lengths": np.zeros(2 * Lc, np.float32),
+             "priors": np.zeros(Lc, np.float32), "node_states": np.zeros(N, np.uint32),
+             "occupancy": np.zeros(Lc, np.float64)}
+        m = ctypes.c_double(0.0)
+        L.check(L.lib().tsr_field_download(self.ctx.h, self.h, _fp(o["kernel_sums"]), _fp(o["cone_lengths"]),
+                                           _fp(o["priors"]), _up(o["node_states"]), _dp(o["occupancy"]),
+                                           ctypes.byref(m)))
+        o["cone_count"] = m.value
+        return o
+
+
+class DualMarchingCells:
+    """DualMarchingCells::extractMesh on the GPU (raw crust mesh)."""
+
+    def __init__(self, ctx, tree, occupancy):
+        self.ctx = ctx
+        h = L.vp()
+        L.check(L.lib().tsr_dmc_extract(ctx.h, tree.h, occupancy.h, ctypes.byref(h)))
+        self.h = h
+        info = L.MeshInfo()
+        L.check(L.lib().tsr_mesh_get_info(h, ctypes.byref(info)))
+        self.vertex_count, self.triangle_count = info.vertex_count, info.triangle_count
+
+    def close(self):
+        if self.h:
+            L.lib().tsr_mesh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def download(self):
+        V, T = self.vertex_count, self.triangle_count
+        o = {"positions": np.zeros(3 * V, np.float64), "scales": np.zeros(V, np.float64),
+             "edge_keys": np.zeros(2 * V, np.uint32), "triangles": np.zeros(3 * T, np.uint32)}
+        L.check(L.lib().tsr_mesh_download(self.ctx.h, self.h, _dp(o["positions"]), _dp(o["scales"]),
+                                          _up(o["edge_keys"]), _up(o["triangles"])))
+        return o
+
+
+def reconstruct(ctx, samples, cameras, params=None, flags=L.TSR_ACCUM_KAHAN_SEGMENTED, culler=None):
+    """Scene::reconstruct (Scene.cpp:169-197) up to the raw crust mesh."""
+    scene = Scene(ctx, samples, cameras)
+    tree = Tree(ctx, scene, params)
+    occ = Occupancy(ctx, tree, scene, params, flags, culler)
+    mesh = DualMarchingCells(ctx, tree, occ)
+    return scene, tree, occ, mesh

@@ -1,0 +1,283 @@
+// Definitions shared by the host and device code of the library: constants,
+// the FP64 geometry frame of the octree, location codes, and the BaseProject
+// stand-ins (EPSILON, plane distance, sphere/box and segment/box tests) that
+// the reference's tree and occupancy code call.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define TSR_HD __host__ __device__ __forceinline__
+
+namespace tsr {
+
+typedef uint32_t u32;
+typedef uint64_t u64;
+
+constexpr float EPS_F = 1e-6f;                       // Math::EPSILON (BaseProject; value fixed here)
+constexpr double EPS_D = (double)EPS_F;
+constexpr u32 INVALID = 0xffffffffu;                 // Nodes::INVALID_INDEX (Nodes.cpp:30)
+constexpr u32 MAX_DEPTH = 19;                        // deepest node level
+constexpr u32 MAX_DEPTH_DIFFERENCE = 3;              // Occupancy.cpp:72
+constexpr u32 FLAG_S = 1u;                           // NODE_FLAG_SAMPLENESS (Occupancy.h:38)
+constexpr u32 FLAG_E = 2u;                           // NODE_FLAG_EMPTINESS (Occupancy.h:39)
+constexpr u32 LEAF_BIT = 0x80000000u;                // child field of a leaf: LEAF_BIT | leaf index
+constexpr double PI_D = 3.14159265358979323846;
+
+// ---------------------------------------------------------------------------
+// Location codes: loc = (1 << 3d) | morton_d(kx, ky, kz); child slot c has
+// bits x = 1, y = 2, z = 4 as in Nodes::getCoords (Nodes.cpp:46-58).
+// ---------------------------------------------------------------------------
+TSR_HD u64 spread3(u64 v) {   // 21 bits -> every third bit
+    v &= 0x1fffffull;
+    v = (v | (v << 32)) & 0x1f00000000ffffull;
+    v = (v | (v << 16)) & 0x1f0000ff0000ffull;
+    v = (v | (v << 8)) & 0x100f00f00f00f00full;
+    v = (v | (v << 4)) & 0x10c30c30c30c30c3ull;
+    v = (v | (v << 2)) & 0x1249249249249249ull;
+    return v;
+}
+TSR_HD u32 compact3(u64 v) {
+    v &= 0x1249249249249249ull;
+    v = (v ^ (v >> 2)) & 0x10c30c30c30c30c3ull;
+    v = (v ^ (v >> 4)) & 0x100f00f00f00f00full;
+    v = (v ^ (v >> 8)) & 0x1f0000ff0000ffull;
+    v = (v ^ (v >> 16)) & 0x1f00000000ffffull;
+    v = (v ^ (v >> 32)) & 0x1fffffull;
+    return (u32)v;
+}
+TSR_HD u64 makeLoc(u32 kx, u32 ky, u32 kz, u32 d) {
+    return (1ull << (3 * d)) | spread3(kx) | (spread3(ky) << 1) | (spread3(kz) << 2);
+}
+TSR_HD u32 locDepth(u64 loc) {   // position of the leading marker bit / 3
+    return (63u - (u32)__builtin_clzll(loc)) / 3u;
+}
+TSR_HD void locCoords(u64 loc, u32 &kx, u32 &ky, u32 &kz) {
+    const u32 d = locDepth(loc);
+    const u64 m = loc ^ (1ull << (3 * d));
+    kx = compact3(m);
+    ky = compact3(m >> 1);
+    kz = compact3(m >> 2);
+}
+// pre-order DFS key of a node: Morton code padded to MAX_DEPTH, depth in the low bits
+TSR_HD u64 dfsKey(u64 loc) {
+    const u32 d = locDepth(loc);
+    const u64 m = loc ^ (1ull << (3 * d));
+    return (m << (3 * (MAX_DEPTH - d) + 5)) | (u64)d;
+}
+
+// ---------------------------------------------------------------------------
+// FP64 frame: boundary k at depth d is B(k,d) = rootMin + k * (rootSize 2^-d),
+// one rounding; centres are B(2k+1, d+1).  Identical on host and device.
+// ---------------------------------------------------------------------------
+struct Frame {
+    double rootMin[3];
+    double rootSize;
+    float rootMinF[3];
+    float rootSizeF;
+    TSR_HD double h(u32 d) const { return ldexp(rootSize, -(int)d); }
+    TSR_HD double B(int a, u64 k, u32 d) const { return rootMin[a] + (double)k * h(d); }
+    TSR_HD double centre(int a, u64 k, u32 d) const { return B(a, 2 * k + 1, d + 1); }
+};
+
+// Samples::getDistanceToPlane (Samples.cpp:451-455): dot(x - p, n)
+TSR_HD double planeDistance(double x, double y, double z, const double p[3], const double n[3]) {
+    return (x - p[0]) * n[0] + (y - p[1]) * n[1] + (z - p[2]) * n[2];
+}
+
+// Per-unit data of the sampling refinement (Nodes.cpp:271-337): sample position,
+// normal and max sampling distance, all promoted from float32.
+struct SamplingUnit {
+    double p[3], n[3], msd;
+};
+
+// Nodes.cpp:471-488 fittingSamplingNodeCenter for node (k, d)
+TSR_HD bool fittingCentre(const Frame &f, u32 kx, u32 ky, u32 kz, u32 d, bool positive, const SamplingUnit &u) {
+    const double cx = f.B(0, 2ull * kx + 1, d + 1), cy = f.B(1, 2ull * ky + 1, d + 1), cz = f.B(2, 2ull * kz + 1, d + 1);
+    const double dx = cx - u.p[0], dy = cy - u.p[1], dz = cz - u.p[2];
+    const double distSq = dx * dx + dy * dy + dz * dz;
+    if (distSq > u.msd * u.msd) return false;
+    const bool onPositive = planeDistance(cx, cy, cz, u.p, u.n) > 0.0;
+    return onPositive == positive;
+}
+
+// Nodes.cpp:490-517 intersect: a corner on the wanted side of the sample plane
+// and box/sphere overlap (squared distance <= msd^2).
+TSR_HD bool intersectNode(const Frame &f, u32 kx, u32 ky, u32 kz, u32 d, bool positive, const SamplingUnit &u) {
+    bool anyRight = false;
+    for (u32 c = 0; c < 8 && !anyRight; ++c) {
+        const double x = f.B(0, (u64)kx + (c & 1), d), y = f.B(1, (u64)ky + ((c >> 1) & 1), d),
+                     z = f.B(2, (u64)kz + ((c >> 2) & 1), d);
+        const bool inPositive = planeDistance(x, y, z, u.p, u.n) > 0.0;
+        if (inPositive == positive) anyRight = true;
+    }
+    if (!anyRight) return false;
+    const u32 k[3] = {kx, ky, kz};
+    double d2 = 0.0;
+    for (int a = 0; a < 3; ++a) {
+        const double lo = f.B(a, k[a], d), hi = f.B(a, (u64)k[a] + 1, d);
+        if (u.p[a] < lo) d2 += (lo - u.p[a]) * (lo - u.p[a]);
+        else if (u.p[a] > hi) d2 += (u.p[a] - hi) * (u.p[a] - hi);
+    }
+    return d2 <= u.msd * u.msd;
+}
+
+// Slab clip of segment st->en against a box, parameters in [0,1]
+// (CollisionDetection::intersectAABBWithLineSegment stand-in, Nodes.cpp:319).
+TSR_HD bool segmentBox(double &tIn, double &tOut, const double st[3], const double en[3], const double lo[3],
+                       const double hi[3]) {
+    tIn = 0.0;
+    tOut = 1.0;
+    for (int a = 0; a < 3; ++a) {
+        const double dir = en[a] - st[a];
+        if (dir == 0.0) {
+            if (st[a] < lo[a] || st[a] > hi[a]) return false;
+            continue;
+        }
+        double t1 = (lo[a] - st[a]) / dir, t2 = (hi[a] - st[a]) / dir;
+        if (t1 > t2) { const double tmp = t1; t1 = t2; t2 = tmp; }
+        if (t1 > tIn) tIn = t1;
+        if (t2 < tOut) tOut = t2;
+    }
+    return tIn <= tOut;
+}
+
+// Tree::getSampleDepth restated exactly (Tree.cpp:86-91 / Nodes.cpp:250):
+// smallest d with rootSize 2^-d <= max sampling distance; MAX_DEPTH+1 = too deep.
+TSR_HD u32 containerDepth(const Frame &f, double msd) {
+    for (u32 d = 0; d <= MAX_DEPTH; ++d)
+        if (f.h(d) <= msd) return d;
+    return MAX_DEPTH + 1;
+}
+
+// Integer cell of coordinate x at depth d, consistent with the FP64 boundaries:
+// the largest k in [0, 2^d) with B(k, d) <= x.
+TSR_HD u32 cellCoord(const Frame &f, int a, double x, u32 d) {
+    const u64 lim = 1ull << d;
+    double q = floor((x - f.rootMin[a]) / f.h(d));
+    if (!(q >= 0.0)) q = 0.0;
+    if (q > (double)(lim - 1)) q = (double)(lim - 1);
+    u64 k = (u64)q;
+    while (k > 0 && f.B(a, k, d) > x) --k;
+    while (k + 1 < lim && f.B(a, k + 1, d) <= x) ++k;
+    return (u32)k;
+}
+
+// ---------------------------------------------------------------------------
+// Cone (ObliqueCircularCone stand-in) and the cone kernels, Occupancy.cpp:279-439.
+//   apex a, end centre e, outside normal n of the end disc, end radius R, cut length L:
+//   tComplete(x) = dot(x - a, n) / dot(e - a, n); disc at t centred a + (e - a) t, radius R t;
+//   cut cone t in [tStart, 1], tStart = max(0, 1 - L/|e - a|), getLength() = min(L, |e - a|),
+//   tCut = (tComplete - tStart) / (1 - tStart); degenerate iff dot(e - a, n) <= 0 or |e-a| = 0 or R <= 0.
+// ---------------------------------------------------------------------------
+struct ConeD {
+    double a[3], v[3], n[3];
+    double R, invDenom, tStart, invCut, length;
+    bool degenerate;
+};
+
+TSR_HD void coneInit(ConeD &c, const double apex[3], const double endNormal[3], const double end[3], double R,
+                     double L) {
+    for (int i = 0; i < 3; ++i) {
+        c.a[i] = apex[i];
+        c.n[i] = endNormal[i];
+        c.v[i] = end[i] - apex[i];
+    }
+    const double denom = c.v[0] * c.n[0] + c.v[1] * c.n[1] + c.v[2] * c.n[2];
+    const double A = sqrt(c.v[0] * c.v[0] + c.v[1] * c.v[1] + c.v[2] * c.v[2]);
+    c.R = R;
+    c.degenerate = !(denom > 0.0) || !(A > 0.0) || !(R > 0.0);
+    c.invDenom = 0; c.tStart = 0; c.invCut = 1; c.length = 0;
+    if (c.degenerate) return;
+    c.invDenom = 1.0 / denom;
+    if (L >= A) {
+        c.tStart = 0.0;
+        c.length = A;
+        c.invCut = 1.0;
+    } else {
+        c.tStart = 1.0 - L / A;
+        c.length = L;
+        c.invCut = 1.0 / (1.0 - c.tStart);
+    }
+}
+
+// Occupancy.cpp:429-439 computeRayKernel (negative parabola)
+TSR_HD double rayKernel(double z, double Z) {
+    if (z < 0.0 || z >= Z) return 0.0;
+    const double z2 = z * z, Z2 = Z * Z;
+    return (3.0 / (2.0 * Z)) * (-z2 / Z2 + 1.0);
+}
+// Occupancy.cpp:392-400 computeCircularDiscKernelFromSquared (negative parabola)
+TSR_HD double discKernel(double r2, double R) {
+    const double R2 = R * R;
+    if (r2 >= R2) return 0.0;
+    const double notNormed = 1.0 - r2 / R2;
+    return (2.0 / (PI_D * R2)) * notNormed;
+}
+// Occupancy.cpp:279-346 addKernel: scaled kernel at x, or 0 when rejected.
+TSR_HD double coneKernel(const ConeD &c, const double x[3], bool sampleness, double confidence) {
+    const double d0 = x[0] - c.a[0], d1 = x[1] - c.a[1], d2 = x[2] - c.a[2];
+    const double tc = (d0 * c.n[0] + d1 * c.n[1] + d2 * c.n[2]) * c.invDenom;
+    const double tcut = (tc - c.tStart) * c.invCut;
+    if (tcut < 0.0 || tcut >= 1.0) return 0.0;
+    const double q0 = x[0] - (c.a[0] + c.v[0] * tc), q1 = x[1] - (c.a[1] + c.v[1] * tc),
+                 q2 = x[2] - (c.a[2] + c.v[2] * tc);
+    const double r2 = q0 * q0 + q1 * q1 + q2 * q2;
+    const double pcdR = c.R * tc;
+    if (r2 >= pcdR * pcdR) return 0.0;
+    double Z, z, norm;
+    if (!sampleness) {
+        Z = c.length;
+        z = tcut * Z;
+        norm = 1.0;
+    } else {
+        const double tc2 = fabs(tcut - 0.5) * 2.0;
+        Z = 0.5 * c.length;
+        z = tc2 * Z;
+        norm = 0.5;
+    }
+    const double k = rayKernel(z, Z) * norm * discKernel(r2, pcdR);
+    if (EPS_D > k) return 0.0;
+    const double sk = confidence * k;
+    if (EPS_D > sk) return 0.0;
+    return sk;
+}
+
+// Conservative cone/box culling in FP32 (stand-in for
+// ObliqueCircularCone::haveSeparatingAxis, ConeNodesChecker.cpp:65): the cut cone
+// lies inside the union of balls B(a + v t, R t), t in [tStart, 1]; a box is
+// culled only if its bounding ball misses that union by a safety margin.
+struct ConeCull {
+    float a[3], vhat[3];
+    float A, R, tStart, slope;   // slope = R / sqrt(A^2 - R^2), < 0 when R >= A
+};
+
+TSR_HD void cullInit(ConeCull &k, const ConeD &c) {
+    const double A = sqrt(c.v[0] * c.v[0] + c.v[1] * c.v[1] + c.v[2] * c.v[2]);
+    for (int i = 0; i < 3; ++i) {
+        k.a[i] = (float)c.a[i];
+        k.vhat[i] = (float)(c.v[i] / A);
+    }
+    k.A = (float)A;
+    k.R = (float)c.R;
+    k.tStart = (float)c.tStart;
+    k.slope = c.R < A ? (float)(c.R / sqrt(A * A - c.R * c.R)) : -1.0f;
+}
+
+// true if the cube [lo, lo + size]^3 cannot meet the cone
+TSR_HD bool cullSeparated(const ConeCull &k, float lx, float ly, float lz, float size, float absMargin) {
+    const float half = 0.5f * size;
+    const float b0 = lx + half - k.a[0], b1 = ly + half - k.a[1], b2 = lz + half - k.a[2];
+    const float u = b0 * k.vhat[0] + b1 * k.vhat[1] + b2 * k.vhat[2];
+    const float p0 = b0 - u * k.vhat[0], p1 = b1 - u * k.vhat[1], p2 = b2 - u * k.vhat[2];
+    const float w2 = p0 * p0 + p1 * p1 + p2 * p2;
+    const float w = sqrtf(w2);
+    float ts = k.slope < 0.0f ? 1.0f : (u + k.slope * w) / k.A;
+    ts = fminf(1.0f, fmaxf(k.tStart, ts));
+    const float du = u - ts * k.A;
+    const float f = sqrtf(du * du + w2) - k.R * ts;
+    const float rho = half * 1.7320508f * 1.01f + absMargin;
+    return f > rho;
+}
+
+}  // namespace tsr

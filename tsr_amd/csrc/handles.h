@@ -1,0 +1,89 @@
+// Handle structs behind the opaque C ABI types.
+#pragma once
+
+#include <vector>
+
+#include "device_util.h"
+
+struct tsr_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[12] = {};   // start/end pair per phase
+    float phaseMs[6] = {0, 0, 0, 0, 0, 0};
+    tsr::Scratch scratch;
+};
+
+struct tsr_scene {
+    tsr_context *ctx = nullptr;
+    tsr::u32 n = 0, maxCams = 1, camCount = 0;
+    tsr::DBuf<float> px, py, pz, nx, ny, nz, scale, conf;
+    tsr::DBuf<tsr::u32> cams;
+    tsr::DBuf<float> cx, cy, cz;
+    // host copies of the geometry in the current sample order, read by the
+    // order-dependent replay of the sampling refinement
+    std::vector<float> hpx, hpy, hpz, hnx, hny, hnz, hscale;
+};
+
+struct tsr_tree {
+    tsr_context *ctx = nullptr;
+    tsr::Frame frame{};
+    float maxRel = 2.0f;
+    tsr::u32 nodeCount = 0, innerCount = 0, leafCount = 0, cellCount = 0, maxDepth = 0, sampleCount = 0;
+    tsr::DBuf<tsr::u32> child;         // per node: first child id, or LEAF_BIT | leaf index
+    tsr::DBuf<tsr::u32> parents;       // Nodes::mParents
+    tsr::DBuf<tsr::u32> sampleCounts;  // Nodes::mSamplesPerNodes
+    tsr::DBuf<tsr::u32> sampleStarts;  // Nodes::mSampleStartIndices
+    tsr::DBuf<tsr::u32> nodeToLeaf;    // Leaves::mNodeToLeafLinks
+    tsr::DBuf<uint8_t> nodeDepth;
+    tsr::DBuf<tsr::u64> nodeLoc;       // location code per node id
+    tsr::DBuf<tsr::u64> leafLoc;       // location code per leaf (DFS order)
+    tsr::DBuf<tsr::u32> leafNode;      // node id per leaf
+    tsr::DBuf<tsr::u32> cells;         // 8 leaf indices per dual cell (DualCells::mIndices)
+    tsr::DBuf<tsr::u32> sampleTarget;  // new index of each original sample
+    tsr::DBuf<uint8_t> sampleDepth;    // container depth per sample (tree order)
+    tsr::DHash nodeHash;               // location code -> node id (all nodes)
+    tsr::u64 samplingEvents = 0, samplingCreated = 0, samplingUnits = 0;
+};
+
+struct tsr_field {
+    tsr_context *ctx = nullptr;
+    tsr::u32 leafCount = 0, nodeCount = 0;
+    tsr::DBuf<float> ksum;     // [2][L] kernel sums, 0 = EMPTINESS, 1 = SAMPLENESS
+    tsr::DBuf<float> clen;     // [2][L] cone length sums
+    tsr::DBuf<float> prior;    // [L] prior for emptiness
+    tsr::DBuf<tsr::u32> states;
+    tsr::DBuf<double> occ;     // [L] occupancy (FP64)
+    double coneCount = 0;
+    tsr::u64 hits[2] = {0, 0};
+};
+
+struct tsr_mesh {
+    tsr_context *ctx = nullptr;
+    tsr::u32 vertexCount = 0, triangleCount = 0;
+    tsr::DBuf<double> pos;      // 3 per vertex
+    tsr::DBuf<double> scale;    // 1 per vertex
+    tsr::DBuf<tsr::u32> keys;   // 2 per vertex
+    tsr::DBuf<tsr::u32> tris;   // 3 per triangle
+};
+
+namespace tsr {
+// phase: 0 tree, 1 S pass, 2 E pass, 3 priors, 4 occupancy, 5 DMC
+inline void recordPhase(tsr_context *ctx, int phase, bool start) {
+    HIPCHECK(hipEventRecord(ctx->ev[2 * phase + (start ? 0 : 1)], ctx->stream));
+}
+void buildTree(tsr_context *ctx, tsr_scene *scene, const tsr_params &prm, tsr_tree *tree);
+void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *scene, const tsr_params &prm,
+                      int flags, const uint8_t *cullMask, tsr_field *field);
+void extractDMC(tsr_context *ctx, const tsr_tree *tree, const tsr_field *field, tsr_mesh *mesh);
+}  // namespace tsr
+
+struct tsr_culler;
+namespace tsr {
+// per-node culling mask of the learned emptiness-cone culling (1 = skip node)
+void tsrCullerMaskImpl(tsr_context *ctx, const tsr_culler *culler, const tsr_tree *tree, const tsr_scene *scene,
+                       const tsr_params &prm, DBuf<uint8_t> &mask);
+}  // namespace tsr
+inline void tsrCullerMask(tsr_context *ctx, const tsr_culler *culler, const tsr_tree *tree, const tsr_scene *scene,
+                          const tsr_params &prm, tsr::DBuf<uint8_t> &mask) {
+    tsr::tsrCullerMaskImpl(ctx, culler, tree, scene, prm, mask);
+}

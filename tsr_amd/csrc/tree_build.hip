@@ -1,0 +1,819 @@
+// GPU scene-octree build (Tree::Tree, SR/Scene/Tree/Tree.cpp:26-37).
+//
+// Stages (reference -> here):
+//   root scope        Tree.cpp:70-101, Samples.cpp:290-298   block min/max reduction (float, exact)
+//   containment       Nodes.cpp:207-211,222-269              container location codes + ancestor
+//                                                             closure into a GPU hash set
+//   sampling refine   Nodes.cpp:214-219,271-337,433-517      order-dependent: speculative GPU
+//                                                             evaluation of every (sample, side) unit
+//                                                             on the containment tree; exact in-order
+//                                                             replay of the few creating / disturbed
+//                                                             units on the host (see replaySampling)
+//   balancing         Nodes.cpp:339-395                       level-free fixpoint of forced splits
+//   reorder           Nodes.cpp:519-592                       radix sort by pre-order DFS key
+//   sample reorder    Nodes.cpp:594-636, Samples.cpp:825-835 stable radix sort by container node id
+//   leaves            Leaves.cpp:44-80                        leaves sorted by DFS key
+//   dual cells        DualCells.cpp:49-357                    one cell per interior leaf corner, ordered
+//                                                             by the Schaefer-Warren recursion path key
+#include <algorithm>
+#include <cstring>
+#include <queue>
+#include <unordered_set>
+#include <vector>
+
+#include "handles.h"
+
+namespace tsr {
+
+namespace {
+
+constexpr u32 BLOCK = 256;
+constexpr u32 VISIT_CAP = 48;   // visited-leaf records per sampling unit before it counts as an event
+
+// error bits written by kernels
+enum : u32 { ERR_OUTSIDE = 1, ERR_TOO_DEEP = 2, ERR_HASH_FULL = 4, ERR_NOT_INSERTED = 8 };
+
+// ---------------------------------------------------------------------------
+// root scope
+// ---------------------------------------------------------------------------
+__global__ void k_aabb(const float *__restrict__ px, const float *__restrict__ py, const float *__restrict__ pz,
+                       const float *__restrict__ scale, float maxRel, u32 n, float *__restrict__ partial) {
+    __shared__ float s[6][BLOCK];
+    float mn[3] = {3.4028235e38f, 3.4028235e38f, 3.4028235e38f};
+    float mx[3] = {-3.4028235e38f, -3.4028235e38f, -3.4028235e38f};
+    for (u32 i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) {
+        const float msd = scale[i] * maxRel;   // Samples.h:409-412
+        const float r = msd + EPS_F;           // Samples.cpp:445
+        const float p[3] = {px[i], py[i], pz[i]};
+        for (int a = 0; a < 3; ++a) {
+            mn[a] = fminf(mn[a], p[a] - r);
+            mx[a] = fmaxf(mx[a], p[a] + r);
+        }
+    }
+    for (int a = 0; a < 3; ++a) { s[a][threadIdx.x] = mn[a]; s[3 + a][threadIdx.x] = mx[a]; }
+    __syncthreads();
+    for (u32 w = BLOCK / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w)
+            for (int a = 0; a < 3; ++a) {
+                s[a][threadIdx.x] = fminf(s[a][threadIdx.x], s[a][threadIdx.x + w]);
+                s[3 + a][threadIdx.x] = fmaxf(s[3 + a][threadIdx.x], s[3 + a][threadIdx.x + w]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int a = 0; a < 6; ++a) partial[6 * blockIdx.x + a] = s[a][0];
+}
+
+__global__ void k_checkScales(const float *__restrict__ scale, u32 n, u32 *err) {
+    const u32 i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n && !(scale[i] > 0.0f)) atomicOr(err, 16u);
+}
+
+// ---------------------------------------------------------------------------
+// containment (Nodes.cpp:222-269): container = node at the sample depth
+// containing the sample; its ancestors become inner nodes.
+// ---------------------------------------------------------------------------
+__global__ void k_containers(Frame f, const float *__restrict__ px, const float *__restrict__ py,
+                             const float *__restrict__ pz, const float *__restrict__ scale, float maxRel, u32 n,
+                             u64 *__restrict__ containerLoc, uint8_t *__restrict__ depthOut, u32 *err) {
+    const u32 i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const double p[3] = {px[i], py[i], pz[i]};
+    for (int a = 0; a < 3; ++a)   // Nodes.cpp:237-238 (tolerance EPSILON)
+        if (!(p[a] >= f.B(a, 0, 0) - EPS_D && p[a] < f.B(a, 1, 0) + EPS_D)) atomicOr(err, (u32)ERR_OUTSIDE);
+    const double msd = (double)(scale[i] * maxRel);
+    const u32 d = containerDepth(f, msd);
+    if (d > MAX_DEPTH) {
+        atomicOr(err, (u32)ERR_TOO_DEEP);
+        containerLoc[i] = 1;
+        depthOut[i] = 0;
+        return;
+    }
+    containerLoc[i] = makeLoc(cellCoord(f, 0, p[0], d), cellCoord(f, 1, p[1], d), cellCoord(f, 2, p[2], d), d);
+    depthOut[i] = (uint8_t)d;
+}
+
+__global__ void k_insertAncestors(HashView h, const u64 *__restrict__ containerLoc, u32 n, u32 *err) {
+    const u32 i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    u64 loc = containerLoc[i];
+    while (loc > 1) {
+        loc >>= 3;
+        if (!h.insert(loc, 0, err)) break;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// sampling refinement, speculative evaluation on a frozen tree
+// Nodes.cpp:433-469 findReliableSamplingNode as a stackless DFS over location
+// codes (parent = loc >> 3, next sibling = loc + 1).
+// ---------------------------------------------------------------------------
+template <typename IsInner, typename OnLeaf>
+TSR_HD bool findReliable(const Frame &f, u64 start, bool positive, const SamplingUnit &u, IsInner isInner,
+                         OnLeaf onLeaf) {
+    auto reliableLeaf = [&](u64 loc) {
+        u32 kx, ky, kz;
+        locCoords(loc, kx, ky, kz);
+        const u32 d = locDepth(loc);
+        if (!fittingCentre(f, kx, ky, kz, d, positive, u)) return false;
+        for (u32 c = 0; c < 8; ++c)
+            if (fittingCentre(f, 2 * kx + (c & 1), 2 * ky + ((c >> 1) & 1), 2 * kz + ((c >> 2) & 1), d + 1, positive, u))
+                return true;
+        return false;
+    };
+    if (!isInner(start)) {
+        onLeaf(start);
+        return reliableLeaf(start);
+    }
+    u64 loc = start;
+    u32 next = 0;
+    for (;;) {
+        if (next == 8) {
+            if (loc == start) return false;
+            next = (u32)(loc & 7) + 1;
+            loc >>= 3;
+            continue;
+        }
+        const u64 ch = (loc << 3) | next;
+        u32 kx, ky, kz;
+        locCoords(ch, kx, ky, kz);
+        if (intersectNode(f, kx, ky, kz, locDepth(ch), positive, u)) {
+            if (isInner(ch)) {
+                loc = ch;
+                next = 0;
+                continue;
+            }
+            onLeaf(ch);
+            if (reliableLeaf(ch)) return true;
+        }
+        ++next;
+    }
+}
+
+__device__ SamplingUnit loadUnit(const float *px, const float *py, const float *pz, const float *nx, const float *ny,
+                                 const float *nz, const float *scale, float maxRel, u32 i) {
+    SamplingUnit u;
+    u.p[0] = px[i]; u.p[1] = py[i]; u.p[2] = pz[i];
+    u.n[0] = nx[i]; u.n[1] = ny[i]; u.n[2] = nz[i];
+    u.msd = (double)(scale[i] * maxRel);
+    return u;
+}
+
+// mode 0: count visited leaves and flag creating units; mode 1: write (leaf, unit) pairs
+__global__ void k_samplingEval(Frame f, HashView inner, const float *__restrict__ px, const float *__restrict__ py,
+                               const float *__restrict__ pz, const float *__restrict__ nx,
+                               const float *__restrict__ ny, const float *__restrict__ nz,
+                               const float *__restrict__ scale, float maxRel, u32 units, int mode,
+                               u32 *__restrict__ visitCount, uint8_t *__restrict__ isEvent,
+                               const u32 *__restrict__ offsets, u64 *__restrict__ pairLeaf,
+                               u32 *__restrict__ pairUnit) {
+    const u32 unit = blockIdx.x * BLOCK + threadIdx.x;
+    if (unit >= units) return;
+    if (mode == 1 && isEvent[unit]) return;
+    const u32 i = unit >> 1;
+    const bool positive = (unit & 1) == 0;   // Nodes.cpp:217-218: positive side first
+    const SamplingUnit u = loadUnit(px, py, pz, nx, ny, nz, scale, maxRel, i);
+    u32 count = 0;
+    const u32 base = mode == 1 ? offsets[unit] : 0;
+    auto isInner = [&](u64 loc) { return inner.contains(loc); };
+    auto onLeaf = [&](u64 loc) {
+        if (mode == 1 && count < VISIT_CAP) {
+            pairLeaf[base + count] = loc;
+            pairUnit[base + count] = unit;
+        }
+        ++count;
+    };
+    const bool found = findReliable(f, 1ull, positive, u, isInner, onLeaf);
+    if (mode == 0) {
+        const bool event = !found || count > VISIT_CAP;
+        isEvent[unit] = event ? 1 : 0;
+        visitCount[unit] = event ? 0 : count;
+    }
+}
+
+__global__ void k_compactKeys(const u64 *__restrict__ keys, u64 cap, u64 *__restrict__ out, u32 *counter) {
+    const u64 s = (u64)blockIdx.x * BLOCK + threadIdx.x;
+    if (s >= cap) return;
+    const u64 k = keys[s];
+    if (k != 0ull) out[atomicAdd(counter, 1u)] = k;
+}
+
+__global__ void k_insertList(HashView h, const u64 *__restrict__ locs, u32 n, u32 *err) {
+    const u32 i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) h.insert(locs[i], 0, err);
+}
+
+// ---------------------------------------------------------------------------
+// 2:1 face balancing (Nodes.cpp:339-395): for every leaf (child of an inner
+// node that is not inner) and each face, the neighbour leaf must be at most one
+// level coarser; otherwise the neighbour's ancestors down to depth d-2 are split.
+// ---------------------------------------------------------------------------
+__global__ void k_balance(HashView inner, u64 cap, u32 *changed, u32 *err) {
+    const u64 s = (u64)blockIdx.x * BLOCK + threadIdx.x;
+    if (s >= cap) return;
+    const u64 parent = inner.keys[s];
+    if (parent == 0ull) return;
+    const u32 d = locDepth(parent) + 1;
+    if (d < 2) return;
+    for (u32 c = 0; c < 8; ++c) {
+        const u64 leaf = (parent << 3) | c;
+        if (inner.contains(leaf)) continue;
+        u32 k[3];
+        locCoords(leaf, k[0], k[1], k[2]);
+        const u32 lim = 1u << d;
+        for (u32 side = 0; side < 6; ++side) {
+            u32 q[3] = {k[0], k[1], k[2]};
+            const u32 a = side >> 1;
+            if (side & 1) {
+                if (q[a] + 1 >= lim) continue;
+                q[a] += 1;
+            } else {
+                if (q[a] == 0) continue;
+                q[a] -= 1;
+            }
+            const u64 nb = makeLoc(q[0], q[1], q[2], d);
+            if (inner.contains(nb >> 3)) continue;   // neighbour exists at depth d
+            // deepest inner ancestor of nb, depth m < d - 1
+            u32 m = d - 2;
+            while (m > 0 && !inner.contains(nb >> (3 * (d - m)))) --m;
+            if (m + 1 >= d - 1) continue;            // neighbour leaf at depth m+1 >= d-1
+            for (u32 j = m + 1; j <= d - 2; ++j) {
+                if (inner.insert(nb >> (3 * (d - j)), 0, err)) atomicOr(changed, 1u);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// finalisation: node ids in DFS block order, leaves, samples, dual cells
+// ---------------------------------------------------------------------------
+__global__ void k_dfsKeys(const u64 *__restrict__ locs, u32 n, u64 *__restrict__ keys) {
+    const u32 i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) keys[i] = dfsKey(locs[i]);
+}
+
+// children of the inner node with pre-order rank r get ids 1 + 8r + c (Nodes.cpp:536-554)
+__global__ void k_nodeHash(HashView nodes, const u64 *__restrict__ innerSorted, u32 innerCount, u32 *err) {
+    const u32 r = blockIdx.x * BLOCK + threadIdx.x;
+    if (r == 0) nodes.insert(1ull, 0u, err);
+    if (r >= innerCount) return;
+    const u64 loc = innerSorted[r];
+    for (u32 c = 0; c < 8; ++c) nodes.insert((loc << 3) | c, 1u + 8u * r + c, err);
+}
+
+__global__ void k_nodeArrays(HashView nodes, HashView inner, const u64 *__restrict__ innerSorted, u32 innerCount,
+                             u32 *__restrict__ child, u32 *__restrict__ parents, uint8_t *__restrict__ depth,
+                             u64 *__restrict__ nodeLoc, u64 *__restrict__ leafKeys, u32 *__restrict__ leafIds,
+                             u32 *leafCounter) {
+    const u32 r = blockIdx.x * BLOCK + threadIdx.x;
+    if (r == 0) {
+        parents[0] = INVALID;
+        depth[0] = 0;
+        nodeLoc[0] = 1ull;
+        if (innerCount == 0) {   // single-leaf tree
+            child[0] = INVALID;
+            const u32 slot = atomicAdd(leafCounter, 1u);
+            leafKeys[slot] = dfsKey(1ull);
+            leafIds[slot] = 0;
+        }
+    }
+    if (r >= innerCount) return;
+    const u64 loc = innerSorted[r];
+    const u32 id = nodes.find(loc);
+    child[id] = 1u + 8u * r;
+    const u32 d = locDepth(loc) + 1;
+    for (u32 c = 0; c < 8; ++c) {
+        const u64 cl = (loc << 3) | c;
+        const u32 cid = 1u + 8u * r + c;
+        parents[cid] = id;
+        depth[cid] = (uint8_t)d;
+        nodeLoc[cid] = cl;
+        if (!inner.contains(cl)) {
+            const u32 slot = atomicAdd(leafCounter, 1u);
+            leafKeys[slot] = dfsKey(cl);
+            leafIds[slot] = cid;
+        }
+    }
+}
+
+__global__ void k_leafArrays(const u32 *__restrict__ leafIdsSorted, u32 leafCount, const u64 *__restrict__ nodeLoc,
+                             u32 *__restrict__ child, u32 *__restrict__ nodeToLeaf, u64 *__restrict__ leafLoc,
+                             u32 *__restrict__ leafNode) {
+    const u32 l = blockIdx.x * BLOCK + threadIdx.x;
+    if (l >= leafCount) return;
+    const u32 id = leafIdsSorted[l];
+    child[id] = LEAF_BIT | l;
+    nodeToLeaf[id] = l;
+    leafLoc[l] = nodeLoc[id];
+    leafNode[l] = id;
+}
+
+__global__ void k_fillU32(u32 *p, size_t n, u32 v) {
+    const size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_sampleNodes(HashView nodes, const u64 *__restrict__ containerLoc, u32 n, u32 *__restrict__ nodeOf,
+                              u32 *__restrict__ counts, u32 *err) {
+    const u32 i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const u32 id = nodes.find(containerLoc[i]);
+    if (id == INVALID) {
+        atomicOr(err, (u32)ERR_NOT_INSERTED);
+        nodeOf[i] = 0;
+        return;
+    }
+    nodeOf[i] = id;
+    atomicAdd(&counts[id], 1u);
+}
+
+__global__ void k_iota(u32 *p, u32 n) {
+    const u32 i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) p[i] = i;
+}
+
+__global__ void k_scatterTarget(const u32 *__restrict__ sortedOrig, u32 n, u32 *__restrict__ target) {
+    const u32 j = blockIdx.x * BLOCK + threadIdx.x;
+    if (j < n) target[sortedOrig[j]] = j;
+}
+
+template <typename T>
+__global__ void k_gather(const T *__restrict__ src, const u32 *__restrict__ order, u32 n, u32 stride,
+                         T *__restrict__ dst) {
+    const u32 j = blockIdx.x * BLOCK + threadIdx.x;
+    if (j >= n) return;
+    const u32 o = order[j];
+    for (u32 s = 0; s < stride; ++s) dst[(size_t)j * stride + s] = src[(size_t)o * stride + s];
+}
+
+// dual cells: every interior corner of every leaf, packed (x | y << 21 | z << 42) at MAX_DEPTH
+__global__ void k_cornerCodes(const u64 *__restrict__ leafLoc, u32 leafCount, u64 *__restrict__ codes) {
+    const u32 l = blockIdx.x * BLOCK + threadIdx.x;
+    if (l >= leafCount) return;
+    const u64 loc = leafLoc[l];
+    u32 k[3];
+    locCoords(loc, k[0], k[1], k[2]);
+    const u32 sh = MAX_DEPTH - locDepth(loc);
+    const u32 lim = 1u << MAX_DEPTH;
+    for (u32 c = 0; c < 8; ++c) {
+        const u32 x = (k[0] + (c & 1)) << sh, y = (k[1] + ((c >> 1) & 1)) << sh, z = (k[2] + ((c >> 2) & 1)) << sh;
+        const bool interior = x > 0 && x < lim && y > 0 && y < lim && z > 0 && z < lim;
+        codes[8 * (size_t)l + c] = interior ? ((u64)x | ((u64)y << 21) | ((u64)z << 42)) : ~0ull;
+    }
+}
+
+// Schaefer-Warren recursion path of the cell at vertex v (DualCells.cpp:49-357):
+// call types 0 node, 1-3 face normal x/y/z, 4-6 edge along x/y/z; 5 bits per level.
+TSR_HD void swKey(u32 vx, u32 vy, u32 vz, u64 &hi, u64 &lo) {
+    const u32 v[3] = {vx, vy, vz};
+    u32 k[3] = {0, 0, 0};
+    int type = 0;
+    hi = 0;
+    lo = 0;
+    for (u32 lvl = 0; lvl < MAX_DEPTH; ++lvl) {
+        const u32 half = 1u << (MAX_DEPTH - lvl - 1);
+        u32 eq[3], bit[3];
+        for (int a = 0; a < 3; ++a) {
+            const u32 m = (2 * k[a] + 1) * half;
+            eq[a] = v[a] == m;
+            bit[a] = v[a] > m;
+        }
+        u32 idx = 0;
+        int next = -1;   // -1 = terminal vertex call
+        switch (type) {
+        case 0: {
+            const u32 ne = eq[0] + eq[1] + eq[2];
+            if (ne == 0) { idx = bit[0] | (bit[1] << 1) | (bit[2] << 2); next = 0; }
+            else if (ne == 1) {
+                if (eq[0]) { idx = 8 + 2 * bit[1] + bit[2]; next = 1; }
+                else if (eq[1]) { idx = 12 + bit[0] + 2 * bit[2]; next = 2; }
+                else { idx = 16 + bit[0] + 2 * bit[1]; next = 3; }
+            } else if (ne == 2) {
+                if (!eq[2]) { idx = 20 + bit[2]; next = 6; }
+                else if (!eq[1]) { idx = 22 + bit[1]; next = 5; }
+                else { idx = 24 + bit[0]; next = 4; }
+            } else { idx = 26; }
+            break;
+        }
+        case 1:   // face with normal x: y, z free
+            if (!eq[1] && !eq[2]) { idx = 2 * bit[1] + bit[2]; next = 1; }
+            else if (eq[1] && !eq[2]) { idx = 4 + bit[2]; next = 6; }
+            else if (!eq[1] && eq[2]) { idx = 6 + bit[1]; next = 5; }
+            else idx = 8;
+            break;
+        case 2:   // face with normal y: x, z free
+            if (!eq[0] && !eq[2]) { idx = bit[0] + 2 * bit[2]; next = 2; }
+            else if (eq[0] && !eq[2]) { idx = 4 + bit[2]; next = 6; }
+            else if (!eq[0] && eq[2]) { idx = 6 + bit[0]; next = 4; }
+            else idx = 8;
+            break;
+        case 3:   // face with normal z: x, y free
+            if (!eq[0] && !eq[1]) { idx = bit[0] + 2 * bit[1]; next = 3; }
+            else if (eq[0] && !eq[1]) { idx = 4 + bit[1]; next = 5; }
+            else if (!eq[0] && eq[1]) { idx = 6 + bit[0]; next = 4; }
+            else idx = 8;
+            break;
+        default: {   // edges: 4 along x, 5 along y, 6 along z
+            const int a = type - 4;
+            if (!eq[a]) { idx = bit[a]; next = type; }
+            else idx = 2;
+            break;
+        }
+        }
+        if (lvl < 12) hi |= (u64)idx << (5 * (11 - lvl) + 4);
+        else lo |= (u64)idx << (5 * (18 - lvl) + 4);
+        if (next < 0) return;
+        // descend: free axes refine, fixed axes keep their plane
+        const bool fixedAxis[3] = {next == 1 || next == 5 || next == 6, next == 2 || next == 4 || next == 6,
+                                   next == 3 || next == 4 || next == 5};
+        for (int a = 0; a < 3; ++a) k[a] = fixedAxis[a] ? k[a] * 2 : 2 * k[a] + bit[a];
+        type = next;
+    }
+}
+
+__device__ u32 leafOfCell(HashView inner, HashView nodes, const u32 *__restrict__ nodeToLeaf, u32 cx, u32 cy, u32 cz) {
+    const u64 finest = makeLoc(cx, cy, cz, MAX_DEPTH);
+    // largest m with the depth-m ancestor inner (monotone in m)
+    int lo = 0, hi = (int)MAX_DEPTH;   // inner(lo) true (root), inner(hi) false
+    if (!inner.contains(1ull)) return nodeToLeaf[0];
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) / 2;
+        if (inner.contains(finest >> (3 * (MAX_DEPTH - mid)))) lo = mid;
+        else hi = mid;
+    }
+    const u64 leafLoc = finest >> (3 * (MAX_DEPTH - (lo + 1)));
+    const u32 id = nodes.find(leafLoc);
+    return id == INVALID ? INVALID : nodeToLeaf[id];
+}
+
+__global__ void k_checkCells(const u32 *__restrict__ cells, size_t n, u32 *err) {
+    const size_t j = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (j < n && cells[j] == INVALID) atomicOr(err, (u32)ERR_NOT_INSERTED);
+}
+
+__global__ void k_cells(HashView inner, HashView nodes, const u32 *__restrict__ nodeToLeaf,
+                        const u64 *__restrict__ vcodes, u32 count, u32 *__restrict__ cellsUnsorted,
+                        u64 *__restrict__ keyHi, u64 *__restrict__ keyLo, u32 *__restrict__ idx) {
+    const u32 c = blockIdx.x * BLOCK + threadIdx.x;
+    if (c >= count) return;
+    const u64 code = vcodes[c];
+    const u32 v[3] = {(u32)(code & 0x1fffff), (u32)((code >> 21) & 0x1fffff), (u32)((code >> 42) & 0x1fffff)};
+    for (u32 s = 0; s < 8; ++s) {
+        const u32 x = v[0] - 1 + (s & 1), y = v[1] - 1 + ((s >> 1) & 1), z = v[2] - 1 + ((s >> 2) & 1);
+        cellsUnsorted[8 * (size_t)c + s] = leafOfCell(inner, nodes, nodeToLeaf, x, y, z);
+    }
+    u64 h, l;
+    swKey(v[0], v[1], v[2], h, l);
+    keyHi[c] = h;
+    keyLo[c] = l;
+    idx[c] = c;
+}
+
+// ---------------------------------------------------------------------------
+// Host replay of the order-dependent sampling refinement (Nodes.cpp:214-219,
+// 271-337).  Units are (sample, side) in reference order.  Every unit that the
+// GPU found reliable on the containment tree T0 stays a no-op unless a leaf it
+// visited on T0 gets subdivided by an earlier unit; creating units and such
+// disturbed units are re-run here in order, on the evolving tree.
+// ---------------------------------------------------------------------------
+struct HostTree {
+    std::unordered_set<u64> inner;      // current inner nodes
+    const std::unordered_set<u64> *t0;  // inner nodes of T0
+    bool isInner(u64 loc) const { return inner.count(loc) != 0; }
+};
+
+void replaySampling(const Frame &f, const tsr_scene *scene, float maxRel, const std::vector<u64> &t0Inner,
+                    const std::vector<uint8_t> &isEvent, const std::vector<u64> &pairLeaf,
+                    const std::vector<u32> &pairUnit, std::vector<u64> &created, u64 &eventCount) {
+    HostTree tree;
+    std::unordered_set<u64> t0(t0Inner.begin(), t0Inner.end());
+    tree.inner = t0;
+    tree.t0 = &t0;
+    const u32 units = (u32)isEvent.size();
+    std::vector<uint8_t> pending(isEvent.begin(), isEvent.end());
+    std::priority_queue<u32, std::vector<u32>, std::greater<u32>> heap;
+    for (u32 u = 0; u < units; ++u)
+        if (pending[u]) heap.push(u);
+    auto leafIsInner = [&](u64 loc) { return tree.isInner(loc); };
+    auto noLeaf = [](u64) {};
+    auto taint = [&](u64 subdivided, u32 writer) {
+        // the T0 leaf containing the subdivided node
+        u64 l = subdivided;
+        while (l > 1 && t0.count(l >> 3) == 0) l >>= 3;
+        auto range = std::equal_range(pairLeaf.begin(), pairLeaf.end(), l);
+        const size_t b = range.first - pairLeaf.begin(), e = range.second - pairLeaf.begin();
+        auto it = std::upper_bound(pairUnit.begin() + b, pairUnit.begin() + e, writer);
+        for (; it != pairUnit.begin() + e; ++it)
+            if (!pending[*it]) {
+                pending[*it] = 1;
+                heap.push(*it);
+            }
+    };
+    eventCount = 0;
+    while (!heap.empty()) {
+        const u32 unit = heap.top();
+        heap.pop();
+        ++eventCount;
+        const u32 i = unit >> 1;
+        const bool positive = (unit & 1) == 0;
+        SamplingUnit u;
+        u.p[0] = scene->hpx[i]; u.p[1] = scene->hpy[i]; u.p[2] = scene->hpz[i];
+        u.n[0] = scene->hnx[i]; u.n[1] = scene->hny[i]; u.n[2] = scene->hnz[i];
+        u.msd = (double)(scene->hscale[i] * maxRel);
+        if (findReliable(f, 1ull, positive, u, leafIsInner, noLeaf)) continue;
+        // Nodes.cpp:284-336
+        const double sgn = positive ? u.msd : -u.msd;
+        const double en[3] = {u.p[0] + u.n[0] * sgn, u.p[1] + u.n[1] * sgn, u.p[2] + u.n[2] * sgn};
+        u64 n = 1ull;
+        for (;;) {
+            if (!tree.isInner(n)) {
+                if (locDepth(n) + 1 > MAX_DEPTH) throw tsr_error(TSR_ERR_TREE_TOO_DEEP, "octree deeper than 19 levels");
+                tree.inner.insert(n);
+                created.push_back(n);
+                taint(n, unit);
+                if (findReliable(f, n, positive, u, leafIsInner, noLeaf)) break;
+            }
+            const u32 d = locDepth(n) + 1;
+            if (f.h(d) == 0.0) throw tsr_error(TSR_ERR_ZERO_SIZE_NODE, "Reached a tree node of size 0");
+            double score = -1.0e300;
+            u32 best = INVALID;
+            for (u32 c = 0; c < 8; ++c) {
+                const u64 ch = (n << 3) | c;
+                u32 k[3];
+                locCoords(ch, k[0], k[1], k[2]);
+                double lo[3], hi[3];
+                for (int a = 0; a < 3; ++a) { lo[a] = f.B(a, k[a], d); hi[a] = f.B(a, (u64)k[a] + 1, d); }
+                double tIn, tOut;
+                if (!segmentBox(tIn, tOut, u.p, en, lo, hi)) continue;
+                const double overlap = tOut - tIn;
+                if (overlap <= score) continue;
+                score = overlap;
+                best = c;
+                if (overlap >= 1.0 - EPS_D) break;
+            }
+            if (best == INVALID) throw tsr_error(TSR_ERR_INVALID_ARGUMENT, "sampling refinement: segment leaves the node");
+            n = (n << 3) | best;
+        }
+    }
+}
+
+void checkErr(u32 e) {
+    if (e & ERR_OUTSIDE) throw tsr_error(TSR_ERR_SAMPLE_OUTSIDE_ROOT, "Cannot find a container node for a position which is not contained by the start node for the search.");
+    if (e & ERR_TOO_DEEP) throw tsr_error(TSR_ERR_TREE_TOO_DEEP, "octree deeper than 19 levels");
+    if (e & ERR_HASH_FULL) throw tsr_error(TSR_ERR_NODE_OVERFLOW, "octree node table full");
+    if (e & ERR_NOT_INSERTED) throw tsr_error(TSR_ERR_SAMPLE_NOT_INSERTED, "There is a sample which was not inserted into the Tree object!");
+    if (e & 16u) throw tsr_error(TSR_ERR_INVALID_SCALE, "Invalid (non-positive) sample scale detected.");
+}
+
+}  // namespace
+
+void buildTree(tsr_context *ctx, tsr_scene *scene, const tsr_params &prm, tsr_tree *tree) {
+    hipStream_t st = ctx->stream;
+    Scratch &sc = ctx->scratch;
+    const u32 n = scene->n;
+    if (n == 0) throw tsr_error(TSR_ERR_NO_SAMPLES, "no samples");
+    const float maxRel = prm.max_relative_sampling_distance;
+    tree->maxRel = maxRel;
+    tree->sampleCount = n;
+    DBuf<u32> err(1);
+    HIPCHECK(hipMemsetAsync(err.p, 0, 4, st));
+    k_checkScales<<<gridFor(n, BLOCK), BLOCK, 0, st>>>(scene->scale.p, n, err.p);
+    checkErr(readScalar(err.p, st));
+
+    // ---- root scope (float, Real = float semantics of the reference)
+    const u32 nb = std::min<u32>(1024, gridFor(n, BLOCK));
+    DBuf<float> partial(6 * (size_t)nb);
+    k_aabb<<<nb, BLOCK, 0, st>>>(scene->px.p, scene->py.p, scene->pz.p, scene->scale.p, maxRel, n, partial.p);
+    std::vector<float> hp(6 * (size_t)nb);
+    HIPCHECK(hipMemcpyAsync(hp.data(), partial.p, partial.bytes(), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    float mn[3] = {3.4028235e38f, 3.4028235e38f, 3.4028235e38f}, mx[3] = {-3.4028235e38f, -3.4028235e38f, -3.4028235e38f};
+    for (u32 b = 0; b < nb; ++b)
+        for (int a = 0; a < 3; ++a) {
+            mn[a] = std::min(mn[a], hp[6 * b + a]);
+            mx[a] = std::max(mx[a], hp[6 * b + 3 + a]);
+        }
+    Frame f{};
+    {
+        float c[3], e[3];
+        for (int a = 0; a < 3; ++a) { c[a] = (mn[a] + mx[a]) * 0.5f; e[a] = mx[a] - mn[a]; }
+        const float size = std::max(std::max(e[0], e[1]), e[2]);
+        const float half = size * 0.5f;
+        f.rootSizeF = size;
+        f.rootSize = (double)size;
+        for (int a = 0; a < 3; ++a) {
+            f.rootMinF[a] = c[a] - half;
+            f.rootMin[a] = (double)f.rootMinF[a];
+        }
+    }
+    tree->frame = f;
+
+    // ---- containment
+    DBuf<u64> containerLoc(n);
+    DBuf<uint8_t> cdepth(n);
+    k_containers<<<gridFor(n, BLOCK), BLOCK, 0, st>>>(f, scene->px.p, scene->py.p, scene->pz.p, scene->scale.p, maxRel,
+                                                       n, containerLoc.p, cdepth.p, err.p);
+    checkErr(readScalar(err.p, st));
+    DHash inner;
+    inner.alloc(std::max<u64>(4096, (u64)n), st);
+    k_insertAncestors<<<gridFor(n, BLOCK), BLOCK, 0, st>>>(inner.view(), containerLoc.p, n, err.p);
+    checkErr(readScalar(err.p, st));
+
+    // ---- sampling refinement: speculative evaluation on T0
+    const u32 units = 2 * n;
+    DBuf<u32> visitCount(units), offsets(units);
+    DBuf<uint8_t> isEvent(units);
+    k_samplingEval<<<gridFor(units, BLOCK), BLOCK, 0, st>>>(f, inner.view(), scene->px.p, scene->py.p, scene->pz.p,
+                                                            scene->nx.p, scene->ny.p, scene->nz.p, scene->scale.p,
+                                                            maxRel, units, 0, visitCount.p, isEvent.p, nullptr,
+                                                            nullptr, nullptr);
+    exclusiveScan(sc, visitCount.p, offsets.p, units, st);
+    const u32 lastOff = readScalar(offsets.p + units - 1, st), lastCnt = readScalar(visitCount.p + units - 1, st);
+    const size_t pairs = (size_t)lastOff + lastCnt;
+    DBuf<u64> pairLeaf(std::max<size_t>(pairs, 1)), pairLeafSorted(std::max<size_t>(pairs, 1));
+    DBuf<u32> pairUnit(std::max<size_t>(pairs, 1)), pairUnitSorted(std::max<size_t>(pairs, 1));
+    k_samplingEval<<<gridFor(units, BLOCK), BLOCK, 0, st>>>(f, inner.view(), scene->px.p, scene->py.p, scene->pz.p,
+                                                            scene->nx.p, scene->ny.p, scene->nz.p, scene->scale.p,
+                                                            maxRel, units, 1, visitCount.p, isEvent.p, offsets.p,
+                                                            pairLeaf.p, pairUnit.p);
+    // pairs are emitted in unit order; a stable sort by leaf keeps units ascending per leaf
+    sortPairs(sc, pairLeaf.p, pairLeafSorted.p, pairUnit.p, pairUnitSorted.p, pairs, 0, 64, st);
+    std::vector<uint8_t> hEvent(units);
+    std::vector<u64> hPairLeaf(pairs);
+    std::vector<u32> hPairUnit(pairs);
+    HIPCHECK(hipMemcpyAsync(hEvent.data(), isEvent.p, units, hipMemcpyDeviceToHost, st));
+    if (pairs) {
+        HIPCHECK(hipMemcpyAsync(hPairLeaf.data(), pairLeafSorted.p, pairs * 8, hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipMemcpyAsync(hPairUnit.data(), pairUnitSorted.p, pairs * 4, hipMemcpyDeviceToHost, st));
+    }
+    // T0 inner node list
+    DBuf<u64> innerList(inner.capacity);
+    DBuf<u32> counter(1);
+    HIPCHECK(hipMemsetAsync(counter.p, 0, 4, st));
+    k_compactKeys<<<gridFor(inner.capacity, BLOCK), BLOCK, 0, st>>>(inner.keys.p, inner.capacity, innerList.p, counter.p);
+    const u32 t0Count = readScalar(counter.p, st);
+    std::vector<u64> hInner(t0Count);
+    HIPCHECK(hipMemcpyAsync(hInner.data(), innerList.p, (size_t)t0Count * 8, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    pairLeaf.release(); pairUnit.release(); pairLeafSorted.release(); pairUnitSorted.release();
+    std::vector<u64> created;
+    u64 events = 0;
+    replaySampling(f, scene, maxRel, hInner, hEvent, hPairLeaf, hPairUnit, created, events);
+    tree->samplingEvents = events;
+    tree->samplingCreated = 8 * (u64)created.size();
+    tree->samplingUnits = units;
+
+    // re-size the inner hash with room for balancing, insert T0 + created
+    {
+        const u64 need = (u64)t0Count + created.size();
+        DHash grown;
+        grown.alloc(std::max<u64>(4096, 4 * need), st);
+        if (t0Count)
+            k_insertList<<<gridFor(t0Count, BLOCK), BLOCK, 0, st>>>(grown.view(), innerList.p, t0Count, err.p);
+        if (!created.empty()) {
+            DBuf<u64> dc(created.size());
+            HIPCHECK(hipMemcpyAsync(dc.p, created.data(), created.size() * 8, hipMemcpyHostToDevice, st));
+            k_insertList<<<gridFor(created.size(), BLOCK), BLOCK, 0, st>>>(grown.view(), dc.p, (u32)created.size(), err.p);
+            HIPCHECK(hipStreamSynchronize(st));
+        }
+        inner = std::move(grown);
+    }
+
+    // ---- balancing fixpoint
+    for (int iter = 0;; ++iter) {
+        HIPCHECK(hipMemsetAsync(counter.p, 0, 4, st));
+        k_balance<<<gridFor(inner.capacity, BLOCK), BLOCK, 0, st>>>(inner.view(), inner.capacity, counter.p, err.p);
+        const u32 changed = readScalar(counter.p, st);
+        checkErr(readScalar(err.p, st));
+        if (!changed) break;
+        if (iter > 64) throw tsr_error(TSR_ERR_HIP, "balancing did not converge");
+    }
+
+    // ---- final inner list sorted in pre-order
+    innerList.alloc(inner.capacity);
+    HIPCHECK(hipMemsetAsync(counter.p, 0, 4, st));
+    k_compactKeys<<<gridFor(inner.capacity, BLOCK), BLOCK, 0, st>>>(inner.keys.p, inner.capacity, innerList.p, counter.p);
+    const u32 I = readScalar(counter.p, st);
+    if ((u64)1 + 8ull * I >= (u64)INVALID) throw tsr_error(TSR_ERR_NODE_OVERFLOW, "Required scene tree node count is larger than supported.");
+    DBuf<u64> keys(std::max<u32>(I, 1)), keysSorted(std::max<u32>(I, 1)), innerSorted(std::max<u32>(I, 1));
+    if (I) {
+        k_dfsKeys<<<gridFor(I, BLOCK), BLOCK, 0, st>>>(innerList.p, I, keys.p);
+        sortPairs(sc, keys.p, keysSorted.p, innerList.p, innerSorted.p, I, 0, 64, st);
+    }
+    const u32 N = 1 + 8 * I;
+    tree->innerCount = I;
+    tree->nodeCount = N;
+    tree->nodeHash.alloc(N, st);
+    k_nodeHash<<<gridFor(std::max<u32>(I, 1), BLOCK), BLOCK, 0, st>>>(tree->nodeHash.view(), innerSorted.p, I, err.p);
+    tree->child.alloc(N);
+    tree->parents.alloc(N);
+    tree->nodeDepth.alloc(N);
+    tree->nodeLoc.alloc(N);
+    tree->nodeToLeaf.alloc(N);
+    k_fillU32<<<gridFor(N, BLOCK), BLOCK, 0, st>>>(tree->nodeToLeaf.p, N, INVALID);
+    const u32 Lmax = I ? 7 * I + 1 : 1;
+    DBuf<u64> leafKeys(Lmax), leafKeysSorted(Lmax);
+    DBuf<u32> leafIds(Lmax), leafIdsSorted(Lmax);
+    HIPCHECK(hipMemsetAsync(counter.p, 0, 4, st));
+    k_nodeArrays<<<gridFor(std::max<u32>(I, 1), BLOCK), BLOCK, 0, st>>>(
+        tree->nodeHash.view(), inner.view(), innerSorted.p, I, tree->child.p, tree->parents.p, tree->nodeDepth.p,
+        tree->nodeLoc.p, leafKeys.p, leafIds.p, counter.p);
+    const u32 L = readScalar(counter.p, st);
+    checkErr(readScalar(err.p, st));
+    sortPairs(sc, leafKeys.p, leafKeysSorted.p, leafIds.p, leafIdsSorted.p, L, 0, 64, st);
+    tree->leafCount = L;
+    tree->leafLoc.alloc(L);
+    tree->leafNode.alloc(L);
+    k_leafArrays<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(leafIdsSorted.p, L, tree->nodeLoc.p, tree->child.p,
+                                                      tree->nodeToLeaf.p, tree->leafLoc.p, tree->leafNode.p);
+    {
+        std::vector<u64> lastKey(1);
+        HIPCHECK(hipMemcpyAsync(lastKey.data(), leafKeysSorted.p, 8, hipMemcpyDeviceToHost, st));
+    }
+    // max depth
+    {
+        std::vector<uint8_t> hd(N);
+        HIPCHECK(hipMemcpyAsync(hd.data(), tree->nodeDepth.p, N, hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        u32 md = 0;
+        for (uint8_t d : hd) md = std::max<u32>(md, d);
+        tree->maxDepth = md;
+    }
+
+    // ---- samples: node ids, counts, starts, stable order by node id
+    DBuf<u32> nodeOf(n), orig(n), nodeOfSorted(n), origSorted(n);
+    tree->sampleCounts.alloc(N);
+    tree->sampleStarts.alloc(N);
+    HIPCHECK(hipMemsetAsync(tree->sampleCounts.p, 0, (size_t)N * 4, st));
+    k_sampleNodes<<<gridFor(n, BLOCK), BLOCK, 0, st>>>(tree->nodeHash.view(), containerLoc.p, n, nodeOf.p,
+                                                       tree->sampleCounts.p, err.p);
+    checkErr(readScalar(err.p, st));
+    exclusiveScan(sc, tree->sampleCounts.p, tree->sampleStarts.p, N, st);
+    k_iota<<<gridFor(n, BLOCK), BLOCK, 0, st>>>(orig.p, n);
+    int bits = 1;
+    while (bits < 32 && (1ull << bits) < N) ++bits;
+    sortPairs(sc, nodeOf.p, nodeOfSorted.p, orig.p, origSorted.p, n, 0, bits, st);
+    tree->sampleTarget.alloc(n);
+    k_scatterTarget<<<gridFor(n, BLOCK), BLOCK, 0, st>>>(origSorted.p, n, tree->sampleTarget.p);
+    // permute the scene's sample arrays (Samples::reorder)
+    auto permuteF = [&](DBuf<float> &buf) {
+        DBuf<float> out(n);
+        k_gather<float><<<gridFor(n, BLOCK), BLOCK, 0, st>>>(buf.p, origSorted.p, n, 1, out.p);
+        buf = std::move(out);
+    };
+    permuteF(scene->px); permuteF(scene->py); permuteF(scene->pz);
+    permuteF(scene->nx); permuteF(scene->ny); permuteF(scene->nz);
+    permuteF(scene->scale); permuteF(scene->conf);
+    {
+        DBuf<u32> out((size_t)n * scene->maxCams);
+        k_gather<u32><<<gridFor(n, BLOCK), BLOCK, 0, st>>>(scene->cams.p, origSorted.p, n, scene->maxCams, out.p);
+        scene->cams = std::move(out);
+        tree->sampleDepth.alloc(n);
+        k_gather<uint8_t><<<gridFor(n, BLOCK), BLOCK, 0, st>>>(cdepth.p, origSorted.p, n, 1, tree->sampleDepth.p);
+    }
+    {   // keep the host copies in the same order
+        std::vector<u32> perm(n);
+        HIPCHECK(hipMemcpyAsync(perm.data(), origSorted.p, (size_t)n * 4, hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        auto permH = [&](std::vector<float> &v) {
+            std::vector<float> o(n);
+            for (u32 j = 0; j < n; ++j) o[j] = v[perm[j]];
+            v.swap(o);
+        };
+        permH(scene->hpx); permH(scene->hpy); permH(scene->hpz);
+        permH(scene->hnx); permH(scene->hny); permH(scene->hnz); permH(scene->hscale);
+    }
+
+    // ---- dual cells
+    if (L > 0) {
+        const size_t corners = 8 * (size_t)L;
+        DBuf<u64> codes(corners), codesSorted(corners), uniq(corners);
+        k_cornerCodes<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(tree->leafLoc.p, L, codes.p);
+        sortKeys(sc, codes.p, codesSorted.p, corners, 0, 64, st);
+        uniqueKeys(sc, codesSorted.p, uniq.p, counter.p, corners, st);
+        u32 V = readScalar(counter.p, st);
+        if (V > 0) {
+            const u64 last = readScalar(uniq.p + V - 1, st);
+            if (last == ~0ull) --V;   // drop the boundary marker
+        }
+        codes.release(); codesSorted.release();
+        tree->cellCount = V;
+        if (V > 0) {
+            DBuf<u32> cellsUnsorted(8 * (size_t)V), idx(V), idxA(V), idxB(V);
+            DBuf<u64> kh(V), kl(V), khS(V), klS(V);
+            k_cells<<<gridFor(V, BLOCK), BLOCK, 0, st>>>(inner.view(), tree->nodeHash.view(), tree->nodeToLeaf.p,
+                                                         uniq.p, V, cellsUnsorted.p, kh.p, kl.p, idx.p);
+            // lexicographic (hi, lo): stable sort by lo, then by hi
+            sortPairs(sc, kl.p, klS.p, idx.p, idxA.p, V, 0, 64, st);
+            k_gather<u64><<<gridFor(V, BLOCK), BLOCK, 0, st>>>(kh.p, idxA.p, V, 1, khS.p);
+            sortPairs(sc, khS.p, kh.p, idxA.p, idxB.p, V, 0, 64, st);
+            tree->cells.alloc(8 * (size_t)V);
+            k_gather<u32><<<gridFor(V, BLOCK), BLOCK, 0, st>>>(cellsUnsorted.p, idxB.p, V, 8, tree->cells.p);
+            k_checkCells<<<gridFor(8 * (size_t)V, BLOCK), BLOCK, 0, st>>>(tree->cells.p, 8 * (size_t)V, err.p);
+        }
+    }
+    HIPCHECK(hipStreamSynchronize(st));
+    checkErr(readScalar(err.p, st));
+}
+
+}  // namespace tsr
