@@ -1,0 +1,177 @@
+"""Benchmark: input samples/s from resident SoA samples to occupancy field + DMC mesh.
+
+One step = Tree() + Occupancy(tree) + DualMarchingCells::extractMesh on one MI355X
+(HIP), for the thin-wedge + sphere scene (64 cameras, 640x480 depth maps, 0.3 %
+depth noise, 1 % outliers; samples resident in HBM before timing starts).
+Prints one JSON line.  `--gpus N` with N > 1 exits: this build runs on one GPU.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def cpu_info():
+    model, cores = "unknown", os.cpu_count() or 1
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        pass
+    return model, cores
+
+
+def cpu_baseline(config, res, threads):
+    """The oracle in its OpenMP float mode (reference parallel structure) on a bounded sample."""
+    from oracle.oracle import Oracle
+    from tsr_amd import SyntheticScene
+
+    syn = SyntheticScene(config, seed=0, resolution_scale=res)
+    o = Oracle(syn.samples, syn.cameras)
+    t0 = time.perf_counter()
+    o.build_tree()
+    o.occupancy(threads=threads)
+    o.dmc()
+    dt = time.perf_counter() - t0
+    ph = o.timings()
+    return syn.count, dt, ph
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", type=int, default=1)
+    ap.add_argument("--res", type=float, default=1.0)
+    ap.add_argument("--cpu-res", type=float, default=0.25, help="resolution scale of the CPU baseline sample")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", type=float, default=0,
+                    help="also run a plain copy kernel of this many bytes (reference for PMC traffic scaling)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
+              "(the multi-GPU sharding plan is design-only)", file=sys.stderr)
+        sys.exit(2)
+
+    from tsr_amd import Context, DualMarchingCells, Occupancy, Scene, SyntheticScene, Tree
+
+    t0 = time.perf_counter()
+    syn = SyntheticScene(args.config, seed=0, resolution_scale=args.res)
+    gen_s = time.perf_counter() - t0
+    n = syn.count
+    print(f"[bench] config {args.config} res {args.res}: {n} samples, {len(syn.cameras['x'])} cameras "
+          f"(generated in {gen_s:.1f}s)", file=sys.stderr, flush=True)
+    ctx = Context(0)
+    if args.calibrate_copy > 0:
+        ctx.calibration_copy(int(args.calibrate_copy))
+    scene = Scene(ctx, syn.samples, syn.cameras)
+
+    def step():
+        tree = Tree(ctx, scene)
+        occ = Occupancy(ctx, tree, scene)
+        mesh = DualMarchingCells(ctx, tree, occ)
+        info = (tree.node_count, tree.leaf_count, tree.cell_count, mesh.vertex_count, mesh.triangle_count,
+                occ.hit_counts())
+        mesh.close()
+        occ.close()
+        tree.close()
+        return info
+
+    for w in range(args.warmup):
+        info = step()
+        print(f"[bench] warmup {w} done", file=sys.stderr, flush=True)
+    ctx.synchronize()
+    ctx.reset_kernel_stats()
+    phases = []
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        info = step()
+        phases.append(ctx.phase_times())
+        print(f"[bench] step {k} done", file=sys.stderr, flush=True)
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    ms_per_step = 1000.0 * dt / args.steps
+    value = n * args.steps / dt
+    nodes, leaves, cells, verts, tris, hits = info
+    ph = [float(sum(p[i] for p in phases) / len(phases)) for i in range(6)]
+
+    out = {
+        "metric": "input samples/sec to occupancy field + DMC mesh",
+        "value": value,
+        "unit": "samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 (fp64 predicates/ratio)",
+        "data": "synthetic",
+        "config": {
+            "workload": "thin wedge (two 1x1 sheets, 0.01 thick, 10 deg) + sphere r=0.4; 64 cameras at 2..8, "
+                        "640x480 depth maps, 0.3% depth noise, 1% outliers",
+            "config_index": args.config,
+            "resolution_scale": args.res,
+            "samples": n,
+            "nodes": nodes, "leaves": leaves, "dual_cells": cells,
+            "mesh_vertices": verts, "mesh_triangles": tris,
+            "hits_emptiness": int(hits[0]), "hits_sampleness": int(hits[1]),
+            "phase_ms": {"tree": ph[0], "sampleness": ph[1], "emptiness": ph[2], "priors": ph[3],
+                         "occupancy": ph[4], "dmc": ph[5]},
+        },
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    # roofline of the dominant instrumented kernel: algorithmic bytes per launch / average
+    # launch time (HIP events on the library's stream)
+    stats = {k: ctx.kernel_stats(k) for k in Context.KERNELS}
+    out["kernels"] = {k: {"ms_per_launch": v[0] / v[1], "algorithmic_bytes_per_launch": v[2] / v[1],
+                          "GBps": (v[2] / v[1]) / (v[0] / v[1] * 1e-3) / 1e9}
+                      for k, v in stats.items() if v[1] > 0}
+    if out["kernels"]:
+        dom = max(out["kernels"], key=lambda k: out["kernels"][k]["ms_per_launch"])
+        kd = out["kernels"][dom]
+        traffic = None
+        tfile = os.path.join(ROOT, "profiles", f"pmc_traffic_{dom}.txt")
+        if os.path.exists(tfile):
+            try:
+                with open(tfile) as f:
+                    traffic = json.loads(f.readline())["hbm_bytes_per_launch"]
+            except (ValueError, KeyError):
+                traffic = None
+        out["roofline"] = {"bound": "hbm", "kernel": dom, "achieved": kd["GBps"], "peak": HBM_PEAK_GBS,
+                           "unit": "GB/s", "frac": kd["GBps"] / HBM_PEAK_GBS, "traffic": traffic,
+                           "algorithmic_bytes_per_launch": kd["algorithmic_bytes_per_launch"],
+                           "ms_per_launch": kd["ms_per_launch"]}
+    if not args.no_cpu_baseline:
+        model, cores = cpu_info()
+        threads = min(cores, 64)
+        cn, cdt, cph = cpu_baseline(args.config, args.cpu_res, threads)
+        out["cpu_baseline"] = {
+            "value": cn / cdt, "unit": "samples/s", "cores": threads, "kind": "port",
+            "sample": f"same scene at resolution scale {args.cpu_res} ({cn} samples), full path "
+                      f"(tree serial as in the reference, OpenMP float occupancy/priors, serial DMC); {model}",
+            "phase_s": {"tree": cph[0], "sampleness": cph[1], "emptiness": cph[2], "priors": cph[3],
+                        "occupancy": cph[4], "dmc": cph[5]},
+        }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

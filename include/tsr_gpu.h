@@ -33,7 +33,7 @@ typedef enum tsr_status {
     TSR_ERR_NO_SAMPLES = 2,            /* Scene.cpp:137-151: reconstruct() returns false          */
     TSR_ERR_SAMPLE_OUTSIDE_ROOT = 3,   /* Nodes.cpp:237-238 findContainer exception                */
     TSR_ERR_NODE_OVERFLOW = 4,         /* Nodes.cpp:399-407 more than 2^32-2 nodes                  */
-    TSR_ERR_TREE_TOO_DEEP = 5,         /* octree deeper than 21 levels (63-bit location codes)     */
+    TSR_ERR_TREE_TOO_DEEP = 5,         /* octree deeper than 19 levels                              */
     TSR_ERR_LINK_OVERFLOW = 6,         /* Samples.h:395-402 more than 2^32-1 camera links          */
     TSR_ERR_NON_MANIFOLD_EDGE = 7,     /* FlexibleMesh.cpp:1227-1260 addEdgeConflict "BOOM"        */
     TSR_ERR_HIP = 8,                   /* HIP runtime failure                                      */
@@ -56,7 +56,10 @@ typedef struct tsr_params {
 enum {
     TSR_ACCUM_KAHAN_SEGMENTED = 0,   /* FP32 compensated, deterministic segmented reduction (default) */
     TSR_ACCUM_FP32_ATOMIC = 1,       /* plain FP32 atomics (precision study only, nondeterministic)  */
-    TSR_CULL_LEARNED = 0x100         /* skip emptiness cones in nodes the culling MLP marks far     */
+    TSR_CULL_LEARNED = 0x100,        /* skip emptiness cones in nodes the culling MLP marks far     */
+    TSR_PRIORS_ALL_LEAVES = 0x200    /* priors for every leaf; by default only for the leaves whose
+                                        occupancy reads them (E + S >= confidence threshold, the
+                                        others get NaN)                                             */
 };
 
 typedef struct tsr_context tsr_context;
@@ -108,18 +111,31 @@ tsr_status tsr_context_synchronize(tsr_context *ctx);
 /* Milliseconds of the last tree build / occupancy / DMC phases (HIP events):
  * [0] tree, [1] S pass, [2] E pass, [3] priors, [4] occupancy, [5] DMC.  n <= 6. */
 tsr_status tsr_context_phase_times(tsr_context *ctx, float *ms, int n);
+/* Accumulated live timing (HIP events on the context stream) of instrumented
+ * kernels since the last reset, with their algorithmic bytes (each input or
+ * output element counted once per launch) and work units (queries or links):
+ * kernel 0 = priors sphere query, 1/2 = emptiness cone pass count/write,
+ * 3/4 = sampleness cone pass count/write. */
+tsr_status tsr_context_kernel_stats(tsr_context *ctx, int kernel, double *ms, uint64_t *launches, uint64_t *bytes,
+                                    uint64_t *units);
+tsr_status tsr_context_reset_kernel_stats(tsr_context *ctx);
+/* Plain device copy kernel of `bytes` bytes (float4 loads and stores) on the
+ * context stream: the reference point for scaling HBM traffic counters. */
+tsr_status tsr_calibration_copy(tsr_context *ctx, uint64_t bytes);
 
 /* Scene = device copy of samples + cameras (the Scene singleton's data). */
 tsr_status tsr_scene_upload(tsr_context *ctx, const tsr_samples_view *samples, const tsr_cameras_view *cameras,
                             tsr_scene **out);
 void tsr_scene_destroy(tsr_scene *scene);
-/* Download the scene's samples in their current (tree) order. Arrays may be NULL. */
+/* Download the scene's samples (input order). Arrays may be NULL. */
 tsr_status tsr_scene_download(tsr_context *ctx, const tsr_scene *scene, float *px, float *py, float *pz, float *nx,
                               float *ny, float *nz, float *scale, float *confidence, uint32_t *parent_cameras);
 
-/* Tree::Tree(): octree nodes, leaves and dual cells built on the GPU.  The
- * scene's samples are permuted into node order (Nodes.cpp:594-636). */
-tsr_status tsr_tree_build(tsr_context *ctx, tsr_scene *scene, const tsr_params *params, tsr_tree **out);
+/* Tree::Tree(): octree nodes, leaves and dual cells built on the GPU.  The tree
+ * keeps its own copy of the samples permuted into node order (Nodes.cpp:594-636,
+ * Samples::reorder); the scene's arrays stay in input order, so building again
+ * from the same scene gives the same tree. */
+tsr_status tsr_tree_build(tsr_context *ctx, const tsr_scene *scene, const tsr_params *params, tsr_tree **out);
 tsr_status tsr_tree_get_info(const tsr_tree *tree, tsr_tree_info *info);
 /* Host copies of the tables (NULL pointers are skipped):
  *   children/parents/sample_counts/sample_starts: node_count (Nodes.h:143-147)
@@ -132,6 +148,10 @@ tsr_status tsr_tree_download(tsr_context *ctx, const tsr_tree *tree, uint32_t *c
                              uint32_t *sample_counts, uint32_t *sample_starts, uint32_t *node_to_leaf,
                              uint32_t *leaf_nodes, float *leaf_scopes, uint32_t *cells, uint32_t *sample_target,
                              uint32_t *node_depth);
+/* The tree's samples in node order (the reference's reordered Samples). */
+tsr_status tsr_tree_samples_download(tsr_context *ctx, const tsr_tree *tree, float *px, float *py, float *pz,
+                                     float *nx, float *ny, float *nz, float *scale, float *confidence,
+                                     uint32_t *parent_cameras);
 void tsr_tree_destroy(tsr_tree *tree);
 
 /* Occupancy::Occupancy(tree): S pass, flag propagation, E pass, priors and the

@@ -145,3 +145,76 @@ def mc_case(cfg):
 
 def mc_edges(cfg):
     return int(lib().oracle_mc_edges(cfg))
+
+
+def _d3(v):
+    a = np.ascontiguousarray(v, np.float64)
+    return a, a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def ray_kernel(z, Z):
+    L = lib()
+    L.oracle_ray_kernel.restype = ctypes.c_double
+    L.oracle_ray_kernel.argtypes = [ctypes.c_double, ctypes.c_double]
+    return L.oracle_ray_kernel(z, Z)
+
+
+def disc_kernel(r2, R):
+    L = lib()
+    L.oracle_disc_kernel.restype = ctypes.c_double
+    L.oracle_disc_kernel.argtypes = [ctypes.c_double, ctypes.c_double]
+    return L.oracle_disc_kernel(r2, R)
+
+
+def cone_kernel(apex, end_normal, end, R, length, x, sampleness, conf=1.0):
+    """Returns (scaled kernel, tComplete, tCut, getLength(), degenerate)."""
+    L = lib()
+    dp = ctypes.POINTER(ctypes.c_double)
+    L.oracle_cone_kernel.restype = ctypes.c_double
+    L.oracle_cone_kernel.argtypes = [dp, dp, dp, ctypes.c_double, ctypes.c_double, dp, ctypes.c_int,
+                                     ctypes.c_double, dp]
+    a, ap = _d3(apex)
+    n, np_ = _d3(end_normal)
+    e, ep = _d3(end)
+    xx, xp = _d3(x)
+    out, op = _d3(np.zeros(4))
+    k = L.oracle_cone_kernel(ap, np_, ep, R, length, xp, int(sampleness), conf, op)
+    return k, out[0], out[1], out[2], bool(out[3])
+
+
+def cone_separated(apex, end_normal, end, R, length, lo, size):
+    L = lib()
+    dp = ctypes.POINTER(ctypes.c_double)
+    L.oracle_cone_separated.restype = ctypes.c_int
+    L.oracle_cone_separated.argtypes = [dp, dp, dp, ctypes.c_double, ctypes.c_double, dp, ctypes.c_double]
+    a, ap = _d3(apex)
+    n, np_ = _d3(end_normal)
+    e, ep = _d3(end)
+    l, lp = _d3(lo)
+    return bool(L.oracle_cone_separated(ap, np_, ep, R, length, lp, size))
+
+
+def uniform(depth):
+    """Oracle handle holding a uniform tree of `depth` levels over [0,1)^3."""
+    one = np.ones(1, np.float32)
+    samples = {k: one.copy() * 0.5 for k in ("px", "py", "pz")}
+    samples.update({"nx": one * 0, "ny": one * 0, "nz": one.copy(), "scale": one.copy() * 0.1,
+                    "confidence": one.copy(), "parent_cameras": np.zeros(1, np.uint32)})
+    o = Oracle(samples, {"x": one * 3, "y": one * 0, "z": one * 0})
+    L = lib()
+    L.oracle_uniform_tree.argtypes = [ctypes.c_void_p, ctypes.c_uint32]
+    if L.oracle_uniform_tree(o.h, depth) != 0:
+        raise OracleError(L.oracle_error(o.h).decode())
+    return o
+
+
+def leaf_centres(o):
+    L = lib()
+    L.oracle_leaf_centre.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double)]
+    n = o.count("leaves")
+    out = np.zeros((n, 3))
+    buf = np.zeros(3)
+    for i in range(n):
+        L.oracle_leaf_centre(o.h, i, buf.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        out[i] = buf
+    return out

@@ -1083,3 +1083,80 @@ const char *oracle_mc_case(u32 cfg) { return cfg < 256 ? TRI[cfg] : ""; }
 u32 oracle_mc_edges(u32 cfg) { return edgeTableEntry(cfg); }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Test hooks: closed-form kernels, one cone evaluation, uniform trees
+// ---------------------------------------------------------------------------
+extern "C" {
+
+double oracle_ray_kernel(double z, double Z) { return OccupancyT<double>::rayKernel(z, Z); }
+double oracle_disc_kernel(double r2, double R) { return OccupancyT<double>::discKernel(r2, R); }
+
+// Cone(apex, endNormal, end, R, L) evaluated at x; out[0] = tComplete, out[1] = tCut,
+// out[2] = getLength(), out[3] = degenerate flag.  Returns the scaled kernel (0 = rejected).
+double oracle_cone_kernel(const double *apex, const double *endNormal, const double *end, double R, double L,
+                          const double *x, int sampleness, double conf, double *out) {
+    Cone<double> c;
+    c.init(apex, endNormal, end, R, L);
+    if (out) {
+        out[3] = c.degenerate ? 1.0 : 0.0;
+        if (!c.degenerate) {
+            const double tc = ((x[0] - c.a[0]) * c.n[0] + (x[1] - c.a[1]) * c.n[1] + (x[2] - c.a[2]) * c.n[2]) * c.invDenom;
+            out[0] = tc;
+            out[1] = (tc - c.tStart) * c.invCut;
+            out[2] = c.length;
+        }
+    }
+    if (c.degenerate) return 0.0;
+    return OccupancyT<double>::kernelAt(c, x, sampleness != 0, conf);
+}
+
+// conservative culling bound of the cone for the cube [lo, lo + size]^3 (1 = culled)
+int oracle_cone_separated(const double *apex, const double *endNormal, const double *end, double R, double L,
+                          const double *lo, double size) {
+    Cone<double> c;
+    c.init(apex, endNormal, end, R, L);
+    if (c.degenerate) return 1;
+    return c.separated(lo, size) ? 1 : 0;
+}
+
+// Replace the handle's tree by a uniform tree of the given depth over the unit
+// cube [0,1)^3 (leaves in Morton order) and build leaves + dual cells; used by
+// the closed-form DMC tests.
+int oracle_uniform_tree(void *hp, u32 depth) {
+    Handle *h = (Handle *)hp;
+    try {
+        h->t = Tree();
+        Tree &t = h->t;
+        t.rootSizeF = 1.0f; t.rootSize = 1.0;
+        t.newNode(INVALID, 0, 0, 0, 0);
+        std::vector<u32> frontier{0};
+        for (u32 d = 0; d < depth; ++d) {
+            std::vector<u32> next;
+            for (u32 nd : frontier) {
+                t.createChildren(nd);
+                for (u32 c = 0; c < 8; ++c) next.push_back(t.children[nd] + c);
+            }
+            frontier.swap(next);
+        }
+        Builder b(h->s, t);
+        b.reorder();
+        t.nodeToLeaf.assign(t.count(), INVALID);
+        b.gather(0);
+        b.byNode(0);
+        h->timings.assign(6, 0.0);
+        return 0;
+    } catch (const std::exception &e) {
+        h->err = e.what();
+        return 1;
+    }
+}
+
+// leaf centre of leaf l (3 doubles)
+void oracle_leaf_centre(void *hp, u32 l, double *out) {
+    Handle *h = (Handle *)hp;
+    const u32 n = h->t.leafNode[l];
+    for (int a = 0; a < 3; ++a) out[a] = h->t.centre(a, n);
+}
+
+}  // extern "C"

@@ -33,13 +33,14 @@ def _oracle(samples, cameras):
 @pytest.fixture(scope="module", params=CASES, ids=lambda c: f"cfg{c[0]}x{c[1]}")
 def case(request, gpu_ctx):
     from tsr_amd import DualMarchingCells, Occupancy, Scene, Tree
+    from tsr_amd._lib import TSR_PRIORS_ALL_LEAVES
     cfg, res = request.param
     samples, cameras = scene(cfg, res)
     print(f"\n[case] config={cfg} res={res} samples={len(samples['px'])}", flush=True)
     orc = _oracle(samples, cameras)
     sc = Scene(gpu_ctx, samples, cameras)
     tree = Tree(gpu_ctx, sc)
-    occ = Occupancy(gpu_ctx, tree, sc)
+    occ = Occupancy(gpu_ctx, tree, sc, flags=TSR_PRIORS_ALL_LEAVES)
     mesh = DualMarchingCells(gpu_ctx, tree, occ)
     return {"orc": orc, "tree": tree.download(), "field": occ.download(), "mesh": mesh.download(),
             "gpu_tree": tree, "gpu_occ": occ, "samples": samples, "scene": sc}
@@ -58,11 +59,14 @@ def test_tree_tables_bit_exact(case):
 def test_samples_reordered(case):
     t = case["tree"]
     s = case["samples"]
-    d = case["scene"].download()
+    d = case["gpu_tree"].download_samples()
     inv = np.empty_like(t["sample_target"])
     inv[t["sample_target"]] = np.arange(len(inv), dtype=np.uint32)
     for k in ("px", "py", "pz", "nx", "scale"):
         assert np.array_equal(d[k], s[k][inv])
+    orig = case["scene"].download()
+    for k in ("px", "py", "pz", "nx", "scale"):
+        assert np.array_equal(orig[k], s[k])
 
 
 def test_field_parity(case):
@@ -95,7 +99,21 @@ def test_mesh_parity(case):
 
 def test_deterministic_rerun(case, gpu_ctx):
     from tsr_amd import Occupancy
-    occ2 = Occupancy(gpu_ctx, case["gpu_tree"], case["scene"]).download()
+    from tsr_amd._lib import TSR_PRIORS_ALL_LEAVES
+    occ2 = Occupancy(gpu_ctx, case["gpu_tree"], case["scene"], flags=TSR_PRIORS_ALL_LEAVES).download()
     g = case["field"]
     for k in ("kernel_sums", "cone_lengths", "priors", "occupancy", "node_states"):
         assert np.array_equal(occ2[k], g[k]), k
+
+
+def test_default_priors_only_where_used(case, gpu_ctx):
+    # the default mode computes priors only for leaves whose occupancy reads them;
+    # the occupancy field is identical
+    from tsr_amd import Occupancy
+    occ = Occupancy(gpu_ctx, case["gpu_tree"], case["scene"]).download()
+    g = case["field"]
+    assert np.array_equal(occ["occupancy"], g["occupancy"])
+    L = len(g["occupancy"])
+    used = g["kernel_sums"][:L].astype(np.float64) + g["kernel_sums"][L:] >= 100.0
+    assert np.array_equal(occ["priors"][used], g["priors"][used])
+    assert np.all(np.isnan(occ["priors"][~used]))

@@ -78,6 +78,7 @@ STATUS_NAMES = {
 TSR_ACCUM_KAHAN_SEGMENTED = 0
 TSR_ACCUM_FP32_ATOMIC = 1
 TSR_CULL_LEARNED = 0x100
+TSR_PRIORS_ALL_LEAVES = 0x200
 MLP_PARAM_COUNT = 64 * 16 + 64 + 64 * 64 + 64 + 64 * 64 + 64 + 64 + 1
 
 # (name, restype, argtypes) of every exported entry point of include/tsr_gpu.h
@@ -88,12 +89,16 @@ SIGNATURES = [
     ("tsr_context_destroy", None, [vp]),
     ("tsr_context_synchronize", ctypes.c_int, [vp]),
     ("tsr_context_phase_times", ctypes.c_int, [vp, f32p, ctypes.c_int]),
+    ("tsr_context_kernel_stats", ctypes.c_int, [vp, ctypes.c_int, f64p, u64p, u64p, u64p]),
+    ("tsr_context_reset_kernel_stats", ctypes.c_int, [vp]),
+    ("tsr_calibration_copy", ctypes.c_int, [vp, u64]),
     ("tsr_scene_upload", ctypes.c_int, [vp, ctypes.POINTER(SamplesView), ctypes.POINTER(CamerasView), ctypes.POINTER(vp)]),
     ("tsr_scene_destroy", None, [vp]),
     ("tsr_scene_download", ctypes.c_int, [vp, vp, f32p, f32p, f32p, f32p, f32p, f32p, f32p, f32p, u32p]),
     ("tsr_tree_build", ctypes.c_int, [vp, vp, ctypes.POINTER(Params), ctypes.POINTER(vp)]),
     ("tsr_tree_get_info", ctypes.c_int, [vp, ctypes.POINTER(TreeInfo)]),
     ("tsr_tree_download", ctypes.c_int, [vp, vp, u32p, u32p, u32p, u32p, u32p, u32p, f32p, u32p, u32p, u32p]),
+    ("tsr_tree_samples_download", ctypes.c_int, [vp, vp, f32p, f32p, f32p, f32p, f32p, f32p, f32p, f32p, u32p]),
     ("tsr_tree_destroy", None, [vp]),
     ("tsr_occupancy_compute", ctypes.c_int, [vp, vp, vp, ctypes.POINTER(Params), ctypes.c_int, vp, ctypes.POINTER(vp)]),
     ("tsr_field_download", ctypes.c_int, [vp, vp, f32p, f32p, f32p, u32p, f64p, f64p]),

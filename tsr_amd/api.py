@@ -52,6 +52,24 @@ class Context:
         return ms
 
 
+    KERNELS = {"priors": 0, "emptiness_count": 1, "emptiness_write": 2, "sampleness_count": 3,
+               "sampleness_write": 4}
+
+    def kernel_stats(self, name):
+        """(ms, launches, algorithmic bytes, units) accumulated for an instrumented kernel."""
+        ms = ctypes.c_double(0)
+        la, by, un = ctypes.c_uint64(0), ctypes.c_uint64(0), ctypes.c_uint64(0)
+        L.check(L.lib().tsr_context_kernel_stats(self.h, self.KERNELS[name], ctypes.byref(ms), ctypes.byref(la),
+                                                 ctypes.byref(by), ctypes.byref(un)))
+        return ms.value, la.value, by.value, un.value
+
+    def reset_kernel_stats(self):
+        L.check(L.lib().tsr_context_reset_kernel_stats(self.h))
+
+    def calibration_copy(self, nbytes):
+        L.check(L.lib().tsr_calibration_copy(self.h, int(nbytes)))
+
+
 class SyntheticScene:
     """Seeded analytic scene (config 0..4) generated on the host (tsr_synthetic_*)."""
 
@@ -152,6 +170,7 @@ class Tree:
         self.root_min = np.array(info.root_min[:], np.float32)
         self.root_size = np.float32(info.root_size)
         self.sample_count = scene.count
+        self.max_cams = scene.max_cams
 
     def close(self):
         if self.h:
@@ -178,6 +197,17 @@ class Tree:
                                           _up(o["leaf_nodes"]), _fp(o["leaf_scopes"]), _up(o["cells"]),
                                           _up(o["sample_target"]), _up(o["node_depth"])))
         return o
+
+
+    def download_samples(self):
+        """The samples in node order (Samples::reorder)."""
+        n = self.sample_count
+        out = {k: np.zeros(n, np.float32) for k in ("px", "py", "pz", "nx", "ny", "nz", "scale", "confidence")}
+        out["parent_cameras"] = np.zeros(n * self.max_cams, np.uint32)
+        L.check(L.lib().tsr_tree_samples_download(self.ctx.h, self.h, *[_fp(out[k]) for k in
+                                                                        ("px", "py", "pz", "nx", "ny", "nz", "scale",
+                                                                         "confidence")], _up(out["parent_cameras"])))
+        return out
 
 
 class Occupancy:

@@ -71,6 +71,7 @@ tsr_status tsr_context_create(int device, tsr_context **out) {
         c->device = device;
         HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         for (hipEvent_t &e : c->ev) HIPCHECK(hipEventCreate(&e));
+        for (hipEvent_t &e : c->kev) HIPCHECK(hipEventCreate(&e));
         *out = c;
     });
 }
@@ -81,6 +82,7 @@ void tsr_context_destroy(tsr_context *ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     ctx->scratch.buf.release();
     for (hipEvent_t &e : ctx->ev) (void)hipEventDestroy(e);
+    for (hipEvent_t &e : ctx->kev) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -103,6 +105,31 @@ tsr_status tsr_context_phase_times(tsr_context *ctx, float *ms, int n) {
     });
 }
 
+tsr_status tsr_context_kernel_stats(tsr_context *ctx, int kernel, double *ms, uint64_t *launches, uint64_t *bytes,
+                                    uint64_t *units) {
+    if (!ctx || kernel < 0 || kernel >= 8) return fail(TSR_ERR_INVALID_ARGUMENT, "bad arguments");
+    const tsr_kernel_stat &k = ctx->kstat[kernel];
+    if (ms) *ms = k.ms;
+    if (launches) *launches = k.launches;
+    if (bytes) *bytes = k.bytes;
+    if (units) *units = k.units;
+    return TSR_OK;
+}
+
+tsr_status tsr_context_reset_kernel_stats(tsr_context *ctx) {
+    if (!ctx) return fail(TSR_ERR_INVALID_ARGUMENT, "null context");
+    for (tsr_kernel_stat &k : ctx->kstat) k = tsr_kernel_stat();
+    return TSR_OK;
+}
+
+tsr_status tsr_calibration_copy(tsr_context *ctx, uint64_t bytes) {
+    if (!ctx || bytes == 0) return fail(TSR_ERR_INVALID_ARGUMENT, "bad arguments");
+    return guarded([&] {
+        activate(ctx);
+        calibrationCopy(ctx, bytes);
+    });
+}
+
 tsr_status tsr_scene_upload(tsr_context *ctx, const tsr_samples_view *s, const tsr_cameras_view *c,
                             tsr_scene **out) {
     if (!ctx || !s || !c || !out) return fail(TSR_ERR_INVALID_ARGUMENT, "null argument");
@@ -120,6 +147,7 @@ tsr_status tsr_scene_upload(tsr_context *ctx, const tsr_samples_view *s, const t
             hipStream_t st = ctx->stream;
             const u32 n = s->count;
             sc->ctx = ctx;
+            sc->device = ctx->device;
             sc->n = n;
             sc->maxCams = s->max_cameras_per_sample;
             sc->camCount = c->count;
@@ -142,8 +170,8 @@ tsr_status tsr_scene_upload(tsr_context *ctx, const tsr_samples_view *s, const t
 
 void tsr_scene_destroy(tsr_scene *scene) {
     if (!scene) return;
-    (void)hipSetDevice(scene->ctx->device);
-    (void)hipStreamSynchronize(scene->ctx->stream);
+    (void)hipSetDevice(scene->device);
+    (void)hipDeviceSynchronize();
     delete scene;
 }
 
@@ -162,7 +190,7 @@ tsr_status tsr_scene_download(tsr_context *ctx, const tsr_scene *sc, float *px, 
     });
 }
 
-tsr_status tsr_tree_build(tsr_context *ctx, tsr_scene *scene, const tsr_params *params, tsr_tree **out) {
+tsr_status tsr_tree_build(tsr_context *ctx, const tsr_scene *scene, const tsr_params *params, tsr_tree **out) {
     if (!ctx || !scene || !out) return fail(TSR_ERR_INVALID_ARGUMENT, "null argument");
     *out = nullptr;
     tsr_params prm;
@@ -173,6 +201,7 @@ tsr_status tsr_tree_build(tsr_context *ctx, tsr_scene *scene, const tsr_params *
         activate(ctx);
         tsr_tree *t = new tsr_tree();
         t->ctx = ctx;
+        t->device = ctx->device;
         try {
             recordPhase(ctx, 0, true);
             buildTree(ctx, scene, prm, t);
@@ -241,10 +270,25 @@ tsr_status tsr_tree_download(tsr_context *ctx, const tsr_tree *t, uint32_t *chil
     });
 }
 
+tsr_status tsr_tree_samples_download(tsr_context *ctx, const tsr_tree *t, float *px, float *py, float *pz, float *nx,
+                                     float *ny, float *nz, float *scale, float *confidence, uint32_t *parent_cameras) {
+    if (!ctx || !t) return fail(TSR_ERR_INVALID_ARGUMENT, "null argument");
+    return guarded([&] {
+        activate(ctx);
+        hipStream_t st = ctx->stream;
+        const u32 n = t->sampleCount;
+        download(px, t->px, n, st); download(py, t->py, n, st); download(pz, t->pz, n, st);
+        download(nx, t->nx, n, st); download(ny, t->ny, n, st); download(nz, t->nz, n, st);
+        download(scale, t->scale, n, st); download(confidence, t->conf, n, st);
+        download(parent_cameras, t->cams, (size_t)n * t->maxCams, st);
+        HIPCHECK(hipStreamSynchronize(st));
+    });
+}
+
 void tsr_tree_destroy(tsr_tree *t) {
     if (!t) return;
-    (void)hipSetDevice(t->ctx->device);
-    (void)hipStreamSynchronize(t->ctx->stream);
+    (void)hipSetDevice(t->device);
+    (void)hipDeviceSynchronize();
     delete t;
 }
 
@@ -254,6 +298,7 @@ tsr_status tsr_occupancy_compute(tsr_context *ctx, const tsr_tree *tree, const t
     *out = nullptr;
     if (scene->n != tree->sampleCount) return fail(TSR_ERR_INVALID_ARGUMENT, "scene and tree sample counts differ");
     const int mode = flags & 0xff;
+    if (flags & ~(0xff | TSR_CULL_LEARNED | TSR_PRIORS_ALL_LEAVES)) return fail(TSR_ERR_INVALID_ARGUMENT, "unknown flags");
     if (mode != TSR_ACCUM_KAHAN_SEGMENTED && mode != TSR_ACCUM_FP32_ATOMIC)
         return fail(TSR_ERR_INVALID_ARGUMENT, "unknown accumulation mode");
     if ((flags & TSR_CULL_LEARNED) && !culler) return fail(TSR_ERR_INVALID_ARGUMENT, "learned culling needs a culler");
@@ -264,6 +309,7 @@ tsr_status tsr_occupancy_compute(tsr_context *ctx, const tsr_tree *tree, const t
         activate(ctx);
         tsr_field *f = new tsr_field();
         f->ctx = ctx;
+        f->device = ctx->device;
         try {
             DBuf<uint8_t> mask;
             if (flags & TSR_CULL_LEARNED) tsrCullerMask(ctx, culler, tree, scene, prm, mask);
@@ -301,8 +347,8 @@ tsr_status tsr_field_hit_counts(const tsr_field *f, uint64_t *hits2) {
 
 void tsr_field_destroy(tsr_field *f) {
     if (!f) return;
-    (void)hipSetDevice(f->ctx->device);
-    (void)hipStreamSynchronize(f->ctx->stream);
+    (void)hipSetDevice(f->device);
+    (void)hipDeviceSynchronize();
     delete f;
 }
 
@@ -314,6 +360,7 @@ tsr_status tsr_dmc_extract(tsr_context *ctx, const tsr_tree *tree, const tsr_fie
         activate(ctx);
         tsr_mesh *m = new tsr_mesh();
         m->ctx = ctx;
+        m->device = ctx->device;
         try {
             recordPhase(ctx, 5, true);
             extractDMC(ctx, tree, field, m);
@@ -350,8 +397,8 @@ tsr_status tsr_mesh_download(tsr_context *ctx, const tsr_mesh *m, double *positi
 
 void tsr_mesh_destroy(tsr_mesh *m) {
     if (!m) return;
-    (void)hipSetDevice(m->ctx->device);
-    (void)hipStreamSynchronize(m->ctx->stream);
+    (void)hipSetDevice(m->device);
+    (void)hipDeviceSynchronize();
     delete m;
 }
 

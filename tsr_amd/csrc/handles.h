@@ -5,27 +5,37 @@
 
 #include "device_util.h"
 
+// live timing + algorithmic traffic of selected kernels (TSR_KSTAT_*)
+struct tsr_kernel_stat {
+    double ms = 0;
+    uint64_t launches = 0, bytes = 0, units = 0;
+};
+
 struct tsr_context {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev[12] = {};   // start/end pair per phase
+    hipEvent_t kev[2] = {};   // start/end of an instrumented kernel
     float phaseMs[6] = {0, 0, 0, 0, 0, 0};
+    tsr_kernel_stat kstat[8];
     tsr::Scratch scratch;
 };
 
 struct tsr_scene {
     tsr_context *ctx = nullptr;
+    int device = 0;   // destroy needs only the device, not the context
     tsr::u32 n = 0, maxCams = 1, camCount = 0;
     tsr::DBuf<float> px, py, pz, nx, ny, nz, scale, conf;
     tsr::DBuf<tsr::u32> cams;
     tsr::DBuf<float> cx, cy, cz;
-    // host copies of the geometry in the current sample order, read by the
-    // order-dependent replay of the sampling refinement
+    // host copies of the geometry (input order), read by the order-dependent
+    // replay of the sampling refinement
     std::vector<float> hpx, hpy, hpz, hnx, hny, hnz, hscale;
 };
 
 struct tsr_tree {
     tsr_context *ctx = nullptr;
+    int device = 0;   // destroy needs only the device, not the context
     tsr::Frame frame{};
     float maxRel = 2.0f;
     tsr::u32 nodeCount = 0, innerCount = 0, leafCount = 0, cellCount = 0, maxDepth = 0, sampleCount = 0;
@@ -41,12 +51,17 @@ struct tsr_tree {
     tsr::DBuf<tsr::u32> cells;         // 8 leaf indices per dual cell (DualCells::mIndices)
     tsr::DBuf<tsr::u32> sampleTarget;  // new index of each original sample
     tsr::DBuf<uint8_t> sampleDepth;    // container depth per sample (tree order)
+    // the scene's samples permuted into node order (Samples::reorder, Samples.cpp:825-835)
+    tsr::u32 maxCams = 1;
+    tsr::DBuf<float> px, py, pz, nx, ny, nz, scale, conf;
+    tsr::DBuf<tsr::u32> cams;
     tsr::DHash nodeHash;               // location code -> node id (all nodes)
     tsr::u64 samplingEvents = 0, samplingCreated = 0, samplingUnits = 0;
 };
 
 struct tsr_field {
     tsr_context *ctx = nullptr;
+    int device = 0;   // destroy needs only the device, not the context
     tsr::u32 leafCount = 0, nodeCount = 0;
     tsr::DBuf<float> ksum;     // [2][L] kernel sums, 0 = EMPTINESS, 1 = SAMPLENESS
     tsr::DBuf<float> clen;     // [2][L] cone length sums
@@ -55,10 +70,12 @@ struct tsr_field {
     tsr::DBuf<double> occ;     // [L] occupancy (FP64)
     double coneCount = 0;
     tsr::u64 hits[2] = {0, 0};
+    tsr::u32 priorQueries = 0;
 };
 
 struct tsr_mesh {
     tsr_context *ctx = nullptr;
+    int device = 0;   // destroy needs only the device, not the context
     tsr::u32 vertexCount = 0, triangleCount = 0;
     tsr::DBuf<double> pos;      // 3 per vertex
     tsr::DBuf<double> scale;    // 1 per vertex
@@ -67,14 +84,28 @@ struct tsr_mesh {
 };
 
 namespace tsr {
+enum { KSTAT_PRIORS = 0, KSTAT_E_COUNT = 1, KSTAT_E_WRITE = 2, KSTAT_S_COUNT = 3, KSTAT_S_WRITE = 4 };
+// time one kernel launch on the context stream with HIP events (synchronises)
+template <typename Launch>
+void timedLaunch(tsr_context *ctx, int stat, Launch launch) {
+    HIPCHECK(hipEventRecord(ctx->kev[0], ctx->stream));
+    launch();
+    HIPCHECK(hipEventRecord(ctx->kev[1], ctx->stream));
+    HIPCHECK(hipEventSynchronize(ctx->kev[1]));
+    float ms = 0.f;
+    HIPCHECK(hipEventElapsedTime(&ms, ctx->kev[0], ctx->kev[1]));
+    ctx->kstat[stat].ms += ms;
+    ctx->kstat[stat].launches += 1;
+}
 // phase: 0 tree, 1 S pass, 2 E pass, 3 priors, 4 occupancy, 5 DMC
 inline void recordPhase(tsr_context *ctx, int phase, bool start) {
     HIPCHECK(hipEventRecord(ctx->ev[2 * phase + (start ? 0 : 1)], ctx->stream));
 }
-void buildTree(tsr_context *ctx, tsr_scene *scene, const tsr_params &prm, tsr_tree *tree);
+void buildTree(tsr_context *ctx, const tsr_scene *scene, const tsr_params &prm, tsr_tree *tree);
 void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *scene, const tsr_params &prm,
                       int flags, const uint8_t *cullMask, tsr_field *field);
 void extractDMC(tsr_context *ctx, const tsr_tree *tree, const tsr_field *field, tsr_mesh *mesh);
+void calibrationCopy(tsr_context *ctx, u64 bytes);
 }  // namespace tsr
 
 struct tsr_culler;

@@ -71,11 +71,12 @@ __device__ bool setupLink(const LinkInputs &in, u32 link, bool sampleness, ConeD
 // cull tests.  Inner nodes failing a test are skipped with their subtrees.
 template <typename OnLeaf>
 __device__ void traverseCone(const TraversalTree &t, const ConeCull &cull, u32 maxDepth, u32 required,
-                             const uint8_t *cullNode, OnLeaf onLeaf) {
+                             const uint8_t *cullNode, u32 &visits, OnLeaf onLeaf) {
     u32 node = 0, d = 0, k[3] = {0, 0, 0};
     bool descend = true;   // whether to test `node` (true) or move on from it (false)
     for (;;) {
         if (descend) {
+            ++visits;
             bool accept = d <= maxDepth;
             if (accept && required) accept = (t.states[node] & required) == required;
             if (accept && cullNode) accept = cullNode[node] == 0;
@@ -118,13 +119,39 @@ __device__ void traverseCone(const TraversalTree &t, const ConeCull &cull, u32 m
     }
 }
 
+__device__ void coneLink(const TraversalTree &t, const LinkInputs &in, u32 link, int sampleness, int mode,
+                         u32 *__restrict__ counts, const u32 *__restrict__ offsets, u32 *__restrict__ recLeaf,
+                         uint2 *__restrict__ recVal, float *__restrict__ atomicSum, float *__restrict__ atomicLen,
+                         u32 *__restrict__ atomicStates, double *__restrict__ confOut, u32 &visits, u32 &count);
+
+// wave sum of a per-lane count, added once per wave to *acc
+__device__ void waveAccumulate(unsigned long long *acc, u32 v) {
+    unsigned long long x = v;
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+    if ((threadIdx.x & 63) == 0) atomicAdd(acc, x);
+}
+
 // mode 0: count accepted hits per link; 1: write records; 2: FP32 atomics (precision study)
+// stats[0] += node visits, stats[1] += accepted hits
 __global__ void k_conePass(TraversalTree t, LinkInputs in, u32 linkCount, int sampleness, int mode,
                            u32 *__restrict__ counts, const u32 *__restrict__ offsets, u32 *__restrict__ recLeaf,
                            uint2 *__restrict__ recVal, float *__restrict__ atomicSum, float *__restrict__ atomicLen,
-                           u32 *__restrict__ atomicStates, double *__restrict__ confOut) {
+                           u32 *__restrict__ atomicStates, double *__restrict__ confOut,
+                           unsigned long long *__restrict__ stats) {
     const u32 link = blockIdx.x * BLOCK + threadIdx.x;
-    if (link >= linkCount) return;
+    u32 visits = 0, count = 0;
+    if (link < linkCount) coneLink(t, in, link, sampleness, mode, counts, offsets, recLeaf, recVal, atomicSum,
+                                   atomicLen, atomicStates, confOut, visits, count);
+    if (stats) {
+        waveAccumulate(&stats[0], visits);
+        waveAccumulate(&stats[1], count);
+    }
+}
+
+__device__ void coneLink(const TraversalTree &t, const LinkInputs &in, u32 link, int sampleness, int mode,
+                         u32 *__restrict__ counts, const u32 *__restrict__ offsets, u32 *__restrict__ recLeaf,
+                         uint2 *__restrict__ recVal, float *__restrict__ atomicSum, float *__restrict__ atomicLen,
+                         u32 *__restrict__ atomicStates, double *__restrict__ confOut, u32 &visits, u32 &count) {
     ConeD cone;
     u32 i;
     const bool ok = setupLink(in, link, sampleness != 0, cone, i);
@@ -139,9 +166,8 @@ __global__ void k_conePass(TraversalTree t, LinkInputs in, u32 linkCount, int sa
     const float weightedLength = (float)(conf * cone.length);
     const u32 maxDepth = (u32)in.sampleDepth[i] + MAX_DEPTH_DIFFERENCE;
     const u32 required = sampleness ? 0u : FLAG_S;
-    u32 count = 0;
     const u32 base = mode == 1 ? offsets[link] : 0;
-    traverseCone(t, cull, maxDepth, required, sampleness ? nullptr : in.cullNode, [&](u32 leaf, const double x[3]) {
+    traverseCone(t, cull, maxDepth, required, sampleness ? nullptr : in.cullNode, visits, [&](u32 leaf, const double x[3]) {
         const double kv = coneKernel(cone, x, sampleness != 0, conf);
         if (kv == 0.0) return;
         if (mode == 1) {
@@ -223,106 +249,162 @@ TSR_HD void twoSumAdd(float &hi, float &lo, float x) {   // hi + lo += x (Kahan-
     lo += err;
 }
 
-// subtree sums (hi, lo) of E and S cone lengths and deepest leaf, one depth level at a time
+// Subtree sums (hi, lo) of E and S cone lengths per node, plus aux = node state
+// flags | deepest leaf depth << 8, built bottom-up one depth level at a time.
 __global__ void k_subtreeLeaves(const u32 *__restrict__ leafNode, u32 leafCount, const float *__restrict__ clen,
-                                const uint8_t *__restrict__ depth, float4 *__restrict__ sums,
-                                uint8_t *__restrict__ maxLeafDepth) {
+                                const uint8_t *__restrict__ depth, const u32 *__restrict__ states,
+                                float4 *__restrict__ sums, u32 *__restrict__ aux) {
     const u32 l = blockIdx.x * BLOCK + threadIdx.x;
     if (l >= leafCount) return;
     const u32 nd = leafNode[l];
     sums[nd] = make_float4(clen[l], 0.f, clen[leafCount + l], 0.f);
-    maxLeafDepth[nd] = depth[nd];
+    aux[nd] = states[nd] | ((u32)depth[nd] << 8);
 }
 
 __global__ void k_subtreeLevel(const u32 *__restrict__ child, const uint8_t *__restrict__ depth, u32 nodeCount,
-                               u32 level, float4 *__restrict__ sums, uint8_t *__restrict__ maxLeafDepth) {
+                               u32 level, const u32 *__restrict__ states, float4 *__restrict__ sums,
+                               u32 *__restrict__ aux) {
     const u32 nd = blockIdx.x * BLOCK + threadIdx.x;
     if (nd >= nodeCount || depth[nd] != level) return;
     const u32 c = child[nd];
     if (c & LEAF_BIT) return;
     float eh = 0.f, el = 0.f, sh = 0.f, sl = 0.f;
-    uint8_t md = 0;
+    u32 md = 0;
     for (u32 s = 0; s < 8; ++s) {
         const float4 v = sums[c + s];
         twoSumAdd(eh, el, v.x);
         el += v.y;
         twoSumAdd(sh, sl, v.z);
         sl += v.w;
-        md = max(md, maxLeafDepth[c + s]);
+        md = max(md, aux[c + s] >> 8);
     }
     sums[nd] = make_float4(eh, el, sh, sl);
-    maxLeafDepth[nd] = md;
+    aux[nd] = states[nd] | (md << 8);
 }
 
-__global__ void k_priors(TraversalTree t, const u32 *__restrict__ leafNode, const u64 *__restrict__ leafLoc,
-                         u32 leafCount, const uint8_t *__restrict__ depthArr, const float *__restrict__ clen,
-                         const float4 *__restrict__ sums, const uint8_t *__restrict__ maxLeafDepth,
-                         float relRadius, float *__restrict__ prior) {
+// leaves whose prior the occupancy ratio reads: E + S kernel sums >= confidence
+// threshold (Occupancy.cpp:575-576); all leaves when `all` is set
+__global__ void k_priorQueries(const float *__restrict__ ksum, u32 leafCount, double threshold, int all,
+                               float *__restrict__ prior, u32 *__restrict__ queries, u32 *counter) {
     const u32 l = blockIdx.x * BLOCK + threadIdx.x;
     if (l >= leafCount) return;
-    const u64 loc = leafLoc[l];
-    u32 kq[3];
-    locCoords(loc, kq[0], kq[1], kq[2]);
-    const u32 dq = locDepth(loc);
-    double c[3];
-    for (int a = 0; a < 3; ++a) c[a] = t.f.centre(a, kq[a], dq);
-    const double radius = t.f.h(dq) * (double)relRadius;
-    const double r2 = radius * radius, rInner2 = r2 * (1.0 - 1e-9);
-    const u32 maxDepth = dq + MAX_DEPTH_DIFFERENCE;
-    float eh = 0.f, el = 0.f, sh = 0.f, sl = 0.f;
-    u32 node = 0, d = 0, k[3] = {0, 0, 0};
-    bool test = true;
-    for (;;) {
-        if (test && d <= maxDepth && (t.states[node] & FLAG_S)) {
-            // SphereNodesChecker.cpp:29-40 (AABB/sphere, squared distance <= r^2)
-            double dmin = 0.0, dmax = 0.0;
-            for (int a = 0; a < 3; ++a) {
-                const double lo = t.f.B(a, k[a], d), hi = t.f.B(a, (u64)k[a] + 1, d);
-                if (c[a] < lo) dmin += (lo - c[a]) * (lo - c[a]);
-                else if (c[a] > hi) dmin += (c[a] - hi) * (c[a] - hi);
-                const double far = fmax(fabs(c[a] - lo), fabs(hi - c[a]));
-                dmax += far * far;
-            }
-            if (dmin <= r2) {
-                const u32 ch = t.child[node];
-                if (ch & LEAF_BIT) {
-                    const u32 j = ch & ~LEAF_BIT;
-                    twoSumAdd(eh, el, clen[j]);
-                    twoSumAdd(sh, sl, clen[leafCount + j]);
-                } else if (dmax <= rInner2 && maxLeafDepth[node] <= maxDepth) {
-                    const float4 v = sums[node];
-                    twoSumAdd(eh, el, v.x);
-                    el += v.y;
-                    twoSumAdd(sh, sl, v.z);
-                    sl += v.w;
+    const bool used = all || ((double)ksum[l] + (double)ksum[leafCount + l] >= threshold);
+    if (used) queries[atomicAdd(counter, 1u)] = l;
+    else prior[l] = __int_as_float(0x7fc00000);   // not needed by the field: NaN
+}
+
+// One wavefront per query leaf (SphereNodeStatesChecker + LeavesIterator,
+// Occupancy.cpp:541-566).  The wave keeps an LDS stack of inner nodes still to
+// open; each step pops up to 8 of them and tests their 8 children on 64 lanes
+// (sibling records are contiguous, so the loads coalesce).  Children entirely
+// inside the sphere whose deepest leaf respects the depth limit contribute their
+// subtree sums; straddling inner children are pushed.
+constexpr u32 PRIOR_STACK = 1344;   // >= 64 * (MAX_DEPTH + 1)
+
+__global__ __launch_bounds__(64) void k_priorsWave(Frame f, const u32 *__restrict__ child,
+                                                   const u64 *__restrict__ nodeLoc, const u32 *__restrict__ aux,
+                                                   const float4 *__restrict__ sums,
+                                                   const u64 *__restrict__ leafLoc, const u32 *__restrict__ queries,
+                                                   u32 queryCount, float relRadius, float *__restrict__ prior,
+                                                   u32 *err, unsigned long long *__restrict__ stats) {
+    __shared__ u32 stack[PRIOR_STACK];
+    __shared__ u32 sp;
+    const u32 lane = threadIdx.x;
+    u32 nPops = 0, nTests = 0, nAccum = 0;   // traffic counters (stats[0..2])
+    for (u32 q = blockIdx.x; q < queryCount; q += gridDim.x) {
+        const u32 l = queries[q];
+        const u64 qloc = leafLoc[l];
+        u32 kq[3];
+        locCoords(qloc, kq[0], kq[1], kq[2]);
+        const u32 dq = locDepth(qloc);
+        double c[3];
+        for (int a = 0; a < 3; ++a) c[a] = f.centre(a, kq[a], dq);
+        const double radius = f.h(dq) * (double)relRadius;
+        const double r2 = radius * radius, rInner2 = r2 * (1.0 - 1e-9);
+        const u32 maxDepth = dq + MAX_DEPTH_DIFFERENCE;
+        float eh = 0.f, el = 0.f, sh = 0.f, sl = 0.f;
+        if (lane == 0) {
+            sp = 0;
+            const u32 ax = aux[0];
+            if (ax & FLAG_S) {   // the root always overlaps the sphere (its centre is inside)
+                if (child[0] & LEAF_BIT) {
+                    const float4 v = sums[0];
+                    eh = v.x; sh = v.z;
                 } else {
-                    node = ch;
-                    ++d;
-                    k[0] *= 2; k[1] *= 2; k[2] *= 2;
-                    continue;
+                    stack[0] = 0;
+                    sp = 1;
                 }
             }
         }
-        test = true;
+        __syncthreads();
         for (;;) {
-            if (node == 0) goto done;
-            const u32 slot = (node - 1) & 7;
-            if (slot < 7) {
-                ++node;
-                const u32 s1 = slot + 1;
-                k[0] = (k[0] & ~1u) | (s1 & 1);
-                k[1] = (k[1] & ~1u) | ((s1 >> 1) & 1);
-                k[2] = (k[2] & ~1u) | ((s1 >> 2) & 1);
-                break;
+            const u32 top = sp;
+            if (top == 0) break;
+            const u32 take = min(top, 8u);
+            const u32 base = top - take;
+            const u32 a = lane >> 3, slot = lane & 7;
+            bool push = false;
+            u32 cid = 0;
+            nPops += lane == 0 ? take : 0;
+            if (a < take) {
+                ++nTests;
+                const u32 parent = stack[base + a];
+                const u64 ploc = nodeLoc[parent];
+                cid = child[parent] + slot;
+                const u32 d = locDepth(ploc) + 1;
+                const u32 ax = aux[cid];
+                if (d <= maxDepth && (ax & FLAG_S)) {
+                    u32 k[3];
+                    locCoords((ploc << 3) | slot, k[0], k[1], k[2]);
+                    double dmin = 0.0, dmax = 0.0;
+                    for (int q3 = 0; q3 < 3; ++q3) {   // SphereNodesChecker.cpp:29-40
+                        const double lo = f.B(q3, k[q3], d), hi = f.B(q3, (u64)k[q3] + 1, d);
+                        if (c[q3] < lo) dmin += (lo - c[q3]) * (lo - c[q3]);
+                        else if (c[q3] > hi) dmin += (c[q3] - hi) * (c[q3] - hi);
+                        const double far = fmax(fabs(c[q3] - lo), fabs(hi - c[q3]));
+                        dmax += far * far;
+                    }
+                    if (dmin <= r2) {
+                        const bool leaf = (child[cid] & LEAF_BIT) != 0;
+                        if (leaf || (dmax <= rInner2 && (ax >> 8) <= maxDepth)) {
+                            ++nAccum;
+                            const float4 v = sums[cid];
+                            twoSumAdd(eh, el, v.x);
+                            el += v.y;
+                            twoSumAdd(sh, sl, v.z);
+                            sl += v.w;
+                        } else {
+                            push = true;
+                        }
+                    }
+                }
             }
-            node = t.parents[node];
-            --d;
-            k[0] >>= 1; k[1] >>= 1; k[2] >>= 1;
+            __syncthreads();   // every lane has read its parent before the slots are reused
+            const u64 mask = __ballot(push);
+            const u32 pos = (u32)__popcll(mask & ((1ull << lane) - 1ull));
+            const u32 newTop = base + (u32)__popcll(mask);
+            if (newTop > PRIOR_STACK) {
+                if (lane == 0) { atomicOr(err, 1u); sp = 0; }
+            } else {
+                if (push) stack[base + pos] = cid;
+                if (lane == 0) sp = newTop;
+            }
+            __syncthreads();
         }
+        // fixed-order wave reduction in FP64 (deterministic)
+        double e = (double)eh + (double)el, s = (double)sh + (double)sl;
+        for (int off = 32; off > 0; off >>= 1) {
+            e += __shfl_xor(e, off);
+            s += __shfl_xor(s, off);
+        }
+        if (lane == 0) prior[l] = (float)(e / (e + s));
+        __syncthreads();
     }
-done:
-    const double E = (double)eh + (double)el, S = (double)sh + (double)sl;
-    prior[l] = (float)(E / (E + S));
+    if (stats) {
+        waveAccumulate(&stats[0], nPops);
+        waveAccumulate(&stats[1], nTests);
+        waveAccumulate(&stats[2], nAccum);
+    }
 }
 
 // Occupancy.cpp:570-587, FP64
@@ -372,14 +454,19 @@ void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *s
 
     TraversalTree tt{tree->child.p, tree->parents.p, field->states.p, tree->frame,
                      (float)(tree->frame.rootSize * 1e-5)};
-    LinkInputs in{scene->px.p, scene->py.p, scene->pz.p, scene->nx.p, scene->ny.p, scene->nz.p, scene->scale.p,
-                  scene->conf.p, scene->cams.p, scene->cx.p, scene->cy.p, scene->cz.p, tree->sampleDepth.p,
-                  scene->maxCams, scene->camCount, prm.bandwidth_factor, prm.relative_sample_cone_length, nullptr};
+    LinkInputs in{tree->px.p, tree->py.p, tree->pz.p, tree->nx.p, tree->ny.p, tree->nz.p, tree->scale.p,
+                  tree->conf.p, tree->cams.p, scene->cx.p, scene->cy.p, scene->cz.p, tree->sampleDepth.p,
+                  tree->maxCams, scene->camCount, prm.bandwidth_factor, prm.relative_sample_cone_length, nullptr};
     in.cullNode = cullMask;
     const bool atomicMode = (flags & 0xff) == TSR_ACCUM_FP32_ATOMIC;
 
     DBuf<u32> counts(links), offsets(links), hitCount(L), segStart(L);
     DBuf<double> confs(links);
+    DBuf<unsigned long long> kst(4);
+    auto readStats = [&](int n, unsigned long long *out) {
+        HIPCHECK(hipMemcpyAsync(out, kst.p, 8 * (size_t)n, hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+    };
     for (int pass = 0; pass < 2; ++pass) {
         const int sampleness = pass == 0 ? 1 : 0;
         const int type = sampleness;   // storage index: 0 = EMPTINESS, 1 = SAMPLENESS
@@ -396,26 +483,49 @@ void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *s
             HIPCHECK(hipMemsetAsync(hitCount.p, 0, (size_t)L * 4, st));
             k_conePass<<<gridFor(links, BLOCK), BLOCK, 0, st>>>(tt, in, links, sampleness, 0, counts.p, nullptr,
                                                                 nullptr, nullptr, nullptr, nullptr, nullptr,
-                                                                sampleness ? nullptr : confs.p);
+                                                                sampleness ? nullptr : confs.p, nullptr);
             k_conePass<<<gridFor(links, BLOCK), BLOCK, 0, st>>>(tt, in, links, sampleness, 2, counts.p, nullptr,
-                                                                nullptr, nullptr, ksum, clen, hitCount.p, nullptr);
+                                                                nullptr, nullptr, ksum, clen, hitCount.p, nullptr,
+                                                                nullptr);
             size_t tmp = 0;
             HIPCHECK(hipcub::DeviceReduce::Sum(nullptr, tmp, counts.p, offsets.p, (int)links, st));
             HIPCHECK(hipcub::DeviceReduce::Sum(sc.get(tmp), tmp, counts.p, offsets.p, (int)links, st));
             field->hits[type] = readScalar(offsets.p, st);
         } else {
-            k_conePass<<<gridFor(links, BLOCK), BLOCK, 0, st>>>(tt, in, links, sampleness, 0, counts.p, nullptr,
-                                                                nullptr, nullptr, nullptr, nullptr, nullptr,
-                                                                sampleness ? nullptr : confs.p);
+            // algorithmic bytes per link: sample SoA 37 B (position, normal, scale, confidence, camera
+            // link, depth) + 4 B count; per visited node 4 B child link (+4 B state in the E pass)
+            const u64 nodeBytes = sampleness ? 4 : 8;
+            HIPCHECK(hipMemsetAsync(kst.p, 0, 32, st));
+            timedLaunch(ctx, sampleness ? KSTAT_S_COUNT : KSTAT_E_COUNT, [&] {
+                k_conePass<<<gridFor(links, BLOCK), BLOCK, 0, st>>>(tt, in, links, sampleness, 0, counts.p, nullptr,
+                                                                    nullptr, nullptr, nullptr, nullptr, nullptr,
+                                                                    sampleness ? nullptr : confs.p, kst.p);
+            });
+            unsigned long long cs[2];
+            readStats(2, cs);
+            {
+                tsr_kernel_stat &k = ctx->kstat[sampleness ? KSTAT_S_COUNT : KSTAT_E_COUNT];
+                k.bytes += (u64)links * 41 + cs[0] * nodeBytes;
+                k.units += links;
+            }
             exclusiveScan(sc, counts.p, offsets.p, links, st);
             const u64 total = (u64)readScalar(offsets.p + links - 1, st) + readScalar(counts.p + links - 1, st);
             if (total >= (u64)INVALID) throw tsr_error(TSR_ERR_OUT_OF_MEMORY, "too many kernel contributions in one pass");
             field->hits[type] = total;
             DBuf<u32> recLeaf(std::max<u64>(total, 1)), recLeafS(std::max<u64>(total, 1));
             DBuf<uint2> recVal(std::max<u64>(total, 1)), recValS(std::max<u64>(total, 1));
-            k_conePass<<<gridFor(links, BLOCK), BLOCK, 0, st>>>(tt, in, links, sampleness, 1, counts.p, offsets.p,
-                                                                recLeaf.p, recVal.p, nullptr, nullptr, nullptr,
-                                                                nullptr);
+            HIPCHECK(hipMemsetAsync(kst.p, 0, 32, st));
+            timedLaunch(ctx, sampleness ? KSTAT_S_WRITE : KSTAT_E_WRITE, [&] {
+                k_conePass<<<gridFor(links, BLOCK), BLOCK, 0, st>>>(tt, in, links, sampleness, 1, counts.p, offsets.p,
+                                                                    recLeaf.p, recVal.p, nullptr, nullptr, nullptr,
+                                                                    nullptr, kst.p);
+            });
+            readStats(2, cs);
+            {   // + 4 B offset per link, 12 B record per accepted hit
+                tsr_kernel_stat &k = ctx->kstat[sampleness ? KSTAT_S_WRITE : KSTAT_E_WRITE];
+                k.bytes += (u64)links * 41 + cs[0] * nodeBytes + cs[1] * 12;
+                k.units += links;
+            }
             int bits = 1;
             while (bits < 32 && (1ull << bits) < L) ++bits;
             sortPairs(sc, recLeaf.p, recLeafS.p, recVal.p, recValS.p, total, 0, bits, st);
@@ -439,15 +549,36 @@ void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *s
     // priors
     {
         DBuf<float4> sums(N);
-        DBuf<uint8_t> maxLeafDepth(N);
+        DBuf<u32> aux(N), queries(L), counter(1), perr(1);
         k_subtreeLeaves<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(tree->leafNode.p, L, field->clen.p, tree->nodeDepth.p,
-                                                             sums.p, maxLeafDepth.p);
+                                                             field->states.p, sums.p, aux.p);
         for (int lvl = (int)tree->maxDepth - 1; lvl >= 0; --lvl)
-            k_subtreeLevel<<<gridFor(N, BLOCK), BLOCK, 0, st>>>(tree->child.p, tree->nodeDepth.p, N, (u32)lvl, sums.p,
-                                                                maxLeafDepth.p);
-        k_priors<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(tt, tree->leafNode.p, tree->leafLoc.p, L, tree->nodeDepth.p,
-                                                      field->clen.p, sums.p, maxLeafDepth.p,
-                                                      prm.relative_radius_for_priors, field->prior.p);
+            k_subtreeLevel<<<gridFor(N, BLOCK), BLOCK, 0, st>>>(tree->child.p, tree->nodeDepth.p, N, (u32)lvl,
+                                                                field->states.p, sums.p, aux.p);
+        HIPCHECK(hipMemsetAsync(counter.p, 0, 4, st));
+        HIPCHECK(hipMemsetAsync(perr.p, 0, 4, st));
+        k_priorQueries<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(field->ksum.p, L, (double)prm.confidence_threshold,
+                                                            (flags & TSR_PRIORS_ALL_LEAVES) ? 1 : 0, field->prior.p,
+                                                            queries.p, counter.p);
+        const u32 nq = readScalar(counter.p, st);
+        field->priorQueries = nq;
+        if (nq) {
+            const u32 grid = std::min<u32>(nq, 256u * 32u);
+            HIPCHECK(hipMemsetAsync(kst.p, 0, 32, st));
+            timedLaunch(ctx, KSTAT_PRIORS, [&] {
+                k_priorsWave<<<grid, 64, 0, st>>>(tree->frame, tree->child.p, tree->nodeLoc.p, aux.p, sums.p,
+                                                  tree->leafLoc.p, queries.p, nq, prm.relative_radius_for_priors,
+                                                  field->prior.p, perr.p, kst.p);
+            });
+            unsigned long long ps[3];
+            readStats(3, ps);
+            // per query: index 4 B + leaf code 8 B + prior 4 B; per opened node: code 8 B + child link 4 B;
+            // per tested child: aux 4 B + child link 4 B; per summed node or leaf: 16 B (E, S hi/lo)
+            tsr_kernel_stat &k = ctx->kstat[KSTAT_PRIORS];
+            k.bytes += (u64)nq * 16 + ps[0] * 12 + ps[1] * 8 + ps[2] * 16;
+            k.units += nq;
+        }
+        if (readScalar(perr.p, st)) throw tsr_error(TSR_ERR_HIP, "priors traversal stack overflow");
     }
     recordPhase(ctx, 3, false);
     recordPhase(ctx, 4, true);
@@ -455,6 +586,18 @@ void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *s
                                                      (double)prm.confidence_threshold, field->occ.p);
     recordPhase(ctx, 4, false);
     HIPCHECK(hipStreamSynchronize(st));
+}
+
+__global__ void k_calibrationCopy(const float4 *__restrict__ src, float4 *__restrict__ dst, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (size_t)gridDim.x * BLOCK) dst[i] = src[i];
+}
+
+void calibrationCopy(tsr_context *ctx, u64 bytes) {
+    const size_t n = (size_t)((bytes + 15) / 16);
+    DBuf<float4> a(n), b(n);
+    HIPCHECK(hipMemsetAsync(a.p, 0, n * 16, ctx->stream));
+    k_calibrationCopy<<<std::min<size_t>(gridFor(n, BLOCK), 256 * 64), BLOCK, 0, ctx->stream>>>(a.p, b.p, n);
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
 }
 
 }  // namespace tsr

@@ -16,9 +16,12 @@
 //   dual cells        DualCells.cpp:49-357                    one cell per interior leaf corner, ordered
 //                                                             by the Schaefer-Warren recursion path key
 #include <algorithm>
+#include <chrono>
+#include <functional>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <queue>
-#include <unordered_set>
 #include <vector>
 
 #include "handles.h"
@@ -29,6 +32,7 @@ namespace {
 
 constexpr u32 BLOCK = 256;
 constexpr u32 VISIT_CAP = 48;   // visited-leaf records per sampling unit before it counts as an event
+constexpr double HOT_RADIUS = 3.0;   // transferred part of the visit index, in max sampling distances
 
 // error bits written by kernels
 enum : u32 { ERR_OUTSIDE = 1, ERR_TOO_DEEP = 2, ERR_HASH_FULL = 4, ERR_NOT_INSERTED = 8 };
@@ -476,34 +480,74 @@ __global__ void k_cells(HashView inner, HashView nodes, const u32 *__restrict__ 
 // visited on T0 gets subdivided by an earlier unit; creating units and such
 // disturbed units are re-run here in order, on the evolving tree.
 // ---------------------------------------------------------------------------
-struct HostTree {
-    std::unordered_set<u64> inner;      // current inner nodes
-    const std::unordered_set<u64> *t0;  // inner nodes of T0
-    bool isInner(u64 loc) const { return inner.count(loc) != 0; }
+// open-addressing set of location codes (host), key 0 = empty
+struct FlatSet {
+    std::vector<u64> keys;
+    u64 mask = 0;
+    void init(size_t expected) {
+        size_t cap = 1024;
+        while (cap < 2 * expected + 16) cap <<= 1;
+        keys.assign(cap, 0ull);
+        mask = cap - 1;
+    }
+    bool contains(u64 k) const {
+        for (u64 s = hashMix(k) & mask;; s = (s + 1) & mask) {
+            const u64 v = keys[s];
+            if (v == k) return true;
+            if (v == 0ull) return false;
+        }
+    }
+    bool insert(u64 k) {
+        for (u64 s = hashMix(k) & mask;; s = (s + 1) & mask) {
+            const u64 v = keys[s];
+            if (v == k) return false;
+            if (v == 0ull) { keys[s] = k; return true; }
+        }
+    }
+};
+
+// host view of the (T0 leaf -> units that visited it) index; leaves outside the
+// transferred "hot" part are fetched from the device on demand
+struct VisitIndex {
+    std::vector<u64> leaf;   // sorted, hot leaves only
+    std::vector<u32> unit;   // ascending per leaf
+    FlatSet hot;             // every hot leaf (also those no unit visited)
+    std::function<void(u64, std::vector<u32> &)> fetch;   // exact lookup on the device
+    u64 misses = 0;
+    void units(u64 loc, std::vector<u32> &out) {
+        out.clear();
+        auto r = std::equal_range(leaf.begin(), leaf.end(), loc);
+        if (r.first != r.second) {
+            out.assign(unit.begin() + (r.first - leaf.begin()), unit.begin() + (r.second - leaf.begin()));
+            return;
+        }
+        if (hot.contains(loc)) return;
+        ++misses;
+        fetch(loc, out);
+    }
 };
 
 void replaySampling(const Frame &f, const tsr_scene *scene, float maxRel, const std::vector<u64> &t0Inner,
-                    const std::vector<uint8_t> &isEvent, const std::vector<u64> &pairLeaf,
-                    const std::vector<u32> &pairUnit, std::vector<u64> &created, u64 &eventCount) {
-    HostTree tree;
-    std::unordered_set<u64> t0(t0Inner.begin(), t0Inner.end());
-    tree.inner = t0;
-    tree.t0 = &t0;
+                    const std::vector<uint8_t> &isEvent, VisitIndex &index, std::vector<u64> &created,
+                    u64 &eventCount) {
+    FlatSet t0, inner;
+    t0.init(t0Inner.size());
+    inner.init(t0Inner.size() * 2 + 1024);
+    for (u64 k : t0Inner) { t0.insert(k); inner.insert(k); }
     const u32 units = (u32)isEvent.size();
     std::vector<uint8_t> pending(isEvent.begin(), isEvent.end());
     std::priority_queue<u32, std::vector<u32>, std::greater<u32>> heap;
     for (u32 u = 0; u < units; ++u)
         if (pending[u]) heap.push(u);
-    auto leafIsInner = [&](u64 loc) { return tree.isInner(loc); };
+    auto isInner = [&](u64 loc) { return inner.contains(loc); };
     auto noLeaf = [](u64) {};
+    std::vector<u32> tmp;
     auto taint = [&](u64 subdivided, u32 writer) {
         // the T0 leaf containing the subdivided node
         u64 l = subdivided;
-        while (l > 1 && t0.count(l >> 3) == 0) l >>= 3;
-        auto range = std::equal_range(pairLeaf.begin(), pairLeaf.end(), l);
-        const size_t b = range.first - pairLeaf.begin(), e = range.second - pairLeaf.begin();
-        auto it = std::upper_bound(pairUnit.begin() + b, pairUnit.begin() + e, writer);
-        for (; it != pairUnit.begin() + e; ++it)
+        while (l > 1 && !t0.contains(l >> 3)) l >>= 3;
+        index.units(l, tmp);
+        for (auto it = std::upper_bound(tmp.begin(), tmp.end(), writer); it != tmp.end(); ++it)
             if (!pending[*it]) {
                 pending[*it] = 1;
                 heap.push(*it);
@@ -520,18 +564,24 @@ void replaySampling(const Frame &f, const tsr_scene *scene, float maxRel, const 
         u.p[0] = scene->hpx[i]; u.p[1] = scene->hpy[i]; u.p[2] = scene->hpz[i];
         u.n[0] = scene->hnx[i]; u.n[1] = scene->hny[i]; u.n[2] = scene->hnz[i];
         u.msd = (double)(scene->hscale[i] * maxRel);
-        if (findReliable(f, 1ull, positive, u, leafIsInner, noLeaf)) continue;
+        if (findReliable(f, 1ull, positive, u, isInner, noLeaf)) continue;
         // Nodes.cpp:284-336
         const double sgn = positive ? u.msd : -u.msd;
         const double en[3] = {u.p[0] + u.n[0] * sgn, u.p[1] + u.n[1] * sgn, u.p[2] + u.n[2] * sgn};
         u64 n = 1ull;
         for (;;) {
-            if (!tree.isInner(n)) {
+            if (!inner.contains(n)) {
                 if (locDepth(n) + 1 > MAX_DEPTH) throw tsr_error(TSR_ERR_TREE_TOO_DEEP, "octree deeper than 19 levels");
-                tree.inner.insert(n);
+                if (inner.keys.size() < 2 * (created.size() + t0Inner.size() + 16)) {   // keep the load factor <= 1/2
+                    FlatSet grown;
+                    grown.init(2 * (created.size() + t0Inner.size()) + 1024);
+                    for (u64 k : inner.keys) if (k) grown.insert(k);
+                    inner = std::move(grown);
+                }
+                inner.insert(n);
                 created.push_back(n);
                 taint(n, unit);
-                if (findReliable(f, n, positive, u, leafIsInner, noLeaf)) break;
+                if (findReliable(f, n, positive, u, isInner, noLeaf)) break;
             }
             const u32 d = locDepth(n) + 1;
             if (f.h(d) == 0.0) throw tsr_error(TSR_ERR_ZERO_SIZE_NODE, "Reached a tree node of size 0");
@@ -557,6 +607,70 @@ void replaySampling(const Frame &f, const tsr_scene *scene, float maxRel, const 
     }
 }
 
+// T0 leaves within HOT_RADIUS x max sampling distance of a unit the speculation could not
+// settle: the writes of the in-order replay (and of short cascades) land there.
+__global__ void k_markHot(Frame f, HashView inner, HashView hot, const float *__restrict__ px,
+                          const float *__restrict__ py, const float *__restrict__ pz, const float *__restrict__ scale,
+                          float maxRel, const uint8_t *__restrict__ isEvent, u32 units, u32 *err) {
+    const u32 unit = blockIdx.x * BLOCK + threadIdx.x;
+    if (unit >= units || !isEvent[unit] || ((unit & 1) && isEvent[unit - 1])) return;
+    const u32 i = unit >> 1;
+    const double p[3] = {px[i], py[i], pz[i]};
+    const double r = HOT_RADIUS * (double)(scale[i] * maxRel);
+    auto overlaps = [&](u64 loc) {
+        u32 k[3];
+        locCoords(loc, k[0], k[1], k[2]);
+        const u32 d = locDepth(loc);
+        double d2 = 0.0;
+        for (int a = 0; a < 3; ++a) {
+            const double lo = f.B(a, k[a], d), hi = f.B(a, (u64)k[a] + 1, d);
+            if (p[a] < lo) d2 += (lo - p[a]) * (lo - p[a]);
+            else if (p[a] > hi) d2 += (p[a] - hi) * (p[a] - hi);
+        }
+        return d2 <= r * r;
+    };
+    if (!inner.contains(1ull)) {
+        hot.insert(1ull, 0, err);
+        return;
+    }
+    u64 loc = 1ull;
+    u32 next = 0;
+    for (;;) {
+        if (next == 8) {
+            if (loc == 1ull) return;
+            next = (u32)(loc & 7) + 1;
+            loc >>= 3;
+            continue;
+        }
+        const u64 ch = (loc << 3) | next;
+        if (overlaps(ch)) {
+            if (inner.contains(ch)) {
+                loc = ch;
+                next = 0;
+                continue;
+            }
+            hot.insert(ch, 0, err);
+        }
+        ++next;
+    }
+}
+
+__global__ void k_hotFlags(HashView hot, const u64 *__restrict__ leaf, size_t n, uint8_t *__restrict__ flag) {
+    const size_t j = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (j < n) flag[j] = hot.contains(leaf[j]) ? 1 : 0;
+}
+
+// [first, last) of `key` in the sorted array (one thread)
+__global__ void k_equalRange(const u64 *__restrict__ keys, size_t n, u64 key, unsigned long long *out) {
+    size_t lo = 0, hi = n;
+    while (lo < hi) { const size_t m = (lo + hi) / 2; if (keys[m] < key) lo = m + 1; else hi = m; }
+    const size_t first = lo;
+    hi = n;
+    while (lo < hi) { const size_t m = (lo + hi) / 2; if (keys[m] <= key) lo = m + 1; else hi = m; }
+    out[0] = first;
+    out[1] = lo;
+}
+
 void checkErr(u32 e) {
     if (e & ERR_OUTSIDE) throw tsr_error(TSR_ERR_SAMPLE_OUTSIDE_ROOT, "Cannot find a container node for a position which is not contained by the start node for the search.");
     if (e & ERR_TOO_DEEP) throw tsr_error(TSR_ERR_TREE_TOO_DEEP, "octree deeper than 19 levels");
@@ -567,8 +681,18 @@ void checkErr(u32 e) {
 
 }  // namespace
 
-void buildTree(tsr_context *ctx, tsr_scene *scene, const tsr_params &prm, tsr_tree *tree) {
+void buildTree(tsr_context *ctx, const tsr_scene *scene, const tsr_params &prm, tsr_tree *tree) {
     hipStream_t st = ctx->stream;
+    // TSR_TRACE=1: per-stage wall times on stderr (synchronises after each stage)
+    static const bool trace = std::getenv("TSR_TRACE") != nullptr;
+    auto t0 = std::chrono::steady_clock::now();
+    auto mark = [&](const char *stage) {
+        if (!trace) return;
+        HIPCHECK(hipStreamSynchronize(st));
+        const auto t1 = std::chrono::steady_clock::now();
+        std::fprintf(stderr, "[tsr tree] %-22s %8.2f ms\n", stage, std::chrono::duration<double, std::milli>(t1 - t0).count());
+        t0 = t1;
+    };
     Scratch &sc = ctx->scratch;
     const u32 n = scene->n;
     if (n == 0) throw tsr_error(TSR_ERR_NO_SAMPLES, "no samples");
@@ -607,6 +731,7 @@ void buildTree(tsr_context *ctx, tsr_scene *scene, const tsr_params &prm, tsr_tr
         }
     }
     tree->frame = f;
+    mark("root scope");
 
     // ---- containment
     DBuf<u64> containerLoc(n);
@@ -619,6 +744,7 @@ void buildTree(tsr_context *ctx, tsr_scene *scene, const tsr_params &prm, tsr_tr
     k_insertAncestors<<<gridFor(n, BLOCK), BLOCK, 0, st>>>(inner.view(), containerLoc.p, n, err.p);
     checkErr(readScalar(err.p, st));
 
+    mark("containment");
     // ---- sampling refinement: speculative evaluation on T0
     const u32 units = 2 * n;
     DBuf<u32> visitCount(units), offsets(units);
@@ -638,27 +764,80 @@ void buildTree(tsr_context *ctx, tsr_scene *scene, const tsr_params &prm, tsr_tr
                                                             pairLeaf.p, pairUnit.p);
     // pairs are emitted in unit order; a stable sort by leaf keeps units ascending per leaf
     sortPairs(sc, pairLeaf.p, pairLeafSorted.p, pairUnit.p, pairUnitSorted.p, pairs, 0, 64, st);
-    std::vector<uint8_t> hEvent(units);
-    std::vector<u64> hPairLeaf(pairs);
-    std::vector<u32> hPairUnit(pairs);
-    HIPCHECK(hipMemcpyAsync(hEvent.data(), isEvent.p, units, hipMemcpyDeviceToHost, st));
+    pairLeaf.release();
+    pairUnit.release();
+    // transfer only the part of the index around unsettled units
+    DHash hot;
+    hot.alloc(std::max<u64>(4096, (u64)units / 2), st);
+    k_markHot<<<gridFor(units, BLOCK), BLOCK, 0, st>>>(f, inner.view(), hot.view(), scene->px.p, scene->py.p,
+                                                       scene->pz.p, scene->scale.p, maxRel, isEvent.p, units, err.p);
+    DBuf<uint8_t> hotFlag(std::max<size_t>(pairs, 1));
+    DBuf<u64> hotLeaf(std::max<size_t>(pairs, 1));
+    DBuf<u32> hotUnit(std::max<size_t>(pairs, 1));
+    DBuf<u32> counter(1);
+    size_t hotPairs = 0;
     if (pairs) {
-        HIPCHECK(hipMemcpyAsync(hPairLeaf.data(), pairLeafSorted.p, pairs * 8, hipMemcpyDeviceToHost, st));
-        HIPCHECK(hipMemcpyAsync(hPairUnit.data(), pairUnitSorted.p, pairs * 4, hipMemcpyDeviceToHost, st));
+        k_hotFlags<<<gridFor(pairs, BLOCK), BLOCK, 0, st>>>(hot.view(), pairLeafSorted.p, pairs, hotFlag.p);
+        size_t tmp = 0;
+        HIPCHECK(hipcub::DeviceSelect::Flagged(nullptr, tmp, pairLeafSorted.p, hotFlag.p, hotLeaf.p, counter.p, (int)pairs, st));
+        HIPCHECK(hipcub::DeviceSelect::Flagged(sc.get(tmp), tmp, pairLeafSorted.p, hotFlag.p, hotLeaf.p, counter.p, (int)pairs, st));
+        HIPCHECK(hipcub::DeviceSelect::Flagged(nullptr, tmp, pairUnitSorted.p, hotFlag.p, hotUnit.p, counter.p, (int)pairs, st));
+        HIPCHECK(hipcub::DeviceSelect::Flagged(sc.get(tmp), tmp, pairUnitSorted.p, hotFlag.p, hotUnit.p, counter.p, (int)pairs, st));
+        hotPairs = readScalar(counter.p, st);
     }
+    checkErr(readScalar(err.p, st));
+    std::vector<uint8_t> hEvent(units);
+    VisitIndex index;
+    index.leaf.resize(hotPairs);
+    index.unit.resize(hotPairs);
+    HIPCHECK(hipMemcpyAsync(hEvent.data(), isEvent.p, units, hipMemcpyDeviceToHost, st));
+    if (hotPairs) {
+        HIPCHECK(hipMemcpyAsync(index.leaf.data(), hotLeaf.p, hotPairs * 8, hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipMemcpyAsync(index.unit.data(), hotUnit.p, hotPairs * 4, hipMemcpyDeviceToHost, st));
+    }
+    hotFlag.release(); hotLeaf.release(); hotUnit.release();
+    {   // the hot leaf set itself, so that hot leaves without visits need no lookup
+        DBuf<u64> hotKeys(hot.capacity);
+        HIPCHECK(hipMemsetAsync(counter.p, 0, 4, st));
+        k_compactKeys<<<gridFor(hot.capacity, BLOCK), BLOCK, 0, st>>>(hot.keys.p, hot.capacity, hotKeys.p, counter.p);
+        const u32 nh = readScalar(counter.p, st);
+        std::vector<u64> hk(nh);
+        HIPCHECK(hipMemcpyAsync(hk.data(), hotKeys.p, (size_t)nh * 8, hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        index.hot.init(nh);
+        for (u64 k : hk) index.hot.insert(k);
+    }
+    DBuf<unsigned long long> range(2);
+    index.fetch = [&](u64 loc, std::vector<u32> &out) {
+        if (!pairs) return;
+        k_equalRange<<<1, 1, 0, st>>>(pairLeafSorted.p, pairs, loc, range.p);
+        unsigned long long r[2];
+        HIPCHECK(hipMemcpyAsync(r, range.p, sizeof(r), hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        out.resize(r[1] - r[0]);
+        if (r[1] > r[0])
+            HIPCHECK(hipMemcpyAsync(out.data(), pairUnitSorted.p + r[0], (r[1] - r[0]) * 4, hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+    };
     // T0 inner node list
     DBuf<u64> innerList(inner.capacity);
-    DBuf<u32> counter(1);
     HIPCHECK(hipMemsetAsync(counter.p, 0, 4, st));
     k_compactKeys<<<gridFor(inner.capacity, BLOCK), BLOCK, 0, st>>>(inner.keys.p, inner.capacity, innerList.p, counter.p);
     const u32 t0Count = readScalar(counter.p, st);
     std::vector<u64> hInner(t0Count);
     HIPCHECK(hipMemcpyAsync(hInner.data(), innerList.p, (size_t)t0Count * 8, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
-    pairLeaf.release(); pairUnit.release(); pairLeafSorted.release(); pairUnitSorted.release();
+    mark("sampling speculation");
     std::vector<u64> created;
     u64 events = 0;
-    replaySampling(f, scene, maxRel, hInner, hEvent, hPairLeaf, hPairUnit, created, events);
+    replaySampling(f, scene, maxRel, hInner, hEvent, index, created, events);
+    if (trace)
+        std::fprintf(stderr, "[tsr tree] units %u pairs %zu hot pairs %zu t0-inner %u events %llu created nodes %zu "
+                     "index misses %llu\n", units, pairs, hotPairs, t0Count, (unsigned long long)events,
+                     8 * created.size(), (unsigned long long)index.misses);
+    pairLeafSorted.release();
+    pairUnitSorted.release();
+    mark("sampling replay (host)");
     tree->samplingEvents = events;
     tree->samplingCreated = 8 * (u64)created.size();
     tree->samplingUnits = units;
@@ -679,6 +858,7 @@ void buildTree(tsr_context *ctx, tsr_scene *scene, const tsr_params &prm, tsr_tr
         inner = std::move(grown);
     }
 
+    mark("inner set rebuild");
     // ---- balancing fixpoint
     for (int iter = 0;; ++iter) {
         HIPCHECK(hipMemsetAsync(counter.p, 0, 4, st));
@@ -689,6 +869,7 @@ void buildTree(tsr_context *ctx, tsr_scene *scene, const tsr_params &prm, tsr_tr
         if (iter > 64) throw tsr_error(TSR_ERR_HIP, "balancing did not converge");
     }
 
+    mark("balancing");
     // ---- final inner list sorted in pre-order
     innerList.alloc(inner.capacity);
     HIPCHECK(hipMemsetAsync(counter.p, 0, 4, st));
@@ -740,6 +921,7 @@ void buildTree(tsr_context *ctx, tsr_scene *scene, const tsr_params &prm, tsr_tr
         tree->maxDepth = md;
     }
 
+    mark("node/leaf tables");
     // ---- samples: node ids, counts, starts, stable order by node id
     DBuf<u32> nodeOf(n), orig(n), nodeOfSorted(n), origSorted(n);
     tree->sampleCounts.alloc(N);
@@ -755,35 +937,21 @@ void buildTree(tsr_context *ctx, tsr_scene *scene, const tsr_params &prm, tsr_tr
     sortPairs(sc, nodeOf.p, nodeOfSorted.p, orig.p, origSorted.p, n, 0, bits, st);
     tree->sampleTarget.alloc(n);
     k_scatterTarget<<<gridFor(n, BLOCK), BLOCK, 0, st>>>(origSorted.p, n, tree->sampleTarget.p);
-    // permute the scene's sample arrays (Samples::reorder)
-    auto permuteF = [&](DBuf<float> &buf) {
-        DBuf<float> out(n);
-        k_gather<float><<<gridFor(n, BLOCK), BLOCK, 0, st>>>(buf.p, origSorted.p, n, 1, out.p);
-        buf = std::move(out);
+    // the samples in node order (Samples::reorder); the scene keeps the input order
+    auto permuteF = [&](const DBuf<float> &src, DBuf<float> &dst) {
+        dst.alloc(n);
+        k_gather<float><<<gridFor(n, BLOCK), BLOCK, 0, st>>>(src.p, origSorted.p, n, 1, dst.p);
     };
-    permuteF(scene->px); permuteF(scene->py); permuteF(scene->pz);
-    permuteF(scene->nx); permuteF(scene->ny); permuteF(scene->nz);
-    permuteF(scene->scale); permuteF(scene->conf);
-    {
-        DBuf<u32> out((size_t)n * scene->maxCams);
-        k_gather<u32><<<gridFor(n, BLOCK), BLOCK, 0, st>>>(scene->cams.p, origSorted.p, n, scene->maxCams, out.p);
-        scene->cams = std::move(out);
-        tree->sampleDepth.alloc(n);
-        k_gather<uint8_t><<<gridFor(n, BLOCK), BLOCK, 0, st>>>(cdepth.p, origSorted.p, n, 1, tree->sampleDepth.p);
-    }
-    {   // keep the host copies in the same order
-        std::vector<u32> perm(n);
-        HIPCHECK(hipMemcpyAsync(perm.data(), origSorted.p, (size_t)n * 4, hipMemcpyDeviceToHost, st));
-        HIPCHECK(hipStreamSynchronize(st));
-        auto permH = [&](std::vector<float> &v) {
-            std::vector<float> o(n);
-            for (u32 j = 0; j < n; ++j) o[j] = v[perm[j]];
-            v.swap(o);
-        };
-        permH(scene->hpx); permH(scene->hpy); permH(scene->hpz);
-        permH(scene->hnx); permH(scene->hny); permH(scene->hnz); permH(scene->hscale);
-    }
+    permuteF(scene->px, tree->px); permuteF(scene->py, tree->py); permuteF(scene->pz, tree->pz);
+    permuteF(scene->nx, tree->nx); permuteF(scene->ny, tree->ny); permuteF(scene->nz, tree->nz);
+    permuteF(scene->scale, tree->scale); permuteF(scene->conf, tree->conf);
+    tree->maxCams = scene->maxCams;
+    tree->cams.alloc((size_t)n * scene->maxCams);
+    k_gather<u32><<<gridFor(n, BLOCK), BLOCK, 0, st>>>(scene->cams.p, origSorted.p, n, scene->maxCams, tree->cams.p);
+    tree->sampleDepth.alloc(n);
+    k_gather<uint8_t><<<gridFor(n, BLOCK), BLOCK, 0, st>>>(cdepth.p, origSorted.p, n, 1, tree->sampleDepth.p);
 
+    mark("sample reorder");
     // ---- dual cells
     if (L > 0) {
         const size_t corners = 8 * (size_t)L;
@@ -814,6 +982,7 @@ void buildTree(tsr_context *ctx, tsr_scene *scene, const tsr_params &prm, tsr_tr
     }
     HIPCHECK(hipStreamSynchronize(st));
     checkErr(readScalar(err.p, st));
+    mark("dual cells");
 }
 
 }  // namespace tsr
