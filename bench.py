@@ -8,7 +8,6 @@ Prints one JSON line.  `--gpus N` with N > 1 exits: this build runs on one GPU.
 import argparse
 import json
 import os
-import subprocess
 import sys
 import time
 
@@ -161,7 +160,7 @@ def main():
                            "ms_per_launch": kd["ms_per_launch"]}
     if not args.no_cpu_baseline:
         model, cores = cpu_info()
-        threads = min(cores, 64)
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(cores, 64)
         cn, cdt, cph = cpu_baseline(args.config, args.cpu_res, threads)
         out["cpu_baseline"] = {
             "value": cn / cdt, "unit": "samples/s", "cores": threads, "kind": "port",

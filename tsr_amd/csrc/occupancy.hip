@@ -16,6 +16,7 @@
 // stably radix-sorted by leaf, and each leaf's records are summed in FP32 with
 // Kahan compensation in that fixed order: deterministic, no float atomics.
 #include <algorithm>
+#include <cstdlib>
 #include <vector>
 
 #include "handles.h"
@@ -25,6 +26,23 @@ namespace tsr {
 namespace {
 
 constexpr u32 BLOCK = 256;
+constexpr u32 XCD_COUNT = 8;
+
+// logical block index that gives every XCD (blocks are dispatched round-robin
+// over the 8 XCDs) a contiguous range of the work; gridDim.x is a multiple of 8
+__device__ __forceinline__ u32 xcdBlock(int xcdMap) {
+    if (!xcdMap) return blockIdx.x;
+    return (blockIdx.x % XCD_COUNT) * (gridDim.x / XCD_COUNT) + blockIdx.x / XCD_COUNT;
+}
+// work-to-XCD mapping switches (TSR_XCD_MAP bit 0: cone passes, bit 1: priors)
+inline int xcdMapMode() {
+    static const int m = std::getenv("TSR_XCD_MAP") ? std::atoi(std::getenv("TSR_XCD_MAP")) : 2;
+    return m;
+}
+inline unsigned xcdGrid(size_t n, unsigned block) {
+    const unsigned g = gridFor(n, block);
+    return (g + XCD_COUNT - 1) / XCD_COUNT * XCD_COUNT;
+}
 
 struct TraversalTree {
     const u32 *child;      // first child id, or LEAF_BIT | leaf index
@@ -42,6 +60,7 @@ struct LinkInputs {
     u32 maxCams, camCount;
     float bandwidth, relConeLength;
     const uint8_t *cullNode;   // learned culling mask (E pass), may be null
+    int xcdMap;
 };
 
 // Cone of link `link` (Occupancy.cpp:174-201, getCameraToSampleLinkData :250-277)
@@ -138,7 +157,7 @@ __global__ void k_conePass(TraversalTree t, LinkInputs in, u32 linkCount, int sa
                            uint2 *__restrict__ recVal, float *__restrict__ atomicSum, float *__restrict__ atomicLen,
                            u32 *__restrict__ atomicStates, double *__restrict__ confOut,
                            unsigned long long *__restrict__ stats) {
-    const u32 link = blockIdx.x * BLOCK + threadIdx.x;
+    const u32 link = xcdBlock(in.xcdMap) * BLOCK + threadIdx.x;
     u32 visits = 0, count = 0;
     if (link < linkCount) coneLink(t, in, link, sampleness, mode, counts, offsets, recLeaf, recVal, atomicSum,
                                    atomicLen, atomicStates, confOut, visits, count);
@@ -284,13 +303,41 @@ __global__ void k_subtreeLevel(const u32 *__restrict__ child, const uint8_t *__r
 
 // leaves whose prior the occupancy ratio reads: E + S kernel sums >= confidence
 // threshold (Occupancy.cpp:575-576); all leaves when `all` is set
-__global__ void k_priorQueries(const float *__restrict__ ksum, u32 leafCount, double threshold, int all,
-                               float *__restrict__ prior, u32 *__restrict__ queries, u32 *counter) {
+__global__ void k_priorQueryFlags(const float *__restrict__ ksum, u32 leafCount, double threshold, int all,
+                                  float *__restrict__ prior, u32 *__restrict__ flag) {
     const u32 l = blockIdx.x * BLOCK + threadIdx.x;
     if (l >= leafCount) return;
     const bool used = all || ((double)ksum[l] + (double)ksum[leafCount + l] >= threshold);
-    if (used) queries[atomicAdd(counter, 1u)] = l;
-    else prior[l] = __int_as_float(0x7fc00000);   // not needed by the field: NaN
+    flag[l] = used ? 1u : 0u;
+    if (!used) prior[l] = __int_as_float(0x7fc00000);   // not needed by the field: NaN
+}
+
+// queries in leaf (DFS = spatial) order
+__global__ void k_priorQueryList(const u32 *__restrict__ flag, const u32 *__restrict__ pos, u32 leafCount,
+                                 u32 *__restrict__ queries) {
+    const u32 l = blockIdx.x * BLOCK + threadIdx.x;
+    if (l < leafCount && flag[l]) queries[pos[l]] = l;
+}
+
+// one 32-byte record per node for the priors traversal: child link, state | deepest
+// leaf << 8, location code, subtree sums (E hi, E lo, S hi, S lo)
+struct PriorNode {
+    u32 child, aux;
+    u64 loc;
+    float4 sums;
+};
+
+__global__ void k_packPriorNodes(const u32 *__restrict__ child, const u32 *__restrict__ aux,
+                                 const u64 *__restrict__ loc, const float4 *__restrict__ sums, u32 nodeCount,
+                                 PriorNode *__restrict__ out) {
+    const u32 nd = blockIdx.x * BLOCK + threadIdx.x;
+    if (nd >= nodeCount) return;
+    PriorNode r;
+    r.child = child[nd];
+    r.aux = aux[nd];
+    r.loc = loc[nd];
+    r.sums = sums[nd];
+    out[nd] = r;
 }
 
 // One wavefront per query leaf (SphereNodeStatesChecker + LeavesIterator,
@@ -301,17 +348,34 @@ __global__ void k_priorQueries(const float *__restrict__ ksum, u32 leafCount, do
 // subtree sums; straddling inner children are pushed.
 constexpr u32 PRIOR_STACK = 1344;   // >= 64 * (MAX_DEPTH + 1)
 
-__global__ __launch_bounds__(64) void k_priorsWave(Frame f, const u32 *__restrict__ child,
-                                                   const u64 *__restrict__ nodeLoc, const u32 *__restrict__ aux,
-                                                   const float4 *__restrict__ sums,
+// XCD-aware work mapping: blocks are dispatched round-robin over the 8 XCDs of
+// the MI355X; block b serves slice (b mod 8) of the work list, so that each XCD's
+// L2 sees a contiguous (spatially coherent) range of it.
+constexpr u32 XCDS = 8;
+__device__ __forceinline__ void xcdRange(u32 count, int xcdMap, u32 &begin, u32 &end, u32 &stride, u32 &first) {
+    if (!xcdMap) {
+        begin = 0; end = count; stride = gridDim.x; first = blockIdx.x;
+        return;
+    }
+    const u32 xcd = blockIdx.x % XCDS, perXcdBlocks = gridDim.x / XCDS;
+    const u32 slice = (count + XCDS - 1) / XCDS;
+    begin = min(count, xcd * slice);
+    end = min(count, begin + slice);
+    stride = perXcdBlocks;
+    first = begin + blockIdx.x / XCDS;
+}
+
+__global__ __launch_bounds__(64) void k_priorsWave(Frame f, const PriorNode *__restrict__ nodes,
                                                    const u64 *__restrict__ leafLoc, const u32 *__restrict__ queries,
                                                    u32 queryCount, float relRadius, float *__restrict__ prior,
-                                                   u32 *err, unsigned long long *__restrict__ stats) {
+                                                   u32 *err, unsigned long long *__restrict__ stats, int xcdMap) {
     __shared__ u32 stack[PRIOR_STACK];
     __shared__ u32 sp;
     const u32 lane = threadIdx.x;
     u32 nPops = 0, nTests = 0, nAccum = 0;   // traffic counters (stats[0..2])
-    for (u32 q = blockIdx.x; q < queryCount; q += gridDim.x) {
+    u32 qBegin, qEnd, qStride, qFirst;
+    xcdRange(queryCount, xcdMap, qBegin, qEnd, qStride, qFirst);
+    for (u32 q = qFirst; q < qEnd; q += qStride) {
         const u32 l = queries[q];
         const u64 qloc = leafLoc[l];
         u32 kq[3];
@@ -325,10 +389,10 @@ __global__ __launch_bounds__(64) void k_priorsWave(Frame f, const u32 *__restric
         float eh = 0.f, el = 0.f, sh = 0.f, sl = 0.f;
         if (lane == 0) {
             sp = 0;
-            const u32 ax = aux[0];
+            const u32 ax = nodes[0].aux;
             if (ax & FLAG_S) {   // the root always overlaps the sphere (its centre is inside)
-                if (child[0] & LEAF_BIT) {
-                    const float4 v = sums[0];
+                if (nodes[0].child & LEAF_BIT) {
+                    const float4 v = nodes[0].sums;
                     eh = v.x; sh = v.z;
                 } else {
                     stack[0] = 0;
@@ -349,10 +413,11 @@ __global__ __launch_bounds__(64) void k_priorsWave(Frame f, const u32 *__restric
             if (a < take) {
                 ++nTests;
                 const u32 parent = stack[base + a];
-                const u64 ploc = nodeLoc[parent];
-                cid = child[parent] + slot;
+                const u64 ploc = nodes[parent].loc;
+                cid = nodes[parent].child + slot;
                 const u32 d = locDepth(ploc) + 1;
-                const u32 ax = aux[cid];
+                const PriorNode cn = nodes[cid];
+                const u32 ax = cn.aux;
                 if (d <= maxDepth && (ax & FLAG_S)) {
                     u32 k[3];
                     locCoords((ploc << 3) | slot, k[0], k[1], k[2]);
@@ -365,10 +430,10 @@ __global__ __launch_bounds__(64) void k_priorsWave(Frame f, const u32 *__restric
                         dmax += far * far;
                     }
                     if (dmin <= r2) {
-                        const bool leaf = (child[cid] & LEAF_BIT) != 0;
+                        const bool leaf = (cn.child & LEAF_BIT) != 0;
                         if (leaf || (dmax <= rInner2 && (ax >> 8) <= maxDepth)) {
                             ++nAccum;
-                            const float4 v = sums[cid];
+                            const float4 v = cn.sums;
                             twoSumAdd(eh, el, v.x);
                             el += v.y;
                             twoSumAdd(sh, sl, v.z);
@@ -458,6 +523,7 @@ void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *s
                   tree->conf.p, tree->cams.p, scene->cx.p, scene->cy.p, scene->cz.p, tree->sampleDepth.p,
                   tree->maxCams, scene->camCount, prm.bandwidth_factor, prm.relative_sample_cone_length, nullptr};
     in.cullNode = cullMask;
+    in.xcdMap = xcdMapMode() & 1;
     const bool atomicMode = (flags & 0xff) == TSR_ACCUM_FP32_ATOMIC;
 
     DBuf<u32> counts(links), offsets(links), hitCount(L), segStart(L);
@@ -481,10 +547,10 @@ void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *s
             HIPCHECK(hipMemsetAsync(ksum, 0, (size_t)L * 4, st));
             HIPCHECK(hipMemsetAsync(clen, 0, (size_t)L * 4, st));
             HIPCHECK(hipMemsetAsync(hitCount.p, 0, (size_t)L * 4, st));
-            k_conePass<<<gridFor(links, BLOCK), BLOCK, 0, st>>>(tt, in, links, sampleness, 0, counts.p, nullptr,
+            k_conePass<<<xcdGrid(links, BLOCK), BLOCK, 0, st>>>(tt, in, links, sampleness, 0, counts.p, nullptr,
                                                                 nullptr, nullptr, nullptr, nullptr, nullptr,
                                                                 sampleness ? nullptr : confs.p, nullptr);
-            k_conePass<<<gridFor(links, BLOCK), BLOCK, 0, st>>>(tt, in, links, sampleness, 2, counts.p, nullptr,
+            k_conePass<<<xcdGrid(links, BLOCK), BLOCK, 0, st>>>(tt, in, links, sampleness, 2, counts.p, nullptr,
                                                                 nullptr, nullptr, ksum, clen, hitCount.p, nullptr,
                                                                 nullptr);
             size_t tmp = 0;
@@ -497,7 +563,7 @@ void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *s
             const u64 nodeBytes = sampleness ? 4 : 8;
             HIPCHECK(hipMemsetAsync(kst.p, 0, 32, st));
             timedLaunch(ctx, sampleness ? KSTAT_S_COUNT : KSTAT_E_COUNT, [&] {
-                k_conePass<<<gridFor(links, BLOCK), BLOCK, 0, st>>>(tt, in, links, sampleness, 0, counts.p, nullptr,
+                k_conePass<<<xcdGrid(links, BLOCK), BLOCK, 0, st>>>(tt, in, links, sampleness, 0, counts.p, nullptr,
                                                                     nullptr, nullptr, nullptr, nullptr, nullptr,
                                                                     sampleness ? nullptr : confs.p, kst.p);
             });
@@ -516,7 +582,7 @@ void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *s
             DBuf<uint2> recVal(std::max<u64>(total, 1)), recValS(std::max<u64>(total, 1));
             HIPCHECK(hipMemsetAsync(kst.p, 0, 32, st));
             timedLaunch(ctx, sampleness ? KSTAT_S_WRITE : KSTAT_E_WRITE, [&] {
-                k_conePass<<<gridFor(links, BLOCK), BLOCK, 0, st>>>(tt, in, links, sampleness, 1, counts.p, offsets.p,
+                k_conePass<<<xcdGrid(links, BLOCK), BLOCK, 0, st>>>(tt, in, links, sampleness, 1, counts.p, offsets.p,
                                                                     recLeaf.p, recVal.p, nullptr, nullptr, nullptr,
                                                                     nullptr, kst.p);
             });
@@ -555,25 +621,30 @@ void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *s
         for (int lvl = (int)tree->maxDepth - 1; lvl >= 0; --lvl)
             k_subtreeLevel<<<gridFor(N, BLOCK), BLOCK, 0, st>>>(tree->child.p, tree->nodeDepth.p, N, (u32)lvl,
                                                                 field->states.p, sums.p, aux.p);
-        HIPCHECK(hipMemsetAsync(counter.p, 0, 4, st));
         HIPCHECK(hipMemsetAsync(perr.p, 0, 4, st));
-        k_priorQueries<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(field->ksum.p, L, (double)prm.confidence_threshold,
-                                                            (flags & TSR_PRIORS_ALL_LEAVES) ? 1 : 0, field->prior.p,
-                                                            queries.p, counter.p);
-        const u32 nq = readScalar(counter.p, st);
+        DBuf<u32> qflag(L), qpos(L);
+        k_priorQueryFlags<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(field->ksum.p, L, (double)prm.confidence_threshold,
+                                                               (flags & TSR_PRIORS_ALL_LEAVES) ? 1 : 0,
+                                                               field->prior.p, qflag.p);
+        exclusiveScan(sc, qflag.p, qpos.p, L, st);
+        k_priorQueryList<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(qflag.p, qpos.p, L, queries.p);
+        const u32 nq = readScalar(qpos.p + L - 1, st) + readScalar(qflag.p + L - 1, st);
+        DBuf<PriorNode> pnodes(N);
+        k_packPriorNodes<<<gridFor(N, BLOCK), BLOCK, 0, st>>>(tree->child.p, aux.p, tree->nodeLoc.p, sums.p, N,
+                                                              pnodes.p);
         field->priorQueries = nq;
         if (nq) {
-            const u32 grid = std::min<u32>(nq, 256u * 32u);
+            const u32 grid = 256u * 32u;   // multiple of the XCD count
             HIPCHECK(hipMemsetAsync(kst.p, 0, 32, st));
             timedLaunch(ctx, KSTAT_PRIORS, [&] {
-                k_priorsWave<<<grid, 64, 0, st>>>(tree->frame, tree->child.p, tree->nodeLoc.p, aux.p, sums.p,
-                                                  tree->leafLoc.p, queries.p, nq, prm.relative_radius_for_priors,
-                                                  field->prior.p, perr.p, kst.p);
+                k_priorsWave<<<grid, 64, 0, st>>>(tree->frame, pnodes.p, tree->leafLoc.p, queries.p, nq,
+                                                  prm.relative_radius_for_priors, field->prior.p, perr.p, kst.p,
+                                                  (xcdMapMode() >> 1) & 1);
             });
             unsigned long long ps[3];
             readStats(3, ps);
             // per query: index 4 B + leaf code 8 B + prior 4 B; per opened node: code 8 B + child link 4 B;
-            // per tested child: aux 4 B + child link 4 B; per summed node or leaf: 16 B (E, S hi/lo)
+            // per tested child: state 4 B + child link 4 B; per summed node or leaf: 16 B (E, S hi/lo)
             tsr_kernel_stat &k = ctx->kstat[KSTAT_PRIORS];
             k.bytes += (u64)nq * 16 + ps[0] * 12 + ps[1] * 8 + ps[2] * 16;
             k.units += nq;
