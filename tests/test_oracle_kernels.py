@@ -9,7 +9,7 @@ def test_ray_kernel_integrates_to_one_and_endpoints():
     for Z in (0.01, 1.0, 7.5):
         z = np.linspace(0, Z, 200001)
         v = np.array([ray_kernel(t, Z) for t in z[:-1]] + [0.0])
-        assert abs(np.trapz(v, z) - 1.0) < 1e-6
+        assert abs(np.trapezoid(v, z) - 1.0) < 1e-6
         assert ray_kernel(0.0, Z) == pytest.approx(1.5 / Z, rel=1e-15)
         assert ray_kernel(Z, Z) == 0.0 and ray_kernel(-1e-9, Z) == 0.0
 
@@ -18,7 +18,7 @@ def test_disc_kernel_integrates_to_one():
     for R in (0.003, 0.5, 2.0):
         r = np.linspace(0, R, 100001)
         v = np.array([disc_kernel(t * t, R) for t in r])
-        assert abs(np.trapz(v * 2 * np.pi * r, r) - 1.0) < 1e-6
+        assert abs(np.trapezoid(v * 2 * np.pi * r, r) - 1.0) < 1e-6
         assert disc_kernel(0.0, R) == pytest.approx(2 / (np.pi * R * R), rel=1e-15)
         assert disc_kernel(R * R, R) == 0.0
 
