@@ -166,6 +166,8 @@ tsr_status tsr_field_download(tsr_context *ctx, const tsr_field *field, float *k
                               float *priors, uint32_t *node_states, double *occupancy, double *cone_count);
 /* counts of (link, leaf) kernel contributions of the last passes: [0] = E, [1] = S */
 tsr_status tsr_field_hit_counts(const tsr_field *field, uint64_t *hits2);
+/* nodes marked by the learned culling and leaves below them (0 without culling) */
+tsr_status tsr_field_cull_counts(const tsr_field *field, uint32_t *culled_nodes, uint32_t *culled_leaves);
 void tsr_field_destroy(tsr_field *field);
 
 /* DualMarchingCells::extractMesh on the GPU (raw crust mesh, before isle
@@ -179,16 +181,38 @@ tsr_status tsr_mesh_download(tsr_context *ctx, const tsr_mesh *mesh, double *pos
                              uint32_t *edge_keys, uint32_t *triangles);
 void tsr_mesh_destroy(tsr_mesh *mesh);
 
-/* Learned node culling (no reference counterpart): MLP 16-64-64-64-1, BF16
- * MFMA with FP32 accumulation.  Weights are a flat FP32 array in the order
- * W1[64x16], b1[64], W2[64x64], b2[64], W3[64x64], b3[64], W4[1x64], b4[1]. */
+/* Learned node culling (no reference counterpart): an MLP 16-64-64-64-1 (ReLU,
+ * sigmoid output; BF16 MFMA with FP32 accumulation) predicts from per-node
+ * features whether a node's subtree can hold a zero crossing of the field.  With
+ * TSR_CULL_LEARNED, nodes predicted below the threshold skip emptiness-cone
+ * evaluation and priors and their leaves are set to -EPSILON (empty).  Weights
+ * are a flat FP32 array W1[64x16], b1[64], W2[64x64], b2[64], W3[64x64], b3[64],
+ * W4[1x64], b4[1] (W row-major [out][in]). */
 #define TSR_MLP_PARAM_COUNT (64 * 16 + 64 + 64 * 64 + 64 + 64 * 64 + 64 + 64 + 1)
+#define TSR_NODE_FEATURES 16
 tsr_status tsr_culler_create(tsr_context *ctx, const float *weights, float threshold, tsr_culler **out);
 tsr_status tsr_culler_set_weights(tsr_context *ctx, tsr_culler *culler, const float *weights, float threshold);
 void tsr_culler_destroy(tsr_culler *culler);
-/* 16 float features per node for a tree + S-pass field. */
+/* TSR_NODE_FEATURES floats per node (row-major) from a tree and its field
+ * (uses the field's sampleness pass plus a 1/16-subsampled emptiness pass). */
 tsr_status tsr_node_features(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *scene, const tsr_field *field,
                              float *features_host);
+/* Training label per node from a plain field: 1 if its subtree holds a leaf with
+ * positive occupancy or a corner of a dual cell whose corner signs differ. */
+tsr_status tsr_node_labels(tsr_context *ctx, const tsr_tree *tree, const tsr_field *field, uint8_t *labels_host);
+/* MLP forward on host features (n rows): sigmoid outputs. */
+tsr_status tsr_mlp_forward(tsr_context *ctx, const float *weights, const float *features, uint32_t n, float *probs);
+/* Loss (weighted BCE, summed) and its gradient w.r.t. the weights for one batch,
+ * computed by the MFMA forward/backward kernels: the training building block. */
+tsr_status tsr_mlp_gradient(tsr_context *ctx, const float *weights, const float *features, const float *labels,
+                            const float *row_weights, uint32_t n, float *grad, double *loss);
+/* Adam training on the device: epochs over n rows (features, labels, per-row
+ * weights in host memory), mini-batches of `batch` rows, shuffled with `seed`.
+ * weights is read as the initial state and overwritten with the result;
+ * loss_per_epoch receives the mean weighted loss of every epoch. */
+tsr_status tsr_mlp_train(tsr_context *ctx, float *weights, const float *features, const float *labels,
+                         const float *row_weights, uint32_t n, uint32_t epochs, uint32_t batch, float learning_rate,
+                         uint64_t seed, float *loss_per_epoch);
 
 /* Seeded synthetic scenes (analytic stand-in for SyntheticScene.cpp): config 0..4,
  * resolution_scale multiplies the image size. Views stay valid until destroy. */

@@ -46,6 +46,11 @@ def test_argument_validation_without_device():
     assert lib.tsr_occupancy_compute(None, None, None, None, 0, None, None) == 1
     assert lib.tsr_dmc_extract(None, None, None, None) == 1
     assert b"null" in lib.tsr_last_error()
+    assert lib.tsr_mlp_forward(None, None, None, 0, None) == 1
+    assert lib.tsr_mlp_gradient(None, None, None, None, None, 0, None, None) == 1
+    assert lib.tsr_mlp_train(None, None, None, None, None, 0, 1, 1, 0.0, 0, None) == 1
+    assert lib.tsr_culler_create(None, None, 0.5, ctypes.byref(L.vp())) == 1
+    assert lib.tsr_node_labels(None, None, None, None) == 1
     h = L.vp()
     assert lib.tsr_synthetic_create(7, 0, 1.0, ctypes.byref(h)) == 1     # unknown config
     assert lib.tsr_synthetic_create(0, 0, 0.0, ctypes.byref(h)) == 1     # bad resolution

@@ -15,6 +15,7 @@ f32p = ctypes.POINTER(ctypes.c_float)
 f64p = ctypes.POINTER(ctypes.c_double)
 u32p = ctypes.POINTER(ctypes.c_uint32)
 u64p = ctypes.POINTER(ctypes.c_uint64)
+u8p = ctypes.POINTER(ctypes.c_uint8)
 vp = ctypes.c_void_p
 
 
@@ -103,6 +104,7 @@ SIGNATURES = [
     ("tsr_occupancy_compute", ctypes.c_int, [vp, vp, vp, ctypes.POINTER(Params), ctypes.c_int, vp, ctypes.POINTER(vp)]),
     ("tsr_field_download", ctypes.c_int, [vp, vp, f32p, f32p, f32p, u32p, f64p, f64p]),
     ("tsr_field_hit_counts", ctypes.c_int, [vp, u64p]),
+    ("tsr_field_cull_counts", ctypes.c_int, [vp, u32p, u32p]),
     ("tsr_field_destroy", None, [vp]),
     ("tsr_dmc_extract", ctypes.c_int, [vp, vp, vp, ctypes.POINTER(vp)]),
     ("tsr_mesh_get_info", ctypes.c_int, [vp, ctypes.POINTER(MeshInfo)]),
@@ -112,6 +114,11 @@ SIGNATURES = [
     ("tsr_culler_set_weights", ctypes.c_int, [vp, vp, f32p, ctypes.c_float]),
     ("tsr_culler_destroy", None, [vp]),
     ("tsr_node_features", ctypes.c_int, [vp, vp, vp, vp, f32p]),
+    ("tsr_node_labels", ctypes.c_int, [vp, vp, vp, u8p]),
+    ("tsr_mlp_forward", ctypes.c_int, [vp, f32p, f32p, ctypes.c_uint32, f32p]),
+    ("tsr_mlp_gradient", ctypes.c_int, [vp, f32p, f32p, f32p, f32p, ctypes.c_uint32, f32p, f64p]),
+    ("tsr_mlp_train", ctypes.c_int, [vp, f32p, f32p, f32p, f32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                     ctypes.c_float, u64, f32p]),
     ("tsr_synthetic_create", ctypes.c_int, [ctypes.c_int, u64, ctypes.c_double, ctypes.POINTER(vp)]),
     ("tsr_synthetic_views", ctypes.c_int, [vp, ctypes.POINTER(SamplesView), ctypes.POINTER(CamerasView)]),
     ("tsr_synthetic_destroy", None, [vp]),

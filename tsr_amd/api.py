@@ -234,6 +234,12 @@ class Occupancy:
         except Exception:
             pass
 
+    def cull_counts(self):
+        """(nodes, leaves) removed by learned culling (0, 0 without it)"""
+        a, b = ctypes.c_uint32(0), ctypes.c_uint32(0)
+        L.check(L.lib().tsr_field_cull_counts(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
     def hit_counts(self):
         h = np.zeros(2, np.uint64)
         L.check(L.lib().tsr_field_hit_counts(self.h, h.ctypes.data_as(L.u64p)))
@@ -282,6 +288,90 @@ class DualMarchingCells:
         L.check(L.lib().tsr_mesh_download(self.ctx.h, self.h, _dp(o["positions"]), _dp(o["scales"]),
                                           _up(o["edge_keys"]), _up(o["triangles"])))
         return o
+
+
+MLP_PARAM_COUNT = 64 * 16 + 64 + 64 * 64 + 64 + 64 * 64 + 64 + 64 + 1
+NODE_FEATURES = 16
+
+
+class Culler:
+    """Learned node culling weights on the device (tsr_culler)."""
+
+    def __init__(self, ctx, weights, threshold=0.05):
+        w = np.ascontiguousarray(weights, np.float32)
+        if w.size != MLP_PARAM_COUNT:
+            raise ValueError(f"expected {MLP_PARAM_COUNT} weights, got {w.size}")
+        h = L.vp()
+        L.check(L.lib().tsr_culler_create(ctx.h, _fp(w), ctypes.c_float(threshold), ctypes.byref(h)))
+        self.h = h
+        self.threshold = threshold
+
+    def close(self):
+        if self.h:
+            L.lib().tsr_culler_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def node_features(ctx, tree, scene, occupancy):
+    """NODE_FEATURES floats per node of a tree (from a computed field)"""
+    f = np.zeros((tree.node_count, NODE_FEATURES), np.float32)
+    L.check(L.lib().tsr_node_features(ctx.h, tree.h, scene.h, occupancy.h, _fp(f)))
+    return f
+
+
+def node_labels(ctx, tree, occupancy):
+    """1 where a node's subtree holds surface (positive leaf or sign-changing cell corner)"""
+    lab = np.zeros(tree.node_count, np.uint8)
+    L.check(L.lib().tsr_node_labels(ctx.h, tree.h, occupancy.h, lab.ctypes.data_as(L.u8p)))
+    return lab
+
+
+def mlp_forward(ctx, weights, features):
+    w = np.ascontiguousarray(weights, np.float32)
+    x = np.ascontiguousarray(features, np.float32)
+    out = np.zeros(len(x), np.float32)
+    L.check(L.lib().tsr_mlp_forward(ctx.h, _fp(w), _fp(x), len(x), _fp(out)))
+    return out
+
+
+def mlp_gradient(ctx, weights, features, labels, row_weights=None):
+    """(summed weighted BCE loss, gradient) of one batch"""
+    w = np.ascontiguousarray(weights, np.float32)
+    x = np.ascontiguousarray(features, np.float32)
+    y = np.ascontiguousarray(labels, np.float32)
+    rw = np.ascontiguousarray(np.ones(len(x)) if row_weights is None else row_weights, np.float32)
+    g = np.zeros(MLP_PARAM_COUNT, np.float32)
+    loss = ctypes.c_double(0.0)
+    L.check(L.lib().tsr_mlp_gradient(ctx.h, _fp(w), _fp(x), _fp(y), _fp(rw), len(x), _fp(g), ctypes.byref(loss)))
+    return loss.value, g
+
+
+def mlp_train(ctx, weights, features, labels, row_weights=None, epochs=10, batch=4096, learning_rate=1e-3, seed=0):
+    """Adam on the device; returns (trained weights, mean loss per epoch)"""
+    w = np.array(weights, np.float32).reshape(-1)
+    x = np.ascontiguousarray(features, np.float32)
+    y = np.ascontiguousarray(labels, np.float32)
+    rw = np.ascontiguousarray(np.ones(len(x)) if row_weights is None else row_weights, np.float32)
+    losses = np.zeros(epochs, np.float32)
+    L.check(L.lib().tsr_mlp_train(ctx.h, _fp(w), _fp(x), _fp(y), _fp(rw), len(x), epochs, batch,
+                                  ctypes.c_float(learning_rate), seed, _fp(losses)))
+    return w, losses
+
+
+def init_mlp_weights(seed=0):
+    """He-initialised weights in the flat layout W1,b1,W2,b2,W3,b3,W4,b4 ([out][in])."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    for fan_in, fan_out in ((16, 64), (64, 64), (64, 64), (64, 1)):
+        parts.append(rng.normal(0.0, np.sqrt(2.0 / fan_in), fan_out * fan_in))
+        parts.append(np.zeros(fan_out))
+    return np.concatenate(parts).astype(np.float32)
 
 
 def reconstruct(ctx, samples, cameras, params=None, flags=L.TSR_ACCUM_KAHAN_SEGMENTED, culler=None):

@@ -46,6 +46,8 @@ void activate(const tsr_context *ctx) { HIPCHECK(hipSetDevice(ctx->device)); }
 
 }  // namespace
 
+void tsrSetLastError(const char *msg) { g_lastError = msg ? msg : ""; }
+
 extern "C" {
 
 void tsr_params_default(tsr_params *out) {
@@ -311,9 +313,7 @@ tsr_status tsr_occupancy_compute(tsr_context *ctx, const tsr_tree *tree, const t
         f->ctx = ctx;
         f->device = ctx->device;
         try {
-            DBuf<uint8_t> mask;
-            if (flags & TSR_CULL_LEARNED) tsrCullerMask(ctx, culler, tree, scene, prm, mask);
-            computeOccupancy(ctx, tree, scene, prm, flags, (flags & TSR_CULL_LEARNED) ? mask.p : nullptr, f);
+            computeOccupancy(ctx, tree, scene, prm, flags, (flags & TSR_CULL_LEARNED) ? culler : nullptr, f);
         } catch (...) {
             delete f;
             throw;
@@ -342,6 +342,13 @@ tsr_status tsr_field_hit_counts(const tsr_field *f, uint64_t *hits2) {
     if (!f || !hits2) return fail(TSR_ERR_INVALID_ARGUMENT, "null argument");
     hits2[0] = f->hits[0];
     hits2[1] = f->hits[1];
+    return TSR_OK;
+}
+
+tsr_status tsr_field_cull_counts(const tsr_field *f, uint32_t *culled_nodes, uint32_t *culled_leaves) {
+    if (!f) return fail(TSR_ERR_INVALID_ARGUMENT, "null argument");
+    if (culled_nodes) *culled_nodes = f->culledNodes;
+    if (culled_leaves) *culled_leaves = f->culledLeaves;
     return TSR_OK;
 }
 

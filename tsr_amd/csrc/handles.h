@@ -71,6 +71,9 @@ struct tsr_field {
     double coneCount = 0;
     tsr::u64 hits[2] = {0, 0};
     tsr::u32 priorQueries = 0;
+    tsr::DBuf<tsr::u32> hitsS;   // sampleness hits per leaf (features of the learned culling)
+    tsr::u32 culledNodes = 0, culledLeaves = 0;
+    tsr_params params{};
 };
 
 struct tsr_mesh {
@@ -103,18 +106,19 @@ inline void recordPhase(tsr_context *ctx, int phase, bool start) {
 }
 void buildTree(tsr_context *ctx, const tsr_scene *scene, const tsr_params &prm, tsr_tree *tree);
 void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *scene, const tsr_params &prm,
-                      int flags, const uint8_t *cullMask, tsr_field *field);
+                      int flags, const tsr_culler *culler, tsr_field *field);
+// per-leaf emptiness hit counts of every `stride`-th link (gated by the S flags in `states`)
+void emptinessSubsample(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *scene, const tsr_params &prm,
+                        const u32 *states, u32 stride, u32 *leafHits);
+void nodeFeatures(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *scene, const tsr_params &prm,
+                  const u32 *states, const u32 *hitsS, const float *ksumS, DBuf<float> &feat);
+void cullerMask(tsr_context *ctx, const tsr_culler *culler, const tsr_tree *tree, const tsr_scene *scene,
+                const tsr_params &prm, const u32 *states, const u32 *hitsS, const float *ksumS,
+                DBuf<uint8_t> &mask);
 void extractDMC(tsr_context *ctx, const tsr_tree *tree, const tsr_field *field, tsr_mesh *mesh);
 void calibrationCopy(tsr_context *ctx, u64 bytes);
 }  // namespace tsr
 
-struct tsr_culler;
-namespace tsr {
-// per-node culling mask of the learned emptiness-cone culling (1 = skip node)
-void tsrCullerMaskImpl(tsr_context *ctx, const tsr_culler *culler, const tsr_tree *tree, const tsr_scene *scene,
-                       const tsr_params &prm, DBuf<uint8_t> &mask);
-}  // namespace tsr
-inline void tsrCullerMask(tsr_context *ctx, const tsr_culler *culler, const tsr_tree *tree, const tsr_scene *scene,
-                          const tsr_params &prm, tsr::DBuf<uint8_t> &mask) {
-    tsr::tsrCullerMaskImpl(ctx, culler, tree, scene, prm, mask);
-}
+
+// set the thread-local message returned by tsr_last_error
+void tsrSetLastError(const char *msg);

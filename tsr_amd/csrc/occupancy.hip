@@ -61,6 +61,7 @@ struct LinkInputs {
     float bandwidth, relConeLength;
     const uint8_t *cullNode;   // learned culling mask (E pass), may be null
     int xcdMap;
+    u32 linkStride;            // process every linkStride-th link (1 = all)
 };
 
 // Cone of link `link` (Occupancy.cpp:174-201, getCameraToSampleLinkData :250-277)
@@ -150,14 +151,15 @@ __device__ void waveAccumulate(unsigned long long *acc, u32 v) {
     if ((threadIdx.x & 63) == 0) atomicAdd(acc, x);
 }
 
-// mode 0: count accepted hits per link; 1: write records; 2: FP32 atomics (precision study)
+// mode 0: count accepted hits per link; 1: write records; 2: FP32 atomics (precision study);
+// 4: hit count per leaf
 // stats[0] += node visits, stats[1] += accepted hits
 __global__ void k_conePass(TraversalTree t, LinkInputs in, u32 linkCount, int sampleness, int mode,
                            u32 *__restrict__ counts, const u32 *__restrict__ offsets, u32 *__restrict__ recLeaf,
                            uint2 *__restrict__ recVal, float *__restrict__ atomicSum, float *__restrict__ atomicLen,
                            u32 *__restrict__ atomicStates, double *__restrict__ confOut,
                            unsigned long long *__restrict__ stats) {
-    const u32 link = xcdBlock(in.xcdMap) * BLOCK + threadIdx.x;
+    const u32 link = (xcdBlock(in.xcdMap) * BLOCK + threadIdx.x) * in.linkStride;
     u32 visits = 0, count = 0;
     if (link < linkCount) coneLink(t, in, link, sampleness, mode, counts, offsets, recLeaf, recVal, atomicSum,
                                    atomicLen, atomicStates, confOut, visits, count);
@@ -196,6 +198,8 @@ __device__ void coneLink(const TraversalTree &t, const LinkInputs &in, u32 link,
             atomicAdd(&atomicSum[leaf], (float)kv);
             atomicAdd(&atomicLen[leaf], weightedLength);
             atomicOr(&atomicStates[leaf], 1u);
+        } else if (mode == 4) {   // hit count per leaf (integer, order-independent)
+            atomicAdd(&atomicStates[leaf], 1u);
         }
         ++count;
     });
@@ -304,10 +308,11 @@ __global__ void k_subtreeLevel(const u32 *__restrict__ child, const uint8_t *__r
 // leaves whose prior the occupancy ratio reads: E + S kernel sums >= confidence
 // threshold (Occupancy.cpp:575-576); all leaves when `all` is set
 __global__ void k_priorQueryFlags(const float *__restrict__ ksum, u32 leafCount, double threshold, int all,
-                                  float *__restrict__ prior, u32 *__restrict__ flag) {
+                                  const uint8_t *__restrict__ culled, float *__restrict__ prior,
+                                  u32 *__restrict__ flag) {
     const u32 l = blockIdx.x * BLOCK + threadIdx.x;
     if (l >= leafCount) return;
-    const bool used = all || ((double)ksum[l] + (double)ksum[leafCount + l] >= threshold);
+    const bool used = (all || ((double)ksum[l] + (double)ksum[leafCount + l] >= threshold)) && !(culled && culled[l]);
     flag[l] = used ? 1u : 0u;
     if (!used) prior[l] = __int_as_float(0x7fc00000);   // not needed by the field: NaN
 }
@@ -472,13 +477,14 @@ __global__ __launch_bounds__(64) void k_priorsWave(Frame f, const PriorNode *__r
     }
 }
 
-// Occupancy.cpp:570-587, FP64
+// Occupancy.cpp:570-587, FP64 (leaves under learned-culled nodes are empty)
 __global__ void k_occupancy(const float *__restrict__ ksum, const float *__restrict__ prior, u32 leafCount,
-                            double coneCount, double threshold, double *__restrict__ occ) {
+                            double coneCount, double threshold, const uint8_t *__restrict__ culled,
+                            double *__restrict__ occ) {
     const u32 l = blockIdx.x * BLOCK + threadIdx.x;
     if (l >= leafCount) return;
     const double e = ksum[l], s = ksum[leafCount + l];
-    if (e + s < threshold) {
+    if (e + s < threshold || (culled && culled[l])) {
         occ[l] = -EPS_D;
         return;
     }
@@ -500,8 +506,23 @@ __global__ void k_fillU(u32 *p, size_t n, u32 v) {
 
 }  // namespace
 
+__global__ void k_leafCulled(const u32 *__restrict__ leafNode, u32 leafCount, const u32 *__restrict__ parents,
+                             const uint8_t *__restrict__ mask, uint8_t *__restrict__ culled, u32 *count) {
+    const u32 l = blockIdx.x * BLOCK + threadIdx.x;
+    if (l >= leafCount) return;
+    uint8_t c = 0;
+    for (u32 nd = leafNode[l]; nd != INVALID && !c; nd = parents[nd]) c = mask[nd];
+    culled[l] = c;
+    if (c) atomicAdd(count, 1u);
+}
+
+__global__ void k_countMask(const uint8_t *__restrict__ mask, u32 n, u32 *count) {
+    const u32 i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n && mask[i]) atomicAdd(count, 1u);
+}
+
 void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *scene, const tsr_params &prm, int flags,
-                      const uint8_t *cullMask, tsr_field *field) {
+                      const tsr_culler *culler, tsr_field *field) {
     hipStream_t st = ctx->stream;
     Scratch &sc = ctx->scratch;
     const u32 L = tree->leafCount, N = tree->nodeCount;
@@ -522,8 +543,11 @@ void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *s
     LinkInputs in{tree->px.p, tree->py.p, tree->pz.p, tree->nx.p, tree->ny.p, tree->nz.p, tree->scale.p,
                   tree->conf.p, tree->cams.p, scene->cx.p, scene->cy.p, scene->cz.p, tree->sampleDepth.p,
                   tree->maxCams, scene->camCount, prm.bandwidth_factor, prm.relative_sample_cone_length, nullptr};
-    in.cullNode = cullMask;
+    in.cullNode = nullptr;
     in.xcdMap = xcdMapMode() & 1;
+    in.linkStride = 1;
+    field->params = prm;
+    DBuf<uint8_t> cullMask, leafCulled;
     const bool atomicMode = (flags & 0xff) == TSR_ACCUM_FP32_ATOMIC;
 
     DBuf<u32> counts(links), offsets(links), hitCount(L), segStart(L);
@@ -542,6 +566,19 @@ void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *s
             k_setLeafFlags<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(hitCount.p, tree->leafNode.p, L, FLAG_S, field->states.p);
             k_propagateFlag<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(tree->leafNode.p, L, tree->parents.p, FLAG_S,
                                                                  field->states.p);
+            if (culler) {   // learned culling: nodes predicted far from any zero crossing
+                cullerMask(ctx, culler, tree, scene, prm, field->states.p, field->hitsS.p,
+                           field->ksum.p + L, cullMask);
+                in.cullNode = cullMask.p;
+                leafCulled.alloc(L);
+                DBuf<u32> cnt(2);
+                HIPCHECK(hipMemsetAsync(cnt.p, 0, 8, st));
+                k_countMask<<<gridFor(N, BLOCK), BLOCK, 0, st>>>(cullMask.p, N, cnt.p);
+                k_leafCulled<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(tree->leafNode.p, L, tree->parents.p, cullMask.p,
+                                                                  leafCulled.p, cnt.p + 1);
+                field->culledNodes = readScalar(cnt.p, st);
+                field->culledLeaves = readScalar(cnt.p + 1, st);
+            }
         }
         if (atomicMode) {
             HIPCHECK(hipMemsetAsync(ksum, 0, (size_t)L * 4, st));
@@ -609,6 +646,10 @@ void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *s
             HIPCHECK(hipcub::DeviceReduce::Sum(sc.get(tmp), tmp, confs.p, m.p, (int)links, st));
             field->coneCount = readScalar(m.p, st);
         }
+        if (sampleness) {
+            field->hitsS.alloc(L);
+            HIPCHECK(hipMemcpyAsync(field->hitsS.p, hitCount.p, (size_t)L * 4, hipMemcpyDeviceToDevice, st));
+        }
         recordPhase(ctx, 1 + pass, false);
     }
     recordPhase(ctx, 3, true);
@@ -625,7 +666,8 @@ void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *s
         DBuf<u32> qflag(L), qpos(L);
         k_priorQueryFlags<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(field->ksum.p, L, (double)prm.confidence_threshold,
                                                                (flags & TSR_PRIORS_ALL_LEAVES) ? 1 : 0,
-                                                               field->prior.p, qflag.p);
+                                                               culler ? leafCulled.p : nullptr, field->prior.p,
+                                                               qflag.p);
         exclusiveScan(sc, qflag.p, qpos.p, L, st);
         k_priorQueryList<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(qflag.p, qpos.p, L, queries.p);
         const u32 nq = readScalar(qpos.p + L - 1, st) + readScalar(qflag.p + L - 1, st);
@@ -654,9 +696,25 @@ void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *s
     recordPhase(ctx, 3, false);
     recordPhase(ctx, 4, true);
     k_occupancy<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(field->ksum.p, field->prior.p, L, field->coneCount,
-                                                     (double)prm.confidence_threshold, field->occ.p);
+                                                     (double)prm.confidence_threshold,
+                                                     culler ? leafCulled.p : nullptr, field->occ.p);
     recordPhase(ctx, 4, false);
     HIPCHECK(hipStreamSynchronize(st));
+}
+
+void emptinessSubsample(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *scene, const tsr_params &prm,
+                        const u32 *states, u32 stride, u32 *leafHits) {
+    hipStream_t st = ctx->stream;
+    const u32 links = (u32)((u64)scene->n * scene->maxCams);
+    TraversalTree tt{tree->child.p, tree->parents.p, states, tree->frame, (float)(tree->frame.rootSize * 1e-5)};
+    LinkInputs in{tree->px.p, tree->py.p, tree->pz.p, tree->nx.p, tree->ny.p, tree->nz.p, tree->scale.p,
+                  tree->conf.p, tree->cams.p, scene->cx.p, scene->cy.p, scene->cz.p, tree->sampleDepth.p,
+                  tree->maxCams, scene->camCount, prm.bandwidth_factor, prm.relative_sample_cone_length, nullptr,
+                  0, stride};
+    HIPCHECK(hipMemsetAsync(leafHits, 0, (size_t)tree->leafCount * 4, st));
+    const u32 work = (links + stride - 1) / stride;
+    k_conePass<<<gridFor(work, BLOCK), BLOCK, 0, st>>>(tt, in, links, 0, 4, nullptr, nullptr, nullptr, nullptr,
+                                                       nullptr, nullptr, leafHits, nullptr, nullptr);
 }
 
 __global__ void k_calibrationCopy(const float4 *__restrict__ src, float4 *__restrict__ dst, size_t n) {
