@@ -168,6 +168,8 @@ tsr_status tsr_field_download(tsr_context *ctx, const tsr_field *field, float *k
 tsr_status tsr_field_hit_counts(const tsr_field *field, uint64_t *hits2);
 /* nodes marked by the learned culling and leaves below them (0 without culling) */
 tsr_status tsr_field_cull_counts(const tsr_field *field, uint32_t *culled_nodes, uint32_t *culled_leaves);
+/* Per leaf (leaf_count bytes): 1 if learned culling removed it, else 0. */
+tsr_status tsr_field_culled_leaves(tsr_context *ctx, const tsr_field *field, uint8_t *culled);
 void tsr_field_destroy(tsr_field *field);
 
 /* DualMarchingCells::extractMesh on the GPU (raw crust mesh, before isle

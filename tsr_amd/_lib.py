@@ -105,6 +105,7 @@ SIGNATURES = [
     ("tsr_field_download", ctypes.c_int, [vp, vp, f32p, f32p, f32p, u32p, f64p, f64p]),
     ("tsr_field_hit_counts", ctypes.c_int, [vp, u64p]),
     ("tsr_field_cull_counts", ctypes.c_int, [vp, u32p, u32p]),
+    ("tsr_field_culled_leaves", ctypes.c_int, [vp, vp, u8p]),
     ("tsr_field_destroy", None, [vp]),
     ("tsr_dmc_extract", ctypes.c_int, [vp, vp, vp, ctypes.POINTER(vp)]),
     ("tsr_mesh_get_info", ctypes.c_int, [vp, ctypes.POINTER(MeshInfo)]),

@@ -240,6 +240,12 @@ class Occupancy:
         L.check(L.lib().tsr_field_cull_counts(self.h, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
 
+    def culled_leaves(self):
+        """per-leaf flags of learned culling (all zero without it)"""
+        c = np.zeros(self.leaf_count, np.uint8)
+        L.check(L.lib().tsr_field_culled_leaves(self.ctx.h, self.h, c.ctypes.data_as(L.u8p)))
+        return c
+
     def hit_counts(self):
         h = np.zeros(2, np.uint64)
         L.check(L.lib().tsr_field_hit_counts(self.h, h.ctypes.data_as(L.u64p)))

@@ -352,6 +352,19 @@ tsr_status tsr_field_cull_counts(const tsr_field *f, uint32_t *culled_nodes, uin
     return TSR_OK;
 }
 
+tsr_status tsr_field_culled_leaves(tsr_context *ctx, const tsr_field *f, uint8_t *culled) {
+    if (!ctx || !f || !culled) return fail(TSR_ERR_INVALID_ARGUMENT, "null argument");
+    return guarded([&] {
+        HIPCHECK(hipSetDevice(ctx->device));
+        if (!f->culled.p) {
+            std::memset(culled, 0, f->leafCount);
+            return;
+        }
+        HIPCHECK(hipMemcpyAsync(culled, f->culled.p, f->leafCount, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHECK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
 void tsr_field_destroy(tsr_field *f) {
     if (!f) return;
     (void)hipSetDevice(f->device);

@@ -547,7 +547,7 @@ void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *s
     in.xcdMap = xcdMapMode() & 1;
     in.linkStride = 1;
     field->params = prm;
-    DBuf<uint8_t> cullMask, leafCulled;
+    DBuf<uint8_t> cullMask;
     const bool atomicMode = (flags & 0xff) == TSR_ACCUM_FP32_ATOMIC;
 
     DBuf<u32> counts(links), offsets(links), hitCount(L), segStart(L);
@@ -570,12 +570,12 @@ void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *s
                 cullerMask(ctx, culler, tree, scene, prm, field->states.p, field->hitsS.p,
                            field->ksum.p + L, cullMask);
                 in.cullNode = cullMask.p;
-                leafCulled.alloc(L);
+                field->culled.alloc(L);
                 DBuf<u32> cnt(2);
                 HIPCHECK(hipMemsetAsync(cnt.p, 0, 8, st));
                 k_countMask<<<gridFor(N, BLOCK), BLOCK, 0, st>>>(cullMask.p, N, cnt.p);
                 k_leafCulled<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(tree->leafNode.p, L, tree->parents.p, cullMask.p,
-                                                                  leafCulled.p, cnt.p + 1);
+                                                                  field->culled.p, cnt.p + 1);
                 field->culledNodes = readScalar(cnt.p, st);
                 field->culledLeaves = readScalar(cnt.p + 1, st);
             }
@@ -666,7 +666,7 @@ void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *s
         DBuf<u32> qflag(L), qpos(L);
         k_priorQueryFlags<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(field->ksum.p, L, (double)prm.confidence_threshold,
                                                                (flags & TSR_PRIORS_ALL_LEAVES) ? 1 : 0,
-                                                               culler ? leafCulled.p : nullptr, field->prior.p,
+                                                               culler ? field->culled.p : nullptr, field->prior.p,
                                                                qflag.p);
         exclusiveScan(sc, qflag.p, qpos.p, L, st);
         k_priorQueryList<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(qflag.p, qpos.p, L, queries.p);
@@ -697,7 +697,7 @@ void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *s
     recordPhase(ctx, 4, true);
     k_occupancy<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(field->ksum.p, field->prior.p, L, field->coneCount,
                                                      (double)prm.confidence_threshold,
-                                                     culler ? leafCulled.p : nullptr, field->occ.p);
+                                                     culler ? field->culled.p : nullptr, field->occ.p);
     recordPhase(ctx, 4, false);
     HIPCHECK(hipStreamSynchronize(st));
 }
