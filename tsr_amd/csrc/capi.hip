@@ -83,6 +83,7 @@ void tsr_context_destroy(tsr_context *ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     ctx->scratch.buf.release();
+    if (ctx->pinned) (void)hipHostFree(ctx->pinned);
     for (hipEvent_t &e : ctx->ev) (void)hipEventDestroy(e);
     for (hipEvent_t &e : ctx->kev) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->stream);

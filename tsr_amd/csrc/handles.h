@@ -1,6 +1,7 @@
 // Handle structs behind the opaque C ABI types.
 #pragma once
 
+#include <functional>
 #include <vector>
 
 #include "device_util.h"
@@ -19,6 +20,9 @@ struct tsr_context {
     float phaseMs[6] = {0, 0, 0, 0, 0, 0};
     tsr_kernel_stat kstat[8];
     tsr::Scratch scratch;
+    // pinned host staging for large device->host transfers (grow-only, freed with the context)
+    void *pinned = nullptr;
+    size_t pinnedBytes = 0;
 };
 
 struct tsr_scene {
@@ -88,6 +92,10 @@ struct tsr_mesh {
 };
 
 namespace tsr {
+// pinned host memory of the context, at least `bytes` (contents not preserved on growth)
+void *pinnedHost(tsr_context *ctx, size_t bytes);
+// runs fn(begin, end) over [0, n) on up to 16 host threads (OMP_NUM_THREADS caps it)
+void parallelFor(size_t n, const std::function<void(size_t, size_t)> &fn);
 enum { KSTAT_PRIORS = 0, KSTAT_E_COUNT = 1, KSTAT_E_WRITE = 2, KSTAT_S_COUNT = 3, KSTAT_S_WRITE = 4 };
 // time one kernel launch on the context stream with HIP events (synchronises)
 template <typename Launch>

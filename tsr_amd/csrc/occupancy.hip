@@ -17,6 +17,7 @@
 // Kahan compensation in that fixed order: deterministic, no float atomics.
 #include <algorithm>
 #include <cstdlib>
+#include <cstring>
 #include <vector>
 
 #include "handles.h"
@@ -62,6 +63,7 @@ struct LinkInputs {
     const uint8_t *cullNode;   // learned culling mask (E pass), may be null
     int xcdMap;
     u32 linkStride;            // process every linkStride-th link (1 = all)
+    const u32 *order;          // thread index -> link (links grouped by camera), may be null
 };
 
 // Cone of link `link` (Occupancy.cpp:174-201, getCameraToSampleLinkData :250-277)
@@ -159,9 +161,10 @@ __global__ void k_conePass(TraversalTree t, LinkInputs in, u32 linkCount, int sa
                            uint2 *__restrict__ recVal, float *__restrict__ atomicSum, float *__restrict__ atomicLen,
                            u32 *__restrict__ atomicStates, double *__restrict__ confOut,
                            unsigned long long *__restrict__ stats) {
-    const u32 link = (xcdBlock(in.xcdMap) * BLOCK + threadIdx.x) * in.linkStride;
+    const u32 idx = (xcdBlock(in.xcdMap) * BLOCK + threadIdx.x) * in.linkStride;
     u32 visits = 0, count = 0;
-    if (link < linkCount) coneLink(t, in, link, sampleness, mode, counts, offsets, recLeaf, recVal, atomicSum,
+    const u32 link = idx < linkCount && in.order ? in.order[idx] : idx;
+    if (idx < linkCount) coneLink(t, in, link, sampleness, mode, counts, offsets, recLeaf, recVal, atomicSum,
                                    atomicLen, atomicStates, confOut, visits, count);
     if (stats) {
         waveAccumulate(&stats[0], visits);
@@ -477,6 +480,272 @@ __global__ __launch_bounds__(64) void k_priorsWave(Frame f, const PriorNode *__r
     }
 }
 
+// Packet form of the same query: one wavefront serves 64 consecutive queries
+// (leaf order = Morton order, so their spheres overlap heavily).  The wave walks
+// the union of the nodes the 64 queries open, once: an LDS stack holds (parent,
+// lane mask of the queries that still need its children); each step loads the 8
+// child records once for all lanes and every lane in the mask applies its own
+// sphere, depth and flag tests (the same FP64 predicates as the per-query form).
+// Contributions are summed per lane in the fixed traversal order.
+constexpr u32 PACKET_STACK = 256;   // DFS: <= 7 per level + 8 entries
+
+__global__ __launch_bounds__(64) void k_priorsPacket(Frame f, const PriorNode *__restrict__ nodes,
+                                                     const u64 *__restrict__ leafLoc,
+                                                     const u32 *__restrict__ queries, u32 queryCount,
+                                                     float relRadius, float *__restrict__ prior, u32 *err,
+                                                     unsigned long long *__restrict__ stats, int xcdMap) {
+    __shared__ u32 stNode[PACKET_STACK];
+    __shared__ u64 stMask[PACKET_STACK];
+    __shared__ u32 sp;
+    const u32 lane = threadIdx.x;
+    u32 nPops = 0, nAccum = 0;
+    const u32 packets = (queryCount + 63) / 64;
+    u32 pBegin, pEnd, pStride, pFirst;
+    xcdRange(packets, xcdMap, pBegin, pEnd, pStride, pFirst);
+    for (u32 pk = pFirst; pk < pEnd; pk += pStride) {
+        const u32 q = pk * 64 + lane;
+        const bool valid = q < queryCount;
+        const u32 l = valid ? queries[q] : 0u;
+        const u64 qloc = valid ? leafLoc[l] : 1ull;
+        u32 kq[3];
+        locCoords(qloc, kq[0], kq[1], kq[2]);
+        const u32 dq = locDepth(qloc);
+        double c[3];
+        for (int a = 0; a < 3; ++a) c[a] = f.centre(a, kq[a], dq);
+        const double radius = f.h(dq) * (double)relRadius;
+        const double r2 = radius * radius, rInner2 = r2 * (1.0 - 1e-9);
+        const u32 maxDepth = dq + MAX_DEPTH_DIFFERENCE;
+        float eh = 0.f, el = 0.f, sh = 0.f, sl = 0.f;
+        const u64 active = __ballot(valid);
+        const PriorNode root = nodes[0];
+        if (root.aux & FLAG_S) {   // the root always overlaps the sphere (its centre is inside)
+            if (root.child & LEAF_BIT) {
+                eh = root.sums.x;
+                sh = root.sums.z;
+            } else if (lane == 0) {
+                stNode[0] = 0;
+                stMask[0] = active;
+                sp = 1;
+            }
+        } else if (lane == 0) {
+            sp = 0;
+        }
+        if (root.child & LEAF_BIT && lane == 0) sp = 0;
+        __syncthreads();
+        for (;;) {
+            const u32 top = sp;
+            if (top == 0) break;
+            const u32 parent = stNode[top - 1];
+            const u64 mask = stMask[top - 1];
+            __syncthreads();   // every lane has read the entry before it is overwritten
+            nPops += lane == 0 ? 1u : 0u;
+            const PriorNode pn = nodes[parent];
+            const u32 d = locDepth(pn.loc) + 1;
+            u32 kp[3];
+            locCoords(pn.loc, kp[0], kp[1], kp[2]);
+            u32 pushBits = 0;
+            if (((mask >> lane) & 1ull) && d <= maxDepth) {
+#pragma unroll
+                for (u32 slot = 0; slot < 8; ++slot) {
+                    const PriorNode cn = nodes[pn.child + slot];
+                    if (!(cn.aux & FLAG_S)) continue;
+                    const u32 k[3] = {2 * kp[0] + (slot & 1), 2 * kp[1] + ((slot >> 1) & 1),
+                                      2 * kp[2] + ((slot >> 2) & 1)};
+                    double dmin = 0.0, dmax = 0.0;
+                    for (int q3 = 0; q3 < 3; ++q3) {   // SphereNodesChecker.cpp:29-40
+                        const double lo = f.B(q3, k[q3], d), hi = f.B(q3, (u64)k[q3] + 1, d);
+                        if (c[q3] < lo) dmin += (lo - c[q3]) * (lo - c[q3]);
+                        else if (c[q3] > hi) dmin += (c[q3] - hi) * (c[q3] - hi);
+                        const double far = fmax(fabs(c[q3] - lo), fabs(hi - c[q3]));
+                        dmax += far * far;
+                    }
+                    if (dmin <= r2) {
+                        const bool leaf = (cn.child & LEAF_BIT) != 0;
+                        if (leaf || (dmax <= rInner2 && (cn.aux >> 8) <= maxDepth)) {
+                            ++nAccum;
+                            twoSumAdd(eh, el, cn.sums.x);
+                            el += cn.sums.y;
+                            twoSumAdd(sh, sl, cn.sums.z);
+                            sl += cn.sums.w;
+                        } else {
+                            pushBits |= 1u << slot;
+                        }
+                    }
+                }
+            }
+            u32 newTop = top - 1;
+            for (u32 slot = 0; slot < 8; ++slot) {
+                const u64 m = __ballot((pushBits >> slot) & 1u);
+                if (m) {
+                    if (newTop < PACKET_STACK && lane == 0) {
+                        stNode[newTop] = pn.child + slot;
+                        stMask[newTop] = m;
+                    }
+                    ++newTop;
+                }
+            }
+            if (lane == 0) {
+                if (newTop > PACKET_STACK) {
+                    atomicOr(err, 1u);
+                    newTop = 0;
+                }
+                sp = newTop;
+            }
+            __syncthreads();
+        }
+        if (valid) {
+            const double e = (double)eh + (double)el, s = (double)sh + (double)sl;
+            prior[l] = (float)(e / (e + s));
+        }
+        __syncthreads();
+    }
+    if (stats) {
+        waveAccumulate(&stats[0], nPops);
+        waveAccumulate(&stats[2], nAccum);
+    }
+}
+
+// Group form: one wavefront serves 8 consecutive queries.  Stack entries carry an
+// 8-bit mask of the queries that still need a node's children; each step pops up
+// to 8 entries and tests their 8 children on 64 lanes (entry = lane / 8, child =
+// lane % 8), every lane looping over the queries of its entry's mask.  A child
+// record is loaded once for all queries of the group; every query applies the
+// same FP64 predicates as the per-query form.  Sums are FP64 per lane and query,
+// reduced across the wave in a fixed order at the end (deterministic).
+constexpr u32 GROUP_Q = 8;
+
+__global__ __launch_bounds__(64) void k_priorsGroup(Frame f, const PriorNode *__restrict__ nodes,
+                                                    const u64 *__restrict__ leafLoc,
+                                                    const u32 *__restrict__ queries, u32 queryCount,
+                                                    float relRadius, float *__restrict__ prior, u32 *err,
+                                                    unsigned long long *__restrict__ stats, int xcdMap) {
+    __shared__ u32 stNode[PRIOR_STACK];
+    __shared__ uint8_t stMask[PRIOR_STACK];
+    __shared__ double qc[GROUP_Q][3], qr2[GROUP_Q], qri2[GROUP_Q];
+    __shared__ u32 qmd[GROUP_Q];
+    __shared__ u32 sp;
+    const u32 lane = threadIdx.x;
+    u32 nPops = 0, nTests = 0;
+    const u32 groups = (queryCount + GROUP_Q - 1) / GROUP_Q;
+    u32 gBegin, gEnd, gStride, gFirst;
+    xcdRange(groups, xcdMap, gBegin, gEnd, gStride, gFirst);
+    for (u32 g = gFirst; g < gEnd; g += gStride) {
+        const u32 nQ = min(GROUP_Q, queryCount - g * GROUP_Q);
+        if (lane < GROUP_Q) {
+            const u32 l = lane < nQ ? queries[g * GROUP_Q + lane] : 0u;
+            const u64 qloc = lane < nQ ? leafLoc[l] : 1ull;
+            u32 kq[3];
+            locCoords(qloc, kq[0], kq[1], kq[2]);
+            const u32 dq = locDepth(qloc);
+            for (int a = 0; a < 3; ++a) qc[lane][a] = f.centre(a, kq[a], dq);
+            const double radius = f.h(dq) * (double)relRadius;
+            qr2[lane] = radius * radius;
+            qri2[lane] = radius * radius * (1.0 - 1e-9);
+            qmd[lane] = dq + MAX_DEPTH_DIFFERENCE;
+        }
+        double accE[GROUP_Q], accS[GROUP_Q];
+#pragma unroll
+        for (u32 j = 0; j < GROUP_Q; ++j) accE[j] = accS[j] = 0.0;
+        const PriorNode root = nodes[0];
+        const uint8_t all = (uint8_t)((1u << nQ) - 1u);
+        if (lane == 0) {
+            sp = 0;
+            if ((root.aux & FLAG_S) && !(root.child & LEAF_BIT)) {
+                stNode[0] = 0;
+                stMask[0] = all;
+                sp = 1;
+            }
+        }
+        if ((root.aux & FLAG_S) && (root.child & LEAF_BIT) && lane == 0) {   // every query sums the root
+#pragma unroll
+            for (u32 j = 0; j < GROUP_Q; ++j) {
+                accE[j] = (double)root.sums.x + (double)root.sums.y;
+                accS[j] = (double)root.sums.z + (double)root.sums.w;
+            }
+        }
+        __syncthreads();
+        for (;;) {
+            const u32 top = sp;
+            if (top == 0) break;
+            const u32 take = min(top, 8u);
+            const u32 base = top - take;
+            const u32 e = lane >> 3, slot = lane & 7;
+            uint8_t pushMask = 0;
+            u32 cid = 0;
+            nPops += lane == 0 ? take : 0;
+            if (e < take) {
+                const u32 parent = stNode[base + e];
+                const u32 mask = stMask[base + e];
+                const PriorNode pn = nodes[parent];
+                cid = pn.child + slot;
+                const PriorNode cn = nodes[cid];
+                const u32 d = locDepth(pn.loc) + 1;
+                if (cn.aux & FLAG_S) {
+                    ++nTests;
+                    u32 k[3];
+                    locCoords((pn.loc << 3) | slot, k[0], k[1], k[2]);
+                    double lo[3], hi[3];
+                    for (int a = 0; a < 3; ++a) {
+                        lo[a] = f.B(a, k[a], d);
+                        hi[a] = f.B(a, (u64)k[a] + 1, d);
+                    }
+                    const bool leaf = (cn.child & LEAF_BIT) != 0;
+                    const u32 mld = cn.aux >> 8;
+#pragma unroll
+                    for (u32 j = 0; j < GROUP_Q; ++j) {
+                        if (!((mask >> j) & 1u) || d > qmd[j]) continue;
+                        double dmin = 0.0, dmax = 0.0;
+                        for (int q3 = 0; q3 < 3; ++q3) {   // SphereNodesChecker.cpp:29-40
+                            const double c = qc[j][q3];
+                            if (c < lo[q3]) dmin += (lo[q3] - c) * (lo[q3] - c);
+                            else if (c > hi[q3]) dmin += (c - hi[q3]) * (c - hi[q3]);
+                            const double far = fmax(fabs(c - lo[q3]), fabs(hi[q3] - c));
+                            dmax += far * far;
+                        }
+                        if (dmin <= qr2[j]) {
+                            if (leaf || (dmax <= qri2[j] && mld <= qmd[j])) {
+                                accE[j] += (double)cn.sums.x + (double)cn.sums.y;
+                                accS[j] += (double)cn.sums.z + (double)cn.sums.w;
+                            } else {
+                                pushMask |= (uint8_t)(1u << j);
+                            }
+                        }
+                    }
+                }
+            }
+            __syncthreads();   // every lane has read its entry before the slots are reused
+            const u64 pm = __ballot(pushMask != 0);
+            const u32 pos = (u32)__popcll(pm & ((1ull << lane) - 1ull));
+            const u32 newTop = base + (u32)__popcll(pm);
+            if (newTop > PRIOR_STACK) {
+                if (lane == 0) { atomicOr(err, 1u); sp = 0; }
+            } else {
+                if (pushMask) {
+                    stNode[base + pos] = cid;
+                    stMask[base + pos] = pushMask;
+                }
+                if (lane == 0) sp = newTop;
+            }
+            __syncthreads();
+        }
+        // fixed-order wave reduction per query (FP64)
+#pragma unroll
+        for (u32 j = 0; j < GROUP_Q; ++j) {
+            double ee = accE[j], ss = accS[j];
+            for (int off = 32; off > 0; off >>= 1) {
+                ee += __shfl_xor(ee, off);
+                ss += __shfl_xor(ss, off);
+            }
+            if (lane == 0 && j < nQ) prior[queries[g * GROUP_Q + j]] = (float)(ee / (ee + ss));
+        }
+        __syncthreads();
+    }
+    if (stats) {
+        waveAccumulate(&stats[0], nPops);
+        waveAccumulate(&stats[1], nTests);
+    }
+}
+
 // Occupancy.cpp:570-587, FP64 (leaves under learned-culled nodes are empty)
 __global__ void k_occupancy(const float *__restrict__ ksum, const float *__restrict__ prior, u32 leafCount,
                             double coneCount, double threshold, const uint8_t *__restrict__ culled,
@@ -521,6 +790,34 @@ __global__ void k_countMask(const uint8_t *__restrict__ mask, u32 n, u32 *count)
     if (i < n && mask[i]) atomicAdd(count, 1u);
 }
 
+__global__ void k_camKeys(const u32 *__restrict__ cams, u32 links, u32 camCount, u32 *__restrict__ key,
+                          u32 *__restrict__ idx) {
+    const u32 l = blockIdx.x * BLOCK + threadIdx.x;
+    if (l >= links) return;
+    key[l] = min(cams[l], camCount);   // invalid links (>= camCount) last
+    idx[l] = l;
+}
+
+// priors kernel: 0 = one wavefront per query, 1 = 64 queries per wavefront,
+// 2 = 8 queries per wavefront (TSR_PRIORS_KERNEL=query|packet|group)
+int priorsPacketMode() {
+    static const int m = [] {
+        const char *e = std::getenv("TSR_PRIORS_KERNEL");
+        if (e && std::strcmp(e, "query") == 0) return 0;
+        if (e && std::strcmp(e, "packet") == 0) return 1;
+        return 2;
+    }();
+    return m;
+}
+
+int linkOrderMode() {   // TSR_LINK_ORDER=0: threads take links in link order
+    static const int m = [] {
+        const char *e = std::getenv("TSR_LINK_ORDER");
+        return e ? std::atoi(e) : 1;
+    }();
+    return m;
+}
+
 void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *scene, const tsr_params &prm, int flags,
                       const tsr_culler *culler, tsr_field *field) {
     hipStream_t st = ctx->stream;
@@ -546,7 +843,20 @@ void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *s
     in.cullNode = nullptr;
     in.xcdMap = xcdMapMode() & 1;
     in.linkStride = 1;
+    in.order = nullptr;
     field->params = prm;
+    // E-pass threads take links grouped by camera, samples in node (Morton) order within
+    // a camera: neighbouring lanes trace nearly parallel cones through the same nodes.
+    // Records stay indexed by link, so results do not depend on this order.
+    DBuf<u32> linkOrder;
+    if (linkOrderMode()) {
+        DBuf<u32> camKey(links), camKeyS(links), iota(links);
+        k_camKeys<<<gridFor(links, BLOCK), BLOCK, 0, st>>>(tree->cams.p, links, scene->camCount, camKey.p, iota.p);
+        linkOrder.alloc(links);
+        int bits = 1;
+        while (bits < 32 && (1ull << bits) <= scene->camCount) ++bits;
+        sortPairs(sc, camKey.p, camKeyS.p, iota.p, linkOrder.p, links, 0, bits, st);
+    }
     DBuf<uint8_t> cullMask;
     const bool atomicMode = (flags & 0xff) == TSR_ACCUM_FP32_ATOMIC;
 
@@ -562,6 +872,7 @@ void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *s
         const int type = sampleness;   // storage index: 0 = EMPTINESS, 1 = SAMPLENESS
         recordPhase(ctx, 1 + pass, true);
         float *ksum = field->ksum.p + (size_t)type * L, *clen = field->clen.p + (size_t)type * L;
+        in.order = sampleness ? nullptr : linkOrder.p;   // the short S cones gain nothing from it
         if (!sampleness) {   // E pass: S flags up the tree first
             k_setLeafFlags<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(hitCount.p, tree->leafNode.p, L, FLAG_S, field->states.p);
             k_propagateFlag<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(tree->leafNode.p, L, tree->parents.p, FLAG_S,
@@ -678,17 +989,34 @@ void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *s
         if (nq) {
             const u32 grid = 256u * 32u;   // multiple of the XCD count
             HIPCHECK(hipMemsetAsync(kst.p, 0, 32, st));
+            const int kind = priorsPacketMode();
+            const bool packet = kind == 1;
             timedLaunch(ctx, KSTAT_PRIORS, [&] {
-                k_priorsWave<<<grid, 64, 0, st>>>(tree->frame, pnodes.p, tree->leafLoc.p, queries.p, nq,
-                                                  prm.relative_radius_for_priors, field->prior.p, perr.p, kst.p,
-                                                  (xcdMapMode() >> 1) & 1);
+                if (kind == 2)
+                    k_priorsGroup<<<grid, 64, 0, st>>>(tree->frame, pnodes.p, tree->leafLoc.p, queries.p, nq,
+                                                       prm.relative_radius_for_priors, field->prior.p, perr.p,
+                                                       kst.p, (xcdMapMode() >> 1) & 1);
+                else if (packet)
+                    k_priorsPacket<<<grid, 64, 0, st>>>(tree->frame, pnodes.p, tree->leafLoc.p, queries.p, nq,
+                                                        prm.relative_radius_for_priors, field->prior.p, perr.p,
+                                                        kst.p, (xcdMapMode() >> 1) & 1);
+                else
+                    k_priorsWave<<<grid, 64, 0, st>>>(tree->frame, pnodes.p, tree->leafLoc.p, queries.p, nq,
+                                                      prm.relative_radius_for_priors, field->prior.p, perr.p, kst.p,
+                                                      (xcdMapMode() >> 1) & 1);
             });
             unsigned long long ps[3];
             readStats(3, ps);
-            // per query: index 4 B + leaf code 8 B + prior 4 B; per opened node: code 8 B + child link 4 B;
-            // per tested child: state 4 B + child link 4 B; per summed node or leaf: 16 B (E, S hi/lo)
             tsr_kernel_stat &k = ctx->kstat[KSTAT_PRIORS];
-            k.bytes += (u64)nq * 16 + ps[0] * 12 + ps[1] * 8 + ps[2] * 16;
+            if (kind == 2)   // per query 16 B; per popped entry: parent record 32 B; per tested
+                             // child: its 32 B record, shared by the group's queries
+                k.bytes += (u64)nq * 16 + ps[0] * 32 + ps[1] * 32;
+            else if (packet)   // per query: index 4 B + leaf code 8 B + prior 4 B; per opened parent: its
+                               // 32 B record + the 8 contiguous 32 B child records, shared by the packet
+                k.bytes += (u64)nq * 16 + ps[0] * 288;
+            else          // per opened node: code 8 B + child link 4 B; per tested child: state 4 B +
+                          // child link 4 B; per summed node or leaf: 16 B (E, S hi/lo)
+                k.bytes += (u64)nq * 16 + ps[0] * 12 + ps[1] * 8 + ps[2] * 16;
             k.units += nq;
         }
         if (readScalar(perr.p, st)) throw tsr_error(TSR_ERR_HIP, "priors traversal stack overflow");
@@ -710,7 +1038,7 @@ void emptinessSubsample(tsr_context *ctx, const tsr_tree *tree, const tsr_scene 
     LinkInputs in{tree->px.p, tree->py.p, tree->pz.p, tree->nx.p, tree->ny.p, tree->nz.p, tree->scale.p,
                   tree->conf.p, tree->cams.p, scene->cx.p, scene->cy.p, scene->cz.p, tree->sampleDepth.p,
                   tree->maxCams, scene->camCount, prm.bandwidth_factor, prm.relative_sample_cone_length, nullptr,
-                  0, stride};
+                  0, stride, nullptr};
     HIPCHECK(hipMemsetAsync(leafHits, 0, (size_t)tree->leafCount * 4, st));
     const u32 work = (links + stride - 1) / stride;
     k_conePass<<<gridFor(work, BLOCK), BLOCK, 0, st>>>(tt, in, links, 0, 4, nullptr, nullptr, nullptr, nullptr,

@@ -22,6 +22,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <queue>
+#include <thread>
 #include <vector>
 
 #include "handles.h"
@@ -154,6 +155,39 @@ TSR_HD bool findReliable(const Frame &f, u64 start, bool positive, const Samplin
     }
 }
 
+// Where the DFS of findReliable can start without changing its result: the
+// deepest existing node whose box strictly contains the unit's search ball
+// (centre p, radius msd) -- every node that intersectNode accepts lies inside
+// it, all its ancestors pass intersectNode (the sample plane crosses their
+// interior), and their other children are rejected, so a DFS from the root
+// reaches this node first and yields the same leaves in the same order.  The
+// ball is inflated far above FP64 rounding before the strict-containment test.
+// Returns the root when the ball is not strictly inside the root box.
+template <typename IsInner>
+TSR_HD u64 searchStart(const Frame &f, const SamplingUnit &u, IsInner isInner) {
+    const double r = u.msd * (1.0 + 1e-9) + 1e-12 * f.rootSize;
+    u32 lo[3], diff = 0;
+    for (int a = 0; a < 3; ++a) {
+        const double a0 = u.p[a] - r, a1 = u.p[a] + r;
+        if (!(a0 > f.rootMin[a]) || !(a1 < f.B(a, 1ull << MAX_DEPTH, MAX_DEPTH))) return 1ull;
+        // cellCoord: B(k) <= x < B(k + 1), so a0 > B(lo) needs a0 != B(lo)
+        lo[a] = cellCoord(f, a, a0, MAX_DEPTH);
+        if (f.B(a, lo[a], MAX_DEPTH) == a0) return 1ull;
+        diff |= lo[a] ^ cellCoord(f, a, a1, MAX_DEPTH);
+    }
+    const u32 dc = diff ? MAX_DEPTH - (32u - (u32)__builtin_clz(diff)) : MAX_DEPTH;
+    auto anc = [&](u32 d) { return makeLoc(lo[0] >> (MAX_DEPTH - d), lo[1] >> (MAX_DEPTH - d), lo[2] >> (MAX_DEPTH - d), d); };
+    if (dc == 0 || !isInner(1ull)) return 1ull;
+    // inner-ness is prefix-closed along the ancestor chain: binary search the deepest inner one
+    u32 good = 0, bad = dc + 1;
+    while (bad - good > 1) {
+        const u32 mid = (good + bad) / 2;
+        if (isInner(anc(mid))) good = mid;
+        else bad = mid;
+    }
+    return good == dc ? anc(dc) : anc(good + 1);
+}
+
 __device__ SamplingUnit loadUnit(const float *px, const float *py, const float *pz, const float *nx, const float *ny,
                                  const float *nz, const float *scale, float maxRel, u32 i) {
     SamplingUnit u;
@@ -187,7 +221,7 @@ __global__ void k_samplingEval(Frame f, HashView inner, const float *__restrict_
         }
         ++count;
     };
-    const bool found = findReliable(f, 1ull, positive, u, isInner, onLeaf);
+    const bool found = findReliable(f, searchStart(f, u, isInner), positive, u, isInner, onLeaf);
     if (mode == 0) {
         const bool event = !found || count > VISIT_CAP;
         isEvent[unit] = event ? 1 : 0;
@@ -504,67 +538,191 @@ struct FlatSet {
             if (v == 0ull) { keys[s] = k; return true; }
         }
     }
+    // insert from several threads at once (no concurrent readers)
+    void insertConcurrent(u64 k) {
+        for (u64 s = hashMix(k) & mask;; s = (s + 1) & mask) {
+            u64 expected = 0ull;
+            if (__atomic_compare_exchange_n(&keys[s], &expected, k, false, __ATOMIC_RELAXED, __ATOMIC_RELAXED) ||
+                expected == k)
+                return;
+        }
+    }
 };
 
-// host view of the (T0 leaf -> units that visited it) index; leaves outside the
-// transferred "hot" part are fetched from the device on demand
+// open-addressing u64 -> u32 map (key 0 = empty slot), host side
+struct FlatMap {
+    std::vector<u64> keys;
+    std::vector<u32> vals;
+    u64 mask = 0;
+    void init(size_t expected) {
+        size_t cap = 1024;
+        while (cap < 2 * expected + 16) cap <<= 1;
+        keys.assign(cap, 0ull);
+        vals.assign(cap, 0u);
+        mask = cap - 1;
+    }
+    const u32 *find(u64 k) const {
+        for (u64 s = hashMix(k) & mask;; s = (s + 1) & mask) {
+            const u64 v = keys[s];
+            if (v == k) return &vals[s];
+            if (v == 0ull) return nullptr;
+        }
+    }
+    void put(u64 k, u32 val) {
+        for (u64 s = hashMix(k) & mask;; s = (s + 1) & mask) {
+            const u64 v = keys[s];
+            if (v == k || v == 0ull) {
+                keys[s] = k;
+                vals[s] = val;
+                return;
+            }
+        }
+    }
+    // from several threads at once (no concurrent readers): insert k -> val if k is
+    // absent; returns the slot of k
+    u64 putIfAbsentConcurrent(u64 k, u32 val) {
+        for (u64 s = hashMix(k) & mask;; s = (s + 1) & mask) {
+            u64 expected = 0ull;
+            if (__atomic_compare_exchange_n(&keys[s], &expected, k, false, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) {
+                vals[s] = val;
+                return s;
+            }
+            if (expected == k) return s;
+        }
+    }
+};
+
+// host view of the (T0 leaf -> units that visited it) index for the "hot" leaves
+// (those near units the speculation could not settle); other leaves are looked
+// up on the device on demand
 struct VisitIndex {
-    std::vector<u64> leaf;   // sorted, hot leaves only
-    std::vector<u32> unit;   // ascending per leaf
-    FlatSet hot;             // every hot leaf (also those no unit visited)
+    FlatMap run;                  // hot leaf -> run index into runStart, INVALID if no unit visited it
+    const u32 *runStart = nullptr;   // runs + 1 offsets into unit
+    const u32 *unit = nullptr;       // visiting units, ascending within a run
     std::function<void(u64, std::vector<u32> &)> fetch;   // exact lookup on the device
     u64 misses = 0;
-    void units(u64 loc, std::vector<u32> &out) {
-        out.clear();
-        auto r = std::equal_range(leaf.begin(), leaf.end(), loc);
-        if (r.first != r.second) {
-            out.assign(unit.begin() + (r.first - leaf.begin()), unit.begin() + (r.second - leaf.begin()));
+    // [first, last) of the units that visited T0 leaf `loc`, ascending; `spill`
+    // holds the list of a leaf outside the hot part
+    void units(u64 loc, const u32 *&first, const u32 *&last, std::vector<u32> &spill) {
+        first = last = nullptr;
+        if (const u32 *r = run.find(loc)) {
+            if (*r != INVALID) {
+                first = unit + runStart[*r];
+                last = unit + runStart[*r + 1];
+            }
             return;
         }
-        if (hot.contains(loc)) return;
         ++misses;
-        fetch(loc, out);
+        fetch(loc, spill);
+        first = spill.data();
+        last = spill.data() + spill.size();
+    }
+};
+
+// Host set of inner nodes for the in-order replay, laid out for the DFS: one map
+// entry per inner node holding the bit mask of its inner children, so that the
+// eight siblings a DFS step tests cost one map lookup (cached) instead of eight
+// scattered ones.  Inner-ness is prefix-closed (parents of inner nodes are inner).
+struct InnerSet {
+    FlatMap map;              // inner node -> mask of its inner children
+    size_t count = 0;
+    mutable u64 cacheKey = 0;
+    mutable u32 cacheMask = 0;
+    void init(size_t expected) { map.init(expected); }
+    bool contains(u64 loc) const {
+        if (loc == 1ull) return map.find(1ull) != nullptr;
+        const u64 parent = loc >> 3;
+        if (parent != cacheKey) {
+            const u32 *m = map.find(parent);
+            if (!m) return false;
+            cacheKey = parent;
+            cacheMask = *m;
+        }
+        return (cacheMask >> (loc & 7)) & 1u;
+    }
+    // loc's parent must be inner already (or loc is the root)
+    void insert(u64 loc) {
+        if (map.find(loc)) return;
+        map.put(loc, 0u);
+        ++count;
+        if (loc != 1ull) {
+            const u64 parent = loc >> 3;
+            const u32 *m = map.find(parent);
+            map.put(parent, (m ? *m : 0u) | (1u << (loc & 7)));
+            if (cacheKey == parent) cacheKey = 0;
+        }
+    }
+    size_t capacity() const { return map.keys.size(); }
+    // all keys of a prefix-closed set at once (several threads)
+    void build(const std::vector<u64> &keys) {
+        parallelFor(keys.size(), [&](size_t b, size_t e) {
+            for (size_t i = b; i < e; ++i) map.putIfAbsentConcurrent(keys[i], 0u);
+        });
+        parallelFor(keys.size(), [&](size_t b, size_t e) {
+            for (size_t i = b; i < e; ++i) {
+                const u64 k = keys[i];
+                if (k == 1ull) continue;
+                const u64 slot = map.putIfAbsentConcurrent(k >> 3, 0u);   // the parent is present
+                __atomic_fetch_or(&map.vals[slot], 1u << (k & 7), __ATOMIC_RELAXED);
+            }
+        });
+        count = keys.size();
     }
 };
 
 void replaySampling(const Frame &f, const tsr_scene *scene, float maxRel, const std::vector<u64> &t0Inner,
                     const std::vector<uint8_t> &isEvent, VisitIndex &index, std::vector<u64> &created,
                     u64 &eventCount) {
-    FlatSet t0, inner;
+    const auto tInit = std::chrono::steady_clock::now();
+    FlatSet t0;
+    InnerSet inner;
     t0.init(t0Inner.size());
     inner.init(t0Inner.size() * 2 + 1024);
-    for (u64 k : t0Inner) { t0.insert(k); inner.insert(k); }
+    parallelFor(t0Inner.size(), [&](size_t b, size_t e) {
+        for (size_t i = b; i < e; ++i) t0.insertConcurrent(t0Inner[i]);
+    });
+    inner.build(t0Inner);
     const u32 units = (u32)isEvent.size();
     std::vector<uint8_t> pending(isEvent.begin(), isEvent.end());
     std::priority_queue<u32, std::vector<u32>, std::greater<u32>> heap;
     for (u32 u = 0; u < units; ++u)
         if (pending[u]) heap.push(u);
+    const size_t initialEvents = heap.size();
+    u64 tainted = 0, creating = 0;
     auto isInner = [&](u64 loc) { return inner.contains(loc); };
     auto noLeaf = [](u64) {};
     std::vector<u32> tmp;
+    auto loadUnitHost = [&](u32 w, SamplingUnit &su) {
+        const u32 i = w >> 1;
+        su.p[0] = scene->hpx[i]; su.p[1] = scene->hpy[i]; su.p[2] = scene->hpz[i];
+        su.n[0] = scene->hnx[i]; su.n[1] = scene->hny[i]; su.n[2] = scene->hnz[i];
+        su.msd = (double)(scene->hscale[i] * maxRel);
+    };
     auto taint = [&](u64 subdivided, u32 writer) {
         // the T0 leaf containing the subdivided node
         u64 l = subdivided;
         while (l > 1 && !t0.contains(l >> 3)) l >>= 3;
-        index.units(l, tmp);
-        for (auto it = std::upper_bound(tmp.begin(), tmp.end(), writer); it != tmp.end(); ++it)
-            if (!pending[*it]) {
-                pending[*it] = 1;
-                heap.push(*it);
-            }
+        const u32 *first, *last;
+        index.units(l, first, last, tmp);
+        for (const u32 *it = std::upper_bound(first, last, writer); it != last; ++it) {
+            if (pending[*it]) continue;
+            ++tainted;
+            pending[*it] = 1;
+            heap.push(*it);
+        }
     };
     eventCount = 0;
+    const auto tLoop = std::chrono::steady_clock::now();
     while (!heap.empty()) {
         const u32 unit = heap.top();
         heap.pop();
         ++eventCount;
-        const u32 i = unit >> 1;
         const bool positive = (unit & 1) == 0;
         SamplingUnit u;
-        u.p[0] = scene->hpx[i]; u.p[1] = scene->hpy[i]; u.p[2] = scene->hpz[i];
-        u.n[0] = scene->hnx[i]; u.n[1] = scene->hny[i]; u.n[2] = scene->hnz[i];
-        u.msd = (double)(scene->hscale[i] * maxRel);
-        if (findReliable(f, 1ull, positive, u, isInner, noLeaf)) continue;
+        loadUnitHost(unit, u);
+        const u64 start = searchStart(f, u, isInner);
+        if (findReliable(f, start, positive, u, isInner, noLeaf)) continue;
+        ++creating;
         // Nodes.cpp:284-336
         const double sgn = positive ? u.msd : -u.msd;
         const double en[3] = {u.p[0] + u.n[0] * sgn, u.p[1] + u.n[1] * sgn, u.p[2] + u.n[2] * sgn};
@@ -572,10 +730,12 @@ void replaySampling(const Frame &f, const tsr_scene *scene, float maxRel, const 
         for (;;) {
             if (!inner.contains(n)) {
                 if (locDepth(n) + 1 > MAX_DEPTH) throw tsr_error(TSR_ERR_TREE_TOO_DEEP, "octree deeper than 19 levels");
-                if (inner.keys.size() < 2 * (created.size() + t0Inner.size() + 16)) {   // keep the load factor <= 1/2
-                    FlatSet grown;
-                    grown.init(2 * (created.size() + t0Inner.size()) + 1024);
-                    for (u64 k : inner.keys) if (k) grown.insert(k);
+                if (inner.capacity() < 2 * (inner.count + 16)) {   // keep the load factor <= 1/2
+                    InnerSet grown;
+                    grown.init(2 * inner.count + 1024);
+                    for (size_t s2 = 0; s2 < inner.map.keys.size(); ++s2)
+                        if (inner.map.keys[s2]) grown.map.put(inner.map.keys[s2], inner.map.vals[s2]);
+                    grown.count = inner.count;
                     inner = std::move(grown);
                 }
                 inner.insert(n);
@@ -605,6 +765,12 @@ void replaySampling(const Frame &f, const tsr_scene *scene, float maxRel, const 
             n = (n << 3) | best;
         }
     }
+    if (std::getenv("TSR_TRACE"))
+        std::fprintf(stderr, "[tsr tree]   replay set-up %.2f ms, in-order loop %.2f ms; initial events %zu, "
+                     "creating %llu, tainted %llu\n",
+                     std::chrono::duration<double, std::milli>(tLoop - tInit).count(),
+                     std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tLoop).count(),
+                     initialEvents, (unsigned long long)creating, (unsigned long long)tainted);
 }
 
 // T0 leaves within HOT_RADIUS x max sampling distance of a unit the speculation could not
@@ -661,14 +827,9 @@ __global__ void k_hotFlags(HashView hot, const u64 *__restrict__ leaf, size_t n,
 }
 
 // [first, last) of `key` in the sorted array (one thread)
-__global__ void k_equalRange(const u64 *__restrict__ keys, size_t n, u64 key, unsigned long long *out) {
-    size_t lo = 0, hi = n;
-    while (lo < hi) { const size_t m = (lo + hi) / 2; if (keys[m] < key) lo = m + 1; else hi = m; }
-    const size_t first = lo;
-    hi = n;
-    while (lo < hi) { const size_t m = (lo + hi) / 2; if (keys[m] <= key) lo = m + 1; else hi = m; }
-    out[0] = first;
-    out[1] = lo;
+__global__ void k_matchLeaf(const u64 *__restrict__ leaf, size_t n, u64 key, uint8_t *__restrict__ flag) {
+    const size_t j = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (j < n) flag[j] = leaf[j] == key ? 1 : 0;
 }
 
 void checkErr(u32 e) {
@@ -680,6 +841,37 @@ void checkErr(u32 e) {
 }
 
 }  // namespace
+
+void *pinnedHost(tsr_context *ctx, size_t bytes) {
+    if (bytes > ctx->pinnedBytes) {
+        if (ctx->pinned) HIPCHECK(hipHostFree(ctx->pinned));
+        ctx->pinned = nullptr;
+        ctx->pinnedBytes = 0;
+        size_t cap = (size_t)1 << 26;
+        while (cap < bytes) cap <<= 1;
+        HIPCHECK(hipHostMalloc(&ctx->pinned, cap, hipHostMallocDefault));
+        ctx->pinnedBytes = cap;
+    }
+    return ctx->pinned;
+}
+
+void parallelFor(size_t n, const std::function<void(size_t, size_t)> &fn) {
+    static const unsigned threads = [] {
+        unsigned t = std::max(1u, std::thread::hardware_concurrency());
+        if (const char *e = std::getenv("OMP_NUM_THREADS")) t = std::max(1, std::atoi(e));
+        return std::min(t, 16u);
+    }();
+    const size_t chunks = std::min<size_t>(threads, (n + 65535) / 65536);
+    if (chunks <= 1) {
+        fn(0, n);
+        return;
+    }
+    std::vector<std::thread> pool;
+    for (size_t c = 1; c < chunks; ++c)
+        pool.emplace_back([&, c] { fn(n * c / chunks, n * (c + 1) / chunks); });
+    fn(0, n / chunks);
+    for (auto &t : pool) t.join();
+}
 
 void buildTree(tsr_context *ctx, const tsr_scene *scene, const tsr_params &prm, tsr_tree *tree) {
     hipStream_t st = ctx->stream;
@@ -744,6 +936,14 @@ void buildTree(tsr_context *ctx, const tsr_scene *scene, const tsr_params &prm, 
     k_insertAncestors<<<gridFor(n, BLOCK), BLOCK, 0, st>>>(inner.view(), containerLoc.p, n, err.p);
     checkErr(readScalar(err.p, st));
 
+    u32 maxContainerDepth = 0;
+    {
+        DBuf<uint8_t> md(1);
+        size_t tmp = 0;
+        HIPCHECK(hipcub::DeviceReduce::Max(nullptr, tmp, cdepth.p, md.p, (int)n, st));
+        HIPCHECK(hipcub::DeviceReduce::Max(sc.get(tmp), tmp, cdepth.p, md.p, (int)n, st));
+        maxContainerDepth = readScalar(md.p, st);
+    }
     mark("containment");
     // ---- sampling refinement: speculative evaluation on T0
     const u32 units = 2 * n;
@@ -756,67 +956,106 @@ void buildTree(tsr_context *ctx, const tsr_scene *scene, const tsr_params &prm, 
     exclusiveScan(sc, visitCount.p, offsets.p, units, st);
     const u32 lastOff = readScalar(offsets.p + units - 1, st), lastCnt = readScalar(visitCount.p + units - 1, st);
     const size_t pairs = (size_t)lastOff + lastCnt;
-    DBuf<u64> pairLeaf(std::max<size_t>(pairs, 1)), pairLeafSorted(std::max<size_t>(pairs, 1));
-    DBuf<u32> pairUnit(std::max<size_t>(pairs, 1)), pairUnitSorted(std::max<size_t>(pairs, 1));
+    DBuf<u64> pairLeaf(std::max<size_t>(pairs, 1));
+    DBuf<u32> pairUnit(std::max<size_t>(pairs, 1));
     k_samplingEval<<<gridFor(units, BLOCK), BLOCK, 0, st>>>(f, inner.view(), scene->px.p, scene->py.p, scene->pz.p,
                                                             scene->nx.p, scene->ny.p, scene->nz.p, scene->scale.p,
                                                             maxRel, units, 1, visitCount.p, isEvent.p, offsets.p,
                                                             pairLeaf.p, pairUnit.p);
-    // pairs are emitted in unit order; a stable sort by leaf keeps units ascending per leaf
-    sortPairs(sc, pairLeaf.p, pairLeafSorted.p, pairUnit.p, pairUnitSorted.p, pairs, 0, 64, st);
-    pairLeaf.release();
-    pairUnit.release();
+    mark("speculation passes");
     // transfer only the part of the index around unsettled units
     DHash hot;
     hot.alloc(std::max<u64>(4096, (u64)units / 2), st);
     k_markHot<<<gridFor(units, BLOCK), BLOCK, 0, st>>>(f, inner.view(), hot.view(), scene->px.p, scene->py.p,
                                                        scene->pz.p, scene->scale.p, maxRel, isEvent.p, units, err.p);
-    DBuf<uint8_t> hotFlag(std::max<size_t>(pairs, 1));
-    DBuf<u64> hotLeaf(std::max<size_t>(pairs, 1));
-    DBuf<u32> hotUnit(std::max<size_t>(pairs, 1));
+    DBuf<uint8_t> pairFlag(std::max<size_t>(pairs, 1));
     DBuf<u32> counter(1);
     size_t hotPairs = 0;
+    DBuf<u64> hotLeafSorted;
+    DBuf<u32> hotUnitSorted;
     if (pairs) {
-        k_hotFlags<<<gridFor(pairs, BLOCK), BLOCK, 0, st>>>(hot.view(), pairLeafSorted.p, pairs, hotFlag.p);
+        DBuf<u64> hotLeaf(pairs);
+        DBuf<u32> hotUnit(pairs);
+        k_hotFlags<<<gridFor(pairs, BLOCK), BLOCK, 0, st>>>(hot.view(), pairLeaf.p, pairs, pairFlag.p);
         size_t tmp = 0;
-        HIPCHECK(hipcub::DeviceSelect::Flagged(nullptr, tmp, pairLeafSorted.p, hotFlag.p, hotLeaf.p, counter.p, (int)pairs, st));
-        HIPCHECK(hipcub::DeviceSelect::Flagged(sc.get(tmp), tmp, pairLeafSorted.p, hotFlag.p, hotLeaf.p, counter.p, (int)pairs, st));
-        HIPCHECK(hipcub::DeviceSelect::Flagged(nullptr, tmp, pairUnitSorted.p, hotFlag.p, hotUnit.p, counter.p, (int)pairs, st));
-        HIPCHECK(hipcub::DeviceSelect::Flagged(sc.get(tmp), tmp, pairUnitSorted.p, hotFlag.p, hotUnit.p, counter.p, (int)pairs, st));
+        HIPCHECK(hipcub::DeviceSelect::Flagged(nullptr, tmp, pairLeaf.p, pairFlag.p, hotLeaf.p, counter.p, (int)pairs, st));
+        HIPCHECK(hipcub::DeviceSelect::Flagged(sc.get(tmp), tmp, pairLeaf.p, pairFlag.p, hotLeaf.p, counter.p, (int)pairs, st));
+        HIPCHECK(hipcub::DeviceSelect::Flagged(nullptr, tmp, pairUnit.p, pairFlag.p, hotUnit.p, counter.p, (int)pairs, st));
+        HIPCHECK(hipcub::DeviceSelect::Flagged(sc.get(tmp), tmp, pairUnit.p, pairFlag.p, hotUnit.p, counter.p, (int)pairs, st));
         hotPairs = readScalar(counter.p, st);
+        // hot pairs are in unit order; a stable sort by leaf code keeps units ascending per
+        // leaf.  Visited leaves are T0 leaves, no deeper than the deepest container.
+        hotLeafSorted.alloc(std::max<size_t>(hotPairs, 1));
+        hotUnitSorted.alloc(std::max<size_t>(hotPairs, 1));
+        if (hotPairs) sortPairs(sc, hotLeaf.p, hotLeafSorted.p, hotUnit.p, hotUnitSorted.p, hotPairs, 0,
+                                (int)(3 * maxContainerDepth + 1), st);
     }
     checkErr(readScalar(err.p, st));
+    mark("hot pairs");
     std::vector<uint8_t> hEvent(units);
     VisitIndex index;
-    index.leaf.resize(hotPairs);
-    index.unit.resize(hotPairs);
     HIPCHECK(hipMemcpyAsync(hEvent.data(), isEvent.p, units, hipMemcpyDeviceToHost, st));
-    if (hotPairs) {
-        HIPCHECK(hipMemcpyAsync(index.leaf.data(), hotLeaf.p, hotPairs * 8, hipMemcpyDeviceToHost, st));
-        HIPCHECK(hipMemcpyAsync(index.unit.data(), hotUnit.p, hotPairs * 4, hipMemcpyDeviceToHost, st));
-    }
-    hotFlag.release(); hotLeaf.release(); hotUnit.release();
-    {   // the hot leaf set itself, so that hot leaves without visits need no lookup
+    {
+        // runs of equal leaves in the sorted hot pairs: unique leaf codes + run offsets + units
+        u32 runs = 0;
+        DBuf<u64> runLeaf(std::max<size_t>(hotPairs, 1));
+        DBuf<u32> runLen(std::max<size_t>(hotPairs, 1)), runOff(std::max<size_t>(hotPairs, 1) + 1);
+        if (hotPairs) {
+            size_t tmp = 0;
+            HIPCHECK(hipcub::DeviceRunLengthEncode::Encode(nullptr, tmp, hotLeafSorted.p, runLeaf.p, runLen.p,
+                                                           counter.p, (int)hotPairs, st));
+            HIPCHECK(hipcub::DeviceRunLengthEncode::Encode(sc.get(tmp), tmp, hotLeafSorted.p, runLeaf.p, runLen.p,
+                                                           counter.p, (int)hotPairs, st));
+            runs = readScalar(counter.p, st);
+            exclusiveScan(sc, runLen.p, runOff.p, runs, st);
+        }
+        // hot leaf codes, run leaves, run offsets and units through the pinned staging buffer
         DBuf<u64> hotKeys(hot.capacity);
-        HIPCHECK(hipMemsetAsync(counter.p, 0, 4, st));
+        HIPCHECK(hipMemsetAsync(counter.p + 0, 0, 4, st));
         k_compactKeys<<<gridFor(hot.capacity, BLOCK), BLOCK, 0, st>>>(hot.keys.p, hot.capacity, hotKeys.p, counter.p);
         const u32 nh = readScalar(counter.p, st);
-        std::vector<u64> hk(nh);
-        HIPCHECK(hipMemcpyAsync(hk.data(), hotKeys.p, (size_t)nh * 8, hipMemcpyDeviceToHost, st));
+        const size_t bytes = (size_t)nh * 8 + (size_t)runs * 8 + ((size_t)runs + 1) * 4 + hotPairs * 4;
+        char *pin = (char *)pinnedHost(ctx, bytes + 64);
+        u64 *hHot = (u64 *)pin;
+        u64 *hRunLeaf = hHot + nh;
+        u32 *hRunStart = (u32 *)(hRunLeaf + runs);
+        u32 *hUnit = hRunStart + runs + 1;
+        if (nh) HIPCHECK(hipMemcpyAsync(hHot, hotKeys.p, (size_t)nh * 8, hipMemcpyDeviceToHost, st));
+        if (runs) {
+            HIPCHECK(hipMemcpyAsync(hRunLeaf, runLeaf.p, (size_t)runs * 8, hipMemcpyDeviceToHost, st));
+            HIPCHECK(hipMemcpyAsync(hRunStart, runOff.p, (size_t)runs * 4, hipMemcpyDeviceToHost, st));
+            HIPCHECK(hipMemcpyAsync(hUnit, hotUnitSorted.p, hotPairs * 4, hipMemcpyDeviceToHost, st));
+        }
         HIPCHECK(hipStreamSynchronize(st));
-        index.hot.init(nh);
-        for (u64 k : hk) index.hot.insert(k);
+        hRunStart[runs] = (u32)hotPairs;
+        index.runStart = hRunStart;
+        index.unit = hUnit;
+        mark("hot index transfer");
+        if (trace) std::fprintf(stderr, "[tsr tree]   hot leaves %u, runs %u\n", nh, runs);
+        // run leaves first (-> run index), then the other hot leaves (-> none)
+        index.run.init(std::max<size_t>(nh, runs));
+        parallelFor(runs, [&](size_t b, size_t e) {
+            for (size_t r = b; r < e; ++r) index.run.putIfAbsentConcurrent(hRunLeaf[r], (u32)r);
+        });
+        parallelFor(nh, [&](size_t b, size_t e) {
+            for (size_t i = b; i < e; ++i) index.run.putIfAbsentConcurrent(hHot[i], INVALID);
+        });
     }
-    DBuf<unsigned long long> range(2);
+    hotLeafSorted.release();
+    hotUnitSorted.release();
+    mark("hot map (host)");
+    // a visited leaf outside the hot part: select its units from the full (unit-ordered) pair list
     index.fetch = [&](u64 loc, std::vector<u32> &out) {
+        out.clear();
         if (!pairs) return;
-        k_equalRange<<<1, 1, 0, st>>>(pairLeafSorted.p, pairs, loc, range.p);
-        unsigned long long r[2];
-        HIPCHECK(hipMemcpyAsync(r, range.p, sizeof(r), hipMemcpyDeviceToHost, st));
-        HIPCHECK(hipStreamSynchronize(st));
-        out.resize(r[1] - r[0]);
-        if (r[1] > r[0])
-            HIPCHECK(hipMemcpyAsync(out.data(), pairUnitSorted.p + r[0], (r[1] - r[0]) * 4, hipMemcpyDeviceToHost, st));
+        k_matchLeaf<<<gridFor(pairs, BLOCK), BLOCK, 0, st>>>(pairLeaf.p, pairs, loc, pairFlag.p);
+        DBuf<u32> all(pairs);
+        size_t tmp = 0;
+        HIPCHECK(hipcub::DeviceSelect::Flagged(nullptr, tmp, pairUnit.p, pairFlag.p, all.p, counter.p, (int)pairs, st));
+        HIPCHECK(hipcub::DeviceSelect::Flagged(sc.get(tmp), tmp, pairUnit.p, pairFlag.p, all.p, counter.p, (int)pairs, st));
+        const u32 m = readScalar(counter.p, st);
+        out.resize(m);
+        if (m) HIPCHECK(hipMemcpyAsync(out.data(), all.p, (size_t)m * 4, hipMemcpyDeviceToHost, st));
         HIPCHECK(hipStreamSynchronize(st));
     };
     // T0 inner node list
@@ -825,7 +1064,12 @@ void buildTree(tsr_context *ctx, const tsr_scene *scene, const tsr_params &prm, 
     k_compactKeys<<<gridFor(inner.capacity, BLOCK), BLOCK, 0, st>>>(inner.keys.p, inner.capacity, innerList.p, counter.p);
     const u32 t0Count = readScalar(counter.p, st);
     std::vector<u64> hInner(t0Count);
-    HIPCHECK(hipMemcpyAsync(hInner.data(), innerList.p, (size_t)t0Count * 8, hipMemcpyDeviceToHost, st));
+    if (t0Count) {   // ascending codes = ascending depth: the host inserts parents first
+        DBuf<u64> sortedInner(t0Count);
+        sortKeys(sc, innerList.p, sortedInner.p, t0Count, 0, (int)(3 * maxContainerDepth + 1), st);
+        HIPCHECK(hipMemcpyAsync(hInner.data(), sortedInner.p, (size_t)t0Count * 8, hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+    }
     HIPCHECK(hipStreamSynchronize(st));
     mark("sampling speculation");
     std::vector<u64> created;
@@ -835,8 +1079,9 @@ void buildTree(tsr_context *ctx, const tsr_scene *scene, const tsr_params &prm, 
         std::fprintf(stderr, "[tsr tree] units %u pairs %zu hot pairs %zu t0-inner %u events %llu created nodes %zu "
                      "index misses %llu\n", units, pairs, hotPairs, t0Count, (unsigned long long)events,
                      8 * created.size(), (unsigned long long)index.misses);
-    pairLeafSorted.release();
-    pairUnitSorted.release();
+    pairLeaf.release();
+    pairUnit.release();
+    pairFlag.release();
     mark("sampling replay (host)");
     tree->samplingEvents = events;
     tree->samplingCreated = 8 * (u64)created.size();
