@@ -94,8 +94,9 @@ struct tsr_mesh {
 namespace tsr {
 // pinned host memory of the context, at least `bytes` (contents not preserved on growth)
 void *pinnedHost(tsr_context *ctx, size_t bytes);
-// runs fn(begin, end) over [0, n) on up to 16 host threads (OMP_NUM_THREADS caps it)
-void parallelFor(size_t n, const std::function<void(size_t, size_t)> &fn);
+// runs fn(begin, end) over [0, n) on up to 16 host threads (OMP_NUM_THREADS caps it),
+// at least `grain` items per thread
+void parallelFor(size_t n, const std::function<void(size_t, size_t)> &fn, size_t grain = 65536);
 enum { KSTAT_PRIORS = 0, KSTAT_E_COUNT = 1, KSTAT_E_WRITE = 2, KSTAT_S_COUNT = 3, KSTAT_S_WRITE = 4 };
 // time one kernel launch on the context stream with HIP events (synchronises)
 template <typename Launch>

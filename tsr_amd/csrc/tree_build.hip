@@ -670,6 +670,24 @@ struct InnerSet {
     }
 };
 
+// read-only view of an InnerSet with its own one-entry cache (one per thread)
+struct InnerView {
+    const FlatMap *map;
+    u64 cacheKey = 0;
+    u32 cacheMask = 0;
+    bool contains(u64 loc) {
+        if (loc == 1ull) return map->find(1ull) != nullptr;
+        const u64 parent = loc >> 3;
+        if (parent != cacheKey) {
+            const u32 *m = map->find(parent);
+            if (!m) return false;
+            cacheKey = parent;
+            cacheMask = *m;
+        }
+        return (cacheMask >> (loc & 7)) & 1u;
+    }
+};
+
 void replaySampling(const Frame &f, const tsr_scene *scene, float maxRel, const std::vector<u64> &t0Inner,
                     const std::vector<uint8_t> &isEvent, VisitIndex &index, std::vector<u64> &created,
                     u64 &eventCount) {
@@ -698,31 +716,52 @@ void replaySampling(const Frame &f, const tsr_scene *scene, float maxRel, const 
         su.n[0] = scene->hnx[i]; su.n[1] = scene->hny[i]; su.n[2] = scene->hnz[i];
         su.msd = (double)(scene->hscale[i] * maxRel);
     };
+    u64 candidates = 0;
+    double taintMs = 0, fullMs = 0, specMs = 0, createMs = 0;
+    using clock = std::chrono::steady_clock;
+    auto msSince = [](clock::time_point t) { return std::chrono::duration<double, std::milli>(clock::now() - t).count(); };
     auto taint = [&](u64 subdivided, u32 writer) {
+        const auto tt = clock::now();
         // the T0 leaf containing the subdivided node
         u64 l = subdivided;
         while (l > 1 && !t0.contains(l >> 3)) l >>= 3;
         const u32 *first, *last;
         index.units(l, first, last, tmp);
-        for (const u32 *it = std::upper_bound(first, last, writer); it != last; ++it) {
+        const u32 *from = std::upper_bound(first, last, writer);
+        candidates += (u64)(last - from);
+        for (const u32 *it = from; it != last; ++it) {
             if (pending[*it]) continue;
             ++tainted;
             pending[*it] = 1;
             heap.push(*it);
         }
+        taintMs += msSince(tt);
     };
     eventCount = 0;
-    const auto tLoop = std::chrono::steady_clock::now();
-    while (!heap.empty()) {
-        const u32 unit = heap.top();
-        heap.pop();
+    // nodes made inner since the current batch was speculated
+    FlatSet createdSince;
+    std::vector<u64> createdList;
+    createdSince.init(1u << 15);
+    // the sequential semantics: evaluate `unit` on the current tree, refine if needed
+    auto processFull = [&](u32 unit, bool knownToFail) {
         ++eventCount;
+        const auto tf = clock::now();
+        struct Acc {
+            double &ms;
+            clock::time_point t;
+            ~Acc() { ms += std::chrono::duration<double, std::milli>(clock::now() - t).count(); }
+        } acc{fullMs, tf};
         const bool positive = (unit & 1) == 0;
         SamplingUnit u;
         loadUnitHost(unit, u);
-        const u64 start = searchStart(f, u, isInner);
-        if (findReliable(f, start, positive, u, isInner, noLeaf)) continue;
+        if (!knownToFail && findReliable(f, searchStart(f, u, isInner), positive, u, isInner, noLeaf)) return;
         ++creating;
+        const auto tc = clock::now();
+        struct AccC {
+            double &ms;
+            clock::time_point t;
+            ~AccC() { ms += std::chrono::duration<double, std::milli>(clock::now() - t).count(); }
+        } accC{createMs, tc};
         // Nodes.cpp:284-336
         const double sgn = positive ? u.msd : -u.msd;
         const double en[3] = {u.p[0] + u.n[0] * sgn, u.p[1] + u.n[1] * sgn, u.p[2] + u.n[2] * sgn};
@@ -740,6 +779,13 @@ void replaySampling(const Frame &f, const tsr_scene *scene, float maxRel, const 
                 }
                 inner.insert(n);
                 created.push_back(n);
+                if (createdList.size() * 2 + 2 > createdSince.keys.size()) {
+                    FlatSet grown;
+                    grown.init(createdList.size() * 2 + 16);
+                    for (u64 k : createdList) grown.insert(k);
+                    createdSince = std::move(grown);
+                }
+                if (createdSince.insert(n)) createdList.push_back(n);
                 taint(n, unit);
                 if (findReliable(f, n, positive, u, isInner, noLeaf)) break;
             }
@@ -764,13 +810,83 @@ void replaySampling(const Frame &f, const tsr_scene *scene, float maxRel, const 
             if (best == INVALID) throw tsr_error(TSR_ERR_INVALID_ARGUMENT, "sampling refinement: segment leaves the node");
             n = (n << 3) | best;
         }
+    };
+    // Batches of the next pending units (in order) are first evaluated in parallel on
+    // the tree as it is when the batch starts, recording the leaves each DFS visits.
+    // Committing in order, a unit whose parallel result was "reliable leaf found" and
+    // none of whose visited leaves has been subdivided since is settled: a DFS only
+    // changes when a leaf it visits becomes an inner node.  Every other unit, and
+    // units made pending meanwhile that precede the rest of the batch, run the
+    // sequential step above.
+    constexpr u32 BATCH = 4096, SPEC_CAP = 48;
+    struct Spec {
+        u32 state;   // 1 = reliable leaf found, 2 = none (refinement needed), 0 = visit cap exceeded
+        u32 count;
+        u64 visited[SPEC_CAP];
+    };
+    std::vector<u32> batch;
+    std::vector<Spec> spec(BATCH);
+    u64 speculated = 0;
+    const auto tLoop = std::chrono::steady_clock::now();
+    while (!heap.empty()) {
+        batch.clear();
+        while (!heap.empty() && batch.size() < BATCH) {
+            batch.push_back(heap.top());
+            heap.pop();
+        }
+        const auto ts = clock::now();
+        parallelFor(batch.size(), [&](size_t b, size_t e) {
+            InnerView view{&inner.map};
+            auto inView = [&](u64 loc) { return view.contains(loc); };
+            for (size_t j = b; j < e; ++j) {
+                Spec &sp = spec[j];
+                sp.count = 0;
+                bool over = false;
+                auto record = [&](u64 loc) {
+                    if (sp.count < SPEC_CAP) sp.visited[sp.count++] = loc;
+                    else over = true;
+                };
+                SamplingUnit su;
+                loadUnitHost(batch[j], su);
+                const bool positive = (batch[j] & 1) == 0;
+                const bool ok = findReliable(f, searchStart(f, su, inView), positive, su, inView, record);
+                sp.state = over ? 0u : ok ? 1u : 2u;
+            }
+        }, 64);
+        specMs += msSince(ts);
+        if (!createdList.empty()) {   // a new batch: nothing created since it was speculated
+            std::fill(createdSince.keys.begin(), createdSince.keys.end(), 0ull);
+            createdList.clear();
+        }
+        for (size_t j = 0; j < batch.size(); ++j) {
+            const u32 u = batch[j];
+            while (!heap.empty() && heap.top() < u) {   // made pending after the batch was formed
+                const u32 v = heap.top();
+                heap.pop();
+                processFull(v, false);
+            }
+            const Spec &sp = spec[j];
+            bool valid = sp.state != 0u;
+            for (u32 i = 0; valid && i < sp.count && !createdList.empty(); ++i)
+                if (createdSince.contains(sp.visited[i])) valid = false;
+            if (valid && sp.state == 1u) {
+                ++eventCount;
+                ++speculated;
+                continue;
+            }
+            processFull(u, valid);   // valid here: the DFS is known to find no reliable leaf
+        }
     }
     if (std::getenv("TSR_TRACE"))
         std::fprintf(stderr, "[tsr tree]   replay set-up %.2f ms, in-order loop %.2f ms; initial events %zu, "
-                     "creating %llu, tainted %llu\n",
+                     "creating %llu, tainted %llu, settled by parallel speculation %llu\n"
+                     "[tsr tree]   parallel speculation %.1f ms, sequential steps %.1f ms (refinement %.1f ms, "
+                     "taints %.1f ms over %llu candidates)\n",
                      std::chrono::duration<double, std::milli>(tLoop - tInit).count(),
                      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tLoop).count(),
-                     initialEvents, (unsigned long long)creating, (unsigned long long)tainted);
+                     initialEvents, (unsigned long long)creating, (unsigned long long)tainted,
+                     (unsigned long long)speculated, specMs, fullMs, createMs, taintMs,
+                     (unsigned long long)candidates);
 }
 
 // T0 leaves within HOT_RADIUS x max sampling distance of a unit the speculation could not
@@ -855,13 +971,13 @@ void *pinnedHost(tsr_context *ctx, size_t bytes) {
     return ctx->pinned;
 }
 
-void parallelFor(size_t n, const std::function<void(size_t, size_t)> &fn) {
+void parallelFor(size_t n, const std::function<void(size_t, size_t)> &fn, size_t grain) {
     static const unsigned threads = [] {
         unsigned t = std::max(1u, std::thread::hardware_concurrency());
         if (const char *e = std::getenv("OMP_NUM_THREADS")) t = std::max(1, std::atoi(e));
         return std::min(t, 16u);
     }();
-    const size_t chunks = std::min<size_t>(threads, (n + 65535) / 65536);
+    const size_t chunks = std::min<size_t>(threads, (n + grain - 1) / std::max<size_t>(grain, 1));
     if (chunks <= 1) {
         fn(0, n);
         return;
