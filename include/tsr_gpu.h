@@ -114,8 +114,9 @@ tsr_status tsr_context_phase_times(tsr_context *ctx, float *ms, int n);
 /* Accumulated live timing (HIP events on the context stream) of instrumented
  * kernels since the last reset, with their algorithmic bytes (each input or
  * output element counted once per launch) and work units (queries or links):
- * kernel 0 = priors sphere query, 1/2 = emptiness cone pass count/write,
- * 3/4 = sampleness cone pass count/write. */
+ * kernel 0 = priors sphere query, 1/2 = emptiness cone pass (one traversal per link,
+ * staging its records) / re-traversal of the links with more records than staging
+ * slots, 3/4 = the same for the sampleness cones. */
 tsr_status tsr_context_kernel_stats(tsr_context *ctx, int kernel, double *ms, uint64_t *launches, uint64_t *bytes,
                                     uint64_t *units);
 tsr_status tsr_context_reset_kernel_stats(tsr_context *ctx);

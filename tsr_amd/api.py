@@ -52,8 +52,8 @@ class Context:
         return ms
 
 
-    KERNELS = {"priors": 0, "emptiness_count": 1, "emptiness_write": 2, "sampleness_count": 3,
-               "sampleness_write": 4}
+    KERNELS = {"priors": 0, "emptiness_pass": 1, "emptiness_overflow": 2, "sampleness_pass": 3,
+               "sampleness_overflow": 4}
 
     def kernel_stats(self, name):
         """(ms, launches, algorithmic bytes, units) accumulated for an instrumented kernel."""

@@ -77,6 +77,7 @@ struct tsr_field {
     tsr::u32 priorQueries = 0;
     tsr::DBuf<tsr::u32> hitsS;   // sampleness hits per leaf (features of the learned culling)
     tsr::u32 culledNodes = 0, culledLeaves = 0;
+    tsr::u32 overflowLinks[2] = {0, 0};   // links traversed twice (more hits than staging slots), E / S
     tsr::DBuf<uint8_t> culled;   // per leaf: 1 if under a learned-culled node (learned mode only)
     tsr_params params{};
 };

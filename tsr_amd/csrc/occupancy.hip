@@ -16,6 +16,7 @@
 // stably radix-sorted by leaf, and each leaf's records are summed in FP32 with
 // Kahan compensation in that fixed order: deterministic, no float atomics.
 #include <algorithm>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <vector>
@@ -45,6 +46,12 @@ inline unsigned xcdGrid(size_t n, unsigned block) {
     return (g + XCD_COUNT - 1) / XCD_COUNT * XCD_COUNT;
 }
 
+// staging slots per link of the single-traversal record pass (as many as a memory
+// budget allows, at most CONE_SLOTS_MAX); links with more accepted hits are traversed
+// again (mode 1) into their final place
+constexpr u32 CONE_SLOTS_MAX = 256;
+constexpr size_t CONE_STAGING_BUDGET = (size_t)24 << 30;   // bytes
+
 struct TraversalTree {
     const u32 *child;      // first child id, or LEAF_BIT | leaf index
     const u32 *parents;
@@ -64,6 +71,7 @@ struct LinkInputs {
     int xcdMap;
     u32 linkStride;            // process every linkStride-th link (1 = all)
     const u32 *order;          // thread index -> link (links grouped by camera), may be null
+    u32 slots;                 // mode 3: staging slots per link, slot j of thread i at j * links + i
 };
 
 // Cone of link `link` (Occupancy.cpp:174-201, getCameraToSampleLinkData :250-277)
@@ -141,7 +149,8 @@ __device__ void traverseCone(const TraversalTree &t, const ConeCull &cull, u32 m
     }
 }
 
-__device__ void coneLink(const TraversalTree &t, const LinkInputs &in, u32 link, int sampleness, int mode,
+__device__ void coneLink(const TraversalTree &t, const LinkInputs &in, u32 link, u32 idx, u32 linkCount,
+                         int sampleness, int mode,
                          u32 *__restrict__ counts, const u32 *__restrict__ offsets, u32 *__restrict__ recLeaf,
                          uint2 *__restrict__ recVal, float *__restrict__ atomicSum, float *__restrict__ atomicLen,
                          u32 *__restrict__ atomicStates, double *__restrict__ confOut, u32 &visits, u32 &count);
@@ -153,18 +162,19 @@ __device__ void waveAccumulate(unsigned long long *acc, u32 v) {
     if ((threadIdx.x & 63) == 0) atomicAdd(acc, x);
 }
 
-// mode 0: count accepted hits per link; 1: write records; 2: FP32 atomics (precision study);
-// 4: hit count per leaf
+// mode 0: count accepted hits per link; 1: write records at offsets[link]; 2: FP32 atomics
+// (precision study); 3: count, and write the first in.slots records into the thread's
+// staging slots (interleaved: record j at j * linkCount + thread index); 4: hit count per leaf
 // stats[0] += node visits, stats[1] += accepted hits
 __global__ void k_conePass(TraversalTree t, LinkInputs in, u32 linkCount, int sampleness, int mode,
                            u32 *__restrict__ counts, const u32 *__restrict__ offsets, u32 *__restrict__ recLeaf,
                            uint2 *__restrict__ recVal, float *__restrict__ atomicSum, float *__restrict__ atomicLen,
                            u32 *__restrict__ atomicStates, double *__restrict__ confOut,
                            unsigned long long *__restrict__ stats) {
-    const u32 idx = (xcdBlock(in.xcdMap) * BLOCK + threadIdx.x) * in.linkStride;
+    const u32 idx = (xcdBlock(in.xcdMap) * blockDim.x + threadIdx.x) * in.linkStride;
     u32 visits = 0, count = 0;
     const u32 link = idx < linkCount && in.order ? in.order[idx] : idx;
-    if (idx < linkCount) coneLink(t, in, link, sampleness, mode, counts, offsets, recLeaf, recVal, atomicSum,
+    if (idx < linkCount) coneLink(t, in, link, idx, linkCount, sampleness, mode, counts, offsets, recLeaf, recVal, atomicSum,
                                    atomicLen, atomicStates, confOut, visits, count);
     if (stats) {
         waveAccumulate(&stats[0], visits);
@@ -172,16 +182,17 @@ __global__ void k_conePass(TraversalTree t, LinkInputs in, u32 linkCount, int sa
     }
 }
 
-__device__ void coneLink(const TraversalTree &t, const LinkInputs &in, u32 link, int sampleness, int mode,
+__device__ void coneLink(const TraversalTree &t, const LinkInputs &in, u32 link, u32 idx, u32 linkCount,
+                         int sampleness, int mode,
                          u32 *__restrict__ counts, const u32 *__restrict__ offsets, u32 *__restrict__ recLeaf,
                          uint2 *__restrict__ recVal, float *__restrict__ atomicSum, float *__restrict__ atomicLen,
                          u32 *__restrict__ atomicStates, double *__restrict__ confOut, u32 &visits, u32 &count) {
     ConeD cone;
     u32 i;
     const bool ok = setupLink(in, link, sampleness != 0, cone, i);
-    if (mode == 0 && confOut) confOut[link] = ok ? (double)in.conf[i] : 0.0;
+    if ((mode == 0 || mode == 3) && confOut) confOut[link] = ok ? (double)in.conf[i] : 0.0;
     if (!ok) {
-        if (mode == 0) counts[link] = 0;
+        if (mode == 0 || mode == 3) counts[link] = 0;
         return;
     }
     ConeCull cull;
@@ -190,13 +201,14 @@ __device__ void coneLink(const TraversalTree &t, const LinkInputs &in, u32 link,
     const float weightedLength = (float)(conf * cone.length);
     const u32 maxDepth = (u32)in.sampleDepth[i] + MAX_DEPTH_DIFFERENCE;
     const u32 required = sampleness ? 0u : FLAG_S;
-    const u32 base = mode == 1 ? offsets[link] : 0;
+    const size_t base = mode == 1 ? (size_t)offsets[link] : 0;
     traverseCone(t, cull, maxDepth, required, sampleness ? nullptr : in.cullNode, visits, [&](u32 leaf, const double x[3]) {
         const double kv = coneKernel(cone, x, sampleness != 0, conf);
         if (kv == 0.0) return;
-        if (mode == 1) {
-            recLeaf[base + count] = leaf;
-            recVal[base + count] = make_uint2(__float_as_uint((float)kv), __float_as_uint(weightedLength));
+        if (mode == 1 || (mode == 3 && count < in.slots)) {
+            const size_t at = mode == 1 ? base + count : (size_t)count * linkCount + idx;
+            recLeaf[at] = leaf;
+            recVal[at] = make_uint2(__float_as_uint((float)kv), __float_as_uint(weightedLength));
         } else if (mode == 2) {
             atomicAdd(&atomicSum[leaf], (float)kv);
             atomicAdd(&atomicLen[leaf], weightedLength);
@@ -206,7 +218,31 @@ __device__ void coneLink(const TraversalTree &t, const LinkInputs &in, u32 link,
         }
         ++count;
     });
-    if (mode == 0) counts[link] = count;
+    if (mode == 0 || mode == 3) counts[link] = count;
+}
+
+// staged records of the links that fit their slots -> link order (offsets = exclusive
+// scan of the counts); a link's records stay in traversal order.  Thread i moves the
+// records staged by thread i (slot j at j * links + i; coalesced reads).
+__global__ void k_gatherSlots(const u32 *__restrict__ counts, const u32 *__restrict__ offsets,
+                              const u32 *__restrict__ order, u32 links, u32 slots, const u32 *__restrict__ stLeaf,
+                              const uint2 *__restrict__ stVal, u32 *__restrict__ recLeaf,
+                              uint2 *__restrict__ recVal) {
+    const u32 i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= links) return;
+    const u32 l = order ? order[i] : i;
+    const u32 c = counts[l];
+    if (c > slots) return;
+    const size_t dst = offsets[l];
+    for (u32 j = 0; j < c; ++j) {
+        recLeaf[dst + j] = stLeaf[(size_t)j * links + i];
+        recVal[dst + j] = stVal[(size_t)j * links + i];
+    }
+}
+
+__global__ void k_overflowFlags(const u32 *__restrict__ counts, u32 links, u32 slots, u32 *__restrict__ flag) {
+    const u32 l = blockIdx.x * BLOCK + threadIdx.x;
+    if (l < links) flag[l] = counts[l] > slots ? 1u : 0u;
 }
 
 // first record of every leaf segment in the sorted record stream
@@ -622,10 +658,19 @@ __global__ __launch_bounds__(64) void k_priorsGroup(Frame f, const PriorNode *__
     __shared__ u32 stNode[PRIOR_STACK];
     __shared__ uint8_t stMask[PRIOR_STACK];
     __shared__ double qc[GROUP_Q][3], qr2[GROUP_Q], qri2[GROUP_Q];
+    __shared__ float qcF[GROUP_Q][3], qr2F[GROUP_Q], qri2F[GROUP_Q], qmarg[GROUP_Q];
     __shared__ u32 qmd[GROUP_Q];
     __shared__ u32 sp;
     const u32 lane = threadIdx.x;
     u32 nPops = 0, nTests = 0;
+    // FP32 pre-test of the FP64 sphere predicates: a coordinate difference rounded to
+    // float (inputs and result) is off by at most e = 4 * 2^-24 * M, M bounding every
+    // |coordinate|; the squared distance near radius r then by less than the margin
+    // 4 e r + 4 e^2 + 2^-20 r^2.  Outside the margin band the FP32 result decides,
+    // inside it the FP64 test does.
+    double M = 0.0;
+    for (int a = 0; a < 3; ++a) M = fmax(M, fabs(f.rootMin[a]) + f.rootSize);
+    const double eps = 4.0 * ldexp(M, -24);
     const u32 groups = (queryCount + GROUP_Q - 1) / GROUP_Q;
     u32 gBegin, gEnd, gStride, gFirst;
     xcdRange(groups, xcdMap, gBegin, gEnd, gStride, gFirst);
@@ -637,10 +682,16 @@ __global__ __launch_bounds__(64) void k_priorsGroup(Frame f, const PriorNode *__
             u32 kq[3];
             locCoords(qloc, kq[0], kq[1], kq[2]);
             const u32 dq = locDepth(qloc);
-            for (int a = 0; a < 3; ++a) qc[lane][a] = f.centre(a, kq[a], dq);
+            for (int a = 0; a < 3; ++a) {
+                qc[lane][a] = f.centre(a, kq[a], dq);
+                qcF[lane][a] = (float)qc[lane][a];
+            }
             const double radius = f.h(dq) * (double)relRadius;
             qr2[lane] = radius * radius;
             qri2[lane] = radius * radius * (1.0 - 1e-9);
+            qr2F[lane] = (float)qr2[lane];
+            qri2F[lane] = (float)qri2[lane];
+            qmarg[lane] = (float)(4.0 * eps * radius + 4.0 * eps * eps + ldexp(radius * radius, -20));
             qmd[lane] = dq + MAX_DEPTH_DIFFERENCE;
         }
         double accE[GROUP_Q], accS[GROUP_Q];
@@ -685,30 +736,56 @@ __global__ __launch_bounds__(64) void k_priorsGroup(Frame f, const PriorNode *__
                     u32 k[3];
                     locCoords((pn.loc << 3) | slot, k[0], k[1], k[2]);
                     double lo[3], hi[3];
+                    float loF[3], hiF[3];
                     for (int a = 0; a < 3; ++a) {
                         lo[a] = f.B(a, k[a], d);
                         hi[a] = f.B(a, (u64)k[a] + 1, d);
+                        loF[a] = (float)lo[a];
+                        hiF[a] = (float)hi[a];
                     }
                     const bool leaf = (cn.child & LEAF_BIT) != 0;
                     const u32 mld = cn.aux >> 8;
 #pragma unroll
                     for (u32 j = 0; j < GROUP_Q; ++j) {
                         if (!((mask >> j) & 1u) || d > qmd[j]) continue;
-                        double dmin = 0.0, dmax = 0.0;
-                        for (int q3 = 0; q3 < 3; ++q3) {   // SphereNodesChecker.cpp:29-40
-                            const double c = qc[j][q3];
-                            if (c < lo[q3]) dmin += (lo[q3] - c) * (lo[q3] - c);
-                            else if (c > hi[q3]) dmin += (c - hi[q3]) * (c - hi[q3]);
-                            const double far = fmax(fabs(c - lo[q3]), fabs(hi[q3] - c));
-                            dmax += far * far;
+                        float dminF = 0.f, dmaxF = 0.f;
+                        for (int q3 = 0; q3 < 3; ++q3) {
+                            const float c = qcF[j][q3];
+                            const float a0 = loF[q3] - c, a1 = c - hiF[q3];
+                            const float o = fmaxf(0.f, fmaxf(a0, a1));
+                            dminF += o * o;
+                            const float far = fmaxf(fabsf(a0), fabsf(a1));
+                            dmaxF += far * far;
                         }
-                        if (dmin <= qr2[j]) {
-                            if (leaf || (dmax <= qri2[j] && mld <= qmd[j])) {
-                                accE[j] += (double)cn.sums.x + (double)cn.sums.y;
-                                accS[j] += (double)cn.sums.z + (double)cn.sums.w;
-                            } else {
-                                pushMask |= (uint8_t)(1u << j);
+                        const float mg = qmarg[j];
+                        if (dminF > qr2F[j] + mg) continue;   // surely outside
+                        if (!(dminF < qr2F[j] - mg)) {        // undecided: the FP64 test
+                            double dmin = 0.0;
+                            for (int q3 = 0; q3 < 3; ++q3) {   // SphereNodesChecker.cpp:29-40
+                                const double c = qc[j][q3];
+                                if (c < lo[q3]) dmin += (lo[q3] - c) * (lo[q3] - c);
+                                else if (c > hi[q3]) dmin += (c - hi[q3]) * (c - hi[q3]);
                             }
+                            if (!(dmin <= qr2[j])) continue;
+                        }
+                        bool whole = leaf;
+                        if (!whole && mld <= qmd[j]) {
+                            if (dmaxF < qri2F[j] - mg) whole = true;
+                            else if (!(dmaxF > qri2F[j] + mg)) {
+                                double dmax = 0.0;
+                                for (int q3 = 0; q3 < 3; ++q3) {
+                                    const double c = qc[j][q3];
+                                    const double far = fmax(fabs(c - lo[q3]), fabs(hi[q3] - c));
+                                    dmax += far * far;
+                                }
+                                whole = dmax <= qri2[j];
+                            }
+                        }
+                        if (whole) {
+                            accE[j] += (double)cn.sums.x + (double)cn.sums.y;
+                            accS[j] += (double)cn.sums.z + (double)cn.sums.w;
+                        } else {
+                            pushMask |= (uint8_t)(1u << j);
                         }
                     }
                 }
@@ -766,6 +843,10 @@ __global__ void k_occupancy(const float *__restrict__ ksum, const float *__restr
 __global__ void k_fillF(float *p, size_t n, float v) {
     const size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x;
     if (i < n) p[i] = v;
+}
+__global__ void k_iotaU(u32 *p, u32 n) {
+    const u32 i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) p[i] = i;
 }
 __global__ void k_fillU(u32 *p, size_t n, u32 v) {
     const size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x;
@@ -844,6 +925,7 @@ void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *s
     in.xcdMap = xcdMapMode() & 1;
     in.linkStride = 1;
     in.order = nullptr;
+    in.slots = 0;
     field->params = prm;
     // E-pass threads take links grouped by camera, samples in node (Morton) order within
     // a camera: neighbouring lanes trace nearly parallel cones through the same nodes.
@@ -906,20 +988,26 @@ void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *s
             HIPCHECK(hipcub::DeviceReduce::Sum(sc.get(tmp), tmp, counts.p, offsets.p, (int)links, st));
             field->hits[type] = readScalar(offsets.p, st);
         } else {
-            // algorithmic bytes per link: sample SoA 37 B (position, normal, scale, confidence, camera
-            // link, depth) + 4 B count; per visited node 4 B child link (+4 B state in the E pass)
+            // One traversal counts every link's accepted hits and stages its first in.slots
+            // records; links with more are traversed again, writing at their final offsets.
+            // Algorithmic bytes per link: sample SoA 37 B (position, normal, scale, confidence,
+            // camera link, depth) + 4 B count; per visited node 4 B child link (+4 B state in
+            // the E pass); 12 B per staged record.
             const u64 nodeBytes = sampleness ? 4 : 8;
+            in.slots = (u32)std::min<size_t>(CONE_SLOTS_MAX, CONE_STAGING_BUDGET / (12 * (size_t)std::max(links, 1u)));
+            DBuf<u32> stLeaf(std::max<size_t>((size_t)links * in.slots, 1));
+            DBuf<uint2> stVal(std::max<size_t>((size_t)links * in.slots, 1));
             HIPCHECK(hipMemsetAsync(kst.p, 0, 32, st));
             timedLaunch(ctx, sampleness ? KSTAT_S_COUNT : KSTAT_E_COUNT, [&] {
-                k_conePass<<<xcdGrid(links, BLOCK), BLOCK, 0, st>>>(tt, in, links, sampleness, 0, counts.p, nullptr,
-                                                                    nullptr, nullptr, nullptr, nullptr, nullptr,
+                k_conePass<<<xcdGrid(links, BLOCK), BLOCK, 0, st>>>(tt, in, links, sampleness, 3, counts.p, nullptr,
+                                                                    stLeaf.p, stVal.p, nullptr, nullptr, nullptr,
                                                                     sampleness ? nullptr : confs.p, kst.p);
             });
             unsigned long long cs[2];
             readStats(2, cs);
             {
                 tsr_kernel_stat &k = ctx->kstat[sampleness ? KSTAT_S_COUNT : KSTAT_E_COUNT];
-                k.bytes += (u64)links * 41 + cs[0] * nodeBytes;
+                k.bytes += (u64)links * 41 + cs[0] * nodeBytes + cs[1] * 12;
                 k.units += links;
             }
             exclusiveScan(sc, counts.p, offsets.p, links, st);
@@ -928,17 +1016,40 @@ void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *s
             field->hits[type] = total;
             DBuf<u32> recLeaf(std::max<u64>(total, 1)), recLeafS(std::max<u64>(total, 1));
             DBuf<uint2> recVal(std::max<u64>(total, 1)), recValS(std::max<u64>(total, 1));
-            HIPCHECK(hipMemsetAsync(kst.p, 0, 32, st));
-            timedLaunch(ctx, sampleness ? KSTAT_S_WRITE : KSTAT_E_WRITE, [&] {
-                k_conePass<<<xcdGrid(links, BLOCK), BLOCK, 0, st>>>(tt, in, links, sampleness, 1, counts.p, offsets.p,
-                                                                    recLeaf.p, recVal.p, nullptr, nullptr, nullptr,
-                                                                    nullptr, kst.p);
-            });
-            readStats(2, cs);
-            {   // + 4 B offset per link, 12 B record per accepted hit
+            k_gatherSlots<<<gridFor(links, BLOCK), BLOCK, 0, st>>>(counts.p, offsets.p, in.order, links, in.slots,
+                                                                  stLeaf.p, stVal.p, recLeaf.p, recVal.p);
+            stLeaf.release();
+            stVal.release();
+            // links beyond their slots: list them and traverse those again
+            DBuf<u32> ovFlag(links), ovList(links), ovCount(1);
+            k_overflowFlags<<<gridFor(links, BLOCK), BLOCK, 0, st>>>(counts.p, links, in.slots, ovFlag.p);
+            {
+                DBuf<u32> iota(links);
+                k_iotaU<<<gridFor(links, BLOCK), BLOCK, 0, st>>>(iota.p, links);
+                size_t tmp = 0;
+                HIPCHECK(hipcub::DeviceSelect::Flagged(nullptr, tmp, iota.p, ovFlag.p, ovList.p, ovCount.p, (int)links, st));
+                HIPCHECK(hipcub::DeviceSelect::Flagged(sc.get(tmp), tmp, iota.p, ovFlag.p, ovList.p, ovCount.p, (int)links, st));
+            }
+            const u32 nOver = readScalar(ovCount.p, st);
+            field->overflowLinks[type] = nOver;
+            if (std::getenv("TSR_TRACE"))
+                std::fprintf(stderr, "[tsr occupancy] %s pass: %u links, %llu hits, %u links over %u slots\n",
+                             sampleness ? "sampleness" : "emptiness", links, (unsigned long long)total, nOver,
+                             in.slots);
+            if (nOver) {
+                LinkInputs ino = in;
+                ino.order = ovList.p;
+                HIPCHECK(hipMemsetAsync(kst.p, 0, 32, st));
+                // few, long traversals: 64-thread blocks spread them over every CU
+                timedLaunch(ctx, sampleness ? KSTAT_S_WRITE : KSTAT_E_WRITE, [&] {
+                    k_conePass<<<xcdGrid(nOver, 64), 64, 0, st>>>(tt, ino, nOver, sampleness, 1, counts.p,
+                                                                        offsets.p, recLeaf.p, recVal.p, nullptr,
+                                                                        nullptr, nullptr, nullptr, kst.p);
+                });
+                readStats(2, cs);
                 tsr_kernel_stat &k = ctx->kstat[sampleness ? KSTAT_S_WRITE : KSTAT_E_WRITE];
-                k.bytes += (u64)links * 41 + cs[0] * nodeBytes + cs[1] * 12;
-                k.units += links;
+                k.bytes += (u64)nOver * 45 + cs[0] * nodeBytes + cs[1] * 12;
+                k.units += nOver;
             }
             int bits = 1;
             while (bits < 32 && (1ull << bits) < L) ++bits;
@@ -1038,7 +1149,7 @@ void emptinessSubsample(tsr_context *ctx, const tsr_tree *tree, const tsr_scene 
     LinkInputs in{tree->px.p, tree->py.p, tree->pz.p, tree->nx.p, tree->ny.p, tree->nz.p, tree->scale.p,
                   tree->conf.p, tree->cams.p, scene->cx.p, scene->cy.p, scene->cz.p, tree->sampleDepth.p,
                   tree->maxCams, scene->camCount, prm.bandwidth_factor, prm.relative_sample_cone_length, nullptr,
-                  0, stride, nullptr};
+                  0, stride, nullptr, 0};
     HIPCHECK(hipMemsetAsync(leafHits, 0, (size_t)tree->leafCount * 4, st));
     const u32 work = (links + stride - 1) / stride;
     k_conePass<<<gridFor(work, BLOCK), BLOCK, 0, st>>>(tt, in, links, 0, 4, nullptr, nullptr, nullptr, nullptr,
