@@ -17,6 +17,17 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
+WORKLOADS = {
+    0: "unit sphere r=1; 12 cameras on a ring of radius 3, 128x96 depth maps, 0.5% depth noise",
+    1: "thin wedge (two 1x1 sheets, 0.01 thick, 10 deg) + sphere r=0.4; 64 cameras at 2..8, "
+       "640x480 depth maps, 0.3% depth noise, 1% outliers",
+    3: "union of 3-8 random spheres/boxes/thin sheets; 32 cameras at 2..8, 320x240 depth maps, "
+       "0.3% depth noise, 1% outliers",
+    4: "30 random primitives incl. thin sheets in [-4,4]^3; 120 cameras at 3..20, 1024x768 depth maps, "
+       "0.3% depth noise, 2% outliers",
+}
+
+
 def cpu_info():
     model, cores = "unknown", os.cpu_count() or 1
     try:
@@ -59,6 +70,10 @@ def main():
     ap.add_argument("--res", type=float, default=1.0)
     ap.add_argument("--cpu-res", type=float, default=0.25, help="resolution scale of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-learned-compare", action="store_true",
+                    help="skip the learned-cull vs plain comparison after the timed steps")
+    ap.add_argument("--learned", action="store_true",
+                    help="learned node culling with the committed weights (tsr_amd/culler_weights.npz)")
     ap.add_argument("--calibrate-copy", type=float, default=0,
                     help="also run a plain copy kernel of this many bytes (reference for PMC traffic scaling)")
     args = ap.parse_args()
@@ -79,10 +94,18 @@ def main():
     if args.calibrate_copy > 0:
         ctx.calibration_copy(int(args.calibrate_copy))
     scene = Scene(ctx, syn.samples, syn.cameras)
+    culler = None
+    if args.learned:
+        from tsr_amd import _lib
+        from tsr_amd.culling import load_culler
+        culler = load_culler(ctx)
 
     def step():
         tree = Tree(ctx, scene)
-        occ = Occupancy(ctx, tree, scene)
+        if culler is not None:
+            occ = Occupancy(ctx, tree, scene, flags=_lib.TSR_CULL_LEARNED, culler=culler)
+        else:
+            occ = Occupancy(ctx, tree, scene)
         mesh = DualMarchingCells(ctx, tree, occ)
         info = (tree.node_count, tree.leaf_count, tree.cell_count, mesh.vertex_count, mesh.triangle_count,
                 occ.hit_counts())
@@ -110,7 +133,7 @@ def main():
     ph = [float(sum(p[i] for p in phases) / len(phases)) for i in range(6)]
 
     out = {
-        "metric": "input samples/sec to occupancy field + DMC mesh",
+        "metric": "input samples/sec to occupancy field + DMC mesh; learned-cull vs plain speedup",
         "value": value,
         "unit": "samples/s",
         "n_gpus": 1,
@@ -123,8 +146,9 @@ def main():
         "dtype": "fp32 (fp64 predicates/ratio)",
         "data": "synthetic",
         "config": {
-            "workload": "thin wedge (two 1x1 sheets, 0.01 thick, 10 deg) + sphere r=0.4; 64 cameras at 2..8, "
-                        "640x480 depth maps, 0.3% depth noise, 1% outliers",
+            "workload": WORKLOADS.get(args.config, f"synthetic config {args.config}")
+                        + (f" at resolution scale {args.res}" if args.res != 1.0 else "")
+                        + ("; learned node culling" if args.learned else ""),
             "config_index": args.config,
             "resolution_scale": args.res,
             "samples": n,
@@ -148,7 +172,8 @@ def main():
         kd = out["kernels"][dom]
         traffic = None
         tfile = os.path.join(ROOT, "profiles", f"pmc_traffic_{dom}.txt")
-        if os.path.exists(tfile):
+        # the committed counter profile covers the default workload only
+        if os.path.exists(tfile) and args.config == 1 and args.res == 1.0 and not args.learned:
             try:
                 with open(tfile) as f:
                     traffic = json.loads(f.readline())["hbm_bytes_per_launch"]
@@ -158,6 +183,35 @@ def main():
                            "unit": "GB/s", "frac": kd["GBps"] / HBM_PEAK_GBS, "traffic": traffic,
                            "algorithmic_bytes_per_launch": kd["algorithmic_bytes_per_launch"],
                            "ms_per_launch": kd["ms_per_launch"]}
+    # learned-cull vs plain: the occupancy + DMC part of one step each on this scene's tree
+    # (the tree build is the same for both), after the timed region
+    if not args.no_learned_compare:
+        from tsr_amd import _lib
+        from tsr_amd.culling import load_culler
+        cul = culler if culler is not None else load_culler(ctx)
+        tree = Tree(ctx, scene)
+        times = {}
+        for mode in ("plain", "learned", "plain", "learned"):   # second round counts
+            ctx.synchronize()
+            t1 = time.perf_counter()
+            if mode == "plain":
+                o = Occupancy(ctx, tree, scene)
+            else:
+                o = Occupancy(ctx, tree, scene, flags=_lib.TSR_CULL_LEARNED, culler=cul)
+            m = DualMarchingCells(ctx, tree, o)
+            ctx.synchronize()
+            times[mode] = 1000.0 * (time.perf_counter() - t1)
+            if mode == "learned":
+                culled = o.cull_counts()[1]
+            m.close()
+            o.close()
+        tree.close()
+        out["learned_vs_plain"] = {
+            "plain_ms": times["plain"], "learned_ms": times["learned"],
+            "speedup": times["plain"] / times["learned"], "culled_leaves": culled,
+            "scope": "occupancy field + DMC on the same tree, one run each after a warm-up run; "
+                     "culler trained on config-3 scenes (tsr_amd/culler_weights.npz)",
+        }
     if not args.no_cpu_baseline:
         model, cores = cpu_info()
         threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(cores, 64)

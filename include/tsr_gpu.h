@@ -161,10 +161,12 @@ void tsr_tree_destroy(tsr_tree *tree);
 tsr_status tsr_occupancy_compute(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *scene,
                                  const tsr_params *params, int flags, const tsr_culler *culler, tsr_field **out);
 /* kernel_sums / cone_lengths: 2 * leaf_count floats, [0..L) = EMPTINESS, [L..2L) = SAMPLENESS
- * (Occupancy.h:137-138); priors: leaf_count (mPriorsForEmptiness); node_states: node_count;
+ * (Occupancy.h:137-138); priors: leaf_count doubles (mPriorsForEmptiness, kept in FP64 because
+ * the occupancy ratio uses 1 - prior, which float rounding of a prior near 1 would spoil);
+ * node_states: node_count;
  * occupancy: leaf_count doubles (getOccupancy, Occupancy.cpp:570-587); cone_count: M. */
 tsr_status tsr_field_download(tsr_context *ctx, const tsr_field *field, float *kernel_sums, float *cone_lengths,
-                              float *priors, uint32_t *node_states, double *occupancy, double *cone_count);
+                              double *priors, uint32_t *node_states, double *occupancy, double *cone_count);
 /* counts of (link, leaf) kernel contributions of the last passes: [0] = E, [1] = S */
 tsr_status tsr_field_hit_counts(const tsr_field *field, uint64_t *hits2);
 /* nodes marked by the learned culling and leaves below them (0 without culling) */

@@ -328,7 +328,7 @@ public:
     double getSamplenessCPD(uint32 leafIdx) const { return mKernelSums[mPriors.size() + leafIdx] / mConeCount; }
     // Occupancy.h:197-210
     double getPriorForEmptiness(uint32 leafIdx) const { return mPriors[leafIdx]; }
-    double getPriorForSampleness(uint32 leafIdx) const { return 1.0 - (double)mPriors[leafIdx]; }
+    double getPriorForSampleness(uint32 leafIdx) const { return 1.0 - mPriors[leafIdx]; }
     double getObservationPD(uint32 leafIdx) const {
         return getPriorForEmptiness(leafIdx) * getEmptinessCPD(leafIdx) +
                getPriorForSampleness(leafIdx) * getSamplenessCPD(leafIdx);
@@ -354,7 +354,8 @@ public:
 private:
     const Tree *mTree;
     tsr_field *mField = nullptr;
-    std::vector<float> mKernelSums, mConeLengths, mPriors;
+    std::vector<float> mKernelSums, mConeLengths;
+    std::vector<double> mPriors;
     std::vector<uint32> mNodeStates;
     std::vector<double> mOccupancy;
     double mConeCount = 0;

@@ -117,3 +117,30 @@ def test_default_priors_only_where_used(case, gpu_ctx):
     used = g["kernel_sums"][:L].astype(np.float64) + g["kernel_sums"][L:] >= 100.0
     assert np.array_equal(occ["priors"][used], g["priors"][used])
     assert np.all(np.isnan(occ["priors"][~used]))
+
+
+def test_link_chunks_match_single_pass(gpu_ctx, monkeypatch):
+    """Cone passes split into link chunks (folded in chunk order with two-sum) give
+    the single-pass field within the parity bound, the same hit counts and mesh."""
+    from tsr_amd import DualMarchingCells, Occupancy, Scene, Tree
+    samples, cameras = scene(1, 0.0625)
+    sc = Scene(gpu_ctx, samples, cameras)
+    tree = Tree(gpu_ctx, sc)
+    one = Occupancy(gpu_ctx, tree, sc)
+    f1, h1 = one.download(), one.hit_counts()
+    m1 = DualMarchingCells(gpu_ctx, tree, one).download()
+    monkeypatch.setenv("TSR_CONE_CHUNK", "7000")   # ~10 chunks on this scene
+    many = Occupancy(gpu_ctx, tree, sc)
+    f2, h2 = many.download(), many.hit_counts()
+    m2 = DualMarchingCells(gpu_ctx, tree, many).download()
+    monkeypatch.delenv("TSR_CONE_CHUNK")
+    assert len(samples["px"]) > 5 * 7000
+    assert np.array_equal(h1, h2)
+    assert np.array_equal(f1["node_states"], f2["node_states"])
+    for k in ("kernel_sums", "cone_lengths"):
+        assert np.max(rel_err(f2[k], f1[k], floor=1e-30)) <= 1e-5, k
+    assert f1["cone_count"] == f2["cone_count"]
+    assert np.sum((f1["occupancy"] > 0) != (f2["occupancy"] > 0)) == 0
+    assert np.array_equal(m1["triangles"], m2["triangles"])
+    for h in (many, one, tree, sc):
+        h.close()

@@ -102,7 +102,7 @@ SIGNATURES = [
     ("tsr_tree_samples_download", ctypes.c_int, [vp, vp, f32p, f32p, f32p, f32p, f32p, f32p, f32p, f32p, u32p]),
     ("tsr_tree_destroy", None, [vp]),
     ("tsr_occupancy_compute", ctypes.c_int, [vp, vp, vp, ctypes.POINTER(Params), ctypes.c_int, vp, ctypes.POINTER(vp)]),
-    ("tsr_field_download", ctypes.c_int, [vp, vp, f32p, f32p, f32p, u32p, f64p, f64p]),
+    ("tsr_field_download", ctypes.c_int, [vp, vp, f32p, f32p, f64p, u32p, f64p, f64p]),
     ("tsr_field_hit_counts", ctypes.c_int, [vp, u64p]),
     ("tsr_field_cull_counts", ctypes.c_int, [vp, u32p, u32p]),
     ("tsr_field_culled_leaves", ctypes.c_int, [vp, vp, u8p]),

@@ -254,11 +254,11 @@ class Occupancy:
     def download(self):
         Lc, N = self.leaf_count, self.node_count
         o = {"kernel_sums": np.zeros(2 * Lc, np.float32), "cone_lengths": np.zeros(2 * Lc, np.float32),
-             "priors": np.zeros(Lc, np.float32), "node_states": np.zeros(N, np.uint32),
+             "priors": np.zeros(Lc, np.float64), "node_states": np.zeros(N, np.uint32),
              "occupancy": np.zeros(Lc, np.float64)}
         m = ctypes.c_double(0.0)
         L.check(L.lib().tsr_field_download(self.ctx.h, self.h, _fp(o["kernel_sums"]), _fp(o["cone_lengths"]),
-                                           _fp(o["priors"]), _up(o["node_states"]), _dp(o["occupancy"]),
+                                           _dp(o["priors"]), _up(o["node_states"]), _dp(o["occupancy"]),
                                            ctypes.byref(m)))
         o["cone_count"] = m.value
         return o

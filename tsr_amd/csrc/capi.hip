@@ -83,6 +83,7 @@ void tsr_context_destroy(tsr_context *ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     ctx->scratch.buf.release();
+    ctx->cone.release();
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
     for (hipEvent_t &e : ctx->ev) (void)hipEventDestroy(e);
     for (hipEvent_t &e : ctx->kev) (void)hipEventDestroy(e);
@@ -324,7 +325,7 @@ tsr_status tsr_occupancy_compute(tsr_context *ctx, const tsr_tree *tree, const t
 }
 
 tsr_status tsr_field_download(tsr_context *ctx, const tsr_field *f, float *kernel_sums, float *cone_lengths,
-                              float *priors, uint32_t *node_states, double *occupancy, double *cone_count) {
+                              double *priors, uint32_t *node_states, double *occupancy, double *cone_count) {
     if (!ctx || !f) return fail(TSR_ERR_INVALID_ARGUMENT, "null argument");
     return guarded([&] {
         activate(ctx);

@@ -4,6 +4,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <hipcub/hipcub.hpp>
 
 #include <string>
@@ -46,6 +47,11 @@ struct DBuf {
         if (p) (void)hipFree(p);
         p = nullptr;
         n = 0;
+    }
+    // grow-only: keeps the buffer when it already holds `count` elements
+    T *ensure(size_t count) {
+        if (count > n || !p) alloc(std::max<size_t>(count, 1));
+        return p;
     }
     size_t bytes() const { return sizeof(T) * n; }
 };

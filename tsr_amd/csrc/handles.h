@@ -12,6 +12,19 @@ struct tsr_kernel_stat {
     uint64_t launches = 0, bytes = 0, units = 0;
 };
 
+namespace tsr {
+// buffers of the cone passes, kept by the context between passes and calls (grow-only):
+// repeated allocation and release of tens of GB costs more than the passes themselves
+struct ConeArena {
+    DBuf<u32> stLeaf, recLeaf, recLeafS;
+    DBuf<uint2> stVal, recVal, recValS;
+    void release() {
+        stLeaf.release(); recLeaf.release(); recLeafS.release();
+        stVal.release(); recVal.release(); recValS.release();
+    }
+};
+}  // namespace tsr
+
 struct tsr_context {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -20,6 +33,7 @@ struct tsr_context {
     float phaseMs[6] = {0, 0, 0, 0, 0, 0};
     tsr_kernel_stat kstat[8];
     tsr::Scratch scratch;
+    tsr::ConeArena cone;
     // pinned host staging for large device->host transfers (grow-only, freed with the context)
     void *pinned = nullptr;
     size_t pinnedBytes = 0;
@@ -69,7 +83,7 @@ struct tsr_field {
     tsr::u32 leafCount = 0, nodeCount = 0;
     tsr::DBuf<float> ksum;     // [2][L] kernel sums, 0 = EMPTINESS, 1 = SAMPLENESS
     tsr::DBuf<float> clen;     // [2][L] cone length sums
-    tsr::DBuf<float> prior;    // [L] prior for emptiness
+    tsr::DBuf<double> prior;   // [L] prior for emptiness, E / (E + S) in FP64
     tsr::DBuf<tsr::u32> states;
     tsr::DBuf<double> occ;     // [L] occupancy (FP64)
     double coneCount = 0;

@@ -16,6 +16,7 @@
 // stably radix-sorted by leaf, and each leaf's records are summed in FP32 with
 // Kahan compensation in that fixed order: deterministic, no float atomics.
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -50,7 +51,24 @@ inline unsigned xcdGrid(size_t n, unsigned block) {
 // budget allows, at most CONE_SLOTS_MAX); links with more accepted hits are traversed
 // again (mode 1) into their final place
 constexpr u32 CONE_SLOTS_MAX = 256;
-constexpr size_t CONE_STAGING_BUDGET = (size_t)24 << 30;   // bytes
+constexpr size_t CONE_STAGING_BUDGET = (size_t)40 << 30;   // bytes
+constexpr u64 CONE_MAX_RECORDS = 1536ull << 20;              // records of one link range (one sort)
+// links per chunk of a cone pass (TSR_CONE_CHUNK overrides, e.g. to exercise the
+// multi-chunk merge on small scenes)
+inline u32 coneChunkLinks() {   // read at every occupancy computation
+    const char *e = std::getenv("TSR_CONE_CHUNK");
+    const long v = e ? std::atol(e) : 0;
+    return v > 0 ? (u32)v : (1u << 24);
+}
+
+// Staging layout of mode 3: each group of STAGE_GROUP consecutive threads owns one
+// contiguous region of slots * STAGE_GROUP records, record j of a thread at row j of
+// its region: lanes writing their j-th record hit one row (coalesced) and a block's
+// writes stay within a few pages.
+constexpr u32 STAGE_GROUP = 256;
+__device__ __forceinline__ size_t stagingIndex(u32 idx, u32 j, u32 slots) {
+    return ((size_t)(idx / STAGE_GROUP) * slots + j) * STAGE_GROUP + (idx % STAGE_GROUP);
+}
 
 struct TraversalTree {
     const u32 *child;      // first child id, or LEAF_BIT | leaf index
@@ -71,7 +89,8 @@ struct LinkInputs {
     int xcdMap;
     u32 linkStride;            // process every linkStride-th link (1 = all)
     const u32 *order;          // thread index -> link (links grouped by camera), may be null
-    u32 slots;                 // mode 3: staging slots per link, slot j of thread i at j * links + i
+    u32 slots;                 // mode 3: staging slots per link (layout: stagingIndex)
+    u32 linkBase;              // without `order`: thread index i -> link linkBase + i
 };
 
 // Cone of link `link` (Occupancy.cpp:174-201, getCameraToSampleLinkData :250-277)
@@ -173,7 +192,7 @@ __global__ void k_conePass(TraversalTree t, LinkInputs in, u32 linkCount, int sa
                            unsigned long long *__restrict__ stats) {
     const u32 idx = (xcdBlock(in.xcdMap) * blockDim.x + threadIdx.x) * in.linkStride;
     u32 visits = 0, count = 0;
-    const u32 link = idx < linkCount && in.order ? in.order[idx] : idx;
+    const u32 link = idx < linkCount && in.order ? in.order[idx] : in.linkBase + idx;
     if (idx < linkCount) coneLink(t, in, link, idx, linkCount, sampleness, mode, counts, offsets, recLeaf, recVal, atomicSum,
                                    atomicLen, atomicStates, confOut, visits, count);
     if (stats) {
@@ -206,7 +225,7 @@ __device__ void coneLink(const TraversalTree &t, const LinkInputs &in, u32 link,
         const double kv = coneKernel(cone, x, sampleness != 0, conf);
         if (kv == 0.0) return;
         if (mode == 1 || (mode == 3 && count < in.slots)) {
-            const size_t at = mode == 1 ? base + count : (size_t)count * linkCount + idx;
+            const size_t at = mode == 1 ? base + count : stagingIndex(idx, count, in.slots);
             recLeaf[at] = leaf;
             recVal[at] = make_uint2(__float_as_uint((float)kv), __float_as_uint(weightedLength));
         } else if (mode == 2) {
@@ -223,20 +242,21 @@ __device__ void coneLink(const TraversalTree &t, const LinkInputs &in, u32 link,
 
 // staged records of the links that fit their slots -> link order (offsets = exclusive
 // scan of the counts); a link's records stay in traversal order.  Thread i moves the
-// records staged by thread i (slot j at j * links + i; coalesced reads).
+// records staged by thread i (stagingIndex; coalesced reads).
 __global__ void k_gatherSlots(const u32 *__restrict__ counts, const u32 *__restrict__ offsets,
-                              const u32 *__restrict__ order, u32 links, u32 slots, const u32 *__restrict__ stLeaf,
+                              const u32 *__restrict__ order, u32 base, u32 links, u32 slots,
+                              const u32 *__restrict__ stLeaf,
                               const uint2 *__restrict__ stVal, u32 *__restrict__ recLeaf,
                               uint2 *__restrict__ recVal) {
     const u32 i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= links) return;
-    const u32 l = order ? order[i] : i;
+    const u32 l = order ? order[i] : base + i;
     const u32 c = counts[l];
     if (c > slots) return;
     const size_t dst = offsets[l];
     for (u32 j = 0; j < c; ++j) {
-        recLeaf[dst + j] = stLeaf[(size_t)j * links + i];
-        recVal[dst + j] = stVal[(size_t)j * links + i];
+        recLeaf[dst + j] = stLeaf[stagingIndex(i, j, slots)];
+        recVal[dst + j] = stVal[stagingIndex(i, j, slots)];
     }
 }
 
@@ -256,7 +276,8 @@ __global__ void k_segmentStarts(const u32 *__restrict__ leafSorted, size_t n, u3
 // Kahan sums of each leaf's records in record order (deterministic)
 __global__ void k_segmentReduce(const u32 *__restrict__ leafSorted, const uint2 *__restrict__ valSorted, size_t n,
                                 const u32 *__restrict__ segStart, u32 leafCount, float *__restrict__ ksum,
-                                float *__restrict__ clen, u32 *__restrict__ hitCount) {
+                                float *__restrict__ clen, u32 *__restrict__ hitCount, float *__restrict__ kcomp,
+                                float *__restrict__ lcomp) {
     const u32 l = blockIdx.x * BLOCK + threadIdx.x;
     if (l >= leafCount) return;
     const u32 s = segStart[l];
@@ -278,6 +299,47 @@ __global__ void k_segmentReduce(const u32 *__restrict__ leafSorted, const uint2 
     ksum[l] = sk;
     clen[l] = sl;
     hitCount[l] = cnt;
+    if (kcomp) {   // the Kahan sums approximate sk - ck and sl - cl
+        kcomp[l] = -ck;
+        lcomp[l] = -cl;
+    }
+}
+
+TSR_HD void twoSumAdd(float &hi, float &lo, float x) {   // hi + lo += x (Kahan-Babuska)
+    const float s = hi + x;
+    const float bp = s - hi;
+    const float err = (hi - (s - bp)) + (x - bp);
+    hi = s;
+    lo += err;
+}
+
+// fold one link chunk's per-leaf Kahan sums (sum, compensation) into running
+// (hi, lo) pairs, chunk after chunk (fixed order: deterministic)
+__global__ void k_mergeChunk(u32 leafCount, const float *__restrict__ ks, const float *__restrict__ kc,
+                             const float *__restrict__ ls, const float *__restrict__ lc, const u32 *__restrict__ hits,
+                             float *__restrict__ kHi, float *__restrict__ kLo, float *__restrict__ lHi,
+                             float *__restrict__ lLo, u32 *__restrict__ hitSum) {
+    const u32 l = blockIdx.x * BLOCK + threadIdx.x;
+    if (l >= leafCount) return;
+    float h = kHi[l], lo = kLo[l];
+    twoSumAdd(h, lo, ks[l]);
+    kHi[l] = h;
+    kLo[l] = lo + kc[l];
+    h = lHi[l];
+    lo = lLo[l];
+    twoSumAdd(h, lo, ls[l]);
+    lHi[l] = h;
+    lLo[l] = lo + lc[l];
+    hitSum[l] += hits[l];
+}
+
+__global__ void k_finishChunks(u32 leafCount, const float *__restrict__ kHi, const float *__restrict__ kLo,
+                               const float *__restrict__ lHi, const float *__restrict__ lLo, float *__restrict__ ksum,
+                               float *__restrict__ clen) {
+    const u32 l = blockIdx.x * BLOCK + threadIdx.x;
+    if (l >= leafCount) return;
+    ksum[l] = kHi[l] + kLo[l];
+    clen[l] = lHi[l] + lLo[l];
 }
 
 // leaf flags from hit counts, then OR into ancestors (Occupancy.cpp:491-516)
@@ -303,13 +365,6 @@ __global__ void k_propagateFlag(const u32 *__restrict__ leafNode, u32 leafCount,
 // ---------------------------------------------------------------------------
 // priors (Occupancy.cpp:530-568)
 // ---------------------------------------------------------------------------
-TSR_HD void twoSumAdd(float &hi, float &lo, float x) {   // hi + lo += x (Kahan-Babuska)
-    const float s = hi + x;
-    const float bp = s - hi;
-    const float err = (hi - (s - bp)) + (x - bp);
-    hi = s;
-    lo += err;
-}
 
 // Subtree sums (hi, lo) of E and S cone lengths per node, plus aux = node state
 // flags | deepest leaf depth << 8, built bottom-up one depth level at a time.
@@ -347,13 +402,13 @@ __global__ void k_subtreeLevel(const u32 *__restrict__ child, const uint8_t *__r
 // leaves whose prior the occupancy ratio reads: E + S kernel sums >= confidence
 // threshold (Occupancy.cpp:575-576); all leaves when `all` is set
 __global__ void k_priorQueryFlags(const float *__restrict__ ksum, u32 leafCount, double threshold, int all,
-                                  const uint8_t *__restrict__ culled, float *__restrict__ prior,
+                                  const uint8_t *__restrict__ culled, double *__restrict__ prior,
                                   u32 *__restrict__ flag) {
     const u32 l = blockIdx.x * BLOCK + threadIdx.x;
     if (l >= leafCount) return;
     const bool used = (all || ((double)ksum[l] + (double)ksum[leafCount + l] >= threshold)) && !(culled && culled[l]);
     flag[l] = used ? 1u : 0u;
-    if (!used) prior[l] = __int_as_float(0x7fc00000);   // not needed by the field: NaN
+    if (!used) prior[l] = __longlong_as_double(0x7ff8000000000000ll);   // not needed by the field: NaN
 }
 
 // queries in leaf (DFS = spatial) order
@@ -411,7 +466,7 @@ __device__ __forceinline__ void xcdRange(u32 count, int xcdMap, u32 &begin, u32 
 
 __global__ __launch_bounds__(64) void k_priorsWave(Frame f, const PriorNode *__restrict__ nodes,
                                                    const u64 *__restrict__ leafLoc, const u32 *__restrict__ queries,
-                                                   u32 queryCount, float relRadius, float *__restrict__ prior,
+                                                   u32 queryCount, float relRadius, double *__restrict__ prior,
                                                    u32 *err, unsigned long long *__restrict__ stats, int xcdMap) {
     __shared__ u32 stack[PRIOR_STACK];
     __shared__ u32 sp;
@@ -506,7 +561,7 @@ __global__ __launch_bounds__(64) void k_priorsWave(Frame f, const PriorNode *__r
             e += __shfl_xor(e, off);
             s += __shfl_xor(s, off);
         }
-        if (lane == 0) prior[l] = (float)(e / (e + s));
+        if (lane == 0) prior[l] = e / (e + s);
         __syncthreads();
     }
     if (stats) {
@@ -528,7 +583,7 @@ constexpr u32 PACKET_STACK = 256;   // DFS: <= 7 per level + 8 entries
 __global__ __launch_bounds__(64) void k_priorsPacket(Frame f, const PriorNode *__restrict__ nodes,
                                                      const u64 *__restrict__ leafLoc,
                                                      const u32 *__restrict__ queries, u32 queryCount,
-                                                     float relRadius, float *__restrict__ prior, u32 *err,
+                                                     float relRadius, double *__restrict__ prior, u32 *err,
                                                      unsigned long long *__restrict__ stats, int xcdMap) {
     __shared__ u32 stNode[PACKET_STACK];
     __shared__ u64 stMask[PACKET_STACK];
@@ -631,7 +686,7 @@ __global__ __launch_bounds__(64) void k_priorsPacket(Frame f, const PriorNode *_
         }
         if (valid) {
             const double e = (double)eh + (double)el, s = (double)sh + (double)sl;
-            prior[l] = (float)(e / (e + s));
+            prior[l] = e / (e + s);
         }
         __syncthreads();
     }
@@ -648,12 +703,11 @@ __global__ __launch_bounds__(64) void k_priorsPacket(Frame f, const PriorNode *_
 // record is loaded once for all queries of the group; every query applies the
 // same FP64 predicates as the per-query form.  Sums are FP64 per lane and query,
 // reduced across the wave in a fixed order at the end (deterministic).
-constexpr u32 GROUP_Q = 8;
-
+template <u32 GROUP_Q>
 __global__ __launch_bounds__(64) void k_priorsGroup(Frame f, const PriorNode *__restrict__ nodes,
                                                     const u64 *__restrict__ leafLoc,
                                                     const u32 *__restrict__ queries, u32 queryCount,
-                                                    float relRadius, float *__restrict__ prior, u32 *err,
+                                                    float relRadius, double *__restrict__ prior, u32 *err,
                                                     unsigned long long *__restrict__ stats, int xcdMap) {
     __shared__ u32 stNode[PRIOR_STACK];
     __shared__ uint8_t stMask[PRIOR_STACK];
@@ -663,14 +717,15 @@ __global__ __launch_bounds__(64) void k_priorsGroup(Frame f, const PriorNode *__
     __shared__ u32 sp;
     const u32 lane = threadIdx.x;
     u32 nPops = 0, nTests = 0;
-    // FP32 pre-test of the FP64 sphere predicates: a coordinate difference rounded to
-    // float (inputs and result) is off by at most e = 4 * 2^-24 * M, M bounding every
-    // |coordinate|; the squared distance near radius r then by less than the margin
-    // 4 e r + 4 e^2 + 2^-20 r^2.  Outside the margin band the FP32 result decides,
-    // inside it the FP64 test does.
+    // FP32 pre-test of the FP64 sphere predicates.  Box bounds are formed in float
+    // (rootMinF + k * hF: two roundings, <= 2 * 2^-24 * M off the FP64 bound, M bounding
+    // every |coordinate|), query centres are rounded to float (<= 2^-24 * M) and their
+    // difference is rounded once more (<= 2 * 2^-24 * M): e = 5 * 2^-24 * M per axis.  The
+    // squared distance near radius r is then off by less than 4 e r + 4 e^2 + 2^-20 r^2.
+    // Outside that margin the FP32 result decides, inside it the exact FP64 test does.
     double M = 0.0;
     for (int a = 0; a < 3; ++a) M = fmax(M, fabs(f.rootMin[a]) + f.rootSize);
-    const double eps = 4.0 * ldexp(M, -24);
+    const double eps = 5.0 * ldexp(M, -24);
     const u32 groups = (queryCount + GROUP_Q - 1) / GROUP_Q;
     u32 gBegin, gEnd, gStride, gFirst;
     xcdRange(groups, xcdMap, gBegin, gEnd, gStride, gFirst);
@@ -735,23 +790,21 @@ __global__ __launch_bounds__(64) void k_priorsGroup(Frame f, const PriorNode *__
                     ++nTests;
                     u32 k[3];
                     locCoords((pn.loc << 3) | slot, k[0], k[1], k[2]);
-                    double lo[3], hi[3];
+                    const float hF = ldexpf(f.rootSizeF, -(int)d);
                     float loF[3], hiF[3];
                     for (int a = 0; a < 3; ++a) {
-                        lo[a] = f.B(a, k[a], d);
-                        hi[a] = f.B(a, (u64)k[a] + 1, d);
-                        loF[a] = (float)lo[a];
-                        hiF[a] = (float)hi[a];
+                        loF[a] = f.rootMinF[a] + (float)k[a] * hF;
+                        hiF[a] = f.rootMinF[a] + (float)(k[a] + 1) * hF;
                     }
                     const bool leaf = (cn.child & LEAF_BIT) != 0;
                     const u32 mld = cn.aux >> 8;
 #pragma unroll
                     for (u32 j = 0; j < GROUP_Q; ++j) {
                         if (!((mask >> j) & 1u) || d > qmd[j]) continue;
+                        const float cq[3] = {qcF[j][0], qcF[j][1], qcF[j][2]};
                         float dminF = 0.f, dmaxF = 0.f;
                         for (int q3 = 0; q3 < 3; ++q3) {
-                            const float c = qcF[j][q3];
-                            const float a0 = loF[q3] - c, a1 = c - hiF[q3];
+                            const float a0 = loF[q3] - cq[q3], a1 = cq[q3] - hiF[q3];
                             const float o = fmaxf(0.f, fmaxf(a0, a1));
                             dminF += o * o;
                             const float far = fmaxf(fabsf(a0), fabsf(a1));
@@ -763,8 +816,9 @@ __global__ __launch_bounds__(64) void k_priorsGroup(Frame f, const PriorNode *__
                             double dmin = 0.0;
                             for (int q3 = 0; q3 < 3; ++q3) {   // SphereNodesChecker.cpp:29-40
                                 const double c = qc[j][q3];
-                                if (c < lo[q3]) dmin += (lo[q3] - c) * (lo[q3] - c);
-                                else if (c > hi[q3]) dmin += (c - hi[q3]) * (c - hi[q3]);
+                                const double lo = f.B(q3, k[q3], d), hi = f.B(q3, (u64)k[q3] + 1, d);
+                                if (c < lo) dmin += (lo - c) * (lo - c);
+                                else if (c > hi) dmin += (c - hi) * (c - hi);
                             }
                             if (!(dmin <= qr2[j])) continue;
                         }
@@ -775,7 +829,8 @@ __global__ __launch_bounds__(64) void k_priorsGroup(Frame f, const PriorNode *__
                                 double dmax = 0.0;
                                 for (int q3 = 0; q3 < 3; ++q3) {
                                     const double c = qc[j][q3];
-                                    const double far = fmax(fabs(c - lo[q3]), fabs(hi[q3] - c));
+                                    const double lo = f.B(q3, k[q3], d), hi = f.B(q3, (u64)k[q3] + 1, d);
+                                    const double far = fmax(fabs(c - lo), fabs(hi - c));
                                     dmax += far * far;
                                 }
                                 whole = dmax <= qri2[j];
@@ -813,7 +868,7 @@ __global__ __launch_bounds__(64) void k_priorsGroup(Frame f, const PriorNode *__
                 ee += __shfl_xor(ee, off);
                 ss += __shfl_xor(ss, off);
             }
-            if (lane == 0 && j < nQ) prior[queries[g * GROUP_Q + j]] = (float)(ee / (ee + ss));
+            if (lane == 0 && j < nQ) prior[queries[g * GROUP_Q + j]] = ee / (ee + ss);
         }
         __syncthreads();
     }
@@ -824,7 +879,7 @@ __global__ __launch_bounds__(64) void k_priorsGroup(Frame f, const PriorNode *__
 }
 
 // Occupancy.cpp:570-587, FP64 (leaves under learned-culled nodes are empty)
-__global__ void k_occupancy(const float *__restrict__ ksum, const float *__restrict__ prior, u32 leafCount,
+__global__ void k_occupancy(const float *__restrict__ ksum, const double *__restrict__ prior, u32 leafCount,
                             double coneCount, double threshold, const uint8_t *__restrict__ culled,
                             double *__restrict__ occ) {
     const u32 l = blockIdx.x * BLOCK + threadIdx.x;
@@ -844,9 +899,9 @@ __global__ void k_fillF(float *p, size_t n, float v) {
     const size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x;
     if (i < n) p[i] = v;
 }
-__global__ void k_iotaU(u32 *p, u32 n) {
+__global__ void k_iotaU(u32 *p, u32 n, u32 base) {
     const u32 i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i < n) p[i] = i;
+    if (i < n) p[i] = base + i;
 }
 __global__ void k_fillU(u32 *p, size_t n, u32 v) {
     const size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x;
@@ -871,12 +926,12 @@ __global__ void k_countMask(const uint8_t *__restrict__ mask, u32 n, u32 *count)
     if (i < n && mask[i]) atomicAdd(count, 1u);
 }
 
-__global__ void k_camKeys(const u32 *__restrict__ cams, u32 links, u32 camCount, u32 *__restrict__ key,
+__global__ void k_camKeys(const u32 *__restrict__ cams, u32 links, u32 camCount, u32 base, u32 *__restrict__ key,
                           u32 *__restrict__ idx) {
     const u32 l = blockIdx.x * BLOCK + threadIdx.x;
     if (l >= links) return;
     key[l] = min(cams[l], camCount);   // invalid links (>= camCount) last
-    idx[l] = l;
+    idx[l] = base + l;
 }
 
 // priors kernel: 0 = one wavefront per query, 1 = 64 queries per wavefront,
@@ -887,6 +942,14 @@ int priorsPacketMode() {
         if (e && std::strcmp(e, "query") == 0) return 0;
         if (e && std::strcmp(e, "packet") == 0) return 1;
         return 2;
+    }();
+    return m;
+}
+
+int priorsGroupSize() {   // TSR_PRIORS_GROUP=4: 4 queries per wavefront (default 8)
+    static const int m = [] {
+        const char *e = std::getenv("TSR_PRIORS_GROUP");
+        return e ? std::atoi(e) : 8;
     }();
     return m;
 }
@@ -926,19 +989,12 @@ void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *s
     in.linkStride = 1;
     in.order = nullptr;
     in.slots = 0;
+    in.linkBase = 0;
     field->params = prm;
-    // E-pass threads take links grouped by camera, samples in node (Morton) order within
-    // a camera: neighbouring lanes trace nearly parallel cones through the same nodes.
-    // Records stay indexed by link, so results do not depend on this order.
-    DBuf<u32> linkOrder;
-    if (linkOrderMode()) {
-        DBuf<u32> camKey(links), camKeyS(links), iota(links);
-        k_camKeys<<<gridFor(links, BLOCK), BLOCK, 0, st>>>(tree->cams.p, links, scene->camCount, camKey.p, iota.p);
-        linkOrder.alloc(links);
-        int bits = 1;
-        while (bits < 32 && (1ull << bits) <= scene->camCount) ++bits;
-        sortPairs(sc, camKey.p, camKeyS.p, iota.p, linkOrder.p, links, 0, bits, st);
-    }
+    // E-pass threads take links grouped by camera (per link chunk), samples in node
+    // (Morton) order within a camera: neighbouring lanes trace nearly parallel cones
+    // through the same nodes.  Records stay indexed by link, so results do not depend
+    // on this order.
     DBuf<uint8_t> cullMask;
     const bool atomicMode = (flags & 0xff) == TSR_ACCUM_FP32_ATOMIC;
 
@@ -954,7 +1010,6 @@ void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *s
         const int type = sampleness;   // storage index: 0 = EMPTINESS, 1 = SAMPLENESS
         recordPhase(ctx, 1 + pass, true);
         float *ksum = field->ksum.p + (size_t)type * L, *clen = field->clen.p + (size_t)type * L;
-        in.order = sampleness ? nullptr : linkOrder.p;   // the short S cones gain nothing from it
         if (!sampleness) {   // E pass: S flags up the tree first
             k_setLeafFlags<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(hitCount.p, tree->leafNode.p, L, FLAG_S, field->states.p);
             k_propagateFlag<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(tree->leafNode.p, L, tree->parents.p, FLAG_S,
@@ -988,76 +1043,174 @@ void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *s
             HIPCHECK(hipcub::DeviceReduce::Sum(sc.get(tmp), tmp, counts.p, offsets.p, (int)links, st));
             field->hits[type] = readScalar(offsets.p, st);
         } else {
-            // One traversal counts every link's accepted hits and stages its first in.slots
-            // records; links with more are traversed again, writing at their final offsets.
+            // Links are processed in chunks of at most coneChunkLinks() (one chunk below that).
+            // Per chunk, one traversal counts every link's accepted hits and stages its first
+            // in.slots records; links with more are traversed again, writing at their final
+            // offsets; the records are sorted by leaf (stable: link order within a leaf) and
+            // reduced with Kahan sums.  Chunk results are folded in chunk order.
             // Algorithmic bytes per link: sample SoA 37 B (position, normal, scale, confidence,
             // camera link, depth) + 4 B count; per visited node 4 B child link (+4 B state in
             // the E pass); 12 B per staged record.
             const u64 nodeBytes = sampleness ? 4 : 8;
-            in.slots = (u32)std::min<size_t>(CONE_SLOTS_MAX, CONE_STAGING_BUDGET / (12 * (size_t)std::max(links, 1u)));
-            DBuf<u32> stLeaf(std::max<size_t>((size_t)links * in.slots, 1));
-            DBuf<uint2> stVal(std::max<size_t>((size_t)links * in.slots, 1));
-            HIPCHECK(hipMemsetAsync(kst.p, 0, 32, st));
-            timedLaunch(ctx, sampleness ? KSTAT_S_COUNT : KSTAT_E_COUNT, [&] {
-                k_conePass<<<xcdGrid(links, BLOCK), BLOCK, 0, st>>>(tt, in, links, sampleness, 3, counts.p, nullptr,
-                                                                    stLeaf.p, stVal.p, nullptr, nullptr, nullptr,
-                                                                    sampleness ? nullptr : confs.p, kst.p);
-            });
-            unsigned long long cs[2];
-            readStats(2, cs);
-            {
-                tsr_kernel_stat &k = ctx->kstat[sampleness ? KSTAT_S_COUNT : KSTAT_E_COUNT];
-                k.bytes += (u64)links * 41 + cs[0] * nodeBytes + cs[1] * 12;
-                k.units += links;
-            }
-            exclusiveScan(sc, counts.p, offsets.p, links, st);
-            const u64 total = (u64)readScalar(offsets.p + links - 1, st) + readScalar(counts.p + links - 1, st);
-            if (total >= (u64)INVALID) throw tsr_error(TSR_ERR_OUT_OF_MEMORY, "too many kernel contributions in one pass");
-            field->hits[type] = total;
-            DBuf<u32> recLeaf(std::max<u64>(total, 1)), recLeafS(std::max<u64>(total, 1));
-            DBuf<uint2> recVal(std::max<u64>(total, 1)), recValS(std::max<u64>(total, 1));
-            k_gatherSlots<<<gridFor(links, BLOCK), BLOCK, 0, st>>>(counts.p, offsets.p, in.order, links, in.slots,
-                                                                  stLeaf.p, stVal.p, recLeaf.p, recVal.p);
-            stLeaf.release();
-            stVal.release();
-            // links beyond their slots: list them and traverse those again
-            DBuf<u32> ovFlag(links), ovList(links), ovCount(1);
-            k_overflowFlags<<<gridFor(links, BLOCK), BLOCK, 0, st>>>(counts.p, links, in.slots, ovFlag.p);
-            {
-                DBuf<u32> iota(links);
-                k_iotaU<<<gridFor(links, BLOCK), BLOCK, 0, st>>>(iota.p, links);
-                size_t tmp = 0;
-                HIPCHECK(hipcub::DeviceSelect::Flagged(nullptr, tmp, iota.p, ovFlag.p, ovList.p, ovCount.p, (int)links, st));
-                HIPCHECK(hipcub::DeviceSelect::Flagged(sc.get(tmp), tmp, iota.p, ovFlag.p, ovList.p, ovCount.p, (int)links, st));
-            }
-            const u32 nOver = readScalar(ovCount.p, st);
-            field->overflowLinks[type] = nOver;
-            if (std::getenv("TSR_TRACE"))
-                std::fprintf(stderr, "[tsr occupancy] %s pass: %u links, %llu hits, %u links over %u slots\n",
-                             sampleness ? "sampleness" : "emptiness", links, (unsigned long long)total, nOver,
-                             in.slots);
-            if (nOver) {
-                LinkInputs ino = in;
-                ino.order = ovList.p;
+            const u32 chunkLinks = coneChunkLinks();
+            const u32 chunks = (u32)((links + (u64)chunkLinks - 1) / chunkLinks);
+            DBuf<float> part, acc;
+            DBuf<u32> partHits;
+            bool merging = false;
+            auto startMerging = [&] {
+                merging = true;
+                part.alloc(4 * (size_t)L);
+                partHits.alloc(L);
+                acc.alloc(4 * (size_t)L);
+                HIPCHECK(hipMemsetAsync(acc.p, 0, 16 * (size_t)L, st));
+                HIPCHECK(hipMemsetAsync(hitCount.p, 0, 4 * (size_t)L, st));
+            };
+            if (chunks > 1) startMerging();
+            u64 totalHits = 0;
+            u32 totalOver = 0;
+            // link ranges in ascending order; the range length adapts to the records per link
+            // seen so far, and a range whose records would exceed what one sort handles is
+            // halved and traversed again
+            const bool trace = std::getenv("TSR_TRACE") != nullptr;
+            auto tmark = std::chrono::steady_clock::now();
+            auto lap = [&](const char *what) {   // TSR_TRACE: synchronised sub-step times
+                if (!trace) return;
+                HIPCHECK(hipStreamSynchronize(st));
+                const auto t = std::chrono::steady_clock::now();
+                std::fprintf(stderr, "[tsr occupancy]   %-28s %8.2f ms\n", what,
+                             std::chrono::duration<double, std::milli>(t - tmark).count());
+                tmark = t;
+            };
+            ConeArena &ar = ctx->cone;
+            u32 rangeLinks = chunkLinks;
+            if ((u64)links * 64 > CONE_MAX_RECORDS) {
+                // records per link estimated on every 64th link (counting traversal), so that the
+                // first range already fits one sort
+                LinkInputs ie = in;
+                ie.linkStride = 64;
+                const u32 sampled = (links + 63) / 64;
                 HIPCHECK(hipMemsetAsync(kst.p, 0, 32, st));
-                // few, long traversals: 64-thread blocks spread them over every CU
-                timedLaunch(ctx, sampleness ? KSTAT_S_WRITE : KSTAT_E_WRITE, [&] {
-                    k_conePass<<<xcdGrid(nOver, 64), 64, 0, st>>>(tt, ino, nOver, sampleness, 1, counts.p,
-                                                                        offsets.p, recLeaf.p, recVal.p, nullptr,
-                                                                        nullptr, nullptr, nullptr, kst.p);
-                });
-                readStats(2, cs);
-                tsr_kernel_stat &k = ctx->kstat[sampleness ? KSTAT_S_WRITE : KSTAT_E_WRITE];
-                k.bytes += (u64)nOver * 45 + cs[0] * nodeBytes + cs[1] * 12;
-                k.units += nOver;
+                k_conePass<<<gridFor(sampled, BLOCK), BLOCK, 0, st>>>(tt, ie, links, sampleness, 0, counts.p, nullptr,
+                                                                      nullptr, nullptr, nullptr, nullptr, nullptr,
+                                                                      nullptr, kst.p);
+                unsigned long long es[2];
+                readStats(2, es);
+                const double perLink = std::max(1.0, 1.25 * (double)es[1] / sampled);
+                rangeLinks = (u32)std::max(1.0, std::min((double)chunkLinks, 0.75 * CONE_MAX_RECORDS / perLink));
+                if (trace)
+                    std::fprintf(stderr, "[tsr occupancy] %s pass: ~%.1f records per link (1/64 sample), ranges of %u links\n",
+                                 sampleness ? "sampleness" : "emptiness", (double)es[1] / sampled, rangeLinks);
             }
-            int bits = 1;
-            while (bits < 32 && (1ull << bits) < L) ++bits;
-            sortPairs(sc, recLeaf.p, recLeafS.p, recVal.p, recValS.p, total, 0, bits, st);
-            k_fillU<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(segStart.p, L, INVALID);
-            k_segmentStarts<<<gridFor(total, BLOCK), BLOCK, 0, st>>>(recLeafS.p, total, segStart.p);
-            k_segmentReduce<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(recLeafS.p, recValS.p, total, segStart.p, L, ksum,
-                                                                 clen, hitCount.p);
+            for (u32 l0 = 0; l0 < links;) {
+                const u32 n = std::min<u32>(rangeLinks, links - l0);
+                lap("(chunk start)");
+                LinkInputs ic = in;
+                ic.linkBase = l0;
+                ic.order = nullptr;
+                DBuf<u32> chunkOrder;
+                if (!sampleness && linkOrderMode()) {   // the chunk's links grouped by camera
+                    DBuf<u32> camKey(n), camKeyS(n), iota(n);
+                    k_camKeys<<<gridFor(n, BLOCK), BLOCK, 0, st>>>(tree->cams.p + l0, n, scene->camCount, l0,
+                                                                   camKey.p, iota.p);
+                    chunkOrder.alloc(n);
+                    int bits = 1;
+                    while (bits < 32 && (1ull << bits) <= scene->camCount) ++bits;
+                    sortPairs(sc, camKey.p, camKeyS.p, iota.p, chunkOrder.p, n, 0, bits, st);
+                    ic.order = chunkOrder.p;
+                }
+                ic.slots = (u32)std::min<size_t>(CONE_SLOTS_MAX, CONE_STAGING_BUDGET / (12 * (size_t)std::max(n, 1u)));
+                const size_t stageCount = ((size_t)n + STAGE_GROUP - 1) / STAGE_GROUP * STAGE_GROUP * ic.slots;
+                u32 *stLeaf = ar.stLeaf.ensure(stageCount);
+                uint2 *stVal = ar.stVal.ensure(stageCount);
+                lap("order + staging");
+                HIPCHECK(hipMemsetAsync(kst.p, 0, 32, st));
+                timedLaunch(ctx, sampleness ? KSTAT_S_COUNT : KSTAT_E_COUNT, [&] {
+                    k_conePass<<<xcdGrid(n, BLOCK), BLOCK, 0, st>>>(tt, ic, n, sampleness, 3, counts.p, nullptr,
+                                                                    stLeaf, stVal, nullptr, nullptr, nullptr,
+                                                                    sampleness ? nullptr : confs.p, kst.p);
+                });
+                unsigned long long cs[2];
+                readStats(2, cs);
+                {
+                    tsr_kernel_stat &k = ctx->kstat[sampleness ? KSTAT_S_COUNT : KSTAT_E_COUNT];
+                    k.bytes += (u64)n * 41 + cs[0] * nodeBytes + cs[1] * 12;
+                    k.units += n;
+                }
+                lap("traversal (mode 3)");
+                exclusiveScan(sc, counts.p + l0, offsets.p + l0, n, st);   // chunk-relative offsets
+                const u64 total = (u64)readScalar(offsets.p + l0 + n - 1, st) + readScalar(counts.p + l0 + n - 1, st);
+                if (total >= CONE_MAX_RECORDS) {
+                    if (n < 2) throw tsr_error(TSR_ERR_OUT_OF_MEMORY, "too many kernel contributions for one link");
+                    if (!merging) startMerging();
+                    rangeLinks = n / 2;
+                    continue;
+                }
+                totalHits += total;
+                u32 *recLeaf = ar.recLeaf.ensure(total), *recLeafS = ar.recLeafS.ensure(total);
+                uint2 *recVal = ar.recVal.ensure(total), *recValS = ar.recValS.ensure(total);
+                k_gatherSlots<<<gridFor(n, BLOCK), BLOCK, 0, st>>>(counts.p, offsets.p, ic.order, l0, n, ic.slots,
+                                                                  stLeaf, stVal, recLeaf, recVal);
+                lap("scan + gather");
+                // links beyond their slots: list them and traverse those again
+                DBuf<u32> ovFlag(n), ovList(n), ovCount(1);
+                k_overflowFlags<<<gridFor(n, BLOCK), BLOCK, 0, st>>>(counts.p + l0, n, ic.slots, ovFlag.p);
+                {
+                    DBuf<u32> iota(n);
+                    k_iotaU<<<gridFor(n, BLOCK), BLOCK, 0, st>>>(iota.p, n, l0);
+                    size_t tmp = 0;
+                    HIPCHECK(hipcub::DeviceSelect::Flagged(nullptr, tmp, iota.p, ovFlag.p, ovList.p, ovCount.p, (int)n, st));
+                    HIPCHECK(hipcub::DeviceSelect::Flagged(sc.get(tmp), tmp, iota.p, ovFlag.p, ovList.p, ovCount.p, (int)n, st));
+                }
+                const u32 nOver = readScalar(ovCount.p, st);
+                totalOver += nOver;
+                if (std::getenv("TSR_TRACE"))
+                    std::fprintf(stderr, "[tsr occupancy] %s pass, links %u..%u: %llu hits, %u links over %u slots\n",
+                                 sampleness ? "sampleness" : "emptiness", l0, l0 + n - 1,
+                                 (unsigned long long)total, nOver, ic.slots);
+                if (nOver) {
+                    LinkInputs ino = ic;
+                    ino.order = ovList.p;
+                    HIPCHECK(hipMemsetAsync(kst.p, 0, 32, st));
+                    // few, long traversals: 64-thread blocks spread them over every CU
+                    timedLaunch(ctx, sampleness ? KSTAT_S_WRITE : KSTAT_E_WRITE, [&] {
+                        k_conePass<<<xcdGrid(nOver, 64), 64, 0, st>>>(tt, ino, nOver, sampleness, 1, counts.p,
+                                                                      offsets.p, recLeaf, recVal, nullptr,
+                                                                      nullptr, nullptr, nullptr, kst.p);
+                    });
+                    readStats(2, cs);
+                    tsr_kernel_stat &k = ctx->kstat[sampleness ? KSTAT_S_WRITE : KSTAT_E_WRITE];
+                    k.bytes += (u64)nOver * 45 + cs[0] * nodeBytes + cs[1] * 12;
+                    k.units += nOver;
+                }
+                lap("overflow list + re-traversal");
+                int bits = 1;
+                while (bits < 32 && (1ull << bits) < L) ++bits;
+                sortPairs(sc, recLeaf, recLeafS, recVal, recValS, total, 0, bits, st);
+                lap("sort by leaf");
+                k_fillU<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(segStart.p, L, INVALID);
+                if (total) k_segmentStarts<<<gridFor(total, BLOCK), BLOCK, 0, st>>>(recLeafS, total, segStart.p);
+                if (!merging) {
+                    k_segmentReduce<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(recLeafS, recValS, total, segStart.p, L,
+                                                                         ksum, clen, hitCount.p, nullptr, nullptr);
+                } else {
+                    float *pk = part.p, *pl = part.p + L, *pkc = part.p + 2 * (size_t)L, *plc = part.p + 3 * (size_t)L;
+                    k_segmentReduce<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(recLeafS, recValS, total, segStart.p, L,
+                                                                         pk, pl, partHits.p, pkc, plc);
+                    k_mergeChunk<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(L, pk, pkc, pl, plc, partHits.p, acc.p,
+                                                                      acc.p + L, acc.p + 2 * (size_t)L,
+                                                                      acc.p + 3 * (size_t)L, hitCount.p);
+                }
+                {   // next range: about 3/4 of the record limit at this range's records per link
+                    const double perLink = std::max(1.0, (double)total / n);
+                    rangeLinks = (u32)std::max(1.0, std::min((double)chunkLinks, 0.75 * CONE_MAX_RECORDS / perLink));
+                }
+                l0 += n;
+            }
+            if (merging)
+                k_finishChunks<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(L, acc.p, acc.p + L, acc.p + 2 * (size_t)L,
+                                                                    acc.p + 3 * (size_t)L, ksum, clen);
+            field->hits[type] = totalHits;
+            field->overflowLinks[type] = totalOver;
         }
         if (!sampleness) {
             k_setLeafFlags<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(hitCount.p, tree->leafNode.p, L, FLAG_E, field->states.p);
@@ -1103,10 +1256,14 @@ void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *s
             const int kind = priorsPacketMode();
             const bool packet = kind == 1;
             timedLaunch(ctx, KSTAT_PRIORS, [&] {
-                if (kind == 2)
-                    k_priorsGroup<<<grid, 64, 0, st>>>(tree->frame, pnodes.p, tree->leafLoc.p, queries.p, nq,
-                                                       prm.relative_radius_for_priors, field->prior.p, perr.p,
-                                                       kst.p, (xcdMapMode() >> 1) & 1);
+                if (kind == 2 && priorsGroupSize() == 4)
+                    k_priorsGroup<4><<<grid, 64, 0, st>>>(tree->frame, pnodes.p, tree->leafLoc.p, queries.p, nq,
+                                                          prm.relative_radius_for_priors, field->prior.p, perr.p,
+                                                          kst.p, (xcdMapMode() >> 1) & 1);
+                else if (kind == 2)
+                    k_priorsGroup<8><<<grid, 64, 0, st>>>(tree->frame, pnodes.p, tree->leafLoc.p, queries.p, nq,
+                                                          prm.relative_radius_for_priors, field->prior.p, perr.p,
+                                                          kst.p, (xcdMapMode() >> 1) & 1);
                 else if (packet)
                     k_priorsPacket<<<grid, 64, 0, st>>>(tree->frame, pnodes.p, tree->leafLoc.p, queries.p, nq,
                                                         prm.relative_radius_for_priors, field->prior.p, perr.p,
@@ -1149,7 +1306,7 @@ void emptinessSubsample(tsr_context *ctx, const tsr_tree *tree, const tsr_scene 
     LinkInputs in{tree->px.p, tree->py.p, tree->pz.p, tree->nx.p, tree->ny.p, tree->nz.p, tree->scale.p,
                   tree->conf.p, tree->cams.p, scene->cx.p, scene->cy.p, scene->cz.p, tree->sampleDepth.p,
                   tree->maxCams, scene->camCount, prm.bandwidth_factor, prm.relative_sample_cone_length, nullptr,
-                  0, stride, nullptr, 0};
+                  0, stride, nullptr, 0, 0};
     HIPCHECK(hipMemsetAsync(leafHits, 0, (size_t)tree->leafCount * 4, st));
     const u32 work = (links + stride - 1) / stride;
     k_conePass<<<gridFor(work, BLOCK), BLOCK, 0, st>>>(tt, in, links, 0, 4, nullptr, nullptr, nullptr, nullptr,

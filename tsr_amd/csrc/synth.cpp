@@ -176,15 +176,37 @@ Prim makeSphere(Vec c, double r) {
 // Cameras at random directions and distances U[dmin,dmax] around `centre`;
 // the horizontal field of view is fitted so that the sphere of radius
 // `sceneRadius` spans the image width (multi-scale sampling of one scene).
-void randomCameras(SceneSpec &sc, int count, double dmin, double dmax, Vec centre, double sceneRadius, u64 seed) {
+void primBox(const Prim &p, Vec &lo, Vec &hi);
+
+// Cameras look at `centre` from distances U[dmin, dmax]; the field of view fits a
+// sphere of `sceneRadius` (90 degrees for a camera inside 1.1 x that radius).  With
+// clearance > 0 a camera position closer than `clearance` to any primitive's box
+// is drawn again (next counter), so that no depth map starts right at a surface.
+void randomCameras(SceneSpec &sc, int count, double dmin, double dmax, Vec centre, double sceneRadius, u64 seed,
+                   double clearance = 0.0) {
     for (int k = 0; k < count; ++k) {
-        u32 r[4];
-        Philox::gen(r, (u32)k, 0xCA3u, 7u, 0u, seed);
-        const Vec dir = randomUnit(r[0], r[1]);
-        const double dist = dmin + (dmax - dmin) * Philox::uniform(r[2]);
-        Camera c = lookAt(centre + dir * dist, centre);
-        c.tanHalf = sceneRadius / std::sqrt(std::max(1e-12, dist * dist - sceneRadius * sceneRadius));
-        sc.cams.push_back(c);
+        for (u32 attempt = 0;; ++attempt) {
+            u32 r[4];
+            Philox::gen(r, (u32)k, 0xCA3u, 7u + 1000u * attempt, 0u, seed);
+            const Vec dir = randomUnit(r[0], r[1]);
+            const double dist = dmin + (dmax - dmin) * Philox::uniform(r[2]);
+            const Vec pos = centre + dir * dist;
+            bool clear = true;
+            for (size_t q = 0; clear && clearance > 0.0 && q < sc.prims.size(); ++q) {
+                Vec a, b;
+                primBox(sc.prims[q], a, b);
+                const double dx = std::max({a.x - pos.x, 0.0, pos.x - b.x}), dy = std::max({a.y - pos.y, 0.0, pos.y - b.y}),
+                             dz = std::max({a.z - pos.z, 0.0, pos.z - b.z});
+                clear = dx * dx + dy * dy + dz * dz >= clearance * clearance;
+            }
+            if (!clear && attempt < 1000) continue;
+            Camera c = lookAt(pos, centre);
+            c.tanHalf = dist > 1.1 * sceneRadius
+                            ? sceneRadius / std::sqrt(dist * dist - sceneRadius * sceneRadius)
+                            : 1.0;
+            sc.cams.push_back(c);
+            break;
+        }
     }
 }
 
@@ -259,7 +281,7 @@ SceneSpec makeScene(int config, u64 seed, double resScale) {
     }
     case 4: {  // 30 random primitives in [-4,4]^3, 120 cameras at 3..20
         randomPrimitives(sc, 30, -3.0, 3.0, 2.0, seed);
-        randomCameras(sc, 120, 3.0, 20.0, {0, 0, 0}, 4.5, seed);
+        randomCameras(sc, 120, 3.0, 20.0, {0, 0, 0}, 4.5, seed, 0.5);
         res(1024, 768);
         sc.sigma = 0.003;
         sc.outlierRate = 0.02;

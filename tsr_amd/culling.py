@@ -77,11 +77,14 @@ def fold_standardisation(w, mean, std):
     return w.astype(np.float32)
 
 
-def mesh_points(mesh, per_edge=3):
-    """vertices plus a barycentric grid of points on every triangle"""
+def mesh_points(mesh, per_edge=None):
+    """vertices plus a barycentric grid of points on every triangle (3 per edge; vertices
+    only for meshes above 2M triangles)"""
     P = mesh["positions"].reshape(-1, 3)
     T = mesh["triangles"].reshape(-1, 3).astype(np.int64)
-    if len(T) == 0:
+    if per_edge is None:
+        per_edge = 3 if len(T) <= 2_000_000 else 1
+    if len(T) == 0 or per_edge <= 1:
         return P
     pts = [P]
     a, b, c = P[T[:, 0]], P[T[:, 1]], P[T[:, 2]]
@@ -106,12 +109,13 @@ def hausdorff(mesh_a, mesh_b):
     return float(max(dab, dba))
 
 
-def evaluate(ctx, culler, seed, res):
+def evaluate(ctx, culler, seed, res, config=SCENE_CONFIG):
     """plain vs learned on one scene: times, work, misses, sign flips, mesh distance"""
-    syn = SyntheticScene(SCENE_CONFIG, seed=seed, resolution_scale=res)
+    syn = SyntheticScene(config, seed=seed, resolution_scale=res)
     sc = Scene(ctx, syn.samples, syn.cameras)
     tree = Tree(ctx, sc)
-    out = {"seed": seed, "samples": syn.count, "nodes": tree.node_count, "leaves": tree.leaf_count}
+    out = {"config": config, "seed": seed, "samples": syn.count, "nodes": tree.node_count,
+           "leaves": tree.leaf_count}
     res_ = {}
     for mode in ("plain", "learned"):
         best = None
@@ -182,7 +186,24 @@ def main(argv=None):
     ap.add_argument("--max-hausdorff", type=float, default=1e-3, help="bound, fraction of the bbox diagonal")
     ap.add_argument("--out", default=DEFAULT_WEIGHTS)
     ap.add_argument("--report", default=None, help="JSON report path")
+    ap.add_argument("--eval-only", action="store_true",
+                    help="no training: evaluate the weights in --out on --holdout seeds of --config")
+    ap.add_argument("--config", type=int, default=SCENE_CONFIG, help="scene config of --eval-only")
     args = ap.parse_args(argv)
+    if args.eval_only:
+        ctx = Context(0)
+        culler = load_culler(ctx, args.out)
+        rs = []
+        for seed in _seeds(args.holdout):
+            r = evaluate(ctx, culler, seed, args.res, args.config)
+            print(json.dumps(r), flush=True)
+            rs.append(r)
+        if args.report:
+            with open(args.report, "w") as f:
+                json.dump(rs, f, indent=1)
+        culler.close()
+        ctx.close()
+        return 0
 
     ctx = Context(0)
     rng = np.random.default_rng(0)
