@@ -76,7 +76,14 @@ struct TraversalTree {
     const u32 *states;     // node state flags (E pass gate), may be null
     Frame f;
     float margin;          // absolute culling margin
+    const uint2 *packed;   // (child, state) per node for gated passes: one load per visit; may be null
 };
+
+__global__ void k_packChildState(const u32 *__restrict__ child, const u32 *__restrict__ states, u32 n,
+                                 uint2 *__restrict__ out) {
+    const u32 i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) out[i] = make_uint2(child[i], states[i]);
+}
 
 struct LinkInputs {
     const float *px, *py, *pz, *nx, *ny, *nz, *scale, *conf;
@@ -127,9 +134,16 @@ __device__ void traverseCone(const TraversalTree &t, const ConeCull &cull, u32 m
         if (descend) {
             ++visits;
             bool accept = d <= maxDepth;
-            if (accept && required) accept = (t.states[node] & required) == required;
+            u32 c = 0u;
+            if (t.packed) {
+                const uint2 cs = t.packed[node];
+                if (accept && required) accept = (cs.y & required) == required;
+                c = cs.x;
+            } else if (accept && required) {
+                accept = (t.states[node] & required) == required;
+            }
             if (accept && cullNode) accept = cullNode[node] == 0;
-            const u32 c = accept ? t.child[node] : 0u;
+            if (!t.packed) c = accept ? t.child[node] : 0u;
             if (accept) {
                 if (c & LEAF_BIT) {
                     double x[3];
@@ -712,7 +726,8 @@ __global__ __launch_bounds__(64) void k_priorsGroup(Frame f, const PriorNode *__
     __shared__ u32 stNode[PRIOR_STACK];
     __shared__ uint8_t stMask[PRIOR_STACK];
     __shared__ double qc[GROUP_Q][3], qr2[GROUP_Q], qri2[GROUP_Q];
-    __shared__ float qcF[GROUP_Q][3], qr2F[GROUP_Q], qri2F[GROUP_Q], qmarg[GROUP_Q];
+    // FP32 query data packed for two LDS reads per query: (centre, r^2), (inner r^2, margin, depth limit)
+    __shared__ float4 qA[GROUP_Q], qB[GROUP_Q];
     __shared__ u32 qmd[GROUP_Q];
     __shared__ u32 sp;
     const u32 lane = threadIdx.x;
@@ -739,14 +754,15 @@ __global__ __launch_bounds__(64) void k_priorsGroup(Frame f, const PriorNode *__
             const u32 dq = locDepth(qloc);
             for (int a = 0; a < 3; ++a) {
                 qc[lane][a] = f.centre(a, kq[a], dq);
-                qcF[lane][a] = (float)qc[lane][a];
+                (void)0;
             }
             const double radius = f.h(dq) * (double)relRadius;
             qr2[lane] = radius * radius;
             qri2[lane] = radius * radius * (1.0 - 1e-9);
-            qr2F[lane] = (float)qr2[lane];
-            qri2F[lane] = (float)qri2[lane];
-            qmarg[lane] = (float)(4.0 * eps * radius + 4.0 * eps * eps + ldexp(radius * radius, -20));
+            qA[lane] = make_float4((float)qc[lane][0], (float)qc[lane][1], (float)qc[lane][2], (float)qr2[lane]);
+            qB[lane] = make_float4((float)qri2[lane],
+                                   (float)(4.0 * eps * radius + 4.0 * eps * eps + ldexp(radius * radius, -20)),
+                                   __uint_as_float(dq + MAX_DEPTH_DIFFERENCE), 0.f);
             qmd[lane] = dq + MAX_DEPTH_DIFFERENCE;
         }
         double accE[GROUP_Q], accS[GROUP_Q];
@@ -800,19 +816,23 @@ __global__ __launch_bounds__(64) void k_priorsGroup(Frame f, const PriorNode *__
                     const u32 mld = cn.aux >> 8;
 #pragma unroll
                     for (u32 j = 0; j < GROUP_Q; ++j) {
-                        if (!((mask >> j) & 1u) || d > qmd[j]) continue;
-                        const float cq[3] = {qcF[j][0], qcF[j][1], qcF[j][2]};
+                        if (!((mask >> j) & 1u)) continue;
+                        const float4 A = qA[j], B = qB[j];
+                        const u32 md = __float_as_uint(B.z);
+                        if (d > md) continue;
+                        const float cq[3] = {A.x, A.y, A.z};
+                        // fused multiply-adds: one rounding each, inside the error bound above
                         float dminF = 0.f, dmaxF = 0.f;
                         for (int q3 = 0; q3 < 3; ++q3) {
                             const float a0 = loF[q3] - cq[q3], a1 = cq[q3] - hiF[q3];
                             const float o = fmaxf(0.f, fmaxf(a0, a1));
-                            dminF += o * o;
+                            dminF = __builtin_fmaf(o, o, dminF);
                             const float far = fmaxf(fabsf(a0), fabsf(a1));
-                            dmaxF += far * far;
+                            dmaxF = __builtin_fmaf(far, far, dmaxF);
                         }
-                        const float mg = qmarg[j];
-                        if (dminF > qr2F[j] + mg) continue;   // surely outside
-                        if (!(dminF < qr2F[j] - mg)) {        // undecided: the FP64 test
+                        const float mg = B.y;
+                        if (dminF > A.w + mg) continue;   // surely outside
+                        if (!(dminF < A.w - mg)) {        // undecided: the FP64 test
                             double dmin = 0.0;
                             for (int q3 = 0; q3 < 3; ++q3) {   // SphereNodesChecker.cpp:29-40
                                 const double c = qc[j][q3];
@@ -823,9 +843,9 @@ __global__ __launch_bounds__(64) void k_priorsGroup(Frame f, const PriorNode *__
                             if (!(dmin <= qr2[j])) continue;
                         }
                         bool whole = leaf;
-                        if (!whole && mld <= qmd[j]) {
-                            if (dmaxF < qri2F[j] - mg) whole = true;
-                            else if (!(dmaxF > qri2F[j] + mg)) {
+                        if (!whole && mld <= md) {
+                            if (dmaxF < B.x - mg) whole = true;
+                            else if (!(dmaxF > B.x + mg)) {
                                 double dmax = 0.0;
                                 for (int q3 = 0; q3 < 3; ++q3) {
                                     const double c = qc[j][q3];
@@ -980,7 +1000,8 @@ void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *s
     HIPCHECK(hipMemsetAsync(field->states.p, 0, (size_t)N * 4, st));
 
     TraversalTree tt{tree->child.p, tree->parents.p, field->states.p, tree->frame,
-                     (float)(tree->frame.rootSize * 1e-5)};
+                     (float)(tree->frame.rootSize * 1e-5), nullptr};
+    DBuf<uint2> packedNodes;
     LinkInputs in{tree->px.p, tree->py.p, tree->pz.p, tree->nx.p, tree->ny.p, tree->nz.p, tree->scale.p,
                   tree->conf.p, tree->cams.p, scene->cx.p, scene->cy.p, scene->cz.p, tree->sampleDepth.p,
                   tree->maxCams, scene->camCount, prm.bandwidth_factor, prm.relative_sample_cone_length, nullptr};
@@ -1014,6 +1035,9 @@ void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *s
             k_setLeafFlags<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(hitCount.p, tree->leafNode.p, L, FLAG_S, field->states.p);
             k_propagateFlag<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(tree->leafNode.p, L, tree->parents.p, FLAG_S,
                                                                  field->states.p);
+            packedNodes.alloc(N);   // (child, state) pairs for the gated emptiness traversal
+            k_packChildState<<<gridFor(N, BLOCK), BLOCK, 0, st>>>(tree->child.p, field->states.p, N, packedNodes.p);
+            tt.packed = packedNodes.p;
             if (culler) {   // learned culling: nodes predicted far from any zero crossing
                 cullerMask(ctx, culler, tree, scene, prm, field->states.p, field->hitsS.p,
                            field->ksum.p + L, cullMask);
@@ -1302,7 +1326,8 @@ void emptinessSubsample(tsr_context *ctx, const tsr_tree *tree, const tsr_scene 
                         const u32 *states, u32 stride, u32 *leafHits) {
     hipStream_t st = ctx->stream;
     const u32 links = (u32)((u64)scene->n * scene->maxCams);
-    TraversalTree tt{tree->child.p, tree->parents.p, states, tree->frame, (float)(tree->frame.rootSize * 1e-5)};
+    TraversalTree tt{tree->child.p, tree->parents.p, states, tree->frame, (float)(tree->frame.rootSize * 1e-5),
+                     nullptr};
     LinkInputs in{tree->px.p, tree->py.p, tree->pz.p, tree->nx.p, tree->ny.p, tree->nz.p, tree->scale.p,
                   tree->conf.p, tree->cams.p, scene->cx.p, scene->cy.p, scene->cz.p, tree->sampleDepth.p,
                   tree->maxCams, scene->camCount, prm.bandwidth_factor, prm.relative_sample_cone_length, nullptr,
