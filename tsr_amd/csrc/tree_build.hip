@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <chrono>
 #include <functional>
+#include <future>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -163,17 +164,18 @@ TSR_HD bool findReliable(const Frame &f, u64 start, bool positive, const Samplin
 // reaches this node first and yields the same leaves in the same order.  The
 // ball is inflated far above FP64 rounding before the strict-containment test.
 // Returns the root when the ball is not strictly inside the root box.
+// Deepest node on the ancestor chain of the box [a0, a1] that strictly contains it:
+// the deepest inner one, or its child on the chain when that child is a leaf
+// (innerOnly = false).  Returns the root when the box is not strictly inside the root.
 template <typename IsInner>
-TSR_HD u64 searchStart(const Frame &f, const SamplingUnit &u, IsInner isInner) {
-    const double r = u.msd * (1.0 + 1e-9) + 1e-12 * f.rootSize;
+TSR_HD u64 deepestContaining(const Frame &f, const double a0[3], const double a1[3], bool innerOnly, IsInner isInner) {
     u32 lo[3], diff = 0;
     for (int a = 0; a < 3; ++a) {
-        const double a0 = u.p[a] - r, a1 = u.p[a] + r;
-        if (!(a0 > f.rootMin[a]) || !(a1 < f.B(a, 1ull << MAX_DEPTH, MAX_DEPTH))) return 1ull;
+        if (!(a0[a] > f.rootMin[a]) || !(a1[a] < f.B(a, 1ull << MAX_DEPTH, MAX_DEPTH))) return 1ull;
         // cellCoord: B(k) <= x < B(k + 1), so a0 > B(lo) needs a0 != B(lo)
-        lo[a] = cellCoord(f, a, a0, MAX_DEPTH);
-        if (f.B(a, lo[a], MAX_DEPTH) == a0) return 1ull;
-        diff |= lo[a] ^ cellCoord(f, a, a1, MAX_DEPTH);
+        lo[a] = cellCoord(f, a, a0[a], MAX_DEPTH);
+        if (f.B(a, lo[a], MAX_DEPTH) == a0[a]) return 1ull;
+        diff |= lo[a] ^ cellCoord(f, a, a1[a], MAX_DEPTH);
     }
     const u32 dc = diff ? MAX_DEPTH - (32u - (u32)__builtin_clz(diff)) : MAX_DEPTH;
     auto anc = [&](u32 d) { return makeLoc(lo[0] >> (MAX_DEPTH - d), lo[1] >> (MAX_DEPTH - d), lo[2] >> (MAX_DEPTH - d), d); };
@@ -185,7 +187,25 @@ TSR_HD u64 searchStart(const Frame &f, const SamplingUnit &u, IsInner isInner) {
         if (isInner(anc(mid))) good = mid;
         else bad = mid;
     }
-    return good == dc ? anc(dc) : anc(good + 1);
+    return (good == dc || innerOnly) ? anc(good) : anc(good + 1);
+}
+
+// Where the DFS of findReliable can start without changing its result: the
+// deepest existing node whose box strictly contains the unit's search ball
+// (centre p, radius msd) -- every node that intersectNode accepts lies inside
+// it, all its ancestors pass intersectNode (the sample plane crosses their
+// interior), and their other children are rejected, so a DFS from the root
+// reaches this node first and yields the same leaves in the same order.  The
+// ball is inflated far above FP64 rounding before the strict-containment test.
+template <typename IsInner>
+TSR_HD u64 searchStart(const Frame &f, const SamplingUnit &u, IsInner isInner) {
+    const double r = u.msd * (1.0 + 1e-9) + 1e-12 * f.rootSize;
+    double a0[3], a1[3];
+    for (int a = 0; a < 3; ++a) {
+        a0[a] = u.p[a] - r;
+        a1[a] = u.p[a] + r;
+    }
+    return deepestContaining(f, a0, a1, false, isInner);
 }
 
 __device__ SamplingUnit loadUnit(const float *px, const float *py, const float *pz, const float *nx, const float *ny,
@@ -688,18 +708,29 @@ struct InnerView {
     }
 };
 
-void replaySampling(const Frame &f, const tsr_scene *scene, float maxRel, const std::vector<u64> &t0Inner,
+// host copies of the containment tree's inner set for the replay: T0 (frozen) and the
+// growing inner set (built on host threads while the GPU runs the speculation)
+struct ReplaySets {
+    FlatSet t0;
+    InnerSet inner;
+};
+ReplaySets buildReplaySets(const std::vector<u64> &t0Inner) {
+    ReplaySets r;
+    r.t0.init(t0Inner.size());
+    r.inner.init(t0Inner.size() * 2 + 1024);
+    parallelFor(t0Inner.size(), [&](size_t b, size_t e) {
+        for (size_t i = b; i < e; ++i) r.t0.insertConcurrent(t0Inner[i]);
+    });
+    r.inner.build(t0Inner);
+    return r;
+}
+
+void replaySampling(const Frame &f, const tsr_scene *scene, float maxRel, ReplaySets &sets,
                     const std::vector<uint8_t> &isEvent, VisitIndex &index, std::vector<u64> &created,
                     u64 &eventCount) {
     const auto tInit = std::chrono::steady_clock::now();
-    FlatSet t0;
-    InnerSet inner;
-    t0.init(t0Inner.size());
-    inner.init(t0Inner.size() * 2 + 1024);
-    parallelFor(t0Inner.size(), [&](size_t b, size_t e) {
-        for (size_t i = b; i < e; ++i) t0.insertConcurrent(t0Inner[i]);
-    });
-    inner.build(t0Inner);
+    FlatSet &t0 = sets.t0;
+    InnerSet &inner = sets.inner;
     const u32 units = (u32)isEvent.size();
     std::vector<uint8_t> pending(isEvent.begin(), isEvent.end());
     std::priority_queue<u32, std::vector<u32>, std::greater<u32>> heap;
@@ -762,10 +793,19 @@ void replaySampling(const Frame &f, const tsr_scene *scene, float maxRel, const 
             clock::time_point t;
             ~AccC() { ms += std::chrono::duration<double, std::milli>(clock::now() - t).count(); }
         } accC{createMs, tc};
-        // Nodes.cpp:284-336
+        // Nodes.cpp:284-336.  The descent from the root passes, without creating anything,
+        // through the inner nodes that strictly contain the segment [p, p +- n msd] (in
+        // each, the one child holding the whole segment has overlap 1 and every other
+        // child misses it), so it starts at the deepest of them.
         const double sgn = positive ? u.msd : -u.msd;
         const double en[3] = {u.p[0] + u.n[0] * sgn, u.p[1] + u.n[1] * sgn, u.p[2] + u.n[2] * sgn};
-        u64 n = 1ull;
+        double s0[3], s1[3];
+        for (int a = 0; a < 3; ++a) {
+            const double m = 1e-9 * std::fabs(en[a] - u.p[a]) + 1e-12 * f.rootSize;
+            s0[a] = std::min(u.p[a], en[a]) - m;
+            s1[a] = std::max(u.p[a], en[a]) + m;
+        }
+        u64 n = deepestContaining(f, s0, s1, true, isInner);
         for (;;) {
             if (!inner.contains(n)) {
                 if (locDepth(n) + 1 > MAX_DEPTH) throw tsr_error(TSR_ERR_TREE_TOO_DEEP, "octree deeper than 19 levels");
@@ -1061,6 +1101,24 @@ void buildTree(tsr_context *ctx, const tsr_scene *scene, const tsr_params &prm, 
         maxContainerDepth = readScalar(md.p, st);
     }
     mark("containment");
+    // T0 inner node list
+    DBuf<u64> innerList(inner.capacity);
+    DBuf<u32> t0Counter(1);
+    HIPCHECK(hipMemsetAsync(t0Counter.p, 0, 4, st));
+    k_compactKeys<<<gridFor(inner.capacity, BLOCK), BLOCK, 0, st>>>(inner.keys.p, inner.capacity, innerList.p,
+                                                                     t0Counter.p);
+    const u32 t0Count = readScalar(t0Counter.p, st);
+    std::vector<u64> hInner(t0Count);
+    if (t0Count) {   // ascending codes = ascending depth: the host inserts parents first
+        DBuf<u64> sortedInner(t0Count);
+        sortKeys(sc, innerList.p, sortedInner.p, t0Count, 0, (int)(3 * maxContainerDepth + 1), st);
+        HIPCHECK(hipMemcpyAsync(hInner.data(), sortedInner.p, (size_t)t0Count * 8, hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+    }
+    HIPCHECK(hipStreamSynchronize(st));
+    // the replay's host sets are built on host threads while the GPU speculates
+    std::future<ReplaySets> setsFuture = std::async(std::launch::async, buildReplaySets, std::cref(hInner));
+
     // ---- sampling refinement: speculative evaluation on T0
     const u32 units = 2 * n;
     DBuf<u32> visitCount(units), offsets(units);
@@ -1174,23 +1232,12 @@ void buildTree(tsr_context *ctx, const tsr_scene *scene, const tsr_params &prm, 
         if (m) HIPCHECK(hipMemcpyAsync(out.data(), all.p, (size_t)m * 4, hipMemcpyDeviceToHost, st));
         HIPCHECK(hipStreamSynchronize(st));
     };
-    // T0 inner node list
-    DBuf<u64> innerList(inner.capacity);
-    HIPCHECK(hipMemsetAsync(counter.p, 0, 4, st));
-    k_compactKeys<<<gridFor(inner.capacity, BLOCK), BLOCK, 0, st>>>(inner.keys.p, inner.capacity, innerList.p, counter.p);
-    const u32 t0Count = readScalar(counter.p, st);
-    std::vector<u64> hInner(t0Count);
-    if (t0Count) {   // ascending codes = ascending depth: the host inserts parents first
-        DBuf<u64> sortedInner(t0Count);
-        sortKeys(sc, innerList.p, sortedInner.p, t0Count, 0, (int)(3 * maxContainerDepth + 1), st);
-        HIPCHECK(hipMemcpyAsync(hInner.data(), sortedInner.p, (size_t)t0Count * 8, hipMemcpyDeviceToHost, st));
-        HIPCHECK(hipStreamSynchronize(st));
-    }
-    HIPCHECK(hipStreamSynchronize(st));
     mark("sampling speculation");
     std::vector<u64> created;
     u64 events = 0;
-    replaySampling(f, scene, maxRel, hInner, hEvent, index, created, events);
+    ReplaySets sets = setsFuture.get();
+    mark("replay sets (host, overlapped)");
+    replaySampling(f, scene, maxRel, sets, hEvent, index, created, events);
     if (trace)
         std::fprintf(stderr, "[tsr tree] units %u pairs %zu hot pairs %zu t0-inner %u events %llu created nodes %zu "
                      "index misses %llu\n", units, pairs, hotPairs, t0Count, (unsigned long long)events,
