@@ -612,11 +612,27 @@ struct FlatMap {
     }
 };
 
+// read-only host view of an open-addressing u64 -> u32 table laid out like the
+// device HashView (same hash, linear probing, key 0 = empty), e.g. one built on the
+// GPU and copied to host memory
+struct FlatMapView {
+    const u64 *keys = nullptr;
+    const u32 *vals = nullptr;
+    u64 mask = 0;
+    const u32 *find(u64 k) const {
+        for (u64 s = hashMix(k) & mask;; s = (s + 1) & mask) {
+            const u64 v = keys[s];
+            if (v == k) return &vals[s];
+            if (v == 0ull) return nullptr;
+        }
+    }
+};
+
 // host view of the (T0 leaf -> units that visited it) index for the "hot" leaves
 // (those near units the speculation could not settle); other leaves are looked
 // up on the device on demand
 struct VisitIndex {
-    FlatMap run;                  // hot leaf -> run index into runStart, INVALID if no unit visited it
+    FlatMapView run;              // hot leaf -> run index into runStart, INVALID if no unit visited it
     const u32 *runStart = nullptr;   // runs + 1 offsets into unit
     const u32 *unit = nullptr;       // visiting units, ascending within a run
     std::function<void(u64, std::vector<u32> &)> fetch;   // exact lookup on the device
@@ -983,6 +999,15 @@ __global__ void k_hotFlags(HashView hot, const u64 *__restrict__ leaf, size_t n,
 }
 
 // [first, last) of `key` in the sorted array (one thread)
+__global__ void k_insertIndexed(HashView h, const u64 *__restrict__ keys, u32 n, u32 *err) {
+    const u32 i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) h.insert(keys[i], i, err);
+}
+__global__ void k_insertConst(HashView h, const u64 *__restrict__ keys, u32 n, u32 val, u32 *err) {
+    const u32 i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) h.insert(keys[i], val, err);
+}
+
 __global__ void k_matchLeaf(const u64 *__restrict__ leaf, size_t n, u64 key, uint8_t *__restrict__ flag) {
     const size_t j = (size_t)blockIdx.x * BLOCK + threadIdx.x;
     if (j < n) flag[j] = leaf[j] == key ? 1 : 0;
@@ -1183,41 +1208,41 @@ void buildTree(tsr_context *ctx, const tsr_scene *scene, const tsr_params &prm, 
             runs = readScalar(counter.p, st);
             exclusiveScan(sc, runLen.p, runOff.p, runs, st);
         }
-        // hot leaf codes, run leaves, run offsets and units through the pinned staging buffer
+        // the hot-leaf -> run map is built on the GPU (run leaves first, then the other hot
+        // leaves with INVALID) in the host map's layout, and copied with the run offsets and
+        // units through the pinned staging buffer
         DBuf<u64> hotKeys(hot.capacity);
         HIPCHECK(hipMemsetAsync(counter.p + 0, 0, 4, st));
         k_compactKeys<<<gridFor(hot.capacity, BLOCK), BLOCK, 0, st>>>(hot.keys.p, hot.capacity, hotKeys.p, counter.p);
         const u32 nh = readScalar(counter.p, st);
-        const size_t bytes = (size_t)nh * 8 + (size_t)runs * 8 + ((size_t)runs + 1) * 4 + hotPairs * 4;
+        DHash runMap;
+        runMap.alloc(std::max<u64>(4096, std::max<u64>(nh, runs)), st);
+        if (runs) k_insertIndexed<<<gridFor(runs, BLOCK), BLOCK, 0, st>>>(runMap.view(), runLeaf.p, runs, err.p);
+        if (nh) k_insertConst<<<gridFor(nh, BLOCK), BLOCK, 0, st>>>(runMap.view(), hotKeys.p, nh, INVALID, err.p);
+        const size_t cap = runMap.capacity;
+        const size_t bytes = cap * 8 + cap * 4 + ((size_t)runs + 1) * 4 + hotPairs * 4;
         char *pin = (char *)pinnedHost(ctx, bytes + 64);
-        u64 *hHot = (u64 *)pin;
-        u64 *hRunLeaf = hHot + nh;
-        u32 *hRunStart = (u32 *)(hRunLeaf + runs);
+        u64 *hKeys = (u64 *)pin;
+        u32 *hVals = (u32 *)(hKeys + cap);
+        u32 *hRunStart = hVals + cap;
         u32 *hUnit = hRunStart + runs + 1;
-        if (nh) HIPCHECK(hipMemcpyAsync(hHot, hotKeys.p, (size_t)nh * 8, hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipMemcpyAsync(hKeys, runMap.keys.p, cap * 8, hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipMemcpyAsync(hVals, runMap.vals.p, cap * 4, hipMemcpyDeviceToHost, st));
         if (runs) {
-            HIPCHECK(hipMemcpyAsync(hRunLeaf, runLeaf.p, (size_t)runs * 8, hipMemcpyDeviceToHost, st));
             HIPCHECK(hipMemcpyAsync(hRunStart, runOff.p, (size_t)runs * 4, hipMemcpyDeviceToHost, st));
             HIPCHECK(hipMemcpyAsync(hUnit, hotUnitSorted.p, hotPairs * 4, hipMemcpyDeviceToHost, st));
         }
         HIPCHECK(hipStreamSynchronize(st));
+        checkErr(readScalar(err.p, st));
         hRunStart[runs] = (u32)hotPairs;
         index.runStart = hRunStart;
         index.unit = hUnit;
+        index.run = FlatMapView{hKeys, hVals, cap - 1};
         mark("hot index transfer");
         if (trace) std::fprintf(stderr, "[tsr tree]   hot leaves %u, runs %u\n", nh, runs);
-        // run leaves first (-> run index), then the other hot leaves (-> none)
-        index.run.init(std::max<size_t>(nh, runs));
-        parallelFor(runs, [&](size_t b, size_t e) {
-            for (size_t r = b; r < e; ++r) index.run.putIfAbsentConcurrent(hRunLeaf[r], (u32)r);
-        });
-        parallelFor(nh, [&](size_t b, size_t e) {
-            for (size_t i = b; i < e; ++i) index.run.putIfAbsentConcurrent(hHot[i], INVALID);
-        });
     }
     hotLeafSorted.release();
     hotUnitSorted.release();
-    mark("hot map (host)");
     // a visited leaf outside the hot part: select its units from the full (unit-ordered) pair list
     index.fetch = [&](u64 loc, std::vector<u32> &out) {
         out.clear();
