@@ -217,7 +217,12 @@ __device__ SamplingUnit loadUnit(const float *px, const float *py, const float *
     return u;
 }
 
-// mode 0: count visited leaves and flag creating units; mode 1: write (leaf, unit) pairs
+// mode 0: count visited leaves and flag creating units; mode 1: write (leaf, unit) pairs;
+// mode 2: as mode 0, and stage the first VISIT_CAP visited leaves in the unit's slots
+// (visitSlot: groups of 256 units own contiguous rows, row j = j-th visit)
+__device__ __forceinline__ size_t visitSlot(u32 unit, u32 j) {
+    return ((size_t)(unit / 256) * VISIT_CAP + j) * 256 + (unit % 256);
+}
 __global__ void k_samplingEval(Frame f, HashView inner, const float *__restrict__ px, const float *__restrict__ py,
                                const float *__restrict__ pz, const float *__restrict__ nx,
                                const float *__restrict__ ny, const float *__restrict__ nz,
@@ -238,11 +243,13 @@ __global__ void k_samplingEval(Frame f, HashView inner, const float *__restrict_
         if (mode == 1 && count < VISIT_CAP) {
             pairLeaf[base + count] = loc;
             pairUnit[base + count] = unit;
+        } else if (mode == 2 && count < VISIT_CAP) {
+            pairLeaf[visitSlot(unit, count)] = loc;
         }
         ++count;
     };
     const bool found = findReliable(f, searchStart(f, u, isInner), positive, u, isInner, onLeaf);
-    if (mode == 0) {
+    if (mode == 0 || mode == 2) {
         const bool event = !found || count > VISIT_CAP;
         isEvent[unit] = event ? 1 : 0;
         visitCount[unit] = event ? 0 : count;
@@ -993,6 +1000,38 @@ __global__ void k_markHot(Frame f, HashView inner, HashView hot, const float *__
     }
 }
 
+// hot visits per unit (non-event units only) from the staged visits; mode 1 writes them
+__global__ void k_hotVisits(HashView hot, const u64 *__restrict__ staged, const u32 *__restrict__ visitCount,
+                            const uint8_t *__restrict__ isEvent, u32 units, int mode, u32 *__restrict__ hotCount,
+                            const u32 *__restrict__ hotOff, u64 *__restrict__ outLeaf, u32 *__restrict__ outUnit) {
+    const u32 u = blockIdx.x * BLOCK + threadIdx.x;
+    if (u >= units) return;
+    const u32 c = isEvent[u] ? 0u : visitCount[u];
+    u32 k = 0;
+    const size_t base = mode == 1 ? hotOff[u] : 0;
+    for (u32 j = 0; j < c; ++j) {
+        const u64 leaf = staged[visitSlot(u, j)];
+        if (!hot.contains(leaf)) continue;
+        if (mode == 1) {
+            outLeaf[base + k] = leaf;
+            outUnit[base + k] = u;
+        }
+        ++k;
+    }
+    if (mode == 0) hotCount[u] = k;
+}
+
+// units (ascending) whose staged visits include `key`
+__global__ void k_matchStaged(const u64 *__restrict__ staged, const u32 *__restrict__ visitCount,
+                              const uint8_t *__restrict__ isEvent, u32 units, u64 key, uint8_t *__restrict__ flag) {
+    const u32 u = blockIdx.x * BLOCK + threadIdx.x;
+    if (u >= units) return;
+    const u32 c = isEvent[u] ? 0u : visitCount[u];
+    uint8_t hit = 0;
+    for (u32 j = 0; j < c && !hit; ++j) hit = staged[visitSlot(u, j)] == key;
+    flag[u] = hit;
+}
+
 __global__ void k_hotFlags(HashView hot, const u64 *__restrict__ leaf, size_t n, uint8_t *__restrict__ flag) {
     const size_t j = (size_t)blockIdx.x * BLOCK + threadIdx.x;
     if (j < n) flag[j] = hot.contains(leaf[j]) ? 1 : 0;
@@ -1148,46 +1187,45 @@ void buildTree(tsr_context *ctx, const tsr_scene *scene, const tsr_params &prm, 
     const u32 units = 2 * n;
     DBuf<u32> visitCount(units), offsets(units);
     DBuf<uint8_t> isEvent(units);
+    // one traversal per unit: visit counts, creating units, and the visits staged
+    DBuf<u64> staged(((size_t)units + 255) / 256 * 256 * VISIT_CAP);
     k_samplingEval<<<gridFor(units, BLOCK), BLOCK, 0, st>>>(f, inner.view(), scene->px.p, scene->py.p, scene->pz.p,
                                                             scene->nx.p, scene->ny.p, scene->nz.p, scene->scale.p,
-                                                            maxRel, units, 0, visitCount.p, isEvent.p, nullptr,
-                                                            nullptr, nullptr);
-    exclusiveScan(sc, visitCount.p, offsets.p, units, st);
-    const u32 lastOff = readScalar(offsets.p + units - 1, st), lastCnt = readScalar(visitCount.p + units - 1, st);
-    const size_t pairs = (size_t)lastOff + lastCnt;
-    DBuf<u64> pairLeaf(std::max<size_t>(pairs, 1));
-    DBuf<u32> pairUnit(std::max<size_t>(pairs, 1));
-    k_samplingEval<<<gridFor(units, BLOCK), BLOCK, 0, st>>>(f, inner.view(), scene->px.p, scene->py.p, scene->pz.p,
-                                                            scene->nx.p, scene->ny.p, scene->nz.p, scene->scale.p,
-                                                            maxRel, units, 1, visitCount.p, isEvent.p, offsets.p,
-                                                            pairLeaf.p, pairUnit.p);
-    mark("speculation passes");
+                                                            maxRel, units, 2, visitCount.p, isEvent.p, nullptr,
+                                                            staged.p, nullptr);
+    mark("speculation pass");
     // transfer only the part of the index around unsettled units
     DHash hot;
     hot.alloc(std::max<u64>(4096, (u64)units / 2), st);
     k_markHot<<<gridFor(units, BLOCK), BLOCK, 0, st>>>(f, inner.view(), hot.view(), scene->px.p, scene->py.p,
                                                        scene->pz.p, scene->scale.p, maxRel, isEvent.p, units, err.p);
-    DBuf<uint8_t> pairFlag(std::max<size_t>(pairs, 1));
     DBuf<u32> counter(1);
     size_t hotPairs = 0;
     DBuf<u64> hotLeafSorted;
     DBuf<u32> hotUnitSorted;
-    if (pairs) {
-        DBuf<u64> hotLeaf(pairs);
-        DBuf<u32> hotUnit(pairs);
-        k_hotFlags<<<gridFor(pairs, BLOCK), BLOCK, 0, st>>>(hot.view(), pairLeaf.p, pairs, pairFlag.p);
-        size_t tmp = 0;
-        HIPCHECK(hipcub::DeviceSelect::Flagged(nullptr, tmp, pairLeaf.p, pairFlag.p, hotLeaf.p, counter.p, (int)pairs, st));
-        HIPCHECK(hipcub::DeviceSelect::Flagged(sc.get(tmp), tmp, pairLeaf.p, pairFlag.p, hotLeaf.p, counter.p, (int)pairs, st));
-        HIPCHECK(hipcub::DeviceSelect::Flagged(nullptr, tmp, pairUnit.p, pairFlag.p, hotUnit.p, counter.p, (int)pairs, st));
-        HIPCHECK(hipcub::DeviceSelect::Flagged(sc.get(tmp), tmp, pairUnit.p, pairFlag.p, hotUnit.p, counter.p, (int)pairs, st));
-        hotPairs = readScalar(counter.p, st);
-        // hot pairs are in unit order; a stable sort by leaf code keeps units ascending per
-        // leaf.  Visited leaves are T0 leaves, no deeper than the deepest container.
+    {   // hot (leaf, unit) pairs in unit order, then a stable sort by leaf code keeps units
+        // ascending per leaf.  Visited leaves are T0 leaves, no deeper than the deepest container.
+        k_hotVisits<<<gridFor(units, BLOCK), BLOCK, 0, st>>>(hot.view(), staged.p, visitCount.p, isEvent.p, units, 0,
+                                                             offsets.p, nullptr, nullptr, nullptr);
+        DBuf<u32> hotOff(units);
+        exclusiveScan(sc, offsets.p, hotOff.p, units, st);
+        hotPairs = (size_t)readScalar(hotOff.p + units - 1, st) + readScalar(offsets.p + units - 1, st);
+        DBuf<u64> hotLeaf(std::max<size_t>(hotPairs, 1));
+        DBuf<u32> hotUnit(std::max<size_t>(hotPairs, 1));
+        k_hotVisits<<<gridFor(units, BLOCK), BLOCK, 0, st>>>(hot.view(), staged.p, visitCount.p, isEvent.p, units, 1,
+                                                             nullptr, hotOff.p, hotLeaf.p, hotUnit.p);
         hotLeafSorted.alloc(std::max<size_t>(hotPairs, 1));
         hotUnitSorted.alloc(std::max<size_t>(hotPairs, 1));
         if (hotPairs) sortPairs(sc, hotLeaf.p, hotLeafSorted.p, hotUnit.p, hotUnitSorted.p, hotPairs, 0,
                                 (int)(3 * maxContainerDepth + 1), st);
+    }
+    size_t pairs = 0;   // all visits of settled units (statistics)
+    {
+        size_t tmp = 0;
+        DBuf<u32> tot(1);
+        HIPCHECK(hipcub::DeviceReduce::Sum(nullptr, tmp, visitCount.p, tot.p, (int)units, st));
+        HIPCHECK(hipcub::DeviceReduce::Sum(sc.get(tmp), tmp, visitCount.p, tot.p, (int)units, st));
+        pairs = readScalar(tot.p, st);
     }
     checkErr(readScalar(err.p, st));
     mark("hot pairs");
@@ -1243,15 +1281,22 @@ void buildTree(tsr_context *ctx, const tsr_scene *scene, const tsr_params &prm, 
     }
     hotLeafSorted.release();
     hotUnitSorted.release();
-    // a visited leaf outside the hot part: select its units from the full (unit-ordered) pair list
+    // a visited leaf outside the hot part: select the units whose staged visits include it
+    DBuf<uint8_t> unitFlag;
+    DBuf<u32> unitIota;
     index.fetch = [&](u64 loc, std::vector<u32> &out) {
         out.clear();
-        if (!pairs) return;
-        k_matchLeaf<<<gridFor(pairs, BLOCK), BLOCK, 0, st>>>(pairLeaf.p, pairs, loc, pairFlag.p);
-        DBuf<u32> all(pairs);
+        if (!unitFlag.p) {
+            unitFlag.alloc(units);
+            unitIota.alloc(units);
+            k_iota<<<gridFor(units, BLOCK), BLOCK, 0, st>>>(unitIota.p, units);
+        }
+        k_matchStaged<<<gridFor(units, BLOCK), BLOCK, 0, st>>>(staged.p, visitCount.p, isEvent.p, units, loc,
+                                                               unitFlag.p);
+        DBuf<u32> all(units);
         size_t tmp = 0;
-        HIPCHECK(hipcub::DeviceSelect::Flagged(nullptr, tmp, pairUnit.p, pairFlag.p, all.p, counter.p, (int)pairs, st));
-        HIPCHECK(hipcub::DeviceSelect::Flagged(sc.get(tmp), tmp, pairUnit.p, pairFlag.p, all.p, counter.p, (int)pairs, st));
+        HIPCHECK(hipcub::DeviceSelect::Flagged(nullptr, tmp, unitIota.p, unitFlag.p, all.p, counter.p, (int)units, st));
+        HIPCHECK(hipcub::DeviceSelect::Flagged(sc.get(tmp), tmp, unitIota.p, unitFlag.p, all.p, counter.p, (int)units, st));
         const u32 m = readScalar(counter.p, st);
         out.resize(m);
         if (m) HIPCHECK(hipMemcpyAsync(out.data(), all.p, (size_t)m * 4, hipMemcpyDeviceToHost, st));
@@ -1267,9 +1312,7 @@ void buildTree(tsr_context *ctx, const tsr_scene *scene, const tsr_params &prm, 
         std::fprintf(stderr, "[tsr tree] units %u pairs %zu hot pairs %zu t0-inner %u events %llu created nodes %zu "
                      "index misses %llu\n", units, pairs, hotPairs, t0Count, (unsigned long long)events,
                      8 * created.size(), (unsigned long long)index.misses);
-    pairLeaf.release();
-    pairUnit.release();
-    pairFlag.release();
+    staged.release();
     mark("sampling replay (host)");
     tree->samplingEvents = events;
     tree->samplingCreated = 8 * (u64)created.size();
