@@ -722,7 +722,8 @@ __global__ __launch_bounds__(64) void k_priorsGroup(Frame f, const PriorNode *__
                                                     const u64 *__restrict__ leafLoc,
                                                     const u32 *__restrict__ queries, u32 queryCount,
                                                     float relRadius, double *__restrict__ prior, u32 *err,
-                                                    unsigned long long *__restrict__ stats, int xcdMap) {
+                                                    unsigned long long *__restrict__ stats, int xcdMap,
+                                                    u32 *__restrict__ nextGroup) {
     __shared__ u32 stNode[PRIOR_STACK];
     __shared__ uint8_t stMask[PRIOR_STACK];
     __shared__ double qc[GROUP_Q][3], qr2[GROUP_Q], qri2[GROUP_Q];
@@ -741,10 +742,20 @@ __global__ __launch_bounds__(64) void k_priorsGroup(Frame f, const PriorNode *__
     double M = 0.0;
     for (int a = 0; a < 3; ++a) M = fmax(M, fabs(f.rootMin[a]) + f.rootSize);
     const double eps = 5.0 * ldexp(M, -24);
+    // groups are taken dynamically from a counter per XCD slice (blocks are dispatched
+    // round-robin over the 8 XCDs, so block b runs on XCD b mod 8): each XCD's L2 sees
+    // one contiguous, spatially coherent range, and waves with cheap groups take more
     const u32 groups = (queryCount + GROUP_Q - 1) / GROUP_Q;
-    u32 gBegin, gEnd, gStride, gFirst;
-    xcdRange(groups, xcdMap, gBegin, gEnd, gStride, gFirst);
-    for (u32 g = gFirst; g < gEnd; g += gStride) {
+    const u32 slice = xcdMap ? (groups + XCDS - 1) / XCDS : groups;
+    const u32 xcd = xcdMap ? blockIdx.x % XCDS : 0u;
+    const u32 gBegin = min(groups, xcd * slice), gEnd = min(groups, gBegin + slice);
+    __shared__ u32 gNext;
+    for (;;) {
+        if (lane == 0) gNext = gBegin + atomicAdd(&nextGroup[xcd], 1u);
+        __syncthreads();
+        const u32 g = gNext;
+        __syncthreads();
+        if (g >= gEnd) break;
         const u32 nQ = min(GROUP_Q, queryCount - g * GROUP_Q);
         if (lane < GROUP_Q) {
             const u32 l = lane < nQ ? queries[g * GROUP_Q + lane] : 0u;
@@ -1278,16 +1289,18 @@ void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *s
             const u32 grid = 256u * 32u;   // multiple of the XCD count
             HIPCHECK(hipMemsetAsync(kst.p, 0, 32, st));
             const int kind = priorsPacketMode();
+            DBuf<u32> groupCounters(8);
+            HIPCHECK(hipMemsetAsync(groupCounters.p, 0, 32, st));
             const bool packet = kind == 1;
             timedLaunch(ctx, KSTAT_PRIORS, [&] {
                 if (kind == 2 && priorsGroupSize() == 4)
                     k_priorsGroup<4><<<grid, 64, 0, st>>>(tree->frame, pnodes.p, tree->leafLoc.p, queries.p, nq,
                                                           prm.relative_radius_for_priors, field->prior.p, perr.p,
-                                                          kst.p, (xcdMapMode() >> 1) & 1);
+                                                          kst.p, (xcdMapMode() >> 1) & 1, groupCounters.p);
                 else if (kind == 2)
                     k_priorsGroup<8><<<grid, 64, 0, st>>>(tree->frame, pnodes.p, tree->leafLoc.p, queries.p, nq,
                                                           prm.relative_radius_for_priors, field->prior.p, perr.p,
-                                                          kst.p, (xcdMapMode() >> 1) & 1);
+                                                          kst.p, (xcdMapMode() >> 1) & 1, groupCounters.p);
                 else if (packet)
                     k_priorsPacket<<<grid, 64, 0, st>>>(tree->frame, pnodes.p, tree->leafLoc.p, queries.p, nq,
                                                         prm.relative_radius_for_priors, field->prior.p, perr.p,

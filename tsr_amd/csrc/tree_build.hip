@@ -24,6 +24,8 @@
 #include <cstring>
 #include <queue>
 #include <thread>
+#include <condition_variable>
+#include <mutex>
 #include <vector>
 
 #include "handles.h"
@@ -1075,22 +1077,98 @@ void *pinnedHost(tsr_context *ctx, size_t bytes) {
     return ctx->pinned;
 }
 
-void parallelFor(size_t n, const std::function<void(size_t, size_t)> &fn, size_t grain) {
+namespace {
+// persistent host worker threads for parallelFor (spawning threads per call costs
+// ~20 us each, and the replay calls it once per batch)
+class HostPool {
+public:
+    explicit HostPool(unsigned workers) {
+        for (unsigned w = 0; w < workers; ++w) threads_.emplace_back([this, w] { loop(w); });
+    }
+    ~HostPool() {
+        {
+            std::lock_guard<std::mutex> g(m_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto &t : threads_) t.join();
+    }
+    unsigned workers() const { return (unsigned)threads_.size(); }
+    // runs job(c) for c in [1, chunks) on the workers; the caller runs chunk 0 itself
+    void run(size_t chunks, const std::function<void(size_t)> &job) {
+        std::unique_lock<std::mutex> g(m_);
+        job_ = &job;
+        chunks_ = chunks;
+        next_ = 1;
+        pending_ = chunks - 1;
+        ++generation_;
+        g.unlock();
+        cv_.notify_all();
+        job(0);
+        g.lock();
+        done_.wait(g, [this] { return pending_ == 0; });
+        job_ = nullptr;
+    }
+    std::mutex busy;   // one run at a time; other callers spawn their own threads
+
+private:
+    void loop(unsigned) {
+        size_t seen = 0;
+        std::unique_lock<std::mutex> g(m_);
+        for (;;) {
+            cv_.wait(g, [&] { return stop_ || (generation_ != seen && next_ < chunks_); });
+            if (stop_) return;
+            while (job_ && next_ < chunks_) {
+                const size_t c = next_++;
+                const std::function<void(size_t)> *job = job_;
+                g.unlock();
+                (*job)(c);
+                g.lock();
+                if (--pending_ == 0) done_.notify_one();
+            }
+            seen = generation_;
+        }
+    }
+    std::vector<std::thread> threads_;
+    std::mutex m_;
+    std::condition_variable cv_, done_;
+    const std::function<void(size_t)> *job_ = nullptr;
+    size_t chunks_ = 0, next_ = 0, pending_ = 0, generation_ = 0;
+    bool stop_ = false;
+};
+
+unsigned hostThreads() {
     static const unsigned threads = [] {
         unsigned t = std::max(1u, std::thread::hardware_concurrency());
         if (const char *e = std::getenv("OMP_NUM_THREADS")) t = std::max(1, std::atoi(e));
         return std::min(t, 16u);
     }();
-    const size_t chunks = std::min<size_t>(threads, (n + grain - 1) / std::max<size_t>(grain, 1));
+    return threads;
+}
+
+HostPool &hostPool() {
+    static HostPool pool(hostThreads() - 1);
+    return pool;
+}
+}  // namespace
+
+void parallelFor(size_t n, const std::function<void(size_t, size_t)> &fn, size_t grain) {
+    const size_t chunks = std::min<size_t>(hostThreads(), (n + grain - 1) / std::max<size_t>(grain, 1));
     if (chunks <= 1) {
         fn(0, n);
         return;
     }
-    std::vector<std::thread> pool;
-    for (size_t c = 1; c < chunks; ++c)
-        pool.emplace_back([&, c] { fn(n * c / chunks, n * (c + 1) / chunks); });
-    fn(0, n / chunks);
-    for (auto &t : pool) t.join();
+    const std::function<void(size_t)> job = [&](size_t c) { fn(n * c / chunks, n * (c + 1) / chunks); };
+    HostPool &pool = hostPool();
+    std::unique_lock<std::mutex> mine(pool.busy, std::try_to_lock);
+    if (mine.owns_lock() && pool.workers() + 1 >= chunks) {
+        pool.run(chunks, job);
+        return;
+    }
+    std::vector<std::thread> extra;
+    for (size_t c = 1; c < chunks; ++c) extra.emplace_back([&job, c] { job(c); });
+    job(0);
+    for (auto &t : extra) t.join();
 }
 
 void buildTree(tsr_context *ctx, const tsr_scene *scene, const tsr_params &prm, tsr_tree *tree) {
