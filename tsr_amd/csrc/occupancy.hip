@@ -254,6 +254,140 @@ __device__ void coneLink(const TraversalTree &t, const LinkInputs &in, u32 link,
     if (mode == 0 || mode == 3) counts[link] = count;
 }
 
+// Mode 3 with persistent threads (after Aila & Laine, "Understanding the efficiency of
+// ray traversal on GPUs", 2009): every lane runs the traversal of traverseCone as an
+// explicit state machine and, when its link is done, takes the next link index from a
+// shared counter (one atomic per wavefront for all idle lanes).  Lanes no longer idle
+// at the end of a wave's longest traversal.  Results, staging and counts are those of
+// k_conePass mode 3.
+__global__ __launch_bounds__(256) void k_conePassPT(TraversalTree t, LinkInputs in, u32 linkCount, int sampleness,
+                                                    u32 *__restrict__ counts, u32 *__restrict__ stLeaf,
+                                                    uint2 *__restrict__ stVal, double *__restrict__ confOut,
+                                                    u32 *__restrict__ nextIdx,
+                                                    unsigned long long *__restrict__ stats) {
+    const u32 lane = threadIdx.x & 63;
+    const u32 required = sampleness ? 0u : FLAG_S;
+    const uint8_t *cullNode = sampleness ? nullptr : in.cullNode;
+    bool active = false, more = true;
+    u32 idx = 0, link = 0, count = 0, maxDepth = 0, node = 0, d = 0, k0 = 0, k1 = 0, k2 = 0;
+    bool descend = true;
+    ConeD cone;
+    ConeCull cull;
+    double conf = 0.0;
+    float wl = 0.f;
+    u32 visits = 0, hits = 0;
+    for (;;) {
+        const u64 idle = __ballot(!active && more);
+        if (idle) {   // refill the idle lanes with consecutive link indices
+            const int leader = __ffsll((long long)idle) - 1;
+            u32 base = 0;
+            if ((int)lane == leader) base = atomicAdd(nextIdx, (u32)__popcll(idle));
+            base = __shfl(base, leader);
+            if (!active && more) {
+                idx = base + (u32)__popcll(idle & ((1ull << lane) - 1ull));
+                if (idx >= linkCount) {
+                    more = false;
+                } else {
+                    link = in.order ? in.order[idx] : in.linkBase + idx;
+                    u32 i;
+                    const bool ok = setupLink(in, link, sampleness != 0, cone, i);
+                    if (confOut) confOut[link] = ok ? (double)in.conf[i] : 0.0;
+                    if (!ok) {
+                        counts[link] = 0;
+                    } else {
+                        cullInit(cull, cone);
+                        conf = in.conf[i];
+                        wl = (float)(conf * cone.length);
+                        maxDepth = (u32)in.sampleDepth[i] + MAX_DEPTH_DIFFERENCE;
+                        count = 0;
+                        node = 0;
+                        d = 0;
+                        k0 = k1 = k2 = 0;
+                        descend = true;
+                        active = true;
+                    }
+                }
+            }
+        }
+        if (!__ballot(active)) {
+            if (!__ballot(more)) break;
+            continue;
+        }
+        if (!active) continue;
+        // one step of traverseCone
+        bool advance = true;
+        if (descend) {
+            ++visits;
+            bool accept = d <= maxDepth;
+            u32 c = 0u;
+            if (t.packed) {
+                const uint2 cs = t.packed[node];
+                if (accept && required) accept = (cs.y & required) == required;
+                c = cs.x;
+            } else {
+                if (accept && required) accept = (t.states[node] & required) == required;
+                c = t.child[node];
+            }
+            if (accept && cullNode) accept = cullNode[node] == 0;
+            if (accept) {
+                if (c & LEAF_BIT) {
+                    const u32 k[3] = {k0, k1, k2};
+                    double x[3];
+                    for (int a = 0; a < 3; ++a) x[a] = t.f.B(a, 2ull * k[a] + 1, d + 1);
+                    const double kv = coneKernel(cone, x, sampleness != 0, conf);
+                    if (kv != 0.0) {
+                        if (count < in.slots) {
+                            const size_t at = stagingIndex(idx, count, in.slots);
+                            stLeaf[at] = c & ~LEAF_BIT;
+                            stVal[at] = make_uint2(__float_as_uint((float)kv), __float_as_uint(wl));
+                        }
+                        ++count;
+                    }
+                } else if (c != INVALID) {
+                    const float h = (float)t.f.h(d);
+                    const float lo0 = t.f.rootMinF[0] + (float)k0 * h, lo1 = t.f.rootMinF[1] + (float)k1 * h,
+                                lo2 = t.f.rootMinF[2] + (float)k2 * h;
+                    if (!cullSeparated(cull, lo0, lo1, lo2, h, t.margin)) {
+                        node = c;   // first child
+                        ++d;
+                        k0 *= 2; k1 *= 2; k2 *= 2;
+                        advance = false;
+                    }
+                }
+            }
+        }
+        if (advance) {   // next sibling, or climb; done when back at the root
+            for (;;) {
+                if (node == 0) {
+                    counts[link] = count;
+                    hits += count;
+                    active = false;
+                    break;
+                }
+                const u32 slot = (node - 1) & 7;
+                if (slot < 7) {
+                    ++node;
+                    const u32 s1 = slot + 1;
+                    k0 = (k0 & ~1u) | (s1 & 1);
+                    k1 = (k1 & ~1u) | ((s1 >> 1) & 1);
+                    k2 = (k2 & ~1u) | ((s1 >> 2) & 1);
+                    descend = true;
+                    break;
+                }
+                node = t.parents[node];
+                --d;
+                k0 >>= 1; k1 >>= 1; k2 >>= 1;
+            }
+        } else {
+            descend = true;
+        }
+    }
+    if (stats) {
+        waveAccumulate(&stats[0], visits);
+        waveAccumulate(&stats[1], hits);
+    }
+}
+
 // staged records of the links that fit their slots -> link order (offsets = exclusive
 // scan of the counts); a link's records stay in traversal order.  Thread i moves the
 // records staged by thread i (stagingIndex; coalesced reads).
@@ -985,6 +1119,16 @@ int priorsGroupSize() {   // TSR_PRIORS_GROUP=4: 4 queries per wavefront (defaul
     return m;
 }
 
+// cone traversal mode 3 with persistent threads (TSR_CONE_PERSISTENT=0: one thread per link)
+int conePersistentMode() {
+    static const int m = [] {
+        const char *e = std::getenv("TSR_CONE_PERSISTENT");
+        return e ? std::atoi(e) : 1;
+    }();
+    return m;
+}
+constexpr u32 CONE_PT_BLOCKS = 2048;   // blocks of 256 threads (about two per SIMD pair at 4 waves/SIMD)
+
 int linkOrderMode() {   // TSR_LINK_ORDER=0: threads take links in link order
     static const int m = [] {
         const char *e = std::getenv("TSR_LINK_ORDER");
@@ -1117,6 +1261,7 @@ void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *s
                 tmark = t;
             };
             ConeArena &ar = ctx->cone;
+            DBuf<u32> linkCursor(1);
             u32 rangeLinks = chunkLinks;
             if ((u64)links * 64 > CONE_MAX_RECORDS) {
                 // records per link estimated on every 64th link (counting traversal), so that the
@@ -1159,10 +1304,18 @@ void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *s
                 uint2 *stVal = ar.stVal.ensure(stageCount);
                 lap("order + staging");
                 HIPCHECK(hipMemsetAsync(kst.p, 0, 32, st));
+                // persistent threads pay off for the long emptiness traversals only
+                const bool persistent = !sampleness && conePersistentMode();
+                if (persistent) HIPCHECK(hipMemsetAsync(linkCursor.p, 0, 4, st));
                 timedLaunch(ctx, sampleness ? KSTAT_S_COUNT : KSTAT_E_COUNT, [&] {
-                    k_conePass<<<xcdGrid(n, BLOCK), BLOCK, 0, st>>>(tt, ic, n, sampleness, 3, counts.p, nullptr,
-                                                                    stLeaf, stVal, nullptr, nullptr, nullptr,
-                                                                    sampleness ? nullptr : confs.p, kst.p);
+                    if (persistent)
+                        k_conePassPT<<<std::min<u32>(gridFor(n, BLOCK), CONE_PT_BLOCKS), BLOCK, 0, st>>>(
+                            tt, ic, n, sampleness, counts.p, stLeaf, stVal, sampleness ? nullptr : confs.p,
+                            linkCursor.p, kst.p);
+                    else
+                        k_conePass<<<xcdGrid(n, BLOCK), BLOCK, 0, st>>>(tt, ic, n, sampleness, 3, counts.p, nullptr,
+                                                                        stLeaf, stVal, nullptr, nullptr, nullptr,
+                                                                        sampleness ? nullptr : confs.p, kst.p);
                 });
                 unsigned long long cs[2];
                 readStats(2, cs);
