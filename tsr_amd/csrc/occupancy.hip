@@ -388,6 +388,116 @@ __global__ __launch_bounds__(256) void k_conePassPT(TraversalTree t, LinkInputs 
     }
 }
 
+// Re-traversal of one heavy link per wavefront (the links with more records than
+// staging slots): an LDS stack of inner nodes still to open; each step pops up to 8
+// and tests their 8 children on the 64 lanes with traverseCone's per-node rules, and
+// evaluates the kernel at accepted leaves.  A link meets every leaf at most once, so
+// its records can be written in any order within its segment [offsets[link], +count):
+// the per-leaf order after the stable sort by leaf is still the link order.
+constexpr u32 CONE_WAVE_STACK = 1344;   // >= 64 * (MAX_DEPTH + 1)
+__global__ __launch_bounds__(64) void k_coneLinkWave(TraversalTree t, LinkInputs in, const u32 *__restrict__ links,
+                                                     u32 count, int sampleness, const u64 *__restrict__ nodeLoc,
+                                                     const u32 *__restrict__ offsets, const u32 *__restrict__ counts,
+                                                     u32 *__restrict__ recLeaf, uint2 *__restrict__ recVal,
+                                                     u32 *__restrict__ err, unsigned long long *__restrict__ stats) {
+    __shared__ u32 stack[CONE_WAVE_STACK];
+    __shared__ u32 sp, written;
+    const u32 lane = threadIdx.x;
+    const u32 required = sampleness ? 0u : FLAG_S;
+    const uint8_t *cullNode = sampleness ? nullptr : in.cullNode;
+    u32 visits = 0;
+    for (u32 w = blockIdx.x; w < count; w += gridDim.x) {
+        const u32 link = links[w];
+        ConeD cone;
+        u32 i;
+        const bool ok = setupLink(in, link, sampleness != 0, cone, i);
+        if (!ok) continue;   // (no records: such links are never listed)
+        ConeCull cull;
+        cullInit(cull, cone);
+        const double conf = in.conf[i];
+        const float wl = (float)(conf * cone.length);
+        const u32 maxDepth = (u32)in.sampleDepth[i] + MAX_DEPTH_DIFFERENCE;
+        const size_t base = offsets[link];
+        // node accept rules of traverseCone
+        auto accepts = [&](u32 node, u32 d, u32 &c) {
+            bool accept = d <= maxDepth;
+            if (accept && required) accept = (t.states[node] & required) == required;
+            if (accept && cullNode) accept = cullNode[node] == 0;
+            c = t.child[node];
+            return accept;
+        };
+        auto leafRecord = [&](u32 c, u64 loc) {
+            u32 k[3];
+            locCoords(loc, k[0], k[1], k[2]);
+            const u32 d = locDepth(loc);
+            double x[3];
+            for (int a = 0; a < 3; ++a) x[a] = t.f.B(a, 2ull * k[a] + 1, d + 1);
+            const double kv = coneKernel(cone, x, sampleness != 0, conf);
+            if (kv != 0.0) {
+                const u32 pos = atomicAdd(&written, 1u);
+                recLeaf[base + pos] = c & ~LEAF_BIT;
+                recVal[base + pos] = make_uint2(__float_as_uint((float)kv), __float_as_uint(wl));
+            }
+        };
+        auto opens = [&](u32 c, u64 loc) {
+            u32 k[3];
+            locCoords(loc, k[0], k[1], k[2]);
+            const u32 d = locDepth(loc);
+            const float h = (float)t.f.h(d);
+            return !cullSeparated(cull, t.f.rootMinF[0] + (float)k[0] * h, t.f.rootMinF[1] + (float)k[1] * h,
+                                  t.f.rootMinF[2] + (float)k[2] * h, h, t.margin);
+        };
+        if (lane == 0) {
+            sp = 0;
+            written = 0;
+            ++visits;
+            u32 c;
+            if (accepts(0, 0, c)) {
+                if (c & LEAF_BIT) leafRecord(c, 1ull);
+                else if (c != INVALID && opens(c, 1ull)) {
+                    stack[0] = 0;
+                    sp = 1;
+                }
+            }
+        }
+        __syncthreads();
+        for (;;) {
+            const u32 top = sp;
+            if (top == 0) break;
+            const u32 take = min(top, 8u);
+            const u32 b = top - take;
+            const u32 e = lane >> 3, slot = lane & 7;
+            bool push = false;
+            u32 cid = 0;
+            if (e < take) {
+                const u32 parent = stack[b + e];
+                cid = t.child[parent] + slot;
+                const u64 loc = (nodeLoc[parent] << 3) | slot;
+                ++visits;
+                u32 c;
+                if (accepts(cid, locDepth(loc), c)) {
+                    if (c & LEAF_BIT) leafRecord(c, loc);
+                    else if (c != INVALID) push = opens(c, loc);
+                }
+            }
+            __syncthreads();   // every lane has read its entry before the slots are reused
+            const u64 mask = __ballot(push);
+            const u32 pos = (u32)__popcll(mask & ((1ull << lane) - 1ull));
+            const u32 newTop = b + (u32)__popcll(mask);
+            if (newTop > CONE_WAVE_STACK) {
+                if (lane == 0) { atomicOr(err, 1u); sp = 0; }
+            } else {
+                if (push) stack[b + pos] = cid;
+                if (lane == 0) sp = newTop;
+            }
+            __syncthreads();
+        }
+        if (lane == 0 && written != counts[link]) atomicOr(err, 2u);   // must match the counting pass
+        __syncthreads();
+    }
+    if (stats) waveAccumulate(&stats[0], visits);
+}
+
 // staged records of the links that fit their slots -> link order (offsets = exclusive
 // scan of the counts); a link's records stay in traversal order.  Thread i moves the
 // records staged by thread i (stagingIndex; coalesced reads).
@@ -1359,12 +1469,17 @@ void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *s
                     LinkInputs ino = ic;
                     ino.order = ovList.p;
                     HIPCHECK(hipMemsetAsync(kst.p, 0, 32, st));
-                    // few, long traversals: 64-thread blocks spread them over every CU
+                    // few, long traversals: one wavefront per link
+                    DBuf<u32> werr(1);
+                    HIPCHECK(hipMemsetAsync(werr.p, 0, 4, st));
                     timedLaunch(ctx, sampleness ? KSTAT_S_WRITE : KSTAT_E_WRITE, [&] {
-                        k_conePass<<<xcdGrid(nOver, 64), 64, 0, st>>>(tt, ino, nOver, sampleness, 1, counts.p,
-                                                                      offsets.p, recLeaf, recVal, nullptr,
-                                                                      nullptr, nullptr, nullptr, kst.p);
+                        k_coneLinkWave<<<std::min<u32>(nOver, 65536), 64, 0, st>>>(
+                            tt, ic, ovList.p, nOver, sampleness, tree->nodeLoc.p, offsets.p, counts.p, recLeaf,
+                            recVal, werr.p, kst.p);
                     });
+                    if (const u32 we = readScalar(werr.p, st))
+                        throw tsr_error(TSR_ERR_HIP, we & 1u ? "cone traversal stack overflow"
+                                                             : "cone re-traversal disagrees with the counting pass");
                     readStats(2, cs);
                     tsr_kernel_stat &k = ctx->kstat[sampleness ? KSTAT_S_WRITE : KSTAT_E_WRITE];
                     k.bytes += (u64)nOver * 45 + cs[0] * nodeBytes + cs[1] * 12;
