@@ -18,9 +18,12 @@ namespace tsr {
 struct ConeArena {
     DBuf<u32> stLeaf, recLeaf, recLeafS;
     DBuf<uint2> stVal, recVal, recValS;
+    DBuf<u32> childPad, blockParent;   // traversal copies of the node arrays
+    DBuf<uint2> packed;
     void release() {
         stLeaf.release(); recLeaf.release(); recLeafS.release();
         stVal.release(); recVal.release(); recValS.release();
+        childPad.release(); blockParent.release(); packed.release();
     }
 };
 }  // namespace tsr
@@ -89,6 +92,7 @@ struct tsr_field {
     double coneCount = 0;
     tsr::u64 hits[2] = {0, 0};
     tsr::u32 priorQueries = 0;
+    tsr::u32 priorsOverflowGroups = 0;   // query groups redone with the long traversal stack
     tsr::DBuf<tsr::u32> hitsS;   // sampleness hits per leaf (features of the learned culling)
     tsr::u32 culledNodes = 0, culledLeaves = 0;
     tsr::u32 overflowLinks[2] = {0, 0};   // links traversed twice (more hits than staging slots), E / S

@@ -37,9 +37,10 @@ __device__ __forceinline__ u32 xcdBlock(int xcdMap) {
     if (!xcdMap) return blockIdx.x;
     return (blockIdx.x % XCD_COUNT) * (gridDim.x / XCD_COUNT) + blockIdx.x / XCD_COUNT;
 }
-// work-to-XCD mapping switches (TSR_XCD_MAP bit 0: cone passes, bit 1: priors)
+// work-to-XCD mapping switches (TSR_XCD_MAP bit 0: static mapping of the one-thread-per-link
+// cone passes, bit 1: priors, bit 2: XCD slices of the persistent-thread emptiness pass)
 inline int xcdMapMode() {
-    static const int m = std::getenv("TSR_XCD_MAP") ? std::atoi(std::getenv("TSR_XCD_MAP")) : 2;
+    static const int m = std::getenv("TSR_XCD_MAP") ? std::atoi(std::getenv("TSR_XCD_MAP")) : 6;
     return m;
 }
 inline unsigned xcdGrid(size_t n, unsigned block) {
@@ -70,19 +71,45 @@ __device__ __forceinline__ size_t stagingIndex(u32 idx, u32 j, u32 slots) {
     return ((size_t)(idx / STAGE_GROUP) * slots + j) * STAGE_GROUP + (idx % STAGE_GROUP);
 }
 
+// Traversal copies of the node arrays store node i at index i + NODE_PAD: the 8
+// children of the r-th inner node (ids 1 + 8r .. 8 + 8r) then start on a 32-byte
+// (child links) or 64-byte ((child, state) pairs) boundary and a sibling scan reads
+// one cache line.  Parents are kept per child block: blockParent[(n - 1) / 8] is the
+// parent of node n (8 times fewer entries than a per-node table).
+constexpr u32 NODE_PAD = 7;
+
 struct TraversalTree {
-    const u32 *child;      // first child id, or LEAF_BIT | leaf index
-    const u32 *parents;
-    const u32 *states;     // node state flags (E pass gate), may be null
+    const u32 *child;        // first child id, or LEAF_BIT | leaf index (node-padded copy)
+    const u32 *blockParent;  // parent per child block
+    const u32 *states;       // node state flags (E pass gate), may be null
     Frame f;
-    float margin;          // absolute culling margin
-    const uint2 *packed;   // (child, state) per node for gated passes: one load per visit; may be null
+    float margin;            // absolute culling margin
+    const uint2 *packed;     // (child, state) per node for gated passes, node-padded: one load per visit; may be null
 };
 
 __global__ void k_packChildState(const u32 *__restrict__ child, const u32 *__restrict__ states, u32 n,
                                  uint2 *__restrict__ out) {
     const u32 i = blockIdx.x * 256 + threadIdx.x;
     if (i < n) out[i] = make_uint2(child[i], states[i]);
+}
+
+__global__ void k_traversalLayout(const u32 *__restrict__ child, const u32 *__restrict__ parents, u32 n,
+                                  u32 *__restrict__ childOut, u32 *__restrict__ blockParent) {
+    const u32 i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    childOut[i] = child[i];
+    if (i >= 1 && ((i - 1) & 7u) == 0) blockParent[(i - 1) >> 3] = parents[i];
+}
+
+// node-padded child links and block parents of `tree` into the given buffers
+void traversalLayout(hipStream_t st, const tsr_tree *tree, DBuf<u32> &childPad, DBuf<u32> &blockParent,
+                     TraversalTree &tt) {
+    const u32 N = tree->nodeCount;
+    u32 *c = childPad.ensure((size_t)N + NODE_PAD);
+    u32 *bp = blockParent.ensure((size_t)N / 8 + 1);
+    k_traversalLayout<<<gridFor(N, 256), 256, 0, st>>>(tree->child.p, tree->parents.p, N, c + NODE_PAD, bp);
+    tt.child = c + NODE_PAD;
+    tt.blockParent = bp;
 }
 
 struct LinkInputs {
@@ -175,7 +202,7 @@ __device__ void traverseCone(const TraversalTree &t, const ConeCull &cull, u32 m
                 descend = true;
                 break;
             }
-            node = t.parents[node];
+            node = t.blockParent[(node - 1) >> 3];
             --d;
             k[0] >>= 1; k[1] >>= 1; k[2] >>= 1;
         }
@@ -260,12 +287,20 @@ __device__ void coneLink(const TraversalTree &t, const LinkInputs &in, u32 link,
 // shared counter (one atomic per wavefront for all idle lanes).  Lanes no longer idle
 // at the end of a wave's longest traversal.  Results, staging and counts are those of
 // k_conePass mode 3.
+//
+// With `slices` = 8 the link list is cut into 8 contiguous slices with a counter
+// each (nextIdx[0..7]); a wave starts on the slice of its XCD (blocks are dispatched
+// round-robin over the 8 XCDs) and moves on to the next slice when its own is used
+// up.  Each XCD's L2 then serves cones of a few cameras towards one part of the
+// scene instead of the whole GPU's window of links.
 __global__ __launch_bounds__(256) void k_conePassPT(TraversalTree t, LinkInputs in, u32 linkCount, int sampleness,
                                                     u32 *__restrict__ counts, u32 *__restrict__ stLeaf,
                                                     uint2 *__restrict__ stVal, double *__restrict__ confOut,
-                                                    u32 *__restrict__ nextIdx,
+                                                    u32 *__restrict__ nextIdx, u32 slices,
                                                     unsigned long long *__restrict__ stats) {
     const u32 lane = threadIdx.x & 63;
+    const u32 sliceLen = (linkCount + slices - 1) / slices;
+    u32 cur = blockIdx.x % slices, tried = 0;   // wave-uniform
     const u32 required = sampleness ? 0u : FLAG_S;
     const uint8_t *cullNode = sampleness ? nullptr : in.cullNode;
     bool active = false, more = true;
@@ -278,15 +313,22 @@ __global__ __launch_bounds__(256) void k_conePassPT(TraversalTree t, LinkInputs 
     u32 visits = 0, hits = 0;
     for (;;) {
         const u64 idle = __ballot(!active && more);
-        if (idle) {   // refill the idle lanes with consecutive link indices
+        if (idle) {   // refill the idle lanes with consecutive link indices of the current slice
             const int leader = __ffsll((long long)idle) - 1;
             u32 base = 0;
-            if ((int)lane == leader) base = atomicAdd(nextIdx, (u32)__popcll(idle));
+            if ((int)lane == leader) base = atomicAdd(&nextIdx[cur], (u32)__popcll(idle));
             base = __shfl(base, leader);
+            const u32 sBegin = min(linkCount, cur * sliceLen), sEnd = min(linkCount, sBegin + sliceLen);
+            if (!__ballot(!active && more && sBegin + base + (u32)__popcll(idle & ((1ull << lane) - 1ull)) < sEnd)) {
+                // this slice is used up: move on; done after the last one
+                cur = cur + 1 == slices ? 0u : cur + 1;
+                if (++tried == slices) more = false;
+                continue;
+            }
             if (!active && more) {
-                idx = base + (u32)__popcll(idle & ((1ull << lane) - 1ull));
-                if (idx >= linkCount) {
-                    more = false;
+                idx = sBegin + base + (u32)__popcll(idle & ((1ull << lane) - 1ull));
+                if (idx >= sEnd) {
+                    // past the slice end: stays idle and refills from the next slice
                 } else {
                     link = in.order ? in.order[idx] : in.linkBase + idx;
                     u32 i;
@@ -374,7 +416,7 @@ __global__ __launch_bounds__(256) void k_conePassPT(TraversalTree t, LinkInputs 
                     descend = true;
                     break;
                 }
-                node = t.parents[node];
+                node = t.blockParent[(node - 1) >> 3];
                 --d;
                 k0 >>= 1; k1 >>= 1; k2 >>= 1;
             }
@@ -676,307 +718,58 @@ __global__ void k_priorQueryList(const u32 *__restrict__ flag, const u32 *__rest
     if (l < leafCount && flag[l]) queries[pos[l]] = l;
 }
 
-// one 32-byte record per node for the priors traversal: child link, state | deepest
-// leaf << 8, location code, subtree sums (E hi, E lo, S hi, S lo)
-struct PriorNode {
-    u32 child, aux;
-    u64 loc;
-    float4 sums;
-};
+// Priors traversal records, in two node-padded arrays (node i at index i + NODE_PAD)
+// so that one popped entry's 8 children are read as one aligned 64-byte run of
+// (child link, state | deepest leaf depth << 8) and one aligned 128-byte run of
+// subtree sums (E hi, E lo, S hi, S lo).
 
-__global__ void k_packPriorNodes(const u32 *__restrict__ child, const u32 *__restrict__ aux,
-                                 const u64 *__restrict__ loc, const float4 *__restrict__ sums, u32 nodeCount,
-                                 PriorNode *__restrict__ out) {
+__global__ void k_packPriorLinks(const u32 *__restrict__ child, const u32 *__restrict__ aux, u32 nodeCount,
+                                 uint2 *__restrict__ links) {
     const u32 nd = blockIdx.x * BLOCK + threadIdx.x;
-    if (nd >= nodeCount) return;
-    PriorNode r;
-    r.child = child[nd];
-    r.aux = aux[nd];
-    r.loc = loc[nd];
-    r.sums = sums[nd];
-    out[nd] = r;
+    if (nd < nodeCount) links[nd] = make_uint2(child[nd], aux[nd]);
 }
 
-// One wavefront per query leaf (SphereNodeStatesChecker + LeavesIterator,
-// Occupancy.cpp:541-566).  The wave keeps an LDS stack of inner nodes still to
-// open; each step pops up to 8 of them and tests their 8 children on 64 lanes
-// (sibling records are contiguous, so the loads coalesce).  Children entirely
-// inside the sphere whose deepest leaf respects the depth limit contribute their
-// subtree sums; straddling inner children are pushed.
-constexpr u32 PRIOR_STACK = 1344;   // >= 64 * (MAX_DEPTH + 1)
+// The priors traversal keeps an LDS stack of inner nodes still to open.  The first
+// pass uses a short stack; a group that would overflow it is abandoned and listed,
+// and a second pass redoes the listed groups with a stack of the proven bound.
+constexpr u32 PRIOR_STACK_SHORT = 512;
+constexpr u32 XCDS = 8;   // MI355X: blocks are dispatched round-robin over 8 XCDs, each with its own L2
+constexpr u32 PRIOR_STACK = 1344;   // >= 64 * (MAX_DEPTH + 1): one step pushes <= 56 more than it pops
 
-// XCD-aware work mapping: blocks are dispatched round-robin over the 8 XCDs of
-// the MI355X; block b serves slice (b mod 8) of the work list, so that each XCD's
-// L2 sees a contiguous (spatially coherent) range of it.
-constexpr u32 XCDS = 8;
-__device__ __forceinline__ void xcdRange(u32 count, int xcdMap, u32 &begin, u32 &end, u32 &stride, u32 &first) {
-    if (!xcdMap) {
-        begin = 0; end = count; stride = gridDim.x; first = blockIdx.x;
-        return;
-    }
-    const u32 xcd = blockIdx.x % XCDS, perXcdBlocks = gridDim.x / XCDS;
-    const u32 slice = (count + XCDS - 1) / XCDS;
-    begin = min(count, xcd * slice);
-    end = min(count, begin + slice);
-    stride = perXcdBlocks;
-    first = begin + blockIdx.x / XCDS;
-}
+// Group form of SphereNodeStatesChecker + LeavesIterator (Occupancy.cpp:541-566):
+// one wavefront serves GROUP_Q consecutive query leaves (DFS order: spatial
+// neighbours, whose spheres overlap almost entirely).  A stack entry holds a node
+// to open (its first child id, its grid coordinates and depth) and a mask of the
+// queries that still need its children; each step pops up to 8 entries and tests their 8
+// children on 64 lanes (entry = lane / 8, child = lane % 8), every lane looping
+// over the queries of its entry's mask.  Each child record is read once for all
+// queries of the group, and only the child records: the entry already carries
+// what the parent record would give.  Children entirely inside the sphere whose
+// deepest leaf respects the depth limit contribute their subtree sums; straddling
+// inner children are pushed.  Every query applies the FP64 predicates of the
+// reference (behind an FP32 pre-test with a proven margin); sums are FP64 per lane
+// and query, reduced across the wave in a fixed order (deterministic).
+#define PRIORS_GROUP_PARAMS                                                                                \
+    Frame f, const uint2 *__restrict__ links, const float4 *__restrict__ sums, const u64 *__restrict__ leafLoc, \
+        const u32 *__restrict__ queries, u32 queryCount, float relRadius, double *__restrict__ prior, u32 *err,  \
+        unsigned long long *__restrict__ stats, int xcdMap, u32 *__restrict__ nextGroup,                        \
+        const u32 *__restrict__ groupList, const u32 *__restrict__ groupListCount,                              \
+        u32 *__restrict__ overflowList, u32 *__restrict__ overflowCount
+#define PRIORS_GROUP_ARGS                                                                                  \
+    f, links, sums, leafLoc, queries, queryCount, relRadius, prior, err, stats, xcdMap, nextGroup, groupList, \
+        groupListCount, overflowList, overflowCount
 
-__global__ __launch_bounds__(64) void k_priorsWave(Frame f, const PriorNode *__restrict__ nodes,
-                                                   const u64 *__restrict__ leafLoc, const u32 *__restrict__ queries,
-                                                   u32 queryCount, float relRadius, double *__restrict__ prior,
-                                                   u32 *err, unsigned long long *__restrict__ stats, int xcdMap) {
-    __shared__ u32 stack[PRIOR_STACK];
-    __shared__ u32 sp;
-    const u32 lane = threadIdx.x;
-    u32 nPops = 0, nTests = 0, nAccum = 0;   // traffic counters (stats[0..2])
-    u32 qBegin, qEnd, qStride, qFirst;
-    xcdRange(queryCount, xcdMap, qBegin, qEnd, qStride, qFirst);
-    for (u32 q = qFirst; q < qEnd; q += qStride) {
-        const u32 l = queries[q];
-        const u64 qloc = leafLoc[l];
-        u32 kq[3];
-        locCoords(qloc, kq[0], kq[1], kq[2]);
-        const u32 dq = locDepth(qloc);
-        double c[3];
-        for (int a = 0; a < 3; ++a) c[a] = f.centre(a, kq[a], dq);
-        const double radius = f.h(dq) * (double)relRadius;
-        const double r2 = radius * radius, rInner2 = r2 * (1.0 - 1e-9);
-        const u32 maxDepth = dq + MAX_DEPTH_DIFFERENCE;
-        float eh = 0.f, el = 0.f, sh = 0.f, sl = 0.f;
-        if (lane == 0) {
-            sp = 0;
-            const u32 ax = nodes[0].aux;
-            if (ax & FLAG_S) {   // the root always overlaps the sphere (its centre is inside)
-                if (nodes[0].child & LEAF_BIT) {
-                    const float4 v = nodes[0].sums;
-                    eh = v.x; sh = v.z;
-                } else {
-                    stack[0] = 0;
-                    sp = 1;
-                }
-            }
-        }
-        __syncthreads();
-        for (;;) {
-            const u32 top = sp;
-            if (top == 0) break;
-            const u32 take = min(top, 8u);
-            const u32 base = top - take;
-            const u32 a = lane >> 3, slot = lane & 7;
-            bool push = false;
-            u32 cid = 0;
-            nPops += lane == 0 ? take : 0;
-            if (a < take) {
-                ++nTests;
-                const u32 parent = stack[base + a];
-                const u64 ploc = nodes[parent].loc;
-                cid = nodes[parent].child + slot;
-                const u32 d = locDepth(ploc) + 1;
-                const PriorNode cn = nodes[cid];
-                const u32 ax = cn.aux;
-                if (d <= maxDepth && (ax & FLAG_S)) {
-                    u32 k[3];
-                    locCoords((ploc << 3) | slot, k[0], k[1], k[2]);
-                    double dmin = 0.0, dmax = 0.0;
-                    for (int q3 = 0; q3 < 3; ++q3) {   // SphereNodesChecker.cpp:29-40
-                        const double lo = f.B(q3, k[q3], d), hi = f.B(q3, (u64)k[q3] + 1, d);
-                        if (c[q3] < lo) dmin += (lo - c[q3]) * (lo - c[q3]);
-                        else if (c[q3] > hi) dmin += (c[q3] - hi) * (c[q3] - hi);
-                        const double far = fmax(fabs(c[q3] - lo), fabs(hi - c[q3]));
-                        dmax += far * far;
-                    }
-                    if (dmin <= r2) {
-                        const bool leaf = (cn.child & LEAF_BIT) != 0;
-                        if (leaf || (dmax <= rInner2 && (ax >> 8) <= maxDepth)) {
-                            ++nAccum;
-                            const float4 v = cn.sums;
-                            twoSumAdd(eh, el, v.x);
-                            el += v.y;
-                            twoSumAdd(sh, sl, v.z);
-                            sl += v.w;
-                        } else {
-                            push = true;
-                        }
-                    }
-                }
-            }
-            __syncthreads();   // every lane has read its parent before the slots are reused
-            const u64 mask = __ballot(push);
-            const u32 pos = (u32)__popcll(mask & ((1ull << lane) - 1ull));
-            const u32 newTop = base + (u32)__popcll(mask);
-            if (newTop > PRIOR_STACK) {
-                if (lane == 0) { atomicOr(err, 1u); sp = 0; }
-            } else {
-                if (push) stack[base + pos] = cid;
-                if (lane == 0) sp = newTop;
-            }
-            __syncthreads();
-        }
-        // fixed-order wave reduction in FP64 (deterministic)
-        double e = (double)eh + (double)el, s = (double)sh + (double)sl;
-        for (int off = 32; off > 0; off >>= 1) {
-            e += __shfl_xor(e, off);
-            s += __shfl_xor(s, off);
-        }
-        if (lane == 0) prior[l] = e / (e + s);
-        __syncthreads();
-    }
-    if (stats) {
-        waveAccumulate(&stats[0], nPops);
-        waveAccumulate(&stats[1], nTests);
-        waveAccumulate(&stats[2], nAccum);
-    }
-}
-
-// Packet form of the same query: one wavefront serves 64 consecutive queries
-// (leaf order = Morton order, so their spheres overlap heavily).  The wave walks
-// the union of the nodes the 64 queries open, once: an LDS stack holds (parent,
-// lane mask of the queries that still need its children); each step loads the 8
-// child records once for all lanes and every lane in the mask applies its own
-// sphere, depth and flag tests (the same FP64 predicates as the per-query form).
-// Contributions are summed per lane in the fixed traversal order.
-constexpr u32 PACKET_STACK = 256;   // DFS: <= 7 per level + 8 entries
-
-__global__ __launch_bounds__(64) void k_priorsPacket(Frame f, const PriorNode *__restrict__ nodes,
-                                                     const u64 *__restrict__ leafLoc,
-                                                     const u32 *__restrict__ queries, u32 queryCount,
-                                                     float relRadius, double *__restrict__ prior, u32 *err,
-                                                     unsigned long long *__restrict__ stats, int xcdMap) {
-    __shared__ u32 stNode[PACKET_STACK];
-    __shared__ u64 stMask[PACKET_STACK];
-    __shared__ u32 sp;
-    const u32 lane = threadIdx.x;
-    u32 nPops = 0, nAccum = 0;
-    const u32 packets = (queryCount + 63) / 64;
-    u32 pBegin, pEnd, pStride, pFirst;
-    xcdRange(packets, xcdMap, pBegin, pEnd, pStride, pFirst);
-    for (u32 pk = pFirst; pk < pEnd; pk += pStride) {
-        const u32 q = pk * 64 + lane;
-        const bool valid = q < queryCount;
-        const u32 l = valid ? queries[q] : 0u;
-        const u64 qloc = valid ? leafLoc[l] : 1ull;
-        u32 kq[3];
-        locCoords(qloc, kq[0], kq[1], kq[2]);
-        const u32 dq = locDepth(qloc);
-        double c[3];
-        for (int a = 0; a < 3; ++a) c[a] = f.centre(a, kq[a], dq);
-        const double radius = f.h(dq) * (double)relRadius;
-        const double r2 = radius * radius, rInner2 = r2 * (1.0 - 1e-9);
-        const u32 maxDepth = dq + MAX_DEPTH_DIFFERENCE;
-        float eh = 0.f, el = 0.f, sh = 0.f, sl = 0.f;
-        const u64 active = __ballot(valid);
-        const PriorNode root = nodes[0];
-        if (root.aux & FLAG_S) {   // the root always overlaps the sphere (its centre is inside)
-            if (root.child & LEAF_BIT) {
-                eh = root.sums.x;
-                sh = root.sums.z;
-            } else if (lane == 0) {
-                stNode[0] = 0;
-                stMask[0] = active;
-                sp = 1;
-            }
-        } else if (lane == 0) {
-            sp = 0;
-        }
-        if (root.child & LEAF_BIT && lane == 0) sp = 0;
-        __syncthreads();
-        for (;;) {
-            const u32 top = sp;
-            if (top == 0) break;
-            const u32 parent = stNode[top - 1];
-            const u64 mask = stMask[top - 1];
-            __syncthreads();   // every lane has read the entry before it is overwritten
-            nPops += lane == 0 ? 1u : 0u;
-            const PriorNode pn = nodes[parent];
-            const u32 d = locDepth(pn.loc) + 1;
-            u32 kp[3];
-            locCoords(pn.loc, kp[0], kp[1], kp[2]);
-            u32 pushBits = 0;
-            if (((mask >> lane) & 1ull) && d <= maxDepth) {
-#pragma unroll
-                for (u32 slot = 0; slot < 8; ++slot) {
-                    const PriorNode cn = nodes[pn.child + slot];
-                    if (!(cn.aux & FLAG_S)) continue;
-                    const u32 k[3] = {2 * kp[0] + (slot & 1), 2 * kp[1] + ((slot >> 1) & 1),
-                                      2 * kp[2] + ((slot >> 2) & 1)};
-                    double dmin = 0.0, dmax = 0.0;
-                    for (int q3 = 0; q3 < 3; ++q3) {   // SphereNodesChecker.cpp:29-40
-                        const double lo = f.B(q3, k[q3], d), hi = f.B(q3, (u64)k[q3] + 1, d);
-                        if (c[q3] < lo) dmin += (lo - c[q3]) * (lo - c[q3]);
-                        else if (c[q3] > hi) dmin += (c[q3] - hi) * (c[q3] - hi);
-                        const double far = fmax(fabs(c[q3] - lo), fabs(hi - c[q3]));
-                        dmax += far * far;
-                    }
-                    if (dmin <= r2) {
-                        const bool leaf = (cn.child & LEAF_BIT) != 0;
-                        if (leaf || (dmax <= rInner2 && (cn.aux >> 8) <= maxDepth)) {
-                            ++nAccum;
-                            twoSumAdd(eh, el, cn.sums.x);
-                            el += cn.sums.y;
-                            twoSumAdd(sh, sl, cn.sums.z);
-                            sl += cn.sums.w;
-                        } else {
-                            pushBits |= 1u << slot;
-                        }
-                    }
-                }
-            }
-            u32 newTop = top - 1;
-            for (u32 slot = 0; slot < 8; ++slot) {
-                const u64 m = __ballot((pushBits >> slot) & 1u);
-                if (m) {
-                    if (newTop < PACKET_STACK && lane == 0) {
-                        stNode[newTop] = pn.child + slot;
-                        stMask[newTop] = m;
-                    }
-                    ++newTop;
-                }
-            }
-            if (lane == 0) {
-                if (newTop > PACKET_STACK) {
-                    atomicOr(err, 1u);
-                    newTop = 0;
-                }
-                sp = newTop;
-            }
-            __syncthreads();
-        }
-        if (valid) {
-            const double e = (double)eh + (double)el, s = (double)sh + (double)sl;
-            prior[l] = e / (e + s);
-        }
-        __syncthreads();
-    }
-    if (stats) {
-        waveAccumulate(&stats[0], nPops);
-        waveAccumulate(&stats[2], nAccum);
-    }
-}
-
-// Group form: one wavefront serves 8 consecutive queries.  Stack entries carry an
-// 8-bit mask of the queries that still need a node's children; each step pops up
-// to 8 entries and tests their 8 children on 64 lanes (entry = lane / 8, child =
-// lane % 8), every lane looping over the queries of its entry's mask.  A child
-// record is loaded once for all queries of the group; every query applies the
-// same FP64 predicates as the per-query form.  Sums are FP64 per lane and query,
-// reduced across the wave in a fixed order at the end (deterministic).
-template <u32 GROUP_Q>
-__global__ __launch_bounds__(64) void k_priorsGroup(Frame f, const PriorNode *__restrict__ nodes,
-                                                    const u64 *__restrict__ leafLoc,
-                                                    const u32 *__restrict__ queries, u32 queryCount,
-                                                    float relRadius, double *__restrict__ prior, u32 *err,
-                                                    unsigned long long *__restrict__ stats, int xcdMap,
-                                                    u32 *__restrict__ nextGroup) {
-    __shared__ u32 stNode[PRIOR_STACK];
-    __shared__ uint8_t stMask[PRIOR_STACK];
+template <u32 GROUP_Q, u32 STACK>
+__device__ __forceinline__ void priorsGroup(PRIORS_GROUP_PARAMS) {
+    __shared__ u32 stFirst[STACK];
+    __shared__ u64 stK[STACK];          // kx | ky << 21 | kz << 42
+    __shared__ uint16_t stMD[STACK];    // query mask | depth << 8
     __shared__ double qc[GROUP_Q][3], qr2[GROUP_Q], qri2[GROUP_Q];
     // FP32 query data packed for two LDS reads per query: (centre, r^2), (inner r^2, margin, depth limit)
     __shared__ float4 qA[GROUP_Q], qB[GROUP_Q];
-    __shared__ u32 qmd[GROUP_Q];
-    __shared__ u32 sp;
+    __shared__ u32 sp, abandoned;
     const u32 lane = threadIdx.x;
-    u32 nPops = 0, nTests = 0;
+    u32 nPops = 0, nTests = 0, nWhole = 0;
     // FP32 pre-test of the FP64 sphere predicates.  Box bounds are formed in float
     // (rootMinF + k * hF: two roundings, <= 2 * 2^-24 * M off the FP64 bound, M bounding
     // every |coordinate|), query centres are rounded to float (<= 2^-24 * M) and their
@@ -986,20 +779,28 @@ __global__ __launch_bounds__(64) void k_priorsGroup(Frame f, const PriorNode *__
     double M = 0.0;
     for (int a = 0; a < 3; ++a) M = fmax(M, fabs(f.rootMin[a]) + f.rootSize);
     const double eps = 5.0 * ldexp(M, -24);
-    // groups are taken dynamically from a counter per XCD slice (blocks are dispatched
+    // Groups are taken dynamically from a counter per XCD slice (blocks are dispatched
     // round-robin over the 8 XCDs, so block b runs on XCD b mod 8): each XCD's L2 sees
-    // one contiguous, spatially coherent range, and waves with cheap groups take more
-    const u32 groups = (queryCount + GROUP_Q - 1) / GROUP_Q;
-    const u32 slice = xcdMap ? (groups + XCDS - 1) / XCDS : groups;
-    const u32 xcd = xcdMap ? blockIdx.x % XCDS : 0u;
-    const u32 gBegin = min(groups, xcd * slice), gEnd = min(groups, gBegin + slice);
+    // one contiguous, spatially coherent range, and waves with cheap groups take more.
+    // A wave whose slice is used up moves on to the next slice.  With a group list,
+    // the listed groups are taken from one counter.
+    const u32 groups = groupList ? *groupListCount : (queryCount + GROUP_Q - 1) / GROUP_Q;
+    const u32 slices = (xcdMap && !groupList) ? XCDS : 1u;
+    const u32 slice = (groups + slices - 1) / slices;
+    u32 cur = blockIdx.x % slices, tried = 0;
     __shared__ u32 gNext;
     for (;;) {
-        if (lane == 0) gNext = gBegin + atomicAdd(&nextGroup[xcd], 1u);
+        const u32 gBegin = min(groups, cur * slice), gEnd = min(groups, gBegin + slice);
+        if (lane == 0) gNext = gBegin + atomicAdd(&nextGroup[cur], 1u);
         __syncthreads();
-        const u32 g = gNext;
+        const u32 gi = gNext;
         __syncthreads();
-        if (g >= gEnd) break;
+        if (gi >= gEnd) {
+            if (++tried == slices) break;
+            cur = cur + 1 == slices ? 0u : cur + 1;
+            continue;
+        }
+        const u32 g = groupList ? groupList[gi] : gi;
         const u32 nQ = min(GROUP_Q, queryCount - g * GROUP_Q);
         if (lane < GROUP_Q) {
             const u32 l = lane < nQ ? queries[g * GROUP_Q + lane] : 0u;
@@ -1007,10 +808,7 @@ __global__ __launch_bounds__(64) void k_priorsGroup(Frame f, const PriorNode *__
             u32 kq[3];
             locCoords(qloc, kq[0], kq[1], kq[2]);
             const u32 dq = locDepth(qloc);
-            for (int a = 0; a < 3; ++a) {
-                qc[lane][a] = f.centre(a, kq[a], dq);
-                (void)0;
-            }
+            for (int a = 0; a < 3; ++a) qc[lane][a] = f.centre(a, kq[a], dq);
             const double radius = f.h(dq) * (double)relRadius;
             qr2[lane] = radius * radius;
             qri2[lane] = radius * radius * (1.0 - 1e-9);
@@ -1018,26 +816,28 @@ __global__ __launch_bounds__(64) void k_priorsGroup(Frame f, const PriorNode *__
             qB[lane] = make_float4((float)qri2[lane],
                                    (float)(4.0 * eps * radius + 4.0 * eps * eps + ldexp(radius * radius, -20)),
                                    __uint_as_float(dq + MAX_DEPTH_DIFFERENCE), 0.f);
-            qmd[lane] = dq + MAX_DEPTH_DIFFERENCE;
         }
         double accE[GROUP_Q], accS[GROUP_Q];
 #pragma unroll
         for (u32 j = 0; j < GROUP_Q; ++j) accE[j] = accS[j] = 0.0;
-        const PriorNode root = nodes[0];
+        const uint2 root = links[0];
         const uint8_t all = (uint8_t)((1u << nQ) - 1u);
         if (lane == 0) {
             sp = 0;
-            if ((root.aux & FLAG_S) && !(root.child & LEAF_BIT)) {
-                stNode[0] = 0;
-                stMask[0] = all;
+            abandoned = 0;
+            if ((root.y & FLAG_S) && !(root.x & LEAF_BIT)) {
+                stFirst[0] = root.x;
+                stK[0] = 0ull;
+                stMD[0] = all;
                 sp = 1;
             }
         }
-        if ((root.aux & FLAG_S) && (root.child & LEAF_BIT) && lane == 0) {   // every query sums the root
+        if ((root.y & FLAG_S) && (root.x & LEAF_BIT) && lane == 0) {   // every query sums the root
+            const float4 rs = sums[0];
 #pragma unroll
             for (u32 j = 0; j < GROUP_Q; ++j) {
-                accE[j] = (double)root.sums.x + (double)root.sums.y;
-                accS[j] = (double)root.sums.z + (double)root.sums.w;
+                accE[j] = (double)rs.x + (double)rs.y;
+                accS[j] = (double)rs.z + (double)rs.w;
             }
         }
         __syncthreads();
@@ -1048,27 +848,32 @@ __global__ __launch_bounds__(64) void k_priorsGroup(Frame f, const PriorNode *__
             const u32 base = top - take;
             const u32 e = lane >> 3, slot = lane & 7;
             uint8_t pushMask = 0;
-            u32 cid = 0;
+            uint2 cn = make_uint2(0u, 0u);
+            u32 k[3] = {0u, 0u, 0u}, d = 0;
             nPops += lane == 0 ? take : 0;
             if (e < take) {
-                const u32 parent = stNode[base + e];
-                const u32 mask = stMask[base + e];
-                const PriorNode pn = nodes[parent];
-                cid = pn.child + slot;
-                const PriorNode cn = nodes[cid];
-                const u32 d = locDepth(pn.loc) + 1;
-                if (cn.aux & FLAG_S) {
+                const u32 cid = stFirst[base + e] + slot;
+                const u64 pk = stK[base + e];
+                const u32 md16 = stMD[base + e];
+                cn = links[cid];
+                const float4 cs = sums[cid];
+                const u32 mask = md16 & 0xffu;
+                d = (md16 >> 8) + 1;
+                // child coordinates: slot bit 0 -> x, bit 1 -> y, bit 2 -> z (location code order)
+                k[0] = 2u * (u32)(pk & 0x1fffffull) + (slot & 1u);
+                k[1] = 2u * (u32)((pk >> 21) & 0x1fffffull) + ((slot >> 1) & 1u);
+                k[2] = 2u * (u32)(pk >> 42) + (slot >> 2);
+                if (cn.y & FLAG_S) {
                     ++nTests;
-                    u32 k[3];
-                    locCoords((pn.loc << 3) | slot, k[0], k[1], k[2]);
                     const float hF = ldexpf(f.rootSizeF, -(int)d);
                     float loF[3], hiF[3];
                     for (int a = 0; a < 3; ++a) {
                         loF[a] = f.rootMinF[a] + (float)k[a] * hF;
                         hiF[a] = f.rootMinF[a] + (float)(k[a] + 1) * hF;
                     }
-                    const bool leaf = (cn.child & LEAF_BIT) != 0;
-                    const u32 mld = cn.aux >> 8;
+                    const bool leaf = (cn.x & LEAF_BIT) != 0;
+                    const u32 mld = cn.y >> 8;
+                    bool summed = false;
 #pragma unroll
                     for (u32 j = 0; j < GROUP_Q; ++j) {
                         if (!((mask >> j) & 1u)) continue;
@@ -1112,30 +917,39 @@ __global__ __launch_bounds__(64) void k_priorsGroup(Frame f, const PriorNode *__
                             }
                         }
                         if (whole) {
-                            accE[j] += (double)cn.sums.x + (double)cn.sums.y;
-                            accS[j] += (double)cn.sums.z + (double)cn.sums.w;
-                        } else {
+                            accE[j] += (double)cs.x + (double)cs.y;
+                            accS[j] += (double)cs.z + (double)cs.w;
+                            summed = true;
+                        } else if (d < md) {   // children at depth d + 1 <= md still count
                             pushMask |= (uint8_t)(1u << j);
                         }
                     }
+                    nWhole += summed ? 1u : 0u;
                 }
             }
             __syncthreads();   // every lane has read its entry before the slots are reused
             const u64 pm = __ballot(pushMask != 0);
             const u32 pos = (u32)__popcll(pm & ((1ull << lane) - 1ull));
             const u32 newTop = base + (u32)__popcll(pm);
-            if (newTop > PRIOR_STACK) {
-                if (lane == 0) { atomicOr(err, 1u); sp = 0; }
+            if (newTop > STACK) {
+                if (lane == 0) {
+                    if (overflowList) overflowList[atomicAdd(overflowCount, 1u)] = gi;
+                    else atomicOr(err, 1u);
+                    abandoned = 1;
+                    sp = 0;
+                }
             } else {
                 if (pushMask) {
-                    stNode[base + pos] = cid;
-                    stMask[base + pos] = pushMask;
+                    stFirst[base + pos] = cn.x;
+                    stK[base + pos] = (u64)k[0] | ((u64)k[1] << 21) | ((u64)k[2] << 42);
+                    stMD[base + pos] = (uint16_t)(pushMask | (d << 8));
                 }
                 if (lane == 0) sp = newTop;
             }
             __syncthreads();
         }
         // fixed-order wave reduction per query (FP64)
+        const bool write = abandoned == 0;
 #pragma unroll
         for (u32 j = 0; j < GROUP_Q; ++j) {
             double ee = accE[j], ss = accS[j];
@@ -1143,14 +957,21 @@ __global__ __launch_bounds__(64) void k_priorsGroup(Frame f, const PriorNode *__
                 ee += __shfl_xor(ee, off);
                 ss += __shfl_xor(ss, off);
             }
-            if (lane == 0 && j < nQ) prior[queries[g * GROUP_Q + j]] = ee / (ee + ss);
+            if (lane == 0 && j < nQ && write) prior[queries[g * GROUP_Q + j]] = ee / (ee + ss);
         }
         __syncthreads();
     }
     if (stats) {
         waveAccumulate(&stats[0], nPops);
         waveAccumulate(&stats[1], nTests);
+        waveAccumulate(&stats[2], nWhole);
     }
+}
+
+// register use capped for 5 waves per SIMD (a few bytes of spill; faster than 4 waves)
+template <u32 GROUP_Q, u32 STACK>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_priorsGroup(PRIORS_GROUP_PARAMS) {
+    priorsGroup<GROUP_Q, STACK>(PRIORS_GROUP_ARGS);
 }
 
 // Occupancy.cpp:570-587, FP64 (leaves under learned-culled nodes are empty)
@@ -1209,26 +1030,6 @@ __global__ void k_camKeys(const u32 *__restrict__ cams, u32 links, u32 camCount,
     idx[l] = base + l;
 }
 
-// priors kernel: 0 = one wavefront per query, 1 = 64 queries per wavefront,
-// 2 = 8 queries per wavefront (TSR_PRIORS_KERNEL=query|packet|group)
-int priorsPacketMode() {
-    static const int m = [] {
-        const char *e = std::getenv("TSR_PRIORS_KERNEL");
-        if (e && std::strcmp(e, "query") == 0) return 0;
-        if (e && std::strcmp(e, "packet") == 0) return 1;
-        return 2;
-    }();
-    return m;
-}
-
-int priorsGroupSize() {   // TSR_PRIORS_GROUP=4: 4 queries per wavefront (default 8)
-    static const int m = [] {
-        const char *e = std::getenv("TSR_PRIORS_GROUP");
-        return e ? std::atoi(e) : 8;
-    }();
-    return m;
-}
-
 // cone traversal mode 3 with persistent threads (TSR_CONE_PERSISTENT=0: one thread per link)
 int conePersistentMode() {
     static const int m = [] {
@@ -1264,9 +1065,9 @@ void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *s
     field->occ.alloc(L);
     HIPCHECK(hipMemsetAsync(field->states.p, 0, (size_t)N * 4, st));
 
-    TraversalTree tt{tree->child.p, tree->parents.p, field->states.p, tree->frame,
-                     (float)(tree->frame.rootSize * 1e-5), nullptr};
-    DBuf<uint2> packedNodes;
+    TraversalTree tt{nullptr, nullptr, field->states.p, tree->frame, (float)(tree->frame.rootSize * 1e-5), nullptr};
+    ConeArena &ar = ctx->cone;
+    traversalLayout(st, tree, ar.childPad, ar.blockParent, tt);
     LinkInputs in{tree->px.p, tree->py.p, tree->pz.p, tree->nx.p, tree->ny.p, tree->nz.p, tree->scale.p,
                   tree->conf.p, tree->cams.p, scene->cx.p, scene->cy.p, scene->cz.p, tree->sampleDepth.p,
                   tree->maxCams, scene->camCount, prm.bandwidth_factor, prm.relative_sample_cone_length, nullptr};
@@ -1300,9 +1101,10 @@ void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *s
             k_setLeafFlags<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(hitCount.p, tree->leafNode.p, L, FLAG_S, field->states.p);
             k_propagateFlag<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(tree->leafNode.p, L, tree->parents.p, FLAG_S,
                                                                  field->states.p);
-            packedNodes.alloc(N);   // (child, state) pairs for the gated emptiness traversal
-            k_packChildState<<<gridFor(N, BLOCK), BLOCK, 0, st>>>(tree->child.p, field->states.p, N, packedNodes.p);
-            tt.packed = packedNodes.p;
+            // (child, state) pairs for the gated emptiness traversal, node-padded
+            uint2 *packed = ar.packed.ensure((size_t)N + NODE_PAD) + NODE_PAD;
+            k_packChildState<<<gridFor(N, BLOCK), BLOCK, 0, st>>>(tree->child.p, field->states.p, N, packed);
+            tt.packed = packed;
             if (culler) {   // learned culling: nodes predicted far from any zero crossing
                 cullerMask(ctx, culler, tree, scene, prm, field->states.p, field->hitsS.p,
                            field->ksum.p + L, cullMask);
@@ -1370,8 +1172,7 @@ void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *s
                              std::chrono::duration<double, std::milli>(t - tmark).count());
                 tmark = t;
             };
-            ConeArena &ar = ctx->cone;
-            DBuf<u32> linkCursor(1);
+            DBuf<u32> linkCursor(8);
             u32 rangeLinks = chunkLinks;
             if ((u64)links * 64 > CONE_MAX_RECORDS) {
                 // records per link estimated on every 64th link (counting traversal), so that the
@@ -1416,12 +1217,12 @@ void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *s
                 HIPCHECK(hipMemsetAsync(kst.p, 0, 32, st));
                 // persistent threads pay off for the long emptiness traversals only
                 const bool persistent = !sampleness && conePersistentMode();
-                if (persistent) HIPCHECK(hipMemsetAsync(linkCursor.p, 0, 4, st));
+                if (persistent) HIPCHECK(hipMemsetAsync(linkCursor.p, 0, 32, st));
                 timedLaunch(ctx, sampleness ? KSTAT_S_COUNT : KSTAT_E_COUNT, [&] {
                     if (persistent)
                         k_conePassPT<<<std::min<u32>(gridFor(n, BLOCK), CONE_PT_BLOCKS), BLOCK, 0, st>>>(
                             tt, ic, n, sampleness, counts.p, stLeaf, stVal, sampleness ? nullptr : confs.p,
-                            linkCursor.p, kst.p);
+                            linkCursor.p, (xcdMapMode() & 4) ? XCD_COUNT : 1u, kst.p);
                     else
                         k_conePass<<<xcdGrid(n, BLOCK), BLOCK, 0, st>>>(tt, ic, n, sampleness, 3, counts.p, nullptr,
                                                                         stLeaf, stVal, nullptr, nullptr, nullptr,
@@ -1533,13 +1334,17 @@ void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *s
     recordPhase(ctx, 3, true);
     // priors
     {
-        DBuf<float4> sums(N);
-        DBuf<u32> aux(N), queries(L), counter(1), perr(1);
+        // node i at index i + NODE_PAD of the traversal records (child blocks line-aligned)
+        DBuf<float4> sumsPad(N + NODE_PAD);
+        DBuf<uint2> linksPad(N + NODE_PAD);
+        float4 *sums = sumsPad.p + NODE_PAD;
+        uint2 *plinks = linksPad.p + NODE_PAD;
+        DBuf<u32> aux(N), queries(L), perr(1);
         k_subtreeLeaves<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(tree->leafNode.p, L, field->clen.p, tree->nodeDepth.p,
-                                                             field->states.p, sums.p, aux.p);
+                                                             field->states.p, sums, aux.p);
         for (int lvl = (int)tree->maxDepth - 1; lvl >= 0; --lvl)
             k_subtreeLevel<<<gridFor(N, BLOCK), BLOCK, 0, st>>>(tree->child.p, tree->nodeDepth.p, N, (u32)lvl,
-                                                                field->states.p, sums.p, aux.p);
+                                                                field->states.p, sums, aux.p);
         HIPCHECK(hipMemsetAsync(perr.p, 0, 4, st));
         DBuf<u32> qflag(L), qpos(L);
         k_priorQueryFlags<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(field->ksum.p, L, (double)prm.confidence_threshold,
@@ -1549,47 +1354,38 @@ void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *s
         exclusiveScan(sc, qflag.p, qpos.p, L, st);
         k_priorQueryList<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(qflag.p, qpos.p, L, queries.p);
         const u32 nq = readScalar(qpos.p + L - 1, st) + readScalar(qflag.p + L - 1, st);
-        DBuf<PriorNode> pnodes(N);
-        k_packPriorNodes<<<gridFor(N, BLOCK), BLOCK, 0, st>>>(tree->child.p, aux.p, tree->nodeLoc.p, sums.p, N,
-                                                              pnodes.p);
+        k_packPriorLinks<<<gridFor(N, BLOCK), BLOCK, 0, st>>>(tree->child.p, aux.p, N, plinks);
         field->priorQueries = nq;
         if (nq) {
+            constexpr u32 Q = 8;
+            const u32 groups = (nq + Q - 1) / Q;
             const u32 grid = 256u * 32u;   // multiple of the XCD count
             HIPCHECK(hipMemsetAsync(kst.p, 0, 32, st));
-            const int kind = priorsPacketMode();
-            DBuf<u32> groupCounters(8);
-            HIPCHECK(hipMemsetAsync(groupCounters.p, 0, 32, st));
-            const bool packet = kind == 1;
+            // counters: 8 XCD slices of the first pass, 1 for the second; overflow count
+            DBuf<u32> counters(16), overflowList(groups);
+            HIPCHECK(hipMemsetAsync(counters.p, 0, 64, st));
+            const int xcdMap = (xcdMapMode() >> 1) & 1;
             timedLaunch(ctx, KSTAT_PRIORS, [&] {
-                if (kind == 2 && priorsGroupSize() == 4)
-                    k_priorsGroup<4><<<grid, 64, 0, st>>>(tree->frame, pnodes.p, tree->leafLoc.p, queries.p, nq,
-                                                          prm.relative_radius_for_priors, field->prior.p, perr.p,
-                                                          kst.p, (xcdMapMode() >> 1) & 1, groupCounters.p);
-                else if (kind == 2)
-                    k_priorsGroup<8><<<grid, 64, 0, st>>>(tree->frame, pnodes.p, tree->leafLoc.p, queries.p, nq,
-                                                          prm.relative_radius_for_priors, field->prior.p, perr.p,
-                                                          kst.p, (xcdMapMode() >> 1) & 1, groupCounters.p);
-                else if (packet)
-                    k_priorsPacket<<<grid, 64, 0, st>>>(tree->frame, pnodes.p, tree->leafLoc.p, queries.p, nq,
-                                                        prm.relative_radius_for_priors, field->prior.p, perr.p,
-                                                        kst.p, (xcdMapMode() >> 1) & 1);
-                else
-                    k_priorsWave<<<grid, 64, 0, st>>>(tree->frame, pnodes.p, tree->leafLoc.p, queries.p, nq,
-                                                      prm.relative_radius_for_priors, field->prior.p, perr.p, kst.p,
-                                                      (xcdMapMode() >> 1) & 1);
+                k_priorsGroup<Q, PRIOR_STACK_SHORT><<<grid, 64, 0, st>>>(
+                    tree->frame, plinks, sums, tree->leafLoc.p, queries.p, nq, prm.relative_radius_for_priors,
+                    field->prior.p, perr.p, kst.p, xcdMap, counters.p, nullptr, nullptr, overflowList.p,
+                    counters.p + 9);
+                k_priorsGroup<Q, PRIOR_STACK><<<1024, 64, 0, st>>>(
+                    tree->frame, plinks, sums, tree->leafLoc.p, queries.p, nq, prm.relative_radius_for_priors,
+                    field->prior.p, perr.p, kst.p, 0, counters.p + 8, overflowList.p, counters.p + 9, nullptr,
+                    nullptr);
             });
             unsigned long long ps[3];
             readStats(3, ps);
+            field->priorsOverflowGroups = readScalar(counters.p + 9, st);
+            if (std::getenv("TSR_TRACE"))
+                std::fprintf(stderr, "[tsr occupancy] priors: %u queries, %llu pops, %llu tests, %llu summed, "
+                                     "%u of %u groups redone with the long stack\n",
+                             nq, ps[0], ps[1], ps[2], field->priorsOverflowGroups, groups);
             tsr_kernel_stat &k = ctx->kstat[KSTAT_PRIORS];
-            if (kind == 2)   // per query 16 B; per popped entry: parent record 32 B; per tested
-                             // child: its 32 B record, shared by the group's queries
-                k.bytes += (u64)nq * 16 + ps[0] * 32 + ps[1] * 32;
-            else if (packet)   // per query: index 4 B + leaf code 8 B + prior 4 B; per opened parent: its
-                               // 32 B record + the 8 contiguous 32 B child records, shared by the packet
-                k.bytes += (u64)nq * 16 + ps[0] * 288;
-            else          // per opened node: code 8 B + child link 4 B; per tested child: state 4 B +
-                          // child link 4 B; per summed node or leaf: 16 B (E, S hi/lo)
-                k.bytes += (u64)nq * 16 + ps[0] * 12 + ps[1] * 8 + ps[2] * 16;
+            // per query: index 4 B + leaf code 8 B + prior 8 B; per popped entry: the 8 child
+            // links (8 B each); per child summed by some query of the group: its 16 B sums
+            k.bytes += (u64)nq * 20 + ps[0] * 64 + ps[2] * 16;
             k.units += nq;
         }
         if (readScalar(perr.p, st)) throw tsr_error(TSR_ERR_HIP, "priors traversal stack overflow");
@@ -1607,8 +1403,9 @@ void emptinessSubsample(tsr_context *ctx, const tsr_tree *tree, const tsr_scene 
                         const u32 *states, u32 stride, u32 *leafHits) {
     hipStream_t st = ctx->stream;
     const u32 links = (u32)((u64)scene->n * scene->maxCams);
-    TraversalTree tt{tree->child.p, tree->parents.p, states, tree->frame, (float)(tree->frame.rootSize * 1e-5),
-                     nullptr};
+    TraversalTree tt{nullptr, nullptr, states, tree->frame, (float)(tree->frame.rootSize * 1e-5), nullptr};
+    DBuf<u32> childPad, blockParent;
+    traversalLayout(st, tree, childPad, blockParent, tt);
     LinkInputs in{tree->px.p, tree->py.p, tree->pz.p, tree->nx.p, tree->ny.p, tree->nz.p, tree->scale.p,
                   tree->conf.p, tree->cams.p, scene->cx.p, scene->cy.p, scene->cz.p, tree->sampleDepth.p,
                   tree->maxCams, scene->camCount, prm.bandwidth_factor, prm.relative_sample_cone_length, nullptr,
