@@ -765,7 +765,8 @@ __device__ __forceinline__ void priorsGroup(PRIORS_GROUP_PARAMS) {
     __shared__ u64 stK[STACK];          // kx | ky << 21 | kz << 42
     __shared__ uint16_t stMD[STACK];    // query mask | depth << 8
     __shared__ double qc[GROUP_Q][3], qr2[GROUP_Q], qri2[GROUP_Q];
-    // FP32 query data packed for two LDS reads per query: (centre, r^2), (inner r^2, margin, depth limit)
+    // FP32 query data packed for two LDS reads per query: (centre, r^2 + margin),
+    // (r^2 - margin, inner r^2 - margin, inner r^2 + margin, depth limit)
     __shared__ float4 qA[GROUP_Q], qB[GROUP_Q];
     __shared__ u32 sp, abandoned;
     const u32 lane = threadIdx.x;
@@ -812,10 +813,11 @@ __device__ __forceinline__ void priorsGroup(PRIORS_GROUP_PARAMS) {
             const double radius = f.h(dq) * (double)relRadius;
             qr2[lane] = radius * radius;
             qri2[lane] = radius * radius * (1.0 - 1e-9);
-            qA[lane] = make_float4((float)qc[lane][0], (float)qc[lane][1], (float)qc[lane][2], (float)qr2[lane]);
-            qB[lane] = make_float4((float)qri2[lane],
-                                   (float)(4.0 * eps * radius + 4.0 * eps * eps + ldexp(radius * radius, -20)),
-                                   __uint_as_float(dq + MAX_DEPTH_DIFFERENCE), 0.f);
+            // decision thresholds in float, formed once per query: r^2 +- margin, inner r^2 +- margin
+            const float r2F = (float)qr2[lane], ri2F = (float)qri2[lane];
+            const float mg = (float)(4.0 * eps * radius + 4.0 * eps * eps + ldexp(radius * radius, -20));
+            qA[lane] = make_float4((float)qc[lane][0], (float)qc[lane][1], (float)qc[lane][2], r2F + mg);
+            qB[lane] = make_float4(r2F - mg, ri2F - mg, ri2F + mg, __uint_as_float(dq + MAX_DEPTH_DIFFERENCE));
         }
         double accE[GROUP_Q], accS[GROUP_Q];
 #pragma unroll
@@ -878,7 +880,7 @@ __device__ __forceinline__ void priorsGroup(PRIORS_GROUP_PARAMS) {
                     for (u32 j = 0; j < GROUP_Q; ++j) {
                         if (!((mask >> j) & 1u)) continue;
                         const float4 A = qA[j], B = qB[j];
-                        const u32 md = __float_as_uint(B.z);
+                        const u32 md = __float_as_uint(B.w);
                         if (d > md) continue;
                         const float cq[3] = {A.x, A.y, A.z};
                         // fused multiply-adds: one rounding each, inside the error bound above
@@ -890,9 +892,8 @@ __device__ __forceinline__ void priorsGroup(PRIORS_GROUP_PARAMS) {
                             const float far = fmaxf(fabsf(a0), fabsf(a1));
                             dmaxF = __builtin_fmaf(far, far, dmaxF);
                         }
-                        const float mg = B.y;
-                        if (dminF > A.w + mg) continue;   // surely outside
-                        if (!(dminF < A.w - mg)) {        // undecided: the FP64 test
+                        if (dminF > A.w) continue;      // surely outside
+                        if (!(dminF < B.x)) {           // undecided: the FP64 test
                             double dmin = 0.0;
                             for (int q3 = 0; q3 < 3; ++q3) {   // SphereNodesChecker.cpp:29-40
                                 const double c = qc[j][q3];
@@ -904,9 +905,9 @@ __device__ __forceinline__ void priorsGroup(PRIORS_GROUP_PARAMS) {
                         }
                         bool whole = leaf;
                         if (!whole && mld <= md) {
-                            if (dmaxF < B.x - mg) whole = true;
-                            else if (!(dmaxF > B.x + mg)) {
-                                double dmax = 0.0;
+                            if (dmaxF < B.y) whole = true;
+                            else if (!(dmaxF > B.z)) {
+                                    double dmax = 0.0;
                                 for (int q3 = 0; q3 < 3; ++q3) {
                                     const double c = qc[j][q3];
                                     const double lo = f.B(q3, k[q3], d), hi = f.B(q3, (u64)k[q3] + 1, d);
