@@ -151,10 +151,13 @@ __device__ bool setupLink(const LinkInputs &in, u32 link, bool sampleness, ConeD
 
 // Depth-first traversal (NodesIterator.cpp:38-86 / LeavesIterator.cpp:24-31):
 // calls onLeaf(leafIndex, centre) for every leaf that passes depth, flag and
-// cull tests.  Inner nodes failing a test are skipped with their subtrees.
+// cull tests.  Inner nodes failing a test are skipped with their subtrees.  With
+// `path` (this thread's column of an LDS array [MAX_DEPTH + 2][BLOCK]) the ancestors
+// are kept there on the way down and climbing reads them back; without it, climbing
+// reads the block-parent table.
 template <typename OnLeaf>
 __device__ void traverseCone(const TraversalTree &t, const ConeCull &cull, u32 maxDepth, u32 required,
-                             const uint8_t *cullNode, u32 &visits, OnLeaf onLeaf) {
+                             const uint8_t *cullNode, u32 &visits, u32 *path, OnLeaf onLeaf) {
     u32 node = 0, d = 0, k[3] = {0, 0, 0};
     bool descend = true;   // whether to test `node` (true) or move on from it (false)
     for (;;) {
@@ -181,6 +184,7 @@ __device__ void traverseCone(const TraversalTree &t, const ConeCull &cull, u32 m
                     const float lo0 = t.f.rootMinF[0] + (float)k[0] * h, lo1 = t.f.rootMinF[1] + (float)k[1] * h,
                                 lo2 = t.f.rootMinF[2] + (float)k[2] * h;
                     if (!cullSeparated(cull, lo0, lo1, lo2, h, t.margin)) {
+                        if (path) path[(d + 1) * BLOCK] = node;
                         node = c;   // first child
                         ++d;
                         k[0] *= 2; k[1] *= 2; k[2] *= 2;
@@ -202,7 +206,7 @@ __device__ void traverseCone(const TraversalTree &t, const ConeCull &cull, u32 m
                 descend = true;
                 break;
             }
-            node = t.blockParent[(node - 1) >> 3];
+            node = path ? path[d * BLOCK] : t.blockParent[(node - 1) >> 3];
             --d;
             k[0] >>= 1; k[1] >>= 1; k[2] >>= 1;
         }
@@ -213,7 +217,8 @@ __device__ void coneLink(const TraversalTree &t, const LinkInputs &in, u32 link,
                          int sampleness, int mode,
                          u32 *__restrict__ counts, const u32 *__restrict__ offsets, u32 *__restrict__ recLeaf,
                          uint2 *__restrict__ recVal, float *__restrict__ atomicSum, float *__restrict__ atomicLen,
-                         u32 *__restrict__ atomicStates, double *__restrict__ confOut, u32 &visits, u32 &count);
+                         u32 *__restrict__ atomicStates, double *__restrict__ confOut, u32 &visits, u32 &count,
+                         u32 *path);
 
 // wave sum of a per-lane count, added once per wave to *acc
 __device__ void waveAccumulate(unsigned long long *acc, u32 v) {
@@ -231,11 +236,12 @@ __global__ void k_conePass(TraversalTree t, LinkInputs in, u32 linkCount, int sa
                            uint2 *__restrict__ recVal, float *__restrict__ atomicSum, float *__restrict__ atomicLen,
                            u32 *__restrict__ atomicStates, double *__restrict__ confOut,
                            unsigned long long *__restrict__ stats) {
+    __shared__ u32 pathL[MAX_DEPTH + 2][BLOCK];   // ancestors per thread (launched with BLOCK threads)
     const u32 idx = (xcdBlock(in.xcdMap) * blockDim.x + threadIdx.x) * in.linkStride;
     u32 visits = 0, count = 0;
     const u32 link = idx < linkCount && in.order ? in.order[idx] : in.linkBase + idx;
     if (idx < linkCount) coneLink(t, in, link, idx, linkCount, sampleness, mode, counts, offsets, recLeaf, recVal, atomicSum,
-                                   atomicLen, atomicStates, confOut, visits, count);
+                                   atomicLen, atomicStates, confOut, visits, count, &pathL[0][threadIdx.x]);
     if (stats) {
         waveAccumulate(&stats[0], visits);
         waveAccumulate(&stats[1], count);
@@ -246,7 +252,8 @@ __device__ void coneLink(const TraversalTree &t, const LinkInputs &in, u32 link,
                          int sampleness, int mode,
                          u32 *__restrict__ counts, const u32 *__restrict__ offsets, u32 *__restrict__ recLeaf,
                          uint2 *__restrict__ recVal, float *__restrict__ atomicSum, float *__restrict__ atomicLen,
-                         u32 *__restrict__ atomicStates, double *__restrict__ confOut, u32 &visits, u32 &count) {
+                         u32 *__restrict__ atomicStates, double *__restrict__ confOut, u32 &visits, u32 &count,
+                         u32 *path) {
     ConeD cone;
     u32 i;
     const bool ok = setupLink(in, link, sampleness != 0, cone, i);
@@ -262,7 +269,8 @@ __device__ void coneLink(const TraversalTree &t, const LinkInputs &in, u32 link,
     const u32 maxDepth = (u32)in.sampleDepth[i] + MAX_DEPTH_DIFFERENCE;
     const u32 required = sampleness ? 0u : FLAG_S;
     const size_t base = mode == 1 ? (size_t)offsets[link] : 0;
-    traverseCone(t, cull, maxDepth, required, sampleness ? nullptr : in.cullNode, visits, [&](u32 leaf, const double x[3]) {
+    traverseCone(t, cull, maxDepth, required, sampleness ? nullptr : in.cullNode, visits, path,
+                 [&](u32 leaf, const double x[3]) {
         const double kv = coneKernel(cone, x, sampleness != 0, conf);
         if (kv == 0.0) return;
         if (mode == 1 || (mode == 3 && count < in.slots)) {
@@ -298,6 +306,9 @@ __global__ __launch_bounds__(256) void k_conePassPT(TraversalTree t, LinkInputs 
                                                     uint2 *__restrict__ stVal, double *__restrict__ confOut,
                                                     u32 *__restrict__ nextIdx, u32 slices,
                                                     unsigned long long *__restrict__ stats) {
+    // ancestors of the current node per lane: pathL[d][x] = parent of thread x's node at depth d
+    // (climbing reads LDS instead of the block-parent table)
+    __shared__ u32 pathL[MAX_DEPTH + 2][256];
     const u32 lane = threadIdx.x & 63;
     const u32 sliceLen = (linkCount + slices - 1) / slices;
     u32 cur = blockIdx.x % slices, tried = 0;   // wave-uniform
@@ -390,6 +401,7 @@ __global__ __launch_bounds__(256) void k_conePassPT(TraversalTree t, LinkInputs 
                     const float lo0 = t.f.rootMinF[0] + (float)k0 * h, lo1 = t.f.rootMinF[1] + (float)k1 * h,
                                 lo2 = t.f.rootMinF[2] + (float)k2 * h;
                     if (!cullSeparated(cull, lo0, lo1, lo2, h, t.margin)) {
+                        pathL[d + 1][threadIdx.x] = node;
                         node = c;   // first child
                         ++d;
                         k0 *= 2; k1 *= 2; k2 *= 2;
@@ -416,7 +428,7 @@ __global__ __launch_bounds__(256) void k_conePassPT(TraversalTree t, LinkInputs 
                     descend = true;
                     break;
                 }
-                node = t.blockParent[(node - 1) >> 3];
+                node = pathL[d][threadIdx.x];
                 --d;
                 k0 >>= 1; k1 >>= 1; k2 >>= 1;
             }
