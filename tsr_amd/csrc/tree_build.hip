@@ -258,11 +258,15 @@ __global__ void k_samplingEval(Frame f, HashView inner, const float *__restrict_
     }
 }
 
-__global__ void k_compactKeys(const u64 *__restrict__ keys, u64 cap, u64 *__restrict__ out, u32 *counter) {
-    const u64 s = (u64)blockIdx.x * BLOCK + threadIdx.x;
-    if (s >= cap) return;
-    const u64 k = keys[s];
-    if (k != 0ull) out[atomicAdd(counter, 1u)] = k;
+// the occupied keys of a hash table, in slot order (a stream compaction: no counter
+// atomics, deterministic order); their number goes to *count
+struct NonZeroKey {
+    __host__ __device__ bool operator()(const u64 &k) const { return k != 0ull; }
+};
+void compactKeys(Scratch &sc, const DHash &h, u64 *out, u32 *count, hipStream_t st) {
+    size_t tmp = 0;
+    HIPCHECK(hipcub::DeviceSelect::If(nullptr, tmp, h.keys.p, out, count, (int)h.capacity, NonZeroKey(), st));
+    HIPCHECK(hipcub::DeviceSelect::If(sc.get(tmp), tmp, h.keys.p, out, count, (int)h.capacity, NonZeroKey(), st));
 }
 
 __global__ void k_insertList(HashView h, const u64 *__restrict__ locs, u32 n, u32 *err) {
@@ -1247,8 +1251,7 @@ void buildTree(tsr_context *ctx, const tsr_scene *scene, const tsr_params &prm, 
     DBuf<u64> innerList(inner.capacity);
     DBuf<u32> t0Counter(1);
     HIPCHECK(hipMemsetAsync(t0Counter.p, 0, 4, st));
-    k_compactKeys<<<gridFor(inner.capacity, BLOCK), BLOCK, 0, st>>>(inner.keys.p, inner.capacity, innerList.p,
-                                                                     t0Counter.p);
+    compactKeys(sc, inner, innerList.p, t0Counter.p, st);
     const u32 t0Count = readScalar(t0Counter.p, st);
     std::vector<u64> hInner(t0Count);
     if (t0Count) {   // ascending codes = ascending depth: the host inserts parents first
@@ -1329,7 +1332,7 @@ void buildTree(tsr_context *ctx, const tsr_scene *scene, const tsr_params &prm, 
         // units through the pinned staging buffer
         DBuf<u64> hotKeys(hot.capacity);
         HIPCHECK(hipMemsetAsync(counter.p + 0, 0, 4, st));
-        k_compactKeys<<<gridFor(hot.capacity, BLOCK), BLOCK, 0, st>>>(hot.keys.p, hot.capacity, hotKeys.p, counter.p);
+        compactKeys(sc, hot, hotKeys.p, counter.p, st);
         const u32 nh = readScalar(counter.p, st);
         DHash runMap;
         runMap.alloc(std::max<u64>(4096, std::max<u64>(nh, runs)), st);
@@ -1427,7 +1430,7 @@ void buildTree(tsr_context *ctx, const tsr_scene *scene, const tsr_params &prm, 
     // ---- final inner list sorted in pre-order
     innerList.alloc(inner.capacity);
     HIPCHECK(hipMemsetAsync(counter.p, 0, 4, st));
-    k_compactKeys<<<gridFor(inner.capacity, BLOCK), BLOCK, 0, st>>>(inner.keys.p, inner.capacity, innerList.p, counter.p);
+    compactKeys(sc, inner, innerList.p, counter.p, st);
     const u32 I = readScalar(counter.p, st);
     if ((u64)1 + 8ull * I >= (u64)INVALID) throw tsr_error(TSR_ERR_NODE_OVERFLOW, "Required scene tree node count is larger than supported.");
     DBuf<u64> keys(std::max<u32>(I, 1)), keysSorted(std::max<u32>(I, 1)), innerSorted(std::max<u32>(I, 1));
