@@ -279,11 +279,14 @@ __global__ void k_insertList(HashView h, const u64 *__restrict__ locs, u32 n, u3
 // node that is not inner) and each face, the neighbour leaf must be at most one
 // level coarser; otherwise the neighbour's ancestors down to depth d-2 are split.
 // ---------------------------------------------------------------------------
-__global__ void k_balance(HashView inner, u64 cap, u32 *changed, u32 *err) {
-    const u64 s = (u64)blockIdx.x * BLOCK + threadIdx.x;
-    if (s >= cap) return;
-    const u64 parent = inner.keys[s];
-    if (parent == 0ull) return;
+// One pass over a list of inner nodes: their leaf children are checked, and the nodes
+// made inner go to `added` (the next pass needs only those: a split can make new
+// leaves too deep for a neighbour region, never an existing leaf).
+__global__ void k_balance(HashView inner, const u64 *__restrict__ parents, u32 n, u64 *__restrict__ added,
+                          u32 *addedCount, u32 addedCap, u32 *err) {
+    const u32 s = blockIdx.x * BLOCK + threadIdx.x;
+    if (s >= n) return;
+    const u64 parent = parents[s];
     const u32 d = locDepth(parent) + 1;
     if (d < 2) return;
     for (u32 c = 0; c < 8; ++c) {
@@ -309,7 +312,12 @@ __global__ void k_balance(HashView inner, u64 cap, u32 *changed, u32 *err) {
             while (m > 0 && !inner.contains(nb >> (3 * (d - m)))) --m;
             if (m + 1 >= d - 1) continue;            // neighbour leaf at depth m+1 >= d-1
             for (u32 j = m + 1; j <= d - 2; ++j) {
-                if (inner.insert(nb >> (3 * (d - j)), 0, err)) atomicOr(changed, 1u);
+                const u64 split = nb >> (3 * (d - j));
+                if (inner.insert(split, 0, err)) {
+                    const u32 at = atomicAdd(addedCount, 1u);
+                    if (at < addedCap) added[at] = split;
+                    else atomicOr(err, (u32)ERR_HASH_FULL);
+                }
             }
         }
     }
@@ -1416,14 +1424,20 @@ void buildTree(tsr_context *ctx, const tsr_scene *scene, const tsr_params &prm, 
     }
 
     mark("inner set rebuild");
-    // ---- balancing fixpoint
-    for (int iter = 0;; ++iter) {
-        HIPCHECK(hipMemsetAsync(counter.p, 0, 4, st));
-        k_balance<<<gridFor(inner.capacity, BLOCK), BLOCK, 0, st>>>(inner.view(), inner.capacity, counter.p, err.p);
-        const u32 changed = readScalar(counter.p, st);
-        checkErr(readScalar(err.p, st));
-        if (!changed) break;
-        if (iter > 64) throw tsr_error(TSR_ERR_HIP, "balancing did not converge");
+    // ---- balancing fixpoint: all inner nodes once, then only the nodes each pass added
+    {
+        const u32 cap = (u32)std::min<u64>(inner.capacity, 0xffffffffull);
+        DBuf<u64> listA(cap), listB(cap);
+        compactKeys(sc, inner, listA.p, counter.p, st);
+        u32 n = readScalar(counter.p, st);
+        for (int iter = 0; n; ++iter) {
+            if (iter > 64) throw tsr_error(TSR_ERR_HIP, "balancing did not converge");
+            HIPCHECK(hipMemsetAsync(counter.p, 0, 4, st));
+            k_balance<<<gridFor(n, BLOCK), BLOCK, 0, st>>>(inner.view(), listA.p, n, listB.p, counter.p, cap, err.p);
+            n = readScalar(counter.p, st);
+            checkErr(readScalar(err.p, st));
+            std::swap(listA, listB);
+        }
     }
 
     mark("balancing");
