@@ -426,18 +426,20 @@ __global__ void k_gather(const T *__restrict__ src, const u32 *__restrict__ orde
 }
 
 // dual cells: every interior corner of every leaf, packed (x | y << 21 | z << 42) at MAX_DEPTH
-__global__ void k_cornerCodes(const u64 *__restrict__ leafLoc, u32 leafCount, u64 *__restrict__ codes) {
+// leaf corners as vertex codes on the grid of the tree's deepest level D:
+// x | y << D | z << 2D (3D bits), boundary corners ~0 (bit 3D set: they sort last)
+__global__ void k_cornerCodes(const u64 *__restrict__ leafLoc, u32 leafCount, u32 D, u64 *__restrict__ codes) {
     const u32 l = blockIdx.x * BLOCK + threadIdx.x;
     if (l >= leafCount) return;
     const u64 loc = leafLoc[l];
     u32 k[3];
     locCoords(loc, k[0], k[1], k[2]);
-    const u32 sh = MAX_DEPTH - locDepth(loc);
-    const u32 lim = 1u << MAX_DEPTH;
+    const u32 sh = D - locDepth(loc);
+    const u32 lim = 1u << D;
     for (u32 c = 0; c < 8; ++c) {
         const u32 x = (k[0] + (c & 1)) << sh, y = (k[1] + ((c >> 1) & 1)) << sh, z = (k[2] + ((c >> 2) & 1)) << sh;
         const bool interior = x > 0 && x < lim && y > 0 && y < lim && z > 0 && z < lim;
-        codes[8 * (size_t)l + c] = interior ? ((u64)x | ((u64)y << 21) | ((u64)z << 42)) : ~0ull;
+        codes[8 * (size_t)l + c] = interior ? ((u64)x | ((u64)y << D) | ((u64)z << (2 * D))) : ~0ull;
     }
 }
 
@@ -530,21 +532,26 @@ __global__ void k_checkCells(const u32 *__restrict__ cells, size_t n, u32 *err) 
     if (j < n && cells[j] == INVALID) atomicOr(err, (u32)ERR_NOT_INSERTED);
 }
 
+// keyHi / keyLo: the path key halves shifted right by hiShift / loShift (bits below
+// that no level of this tree sets), so the sorts start at bit 0
 __global__ void k_cells(HashView inner, HashView nodes, const u32 *__restrict__ nodeToLeaf,
-                        const u64 *__restrict__ vcodes, u32 count, u32 *__restrict__ cellsUnsorted,
-                        u64 *__restrict__ keyHi, u64 *__restrict__ keyLo, u32 *__restrict__ idx) {
+                        const u64 *__restrict__ vcodes, u32 count, u32 D, u32 hiShift, u32 loShift,
+                        u32 *__restrict__ cellsUnsorted, u64 *__restrict__ keyHi, u64 *__restrict__ keyLo,
+                        u32 *__restrict__ idx) {
     const u32 c = blockIdx.x * BLOCK + threadIdx.x;
     if (c >= count) return;
     const u64 code = vcodes[c];
-    const u32 v[3] = {(u32)(code & 0x1fffff), (u32)((code >> 21) & 0x1fffff), (u32)((code >> 42) & 0x1fffff)};
+    const u64 m = (1ull << D) - 1ull;
+    const u32 up = MAX_DEPTH - D;   // vertex coordinates on the MAX_DEPTH grid
+    const u32 v[3] = {(u32)(code & m) << up, (u32)((code >> D) & m) << up, (u32)((code >> (2 * D)) & m) << up};
     for (u32 s = 0; s < 8; ++s) {
         const u32 x = v[0] - 1 + (s & 1), y = v[1] - 1 + ((s >> 1) & 1), z = v[2] - 1 + ((s >> 2) & 1);
         cellsUnsorted[8 * (size_t)c + s] = leafOfCell(inner, nodes, nodeToLeaf, x, y, z);
     }
     u64 h, l;
     swKey(v[0], v[1], v[2], h, l);
-    keyHi[c] = h;
-    keyLo[c] = l;
+    keyHi[c] = h >> hiShift;
+    keyLo[c] = l >> loShift;
     idx[c] = c;
 }
 
@@ -1527,8 +1534,9 @@ void buildTree(tsr_context *ctx, const tsr_scene *scene, const tsr_params &prm, 
     if (L > 0) {
         const size_t corners = 8 * (size_t)L;
         DBuf<u64> codes(corners), codesSorted(corners), uniq(corners);
-        k_cornerCodes<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(tree->leafLoc.p, L, codes.p);
-        sortKeys(sc, codes.p, codesSorted.p, corners, 0, 64, st);
+        const u32 D = std::max<u32>(tree->maxDepth, 1);
+        k_cornerCodes<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(tree->leafLoc.p, L, D, codes.p);
+        sortKeys(sc, codes.p, codesSorted.p, corners, 0, (int)(3 * D + 1), st);
         uniqueKeys(sc, codesSorted.p, uniq.p, counter.p, corners, st);
         u32 V = readScalar(counter.p, st);
         if (V > 0) {
@@ -1540,12 +1548,21 @@ void buildTree(tsr_context *ctx, const tsr_scene *scene, const tsr_params &prm, 
         if (V > 0) {
             DBuf<u32> cellsUnsorted(8 * (size_t)V), idx(V), idxA(V), idxB(V);
             DBuf<u64> kh(V), kl(V), khS(V), klS(V);
+            // lexicographic (hi, lo): stable sort by lo, then by hi.  A vertex on the grid of
+            // depth D has a path of at most D levels of 5 bits: levels 0-11 at hi bits
+            // 4 + 5 (11 - lvl), levels 12-18 at lo bits 4 + 5 (18 - lvl); the key halves are
+            // shifted down past the bits no level < D sets, and the sorts cover the rest.
+            const u32 hiShift = D > 12 ? 4u : 4u + 5u * (12u - D), loShift = D > 12 ? 4u + 5u * (19u - D) : 39u;
             k_cells<<<gridFor(V, BLOCK), BLOCK, 0, st>>>(inner.view(), tree->nodeHash.view(), tree->nodeToLeaf.p,
-                                                         uniq.p, V, cellsUnsorted.p, kh.p, kl.p, idx.p);
-            // lexicographic (hi, lo): stable sort by lo, then by hi
-            sortPairs(sc, kl.p, klS.p, idx.p, idxA.p, V, 0, 64, st);
-            k_gather<u64><<<gridFor(V, BLOCK), BLOCK, 0, st>>>(kh.p, idxA.p, V, 1, khS.p);
-            sortPairs(sc, khS.p, kh.p, idxA.p, idxB.p, V, 0, 64, st);
+                                                         uniq.p, V, D, hiShift, loShift, cellsUnsorted.p, kh.p,
+                                                         kl.p, idx.p);
+            if (D > 12) {
+                sortPairs(sc, kl.p, klS.p, idx.p, idxA.p, V, 0, (int)(39 - loShift), st);
+                k_gather<u64><<<gridFor(V, BLOCK), BLOCK, 0, st>>>(kh.p, idxA.p, V, 1, khS.p);
+                sortPairs(sc, khS.p, kh.p, idxA.p, idxB.p, V, 0, (int)(64 - hiShift), st);
+            } else {   // lo is zero
+                sortPairs(sc, kh.p, khS.p, idx.p, idxB.p, V, 0, (int)(64 - hiShift), st);
+            }
             tree->cells.alloc(8 * (size_t)V);
             k_gather<u32><<<gridFor(V, BLOCK), BLOCK, 0, st>>>(cellsUnsorted.p, idxB.p, V, 8, tree->cells.p);
             k_checkCells<<<gridFor(8 * (size_t)V, BLOCK), BLOCK, 0, st>>>(tree->cells.p, 8 * (size_t)V, err.p);
