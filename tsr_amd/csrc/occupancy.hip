@@ -62,14 +62,10 @@ inline u32 coneChunkLinks() {   // read at every occupancy computation
     return v > 0 ? (u32)v : (1u << 24);
 }
 
-// Staging layout of mode 3: each group of STAGE_GROUP consecutive threads owns one
-// contiguous region of slots * STAGE_GROUP records, record j of a thread at row j of
-// its region: lanes writing their j-th record hit one row (coalesced) and a block's
-// writes stay within a few pages.
-constexpr u32 STAGE_GROUP = 256;
-__device__ __forceinline__ size_t stagingIndex(u32 idx, u32 j, u32 slots) {
-    return ((size_t)(idx / STAGE_GROUP) * slots + j) * STAGE_GROUP + (idx % STAGE_GROUP);
-}
+// Staging layout of mode 3: the records of link position idx are contiguous, record j
+// at idx * slots + j, so that the gather into link order reads and writes whole runs.
+constexpr u32 STAGE_GROUP = 256;   // positions per gather block (the staging area is padded to it)
+__device__ __forceinline__ size_t stagingIndex(u32 idx, u32 j, u32 slots) { return (size_t)idx * slots + j; }
 
 // Traversal copies of the node arrays store node i at index i + NODE_PAD: the 8
 // children of the r-th inner node (ids 1 + 8r .. 8 + 8r) then start on a 32-byte
@@ -553,22 +549,47 @@ __global__ __launch_bounds__(64) void k_coneLinkWave(TraversalTree t, LinkInputs
 }
 
 // staged records of the links that fit their slots -> link order (offsets = exclusive
-// scan of the counts); a link's records stay in traversal order.  Thread i moves the
-// records staged by thread i (stagingIndex; coalesced reads).
-__global__ void k_gatherSlots(const u32 *__restrict__ counts, const u32 *__restrict__ offsets,
-                              const u32 *__restrict__ order, u32 base, u32 links, u32 slots,
-                              const u32 *__restrict__ stLeaf,
-                              const uint2 *__restrict__ stVal, u32 *__restrict__ recLeaf,
-                              uint2 *__restrict__ recVal) {
-    const u32 i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= links) return;
-    const u32 l = order ? order[i] : base + i;
-    const u32 c = counts[l];
-    if (c > slots) return;
-    const size_t dst = offsets[l];
-    for (u32 j = 0; j < c; ++j) {
-        recLeaf[dst + j] = stLeaf[stagingIndex(i, j, slots)];
-        recVal[dst + j] = stVal[stagingIndex(i, j, slots)];
+// scan of the counts); a link's records stay in traversal order.  A block of STAGE_GROUP
+// threads moves the records of STAGE_GROUP consecutive positions together: after a
+// block-wide scan of their counts, thread t copies records t, t + STAGE_GROUP, ... of the
+// block's concatenated runs, so consecutive threads read and write consecutive records.
+__global__ __launch_bounds__(STAGE_GROUP) void k_gatherSlots(const u32 *__restrict__ counts,
+                                                             const u32 *__restrict__ offsets,
+                                                             const u32 *__restrict__ order, u32 base, u32 links,
+                                                             u32 slots, const u32 *__restrict__ stLeaf,
+                                                             const uint2 *__restrict__ stVal, u32 *__restrict__ recLeaf,
+                                                             uint2 *__restrict__ recVal) {
+    __shared__ u32 pre[STAGE_GROUP + 1], dst[STAGE_GROUP];
+    const u32 t = threadIdx.x, i = blockIdx.x * STAGE_GROUP + t;
+    u32 c = 0;
+    if (i < links) {
+        const u32 l = order ? order[i] : base + i;
+        c = counts[l];
+        if (c > slots) c = 0;   // re-traversed into place
+        dst[t] = offsets[l];
+    }
+    pre[t + 1] = c;
+    if (t == 0) pre[0] = 0;
+    __syncthreads();
+    for (u32 w = 1; w < STAGE_GROUP; w <<= 1) {   // inclusive scan of pre[1..]
+        const u32 v = t + 1 > w ? pre[t + 1 - w] : 0u;
+        __syncthreads();
+        pre[t + 1] += v;
+        __syncthreads();
+    }
+    const u32 total = pre[STAGE_GROUP];
+    const size_t src0 = (size_t)blockIdx.x * STAGE_GROUP * slots;
+    for (u32 r = t; r < total; r += STAGE_GROUP) {
+        u32 lo = 0, hi = STAGE_GROUP;   // position s with pre[s] <= r < pre[s + 1]
+        while (hi - lo > 1) {
+            const u32 mid = (lo + hi) >> 1;
+            if (pre[mid] <= r) lo = mid;
+            else hi = mid;
+        }
+        const u32 j = r - pre[lo];
+        const size_t src = src0 + (size_t)lo * slots + j;
+        recLeaf[dst[lo] + j] = stLeaf[src];
+        recVal[dst[lo] + j] = stVal[src];
     }
 }
 
