@@ -599,16 +599,19 @@ __global__ void k_overflowFlags(const u32 *__restrict__ counts, u32 links, u32 s
 }
 
 // first record of every leaf segment in the sorted record stream
-__global__ void k_segmentStarts(const u32 *__restrict__ leafSorted, size_t n, u32 *__restrict__ segStart) {
+// first and one-past-last record of each leaf's run in the sorted records
+__global__ void k_segmentStarts(const u32 *__restrict__ leafSorted, size_t n, u32 *__restrict__ segStart,
+                                u32 *__restrict__ segEnd) {
     const size_t j = (size_t)blockIdx.x * BLOCK + threadIdx.x;
     if (j >= n) return;
     const u32 l = leafSorted[j];
     if (j == 0 || leafSorted[j - 1] != l) segStart[l] = (u32)j;
+    if (j + 1 == n || leafSorted[j + 1] != l) segEnd[l] = (u32)(j + 1);
 }
 
 // Kahan sums of each leaf's records in record order (deterministic)
-__global__ void k_segmentReduce(const u32 *__restrict__ leafSorted, const uint2 *__restrict__ valSorted, size_t n,
-                                const u32 *__restrict__ segStart, u32 leafCount, float *__restrict__ ksum,
+__global__ void k_segmentReduce(const uint2 *__restrict__ valSorted, const u32 *__restrict__ segStart,
+                                const u32 *__restrict__ segEnd, u32 leafCount, float *__restrict__ ksum,
                                 float *__restrict__ clen, u32 *__restrict__ hitCount, float *__restrict__ kcomp,
                                 float *__restrict__ lcomp) {
     const u32 l = blockIdx.x * BLOCK + threadIdx.x;
@@ -617,7 +620,8 @@ __global__ void k_segmentReduce(const u32 *__restrict__ leafSorted, const uint2 
     float sk = 0.f, ck = 0.f, sl = 0.f, cl = 0.f;
     u32 cnt = 0;
     if (s != INVALID) {
-        for (size_t j = s; j < n && leafSorted[j] == l; ++j, ++cnt) {
+        const u32 e = segEnd[l];
+        for (u32 j = s; j < e; ++j, ++cnt) {
             const uint2 v = valSorted[j];
             const float a = __uint_as_float(v.x) - ck;
             const float ta = sk + a;
@@ -1119,7 +1123,7 @@ void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *s
     DBuf<uint8_t> cullMask;
     const bool atomicMode = (flags & 0xff) == TSR_ACCUM_FP32_ATOMIC;
 
-    DBuf<u32> counts(links), offsets(links), hitCount(L), segStart(L);
+    DBuf<u32> counts(links), offsets(links), hitCount(L), segStart(L), segEnd(L);
     DBuf<double> confs(links);
     DBuf<unsigned long long> kst(4);
     auto readStats = [&](int n, unsigned long long *out) {
@@ -1326,13 +1330,14 @@ void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *s
                 sortPairs(sc, recLeaf, recLeafS, recVal, recValS, total, 0, bits, st);
                 lap("sort by leaf");
                 k_fillU<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(segStart.p, L, INVALID);
-                if (total) k_segmentStarts<<<gridFor(total, BLOCK), BLOCK, 0, st>>>(recLeafS, total, segStart.p);
+                if (total)
+                    k_segmentStarts<<<gridFor(total, BLOCK), BLOCK, 0, st>>>(recLeafS, total, segStart.p, segEnd.p);
                 if (!merging) {
-                    k_segmentReduce<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(recLeafS, recValS, total, segStart.p, L,
+                    k_segmentReduce<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(recValS, segStart.p, segEnd.p, L,
                                                                          ksum, clen, hitCount.p, nullptr, nullptr);
                 } else {
                     float *pk = part.p, *pl = part.p + L, *pkc = part.p + 2 * (size_t)L, *plc = part.p + 3 * (size_t)L;
-                    k_segmentReduce<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(recLeafS, recValS, total, segStart.p, L,
+                    k_segmentReduce<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(recValS, segStart.p, segEnd.p, L,
                                                                          pk, pl, partHits.p, pkc, plc);
                     k_mergeChunk<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(L, pk, pkc, pl, plc, partHits.p, acc.p,
                                                                       acc.p + L, acc.p + 2 * (size_t)L,
