@@ -36,7 +36,11 @@ namespace {
 
 constexpr u32 BLOCK = 256;
 constexpr u32 VISIT_CAP = 48;   // visited-leaf records per sampling unit before it counts as an event
-constexpr double HOT_RADIUS = 3.0;   // transferred part of the visit index, in max sampling distances
+// transferred part of the visit index, in max sampling distances: a refinement subdivides
+// only nodes that meet its unit's segment [p, p +- n msd], so the T0 leaves holding them lie
+// within msd of p (5% more covers float rounding of p, n and msd); leaves outside are
+// looked up exactly on the device when a cascade reaches them
+constexpr double HOT_RADIUS = 1.05;
 
 // error bits written by kernels
 enum : u32 { ERR_OUTSIDE = 1, ERR_TOO_DEEP = 2, ERR_HASH_FULL = 4, ERR_NOT_INSERTED = 8 };
@@ -597,31 +601,36 @@ struct FlatSet {
     }
 };
 
-// open-addressing u64 -> u32 map (key 0 = empty slot), host side
+// open-addressing u64 -> u32 map (key 0 = empty slot), host side.  Key and value share a
+// 16-byte slot, so a lookup that hits costs one cache line.
 struct FlatMap {
-    std::vector<u64> keys;
-    std::vector<u32> vals;
+    struct Slot {
+        u64 key;
+        u32 val;
+        u32 pad;
+    };
+    std::vector<Slot> slots;
     u64 mask = 0;
     void init(size_t expected) {
         size_t cap = 1024;
         while (cap < 2 * expected + 16) cap <<= 1;
-        keys.assign(cap, 0ull);
-        vals.assign(cap, 0u);
+        slots.assign(cap, Slot{0ull, 0u, 0u});
         mask = cap - 1;
     }
+    size_t capacity() const { return slots.size(); }
     const u32 *find(u64 k) const {
         for (u64 s = hashMix(k) & mask;; s = (s + 1) & mask) {
-            const u64 v = keys[s];
-            if (v == k) return &vals[s];
+            const u64 v = slots[s].key;
+            if (v == k) return &slots[s].val;
             if (v == 0ull) return nullptr;
         }
     }
     void put(u64 k, u32 val) {
         for (u64 s = hashMix(k) & mask;; s = (s + 1) & mask) {
-            const u64 v = keys[s];
+            const u64 v = slots[s].key;
             if (v == k || v == 0ull) {
-                keys[s] = k;
-                vals[s] = val;
+                slots[s].key = k;
+                slots[s].val = val;
                 return;
             }
         }
@@ -631,8 +640,8 @@ struct FlatMap {
     u64 putIfAbsentConcurrent(u64 k, u32 val) {
         for (u64 s = hashMix(k) & mask;; s = (s + 1) & mask) {
             u64 expected = 0ull;
-            if (__atomic_compare_exchange_n(&keys[s], &expected, k, false, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) {
-                vals[s] = val;
+            if (__atomic_compare_exchange_n(&slots[s].key, &expected, k, false, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) {
+                slots[s].val = val;
                 return s;
             }
             if (expected == k) return s;
@@ -716,7 +725,7 @@ struct InnerSet {
             if (cacheKey == parent) cacheKey = 0;
         }
     }
-    size_t capacity() const { return map.keys.size(); }
+    size_t capacity() const { return map.capacity(); }
     // all keys of a prefix-closed set at once (several threads)
     void build(const std::vector<u64> &keys) {
         parallelFor(keys.size(), [&](size_t b, size_t e) {
@@ -727,7 +736,7 @@ struct InnerSet {
                 const u64 k = keys[i];
                 if (k == 1ull) continue;
                 const u64 slot = map.putIfAbsentConcurrent(k >> 3, 0u);   // the parent is present
-                __atomic_fetch_or(&map.vals[slot], 1u << (k & 7), __ATOMIC_RELAXED);
+                __atomic_fetch_or(&map.slots[slot].val, 1u << (k & 7), __ATOMIC_RELAXED);
             }
         });
         count = keys.size();
@@ -761,7 +770,7 @@ struct ReplaySets {
 ReplaySets buildReplaySets(const std::vector<u64> &t0Inner) {
     ReplaySets r;
     r.t0.init(t0Inner.size());
-    r.inner.init(t0Inner.size() * 2 + 1024);
+    r.inner.init(t0Inner.size() + 1024);   // grown by the replay when needed
     parallelFor(t0Inner.size(), [&](size_t b, size_t e) {
         for (size_t i = b; i < e; ++i) r.t0.insertConcurrent(t0Inner[i]);
     });
@@ -777,9 +786,22 @@ void replaySampling(const Frame &f, const tsr_scene *scene, float maxRel, Replay
     InnerSet &inner = sets.inner;
     const u32 units = (u32)isEvent.size();
     std::vector<uint8_t> pending(isEvent.begin(), isEvent.end());
-    std::priority_queue<u32, std::vector<u32>, std::greater<u32>> heap;
-    for (u32 u = 0; u < units; ++u)
-        if (pending[u]) heap.push(u);
+    // the initial events, ascending: listed per chunk in parallel, then heapified at once
+    std::vector<u32> initial;
+    {
+        constexpr size_t CHUNK = 1 << 20;
+        const size_t chunks = (units + CHUNK - 1) / CHUNK;
+        std::vector<std::vector<u32>> part(chunks);
+        parallelFor(chunks, [&](size_t b, size_t e) {
+            for (size_t c = b; c < e; ++c) {
+                const u32 u0 = (u32)(c * CHUNK), u1 = (u32)std::min<size_t>(units, (c + 1) * CHUNK);
+                for (u32 u = u0; u < u1; ++u)
+                    if (pending[u]) part[c].push_back(u);
+            }
+        }, 1);
+        for (auto &v : part) initial.insert(initial.end(), v.begin(), v.end());
+    }
+    std::priority_queue<u32, std::vector<u32>, std::greater<u32>> heap(std::greater<u32>(), std::move(initial));
     const size_t initialEvents = heap.size();
     u64 tainted = 0, creating = 0;
     auto isInner = [&](u64 loc) { return inner.contains(loc); };
@@ -856,8 +878,8 @@ void replaySampling(const Frame &f, const tsr_scene *scene, float maxRel, Replay
                 if (inner.capacity() < 2 * (inner.count + 16)) {   // keep the load factor <= 1/2
                     InnerSet grown;
                     grown.init(2 * inner.count + 1024);
-                    for (size_t s2 = 0; s2 < inner.map.keys.size(); ++s2)
-                        if (inner.map.keys[s2]) grown.map.put(inner.map.keys[s2], inner.map.vals[s2]);
+                    for (const FlatMap::Slot &sl : inner.map.slots)
+                        if (sl.key) grown.map.put(sl.key, sl.val);
                     grown.count = inner.count;
                     inner = std::move(grown);
                 }
