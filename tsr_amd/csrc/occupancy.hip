@@ -808,15 +808,21 @@ __device__ __forceinline__ void priorsGroup(PRIORS_GROUP_PARAMS) {
     __shared__ u32 sp, abandoned;
     const u32 lane = threadIdx.x;
     u32 nPops = 0, nTests = 0, nWhole = 0;
-    // FP32 pre-test of the FP64 sphere predicates.  Box bounds are formed in float
-    // (rootMinF + k * hF: two roundings, <= 2 * 2^-24 * M off the FP64 bound, M bounding
-    // every |coordinate|), query centres are rounded to float (<= 2^-24 * M) and their
-    // difference is rounded once more (<= 2 * 2^-24 * M): e = 5 * 2^-24 * M per axis.  The
-    // squared distance near radius r is then off by less than 4 e r + 4 e^2 + 2^-20 r^2.
-    // Outside that margin the FP32 result decides, inside it the exact FP64 test does.
+    // FP32 pre-test of the FP64 sphere predicates.  M bounds every |coordinate| (and the
+    // root side); u = 2^-24 is the float rounding unit.  A child cube of side h is taken as
+    // centre c and half side h/2 in float: rootMinF and rootSizeF are within u M of their
+    // FP64 values in total (the odd multiple (2k+1) of the half side is exact), and
+    // c = fma((2k+1), h/2, rootMinF) rounds once (u M), so c is within 2 u M of the FP64
+    // centre.  The query centre rounds to float (u M).  Near a decision every offset is
+    // below r + h/2 <= M: the centre distance rounds once (u M), the nearest / farthest
+    // offsets t -+ h/2 round once more (u M), and the half side itself is within u M.
+    // Per axis the offsets are thus within e = 6 u M of the FP64 ones, and the squared
+    // distance near radius r within 2 sqrt(3) e r + 3 e^2 + 3 u r^2 (three fused
+    // multiply-adds), below the margin 4 e r + 4 e^2 + 2^-20 r^2.  Outside that margin
+    // the FP32 result decides, inside it the exact FP64 test does.
     double M = 0.0;
     for (int a = 0; a < 3; ++a) M = fmax(M, fabs(f.rootMin[a]) + f.rootSize);
-    const double eps = 5.0 * ldexp(M, -24);
+    const double eps = 6.0 * ldexp(M, -24);
     // Groups are taken dynamically from a counter per XCD slice (blocks are dispatched
     // round-robin over the 8 XCDs, so block b runs on XCD b mod 8): each XCD's L2 sees
     // one contiguous, spatially coherent range, and waves with cheap groups take more.
@@ -904,29 +910,29 @@ __device__ __forceinline__ void priorsGroup(PRIORS_GROUP_PARAMS) {
                 k[2] = 2u * (u32)(pk >> 42) + (slot >> 2);
                 if (cn.y & FLAG_S) {
                     ++nTests;
-                    const float hF = ldexpf(f.rootSizeF, -(int)d);
-                    float loF[3], hiF[3];
-                    for (int a = 0; a < 3; ++a) {
-                        loF[a] = f.rootMinF[a] + (float)k[a] * hF;
-                        hiF[a] = f.rootMinF[a] + (float)(k[a] + 1) * hF;
-                    }
+                    // the child cube as centre and half side, in float
+                    const float halfF = ldexpf(f.rootSizeF, -(int)d - 1);
+                    float cenF[3];
+                    for (int a = 0; a < 3; ++a)
+                        cenF[a] = __builtin_fmaf((float)(2u * k[a] + 1u), halfF, f.rootMinF[a]);
                     const bool leaf = (cn.x & LEAF_BIT) != 0;
                     const u32 mld = cn.y >> 8;
                     bool summed = false;
 #pragma unroll
                     for (u32 j = 0; j < GROUP_Q; ++j) {
                         if (!((mask >> j) & 1u)) continue;
+                        // an entry carries query j only if its children are within j's depth limit
                         const float4 A = qA[j], B = qB[j];
                         const u32 md = __float_as_uint(B.w);
-                        if (d > md) continue;
                         const float cq[3] = {A.x, A.y, A.z};
-                        // fused multiply-adds: one rounding each, inside the error bound above
+                        // per axis: distance of the centres t, nearest offset t - half (or 0) and
+                        // farthest offset t + half; fused multiply-adds, one rounding each
                         float dminF = 0.f, dmaxF = 0.f;
                         for (int q3 = 0; q3 < 3; ++q3) {
-                            const float a0 = loF[q3] - cq[q3], a1 = cq[q3] - hiF[q3];
-                            const float o = fmaxf(0.f, fmaxf(a0, a1));
+                            const float t = fabsf(cq[q3] - cenF[q3]);
+                            const float o = fmaxf(t - halfF, 0.f);
                             dminF = __builtin_fmaf(o, o, dminF);
-                            const float far = fmaxf(fabsf(a0), fabsf(a1));
+                            const float far = t + halfF;
                             dmaxF = __builtin_fmaf(far, far, dmaxF);
                         }
                         if (dminF > A.w) continue;      // surely outside
