@@ -902,6 +902,8 @@ __device__ __forceinline__ void priorsGroup(PRIORS_GROUP_PARAMS) {
                 const u32 md16 = stMD[base + e];
                 cn = links[cid];
                 const float4 cs = sums[cid];
+                // both loads in flight together: keep the sums load ahead of the state test
+                asm volatile("" ::"v"(cs.x), "v"(cs.y), "v"(cs.z), "v"(cs.w));
                 const u32 mask = md16 & 0xffu;
                 d = (md16 >> 8) + 1;
                 // child coordinates: slot bit 0 -> x, bit 1 -> y, bit 2 -> z (location code order)
