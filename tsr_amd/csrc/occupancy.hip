@@ -768,8 +768,10 @@ __global__ void k_packPriorLinks(const u32 *__restrict__ child, const u32 *__res
 
 // The priors traversal keeps an LDS stack of inner nodes still to open.  The first
 // pass uses a short stack; a group that would overflow it is abandoned and listed,
-// and a second pass redoes the listed groups with a stack of the proven bound.
-constexpr u32 PRIOR_STACK_SHORT = 512;
+// and a second pass redoes the listed groups with a stack of the proven bound.  The
+// short stack (14 B per entry) leaves room in the 160 KB of LDS per CU for 6 waves per
+// SIMD of one-wave blocks.
+constexpr u32 PRIOR_STACK_SHORT = 384;
 constexpr u32 XCDS = 8;   // MI355X: blocks are dispatched round-robin over 8 XCDs, each with its own L2
 constexpr u32 PRIOR_STACK = 1344;   // >= 64 * (MAX_DEPTH + 1): one step pushes <= 56 more than it pops
 
@@ -1014,9 +1016,11 @@ __device__ __forceinline__ void priorsGroup(PRIORS_GROUP_PARAMS) {
     }
 }
 
-// register use capped for 5 waves per SIMD (a few bytes of spill; faster than 4 waves)
+// register use capped for 6 waves per SIMD: the step's memory round trips are hidden
+// better than with 5 (96 VGPRs, no spill), in spite of ~50 bytes of spill; 7 and 8 waves
+// spill more and run slower
 template <u32 GROUP_Q, u32 STACK>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_priorsGroup(PRIORS_GROUP_PARAMS) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_priorsGroup(PRIORS_GROUP_PARAMS) {
     priorsGroup<GROUP_Q, STACK>(PRIORS_GROUP_ARGS);
 }
 
