@@ -224,10 +224,11 @@ __device__ void waveAccumulate(unsigned long long *acc, u32 v) {
 }
 
 // mode 0: count accepted hits per link; 1: write records at offsets[link]; 2: FP32 atomics
-// (precision study); 3: count, and write the first in.slots records into the thread's
-// staging slots (interleaved: record j at j * linkCount + thread index); 4: hit count per leaf
+// (precision study); 3: count, and write the first in.slots records into the link's
+// staging slots (stagingIndex: contiguous per link); 4: hit count per leaf
 // stats[0] += node visits, stats[1] += accepted hits
-__global__ void k_conePass(TraversalTree t, LinkInputs in, u32 linkCount, int sampleness, int mode,
+// Register use capped for 6 waves per SIMD (80 VGPRs, no spill).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_conePass(TraversalTree t, LinkInputs in, u32 linkCount, int sampleness, int mode,
                            u32 *__restrict__ counts, const u32 *__restrict__ offsets, u32 *__restrict__ recLeaf,
                            uint2 *__restrict__ recVal, float *__restrict__ atomicSum, float *__restrict__ atomicLen,
                            u32 *__restrict__ atomicStates, double *__restrict__ confOut,
