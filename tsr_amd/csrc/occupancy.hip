@@ -811,21 +811,22 @@ __device__ __forceinline__ void priorsGroup(PRIORS_GROUP_PARAMS) {
     __shared__ u32 sp, abandoned;
     const u32 lane = threadIdx.x;
     u32 nPops = 0, nTests = 0, nWhole = 0;
-    // FP32 pre-test of the FP64 sphere predicates.  M bounds every |coordinate| (and the
-    // root side); u = 2^-24 is the float rounding unit.  A child cube of side h is taken as
-    // centre c and half side h/2 in float: rootMinF and rootSizeF are within u M of their
-    // FP64 values in total (the odd multiple (2k+1) of the half side is exact), and
-    // c = fma((2k+1), h/2, rootMinF) rounds once (u M), so c is within 2 u M of the FP64
-    // centre.  The query centre rounds to float (u M).  Near a decision every offset is
-    // below r + h/2 <= M: the centre distance rounds once (u M), the nearest / farthest
-    // offsets t -+ h/2 round once more (u M), and the half side itself is within u M.
-    // Per axis the offsets are thus within e = 6 u M of the FP64 ones, and the squared
-    // distance near radius r within 2 sqrt(3) e r + 3 e^2 + 3 u r^2 (three fused
-    // multiply-adds), below the margin 4 e r + 4 e^2 + 2^-20 r^2.  Outside that margin
-    // the FP32 result decides, inside it the exact FP64 test does.
+    // FP32 pre-test of the FP64 sphere predicates, in coordinates relative to the root
+    // corner (|coordinate| <= S, the root side; u = 2^-24 is the float rounding unit; M
+    // bounds every absolute FP64 coordinate, whose own roundings are far below u S).  A
+    // child cube of side h is taken as centre c = (2k+1) * h/2 and half side h/2 in float:
+    // the half side is within u h/2 of its FP64 value, so (2k+1) h/2 (the odd factor is
+    // exact) is within u S, and the product rounds once (u S).  The query centre rounds
+    // once (u S).  Near a decision the centre distance t is below r + h/2 and rounds once
+    // (u (r + h/2)), the nearest / farthest offsets t -+ h/2 once more (u r) and the half
+    // side is off by u h/2.  With h <= S/2, the offsets per axis are within
+    // e = u (3.5 S + 2 r) (+ 2^-50 M for the FP64 inputs) of the FP64 ones, and the
+    // squared distance near radius r within 2 sqrt(3) e r + 3 e^2 + 5 u r^2 (three fused
+    // multiply-adds, thresholds rounded to float), below the margin
+    // 4 e r + 4 e^2 + 2^-20 r^2.  Outside that margin the FP32 result decides, inside it
+    // the exact FP64 test does.
     double M = 0.0;
     for (int a = 0; a < 3; ++a) M = fmax(M, fabs(f.rootMin[a]) + f.rootSize);
-    const double eps = 6.0 * ldexp(M, -24);
     // Groups are taken dynamically from a counter per XCD slice (blocks are dispatched
     // round-robin over the 8 XCDs, so block b runs on XCD b mod 8): each XCD's L2 sees
     // one contiguous, spatially coherent range, and waves with cheap groups take more.
@@ -861,8 +862,10 @@ __device__ __forceinline__ void priorsGroup(PRIORS_GROUP_PARAMS) {
             qri2[lane] = radius * radius * (1.0 - 1e-9);
             // decision thresholds in float, formed once per query: r^2 +- margin, inner r^2 +- margin
             const float r2F = (float)qr2[lane], ri2F = (float)qri2[lane];
+            const double eps = ldexp(3.5 * f.rootSize + 2.0 * radius, -24) + ldexp(M, -50);
             const float mg = (float)(4.0 * eps * radius + 4.0 * eps * eps + ldexp(radius * radius, -20));
-            qA[lane] = make_float4((float)qc[lane][0], (float)qc[lane][1], (float)qc[lane][2], r2F + mg);
+            qA[lane] = make_float4((float)(qc[lane][0] - f.rootMin[0]), (float)(qc[lane][1] - f.rootMin[1]),
+                                   (float)(qc[lane][2] - f.rootMin[2]), r2F + mg);
             qB[lane] = make_float4(r2F - mg, ri2F - mg, ri2F + mg, __uint_as_float(dq + MAX_DEPTH_DIFFERENCE));
         }
         double accE[GROUP_Q], accS[GROUP_Q];
@@ -915,11 +918,10 @@ __device__ __forceinline__ void priorsGroup(PRIORS_GROUP_PARAMS) {
                 k[2] = 2u * (u32)(pk >> 42) + (slot >> 2);
                 if (cn.y & FLAG_S) {
                     ++nTests;
-                    // the child cube as centre and half side, in float
+                    // the child cube as centre and half side, in float, relative to the root corner
                     const float halfF = ldexpf(f.rootSizeF, -(int)d - 1);
                     float cenF[3];
-                    for (int a = 0; a < 3; ++a)
-                        cenF[a] = __builtin_fmaf((float)(2u * k[a] + 1u), halfF, f.rootMinF[a]);
+                    for (int a = 0; a < 3; ++a) cenF[a] = (float)(2u * k[a] + 1u) * halfF;
                     const bool leaf = (cn.x & LEAF_BIT) != 0;
                     const u32 mld = cn.y >> 8;
                     bool summed = false;
