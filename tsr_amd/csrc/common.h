@@ -264,18 +264,22 @@ TSR_HD void cullInit(ConeCull &k, const ConeD &c) {
     k.slope = c.R < A ? (float)(c.R / sqrt(A * A - c.R * c.R)) : -1.0f;
 }
 
-// true if the cube [lo, lo + size]^3 cannot meet the cone
-TSR_HD bool cullSeparated(const ConeCull &k, float lx, float ly, float lz, float size, float absMargin) {
+// true if the cube [lo, lo + size]^3 cannot meet the cone.  The square roots and the
+// quotient use the hardware approximations (v_sqrt_f32, v_rcp_f32, about 1 ulp): like the
+// FP32 rounding of the inputs, their error is orders of magnitude below the absolute
+// margin (1e-5 of the root side), so the test stays conservative.
+__device__ __forceinline__ bool cullSeparated(const ConeCull &k, float lx, float ly, float lz, float size,
+                                              float absMargin) {
     const float half = 0.5f * size;
     const float b0 = lx + half - k.a[0], b1 = ly + half - k.a[1], b2 = lz + half - k.a[2];
     const float u = b0 * k.vhat[0] + b1 * k.vhat[1] + b2 * k.vhat[2];
     const float p0 = b0 - u * k.vhat[0], p1 = b1 - u * k.vhat[1], p2 = b2 - u * k.vhat[2];
     const float w2 = p0 * p0 + p1 * p1 + p2 * p2;
-    const float w = sqrtf(w2);
-    float ts = k.slope < 0.0f ? 1.0f : (u + k.slope * w) / k.A;
+    const float w = __builtin_amdgcn_sqrtf(w2);
+    float ts = k.slope < 0.0f ? 1.0f : (u + k.slope * w) * __builtin_amdgcn_rcpf(k.A);
     ts = fminf(1.0f, fmaxf(k.tStart, ts));
     const float du = u - ts * k.A;
-    const float f = sqrtf(du * du + w2) - k.R * ts;
+    const float f = __builtin_amdgcn_sqrtf(du * du + w2) - k.R * ts;
     const float rho = half * 1.7320508f * 1.01f + absMargin;
     return f > rho;
 }

@@ -1289,14 +1289,17 @@ void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *s
                     k.units += n;
                 }
                 lap("traversal (mode 3)");
-                exclusiveScan(sc, counts.p + l0, offsets.p + l0, n, st);   // chunk-relative offsets
-                const u64 total = (u64)readScalar(offsets.p + l0 + n - 1, st) + readScalar(counts.p + l0 + n - 1, st);
-                if (total >= CONE_MAX_RECORDS) {
+                // cs[1] is the range's record total in 64 bits: a range over the cap is split
+                // before the 32-bit offsets could wrap
+                if (cs[1] >= CONE_MAX_RECORDS) {
                     if (n < 2) throw tsr_error(TSR_ERR_OUT_OF_MEMORY, "too many kernel contributions for one link");
                     if (!merging) startMerging();
                     rangeLinks = n / 2;
                     continue;
                 }
+                exclusiveScan(sc, counts.p + l0, offsets.p + l0, n, st);   // chunk-relative offsets
+                const u64 total = (u64)readScalar(offsets.p + l0 + n - 1, st) + readScalar(counts.p + l0 + n - 1, st);
+                if (total != cs[1]) throw tsr_error(TSR_ERR_HIP, "cone pass record counts disagree with the hit total");
                 totalHits += total;
                 u32 *recLeaf = ar.recLeaf.ensure(total), *recLeafS = ar.recLeafS.ensure(total);
                 uint2 *recVal = ar.recVal.ensure(total), *recValS = ar.recValS.ensure(total);
