@@ -201,20 +201,11 @@ TSR_HD void coneInit(ConeD &c, const double apex[3], const double endNormal[3], 
     }
 }
 
-// Occupancy.cpp:429-439 computeRayKernel (negative parabola)
-TSR_HD double rayKernel(double z, double Z) {
-    if (z < 0.0 || z >= Z) return 0.0;
-    const double z2 = z * z, Z2 = Z * Z;
-    return (3.0 / (2.0 * Z)) * (-z2 / Z2 + 1.0);
-}
-// Occupancy.cpp:392-400 computeCircularDiscKernelFromSquared (negative parabola)
-TSR_HD double discKernel(double r2, double R) {
-    const double R2 = R * R;
-    if (r2 >= R2) return 0.0;
-    const double notNormed = 1.0 - r2 / R2;
-    return (2.0 / (PI_D * R2)) * notNormed;
-}
-// Occupancy.cpp:279-346 addKernel: scaled kernel at x, or 0 when rejected.
+// Occupancy.cpp:279-346 addKernel: scaled kernel at x, or 0 when rejected.  The kernel is
+// the product of the ray kernel (Occupancy.cpp:429-439 computeRayKernel, a negative
+// parabola 3/(2Z) (1 - z^2/Z^2) on [0, Z)) and the disc kernel (Occupancy.cpp:392-400
+// computeCircularDiscKernelFromSquared, 2/(pi R^2) (1 - r^2/R^2) inside R), formed with
+// the two reciprocals 1/Z and 1/R^2 instead of four quotients (a few ulp of FP64 apart).
 TSR_HD double coneKernel(const ConeD &c, const double x[3], bool sampleness, double confidence) {
     const double d0 = x[0] - c.a[0], d1 = x[1] - c.a[1], d2 = x[2] - c.a[2];
     const double tc = (d0 * c.n[0] + d1 * c.n[1] + d2 * c.n[2]) * c.invDenom;
@@ -224,7 +215,8 @@ TSR_HD double coneKernel(const ConeD &c, const double x[3], bool sampleness, dou
                  q2 = x[2] - (c.a[2] + c.v[2] * tc);
     const double r2 = q0 * q0 + q1 * q1 + q2 * q2;
     const double pcdR = c.R * tc;
-    if (r2 >= pcdR * pcdR) return 0.0;
+    const double R2 = pcdR * pcdR;
+    if (r2 >= R2) return 0.0;
     double Z, z, norm;
     if (!sampleness) {
         Z = c.length;
@@ -236,7 +228,12 @@ TSR_HD double coneKernel(const ConeD &c, const double x[3], bool sampleness, dou
         z = tc2 * Z;
         norm = 0.5;
     }
-    const double k = rayKernel(z, Z) * norm * discKernel(r2, pcdR);
+    if (z < 0.0 || z >= Z) return 0.0;
+    const double invZ = 1.0 / Z, invR2 = 1.0 / R2;
+    const double zq = z * invZ;
+    const double ray = 1.5 * invZ * (1.0 - zq * zq);
+    const double disc = (2.0 / PI_D) * invR2 * (1.0 - r2 * invR2);
+    const double k = ray * norm * disc;
     if (EPS_D > k) return 0.0;
     const double sk = confidence * k;
     if (EPS_D > sk) return 0.0;

@@ -176,7 +176,7 @@ __device__ void traverseCone(const TraversalTree &t, const ConeCull &cull, u32 m
                     for (int a = 0; a < 3; ++a) x[a] = t.f.B(a, 2ull * k[a] + 1, d + 1);
                     onLeaf(c & ~LEAF_BIT, x);
                 } else if (c != INVALID) {
-                    const float h = (float)t.f.h(d);
+                    const float h = ldexpf(t.f.rootSizeF, -(int)d);
                     const float lo0 = t.f.rootMinF[0] + (float)k[0] * h, lo1 = t.f.rootMinF[1] + (float)k[1] * h,
                                 lo2 = t.f.rootMinF[2] + (float)k[2] * h;
                     if (!cullSeparated(cull, lo0, lo1, lo2, h, t.margin)) {
@@ -394,7 +394,7 @@ __global__ __launch_bounds__(256) void k_conePassPT(TraversalTree t, LinkInputs 
                         ++count;
                     }
                 } else if (c != INVALID) {
-                    const float h = (float)t.f.h(d);
+                    const float h = ldexpf(t.f.rootSizeF, -(int)d);
                     const float lo0 = t.f.rootMinF[0] + (float)k0 * h, lo1 = t.f.rootMinF[1] + (float)k1 * h,
                                 lo2 = t.f.rootMinF[2] + (float)k2 * h;
                     if (!cullSeparated(cull, lo0, lo1, lo2, h, t.margin)) {
@@ -494,7 +494,7 @@ __global__ __launch_bounds__(64) void k_coneLinkWave(TraversalTree t, LinkInputs
             u32 k[3];
             locCoords(loc, k[0], k[1], k[2]);
             const u32 d = locDepth(loc);
-            const float h = (float)t.f.h(d);
+            const float h = ldexpf(t.f.rootSizeF, -(int)d);
             return !cullSeparated(cull, t.f.rootMinF[0] + (float)k[0] * h, t.f.rootMinF[1] + (float)k[1] * h,
                                   t.f.rootMinF[2] + (float)k[2] * h, h, t.margin);
         };
