@@ -76,8 +76,9 @@ struct Frame {
     float rootMinF[3];
     float rootSizeF;
     TSR_HD double h(u32 d) const { return ldexp(rootSize, -(int)d); }
-    TSR_HD double B(int a, u64 k, u32 d) const { return rootMin[a] + (double)k * h(d); }
-    TSR_HD double centre(int a, u64 k, u32 d) const { return B(a, 2 * k + 1, d + 1); }
+    // k < 2^21 (grid coordinates at depth <= MAX_DEPTH + 1): a 32-bit integer, one conversion
+    TSR_HD double B(int a, u32 k, u32 d) const { return rootMin[a] + (double)k * h(d); }
+    TSR_HD double centre(int a, u32 k, u32 d) const { return B(a, 2 * k + 1, d + 1); }
 };
 
 // Samples::getDistanceToPlane (Samples.cpp:451-455): dot(x - p, n)

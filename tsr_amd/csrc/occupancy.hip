@@ -1021,9 +1021,10 @@ __device__ __forceinline__ void priorsGroup(PRIORS_GROUP_PARAMS) {
 
 // register use capped for 6 waves per SIMD: the step's memory round trips are hidden
 // better than with 5 (96 VGPRs, no spill), in spite of ~50 bytes of spill; 7 and 8 waves
-// spill more and run slower
+// spill more and run slower.  The long-stack instantiation is held to few waves by its LDS
+// stack anyway and is not capped.
 template <u32 GROUP_Q, u32 STACK>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_priorsGroup(PRIORS_GROUP_PARAMS) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(STACK == PRIOR_STACK_SHORT ? 6 : 1))) void k_priorsGroup(PRIORS_GROUP_PARAMS) {
     priorsGroup<GROUP_Q, STACK>(PRIORS_GROUP_ARGS);
 }
 
