@@ -321,7 +321,10 @@ __global__ __launch_bounds__(256) void k_conePassPT(TraversalTree t, LinkInputs 
     u32 visits = 0, hits = 0;
     for (;;) {
         const u64 idle = __ballot(!active && more);
-        if (idle) {   // refill the idle lanes with consecutive link indices of the current slice
+        // refill the idle lanes with consecutive link indices of the current slice, once at
+        // least 4 are idle (or none is active): a refill costs the wavefront the cone set-up
+        // and a copy of the lanes' traversal state, so fewer, larger refills pay
+        if (idle && (__popcll(idle) >= 4 || !__ballot(active))) {
             const int leader = __ffsll((long long)idle) - 1;
             u32 base = 0;
             if ((int)lane == leader) base = atomicAdd(&nextIdx[cur], (u32)__popcll(idle));
