@@ -762,12 +762,18 @@ __global__ void k_priorQueryList(const u32 *__restrict__ flag, const u32 *__rest
 // Priors traversal records, in two node-padded arrays (node i at index i + NODE_PAD)
 // so that one popped entry's 8 children are read as one aligned 64-byte run of
 // (child link, state | deepest leaf depth << 8) and one aligned 128-byte run of
-// subtree sums (E hi, E lo, S hi, S lo).
+// subtree sums (E, S) in FP64.
 
+// Also turns each (E hi, E lo, S hi, S lo) float4 of subtree sums, in place, into the FP64
+// pair ((double)E hi + (double)E lo, (double)S hi + (double)S lo), once per node: the query
+// traversal adds these doubles per summed child instead of converting per (child, query).
 __global__ void k_packPriorLinks(const u32 *__restrict__ child, const u32 *__restrict__ aux, u32 nodeCount,
-                                 uint2 *__restrict__ links) {
+                                 uint2 *__restrict__ links, float4 *__restrict__ sums) {
     const u32 nd = blockIdx.x * BLOCK + threadIdx.x;
-    if (nd < nodeCount) links[nd] = make_uint2(child[nd], aux[nd]);
+    if (nd >= nodeCount) return;
+    links[nd] = make_uint2(child[nd], aux[nd]);
+    const float4 v = sums[nd];
+    reinterpret_cast<double2 *>(sums)[nd] = make_double2((double)v.x + (double)v.y, (double)v.z + (double)v.w);
 }
 
 // The priors traversal keeps an LDS stack of inner nodes still to open.  The first
@@ -793,7 +799,7 @@ constexpr u32 PRIOR_STACK = 1344;   // >= 64 * (MAX_DEPTH + 1): one step pushes 
 // reference (behind an FP32 pre-test with a proven margin); sums are FP64 per lane
 // and query, reduced across the wave in a fixed order (deterministic).
 #define PRIORS_GROUP_PARAMS                                                                                \
-    Frame f, const uint2 *__restrict__ links, const float4 *__restrict__ sums, const u64 *__restrict__ leafLoc, \
+    Frame f, const uint2 *__restrict__ links, const double2 *__restrict__ sums, const u64 *__restrict__ leafLoc, \
         const u32 *__restrict__ queries, u32 queryCount, float relRadius, double *__restrict__ prior, u32 *err,  \
         unsigned long long *__restrict__ stats, int xcdMap, u32 *__restrict__ nextGroup,                        \
         const u32 *__restrict__ groupList, const u32 *__restrict__ groupListCount,                              \
@@ -887,11 +893,11 @@ __device__ __forceinline__ void priorsGroup(PRIORS_GROUP_PARAMS) {
             }
         }
         if ((root.y & FLAG_S) && (root.x & LEAF_BIT) && lane == 0) {   // every query sums the root
-            const float4 rs = sums[0];
+            const double2 rs = sums[0];
 #pragma unroll
             for (u32 j = 0; j < GROUP_Q; ++j) {
-                accE[j] = (double)rs.x + (double)rs.y;
-                accS[j] = (double)rs.z + (double)rs.w;
+                accE[j] = rs.x;
+                accS[j] = rs.y;
             }
         }
         __syncthreads();
@@ -910,9 +916,9 @@ __device__ __forceinline__ void priorsGroup(PRIORS_GROUP_PARAMS) {
                 const u64 pk = stK[base + e];
                 const u32 md16 = stMD[base + e];
                 cn = links[cid];
-                const float4 cs = sums[cid];
+                const double2 cs = sums[cid];
                 // both loads in flight together: keep the sums load ahead of the state test
-                asm volatile("" ::"v"(cs.x), "v"(cs.y), "v"(cs.z), "v"(cs.w));
+                asm volatile("" ::"v"(cs.x), "v"(cs.y));
                 const u32 mask = md16 & 0xffu;
                 d = (md16 >> 8) + 1;
                 // child coordinates: slot bit 0 -> x, bit 1 -> y, bit 2 -> z (location code order)
@@ -971,8 +977,8 @@ __device__ __forceinline__ void priorsGroup(PRIORS_GROUP_PARAMS) {
                             }
                         }
                         if (whole) {
-                            accE[j] += (double)cs.x + (double)cs.y;
-                            accS[j] += (double)cs.z + (double)cs.w;
+                            accE[j] += cs.x;
+                            accS[j] += cs.y;
                             summed = true;
                         } else if (d < md) {   // children at depth d + 1 <= md still count
                             pushMask |= (uint8_t)(1u << j);
@@ -1415,7 +1421,8 @@ void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *s
         exclusiveScan(sc, qflag.p, qpos.p, L, st);
         k_priorQueryList<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(qflag.p, qpos.p, L, queries.p);
         const u32 nq = readScalar(qpos.p + L - 1, st) + readScalar(qflag.p + L - 1, st);
-        k_packPriorLinks<<<gridFor(N, BLOCK), BLOCK, 0, st>>>(tree->child.p, aux.p, N, plinks);
+        k_packPriorLinks<<<gridFor(N, BLOCK), BLOCK, 0, st>>>(tree->child.p, aux.p, N, plinks, sums);
+        const double2 *sumsD = reinterpret_cast<const double2 *>(sums);
         field->priorQueries = nq;
         if (nq) {
             constexpr u32 Q = 8;
@@ -1428,11 +1435,11 @@ void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *s
             const int xcdMap = (xcdMapMode() >> 1) & 1;
             timedLaunch(ctx, KSTAT_PRIORS, [&] {
                 k_priorsGroup<Q, PRIOR_STACK_SHORT><<<grid, 64, 0, st>>>(
-                    tree->frame, plinks, sums, tree->leafLoc.p, queries.p, nq, prm.relative_radius_for_priors,
+                    tree->frame, plinks, sumsD, tree->leafLoc.p, queries.p, nq, prm.relative_radius_for_priors,
                     field->prior.p, perr.p, kst.p, xcdMap, counters.p, nullptr, nullptr, overflowList.p,
                     counters.p + 9);
                 k_priorsGroup<Q, PRIOR_STACK><<<1024, 64, 0, st>>>(
-                    tree->frame, plinks, sums, tree->leafLoc.p, queries.p, nq, prm.relative_radius_for_priors,
+                    tree->frame, plinks, sumsD, tree->leafLoc.p, queries.p, nq, prm.relative_radius_for_priors,
                     field->prior.p, perr.p, kst.p, 0, counters.p + 8, overflowList.p, counters.p + 9, nullptr,
                     nullptr);
             });
