@@ -216,6 +216,11 @@ __device__ void coneLink(const TraversalTree &t, const LinkInputs &in, u32 link,
                          u32 *__restrict__ atomicStates, double *__restrict__ confOut, u32 &visits, u32 &count,
                          u32 *path);
 
+// set bits of the wave-wide mask m below this lane (v_mbcnt: no per-lane mask register)
+__device__ __forceinline__ u32 lanesBelow(u64 m) {
+    return __builtin_amdgcn_mbcnt_hi((u32)(m >> 32), __builtin_amdgcn_mbcnt_lo((u32)m, 0u));
+}
+
 // wave sum of a per-lane count, added once per wave to *acc
 __device__ void waveAccumulate(unsigned long long *acc, u32 v) {
     unsigned long long x = v;
@@ -330,14 +335,14 @@ __global__ __launch_bounds__(256) void k_conePassPT(TraversalTree t, LinkInputs 
             if ((int)lane == leader) base = atomicAdd(&nextIdx[cur], (u32)__popcll(idle));
             base = __shfl(base, leader);
             const u32 sBegin = min(linkCount, cur * sliceLen), sEnd = min(linkCount, sBegin + sliceLen);
-            if (!__ballot(!active && more && sBegin + base + (u32)__popcll(idle & ((1ull << lane) - 1ull)) < sEnd)) {
+            if (!__ballot(!active && more && sBegin + base + lanesBelow(idle) < sEnd)) {
                 // this slice is used up: move on; done after the last one
                 cur = cur + 1 == slices ? 0u : cur + 1;
                 if (++tried == slices) more = false;
                 continue;
             }
             if (!active && more) {
-                idx = sBegin + base + (u32)__popcll(idle & ((1ull << lane) - 1ull));
+                idx = sBegin + base + lanesBelow(idle);
                 if (idx >= sEnd) {
                     // past the slice end: stays idle and refills from the next slice
                 } else {
@@ -536,7 +541,7 @@ __global__ __launch_bounds__(64) void k_coneLinkWave(TraversalTree t, LinkInputs
             }
             __syncthreads();   // every lane has read its entry before the slots are reused
             const u64 mask = __ballot(push);
-            const u32 pos = (u32)__popcll(mask & ((1ull << lane) - 1ull));
+            const u32 pos = lanesBelow(mask);
             const u32 newTop = b + (u32)__popcll(mask);
             if (newTop > CONE_WAVE_STACK) {
                 if (lane == 0) { atomicOr(err, 1u); sp = 0; }
@@ -989,7 +994,7 @@ __device__ __forceinline__ void priorsGroup(PRIORS_GROUP_PARAMS) {
             }
             __syncthreads();   // every lane has read its entry before the slots are reused
             const u64 pm = __ballot(pushMask != 0);
-            const u32 pos = (u32)__popcll(pm & ((1ull << lane) - 1ull));
+            const u32 pos = lanesBelow(pm);
             const u32 newTop = base + (u32)__popcll(pm);
             if (newTop > STACK) {
                 if (lane == 0) {
