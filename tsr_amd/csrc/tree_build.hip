@@ -516,16 +516,32 @@ TSR_HD void swKey(u32 vx, u32 vy, u32 vz, u64 &hi, u64 &lo) {
     }
 }
 
-__device__ u32 leafOfCell(HashView inner, HashView nodes, const u32 *__restrict__ nodeToLeaf, u32 cx, u32 cy, u32 cz) {
+// The leaf holding finest-grid cell (cx, cy, cz).  `guess` is the inner depth found for a
+// neighbouring cell (-1: none): the depth of the deepest inner ancestor is first tested at
+// guess and guess +- 1 (2:1 balanced neighbours), then found by bisection; the result is
+// the same either way and is returned in `guess`.
+__device__ u32 leafOfCell(HashView inner, HashView nodes, const u32 *__restrict__ nodeToLeaf, u32 cx, u32 cy, u32 cz,
+                          int &guess) {
     const u64 finest = makeLoc(cx, cy, cz, MAX_DEPTH);
+    auto innerAt = [&](int m) { return m < (int)MAX_DEPTH && inner.contains(finest >> (3 * (MAX_DEPTH - m))); };
     // largest m with the depth-m ancestor inner (monotone in m)
     int lo = 0, hi = (int)MAX_DEPTH;   // inner(lo) true (root), inner(hi) false
-    if (!inner.contains(1ull)) return nodeToLeaf[0];
+    if (guess < 0) {
+        if (!inner.contains(1ull)) return nodeToLeaf[0];
+    } else if (innerAt(guess)) {
+        lo = guess;
+        if (!innerAt(guess + 1)) hi = guess + 1;
+        else lo = guess + 1;
+    } else {
+        hi = guess;
+        if (guess > 0 && innerAt(guess - 1)) lo = guess - 1;
+    }
     while (hi - lo > 1) {
         const int mid = (lo + hi) / 2;
         if (inner.contains(finest >> (3 * (MAX_DEPTH - mid)))) lo = mid;
         else hi = mid;
     }
+    guess = lo;
     const u64 leafLoc = finest >> (3 * (MAX_DEPTH - (lo + 1)));
     const u32 id = nodes.find(leafLoc);
     return id == INVALID ? INVALID : nodeToLeaf[id];
@@ -548,9 +564,10 @@ __global__ void k_cells(HashView inner, HashView nodes, const u32 *__restrict__ 
     const u64 m = (1ull << D) - 1ull;
     const u32 up = MAX_DEPTH - D;   // vertex coordinates on the MAX_DEPTH grid
     const u32 v[3] = {(u32)(code & m) << up, (u32)((code >> D) & m) << up, (u32)((code >> (2 * D)) & m) << up};
+    int guess = -1;
     for (u32 s = 0; s < 8; ++s) {
         const u32 x = v[0] - 1 + (s & 1), y = v[1] - 1 + ((s >> 1) & 1), z = v[2] - 1 + ((s >> 2) & 1);
-        cellsUnsorted[8 * (size_t)c + s] = leafOfCell(inner, nodes, nodeToLeaf, x, y, z);
+        cellsUnsorted[8 * (size_t)c + s] = leafOfCell(inner, nodes, nodeToLeaf, x, y, z, guess);
     }
     u64 h, l;
     swKey(v[0], v[1], v[2], h, l);
