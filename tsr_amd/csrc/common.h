@@ -104,21 +104,27 @@ TSR_HD bool fittingCentre(const Frame &f, u32 kx, u32 ky, u32 kz, u32 d, bool po
 
 // Nodes.cpp:490-517 intersect: a corner on the wanted side of the sample plane
 // and box/sphere overlap (squared distance <= msd^2).
+// The corner plane distances are planeDistance's sums ((x - px) nx + (y - py) ny) + (z - pz) nz
+// with the six per-axis products formed once (the same operations in the same order).
 TSR_HD bool intersectNode(const Frame &f, u32 kx, u32 ky, u32 kz, u32 d, bool positive, const SamplingUnit &u) {
+    const u32 k[3] = {kx, ky, kz};
+    double lo[3], hi[3], t[3][2];
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = f.B(a, k[a], d);
+        hi[a] = f.B(a, k[a] + 1, d);
+        t[a][0] = (lo[a] - u.p[a]) * u.n[a];
+        t[a][1] = (hi[a] - u.p[a]) * u.n[a];
+    }
     bool anyRight = false;
     for (u32 c = 0; c < 8 && !anyRight; ++c) {
-        const double x = f.B(0, (u64)kx + (c & 1), d), y = f.B(1, (u64)ky + ((c >> 1) & 1), d),
-                     z = f.B(2, (u64)kz + ((c >> 2) & 1), d);
-        const bool inPositive = planeDistance(x, y, z, u.p, u.n) > 0.0;
+        const bool inPositive = (t[0][c & 1] + t[1][(c >> 1) & 1]) + t[2][c >> 2] > 0.0;
         if (inPositive == positive) anyRight = true;
     }
     if (!anyRight) return false;
-    const u32 k[3] = {kx, ky, kz};
     double d2 = 0.0;
     for (int a = 0; a < 3; ++a) {
-        const double lo = f.B(a, k[a], d), hi = f.B(a, (u64)k[a] + 1, d);
-        if (u.p[a] < lo) d2 += (lo - u.p[a]) * (lo - u.p[a]);
-        else if (u.p[a] > hi) d2 += (u.p[a] - hi) * (u.p[a] - hi);
+        if (u.p[a] < lo[a]) d2 += (lo[a] - u.p[a]) * (lo[a] - u.p[a]);
+        else if (u.p[a] > hi[a]) d2 += (u.p[a] - hi[a]) * (u.p[a] - hi[a]);
     }
     return d2 <= u.msd * u.msd;
 }

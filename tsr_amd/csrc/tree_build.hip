@@ -123,14 +123,27 @@ __global__ void k_insertAncestors(HashView h, const u64 *__restrict__ containerL
 template <typename IsInner, typename OnLeaf>
 TSR_HD bool findReliable(const Frame &f, u64 start, bool positive, const SamplingUnit &u, IsInner isInner,
                          OnLeaf onLeaf) {
+    // the leaf's centre fits, and so does one of its 8 children's (fittingCentre at depth
+    // d + 1, with the two centre offsets per axis, their squares and plane terms formed once:
+    // the same operations in the same order)
     auto reliableLeaf = [&](u64 loc) {
-        u32 kx, ky, kz;
-        locCoords(loc, kx, ky, kz);
+        u32 k[3];
+        locCoords(loc, k[0], k[1], k[2]);
         const u32 d = locDepth(loc);
-        if (!fittingCentre(f, kx, ky, kz, d, positive, u)) return false;
-        for (u32 c = 0; c < 8; ++c)
-            if (fittingCentre(f, 2 * kx + (c & 1), 2 * ky + ((c >> 1) & 1), 2 * kz + ((c >> 2) & 1), d + 1, positive, u))
-                return true;
+        if (!fittingCentre(f, k[0], k[1], k[2], d, positive, u)) return false;
+        double sq[3][2], pl[3][2];
+        for (int a = 0; a < 3; ++a)
+            for (u32 b = 0; b < 2; ++b) {
+                const double dx = f.B(a, 2 * (2 * k[a] + b) + 1, d + 2) - u.p[a];
+                sq[a][b] = dx * dx;
+                pl[a][b] = dx * u.n[a];
+            }
+        const double msd2 = u.msd * u.msd;
+        for (u32 c = 0; c < 8; ++c) {
+            const u32 bx = c & 1, by = (c >> 1) & 1, bz = c >> 2;
+            if ((sq[0][bx] + sq[1][by]) + sq[2][bz] > msd2) continue;
+            if (((pl[0][bx] + pl[1][by]) + pl[2][bz] > 0.0) == positive) return true;
+        }
         return false;
     };
     if (!isInner(start)) {
