@@ -1367,8 +1367,8 @@ void buildTree(tsr_context *ctx, const tsr_scene *scene, const tsr_params &prm, 
         if (hotPairs) sortPairs(sc, hotLeaf.p, hotLeafSorted.p, hotUnit.p, hotUnitSorted.p, hotPairs, 0,
                                 (int)(3 * maxContainerDepth + 1), st);
     }
-    size_t pairs = 0;   // all visits of settled units (statistics)
-    {
+    size_t pairs = 0;   // all visits of settled units (statistics, traced runs only)
+    if (trace) {
         size_t tmp = 0;
         DBuf<u32> tot(1);
         HIPCHECK(hipcub::DeviceReduce::Sum(nullptr, tmp, visitCount.p, tot.p, (int)units, st));
