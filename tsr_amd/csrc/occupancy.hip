@@ -944,6 +944,8 @@ __device__ __forceinline__ void priorsGroup(PRIORS_GROUP_PARAMS) {
                         if (!((mask >> j) & 1u)) continue;
                         // an entry carries query j only if its children are within j's depth limit
                         const float4 A = qA[j], B = qB[j];
+                        // both LDS reads in flight together (one wait, not one per branch)
+                        asm volatile("" ::"v"(A.x), "v"(B.x), "v"(B.y), "v"(B.z), "v"(B.w));
                         const u32 md = __float_as_uint(B.w);
                         const float cq[3] = {A.x, A.y, A.z};
                         // per axis: distance of the centres t, nearest offset t - half (or 0) and
