@@ -7,7 +7,7 @@ oracle (|occ - occ64| / (pS + pE) of the oracle, max and p99), the number of
 sign flips, and the symmetric Hausdorff distance between the GPU mesh and the
 oracle mesh as a fraction of the oracle mesh's bounding-box diagonal.
 
-    python -m tests.precision_study --res 0.25 --out gpurun_out/precision.json
+    python -m tests.precision_study --res 0.25 --out precision.json
 
 Test infrastructure: it runs the oracle, so it lives with the tests.
 """

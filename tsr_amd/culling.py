@@ -4,8 +4,8 @@ The culler has no counterpart in the reference: it predicts, per octree node,
 whether the node's subtree can hold part of the surface (a leaf with positive
 occupancy or a corner of a dual cell whose corner signs differ).  Nodes it
 rejects skip emptiness-cone evaluation and priors, and their leaves become
-empty (-EPSILON), so it may only reject far-empty space; the plain field stays
-the judge of every mesh it changes.
+empty (-EPSILON), so it may only reject far-empty space; every mesh it changes
+is measured against the plain field's mesh.
 
 Training data come from plain fields of seeded synthetic scenes (config 3:
 unions of 3-8 spheres, boxes and thin sheets, 32 cameras, 320x240):
