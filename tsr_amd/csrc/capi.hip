@@ -71,6 +71,7 @@ tsr_status tsr_context_create(int device, tsr_context **out) {
         HIPCHECK(hipSetDevice(device));
         tsr_context *c = new tsr_context();
         c->device = device;
+        c->pool = std::make_shared<tsr::DevicePool>(device);
         HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         for (hipEvent_t &e : c->ev) HIPCHECK(hipEventCreate(&e));
         for (hipEvent_t &e : c->kev) HIPCHECK(hipEventCreate(&e));
@@ -84,6 +85,8 @@ void tsr_context_destroy(tsr_context *ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     ctx->scratch.buf.release();
     ctx->cone.release();
+    if (ctx->pool) ctx->pool->trim();   // blocks still held by live handles are freed by their owners
+    ctx->pool.reset();
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
     for (hipEvent_t &e : ctx->ev) (void)hipEventDestroy(e);
     for (hipEvent_t &e : ctx->kev) (void)hipEventDestroy(e);
@@ -203,6 +206,7 @@ tsr_status tsr_tree_build(tsr_context *ctx, const tsr_scene *scene, const tsr_pa
     if (!(prm.max_relative_sampling_distance > 0.0f)) return fail(TSR_ERR_INVALID_ARGUMENT, "max_relative_sampling_distance must be > 0");
     return guarded([&] {
         activate(ctx);
+        tsr::PoolScope scope(ctx->pool);
         tsr_tree *t = new tsr_tree();
         t->ctx = ctx;
         t->device = ctx->device;
@@ -311,6 +315,7 @@ tsr_status tsr_occupancy_compute(tsr_context *ctx, const tsr_tree *tree, const t
     if (params) prm = *params;
     return guarded([&] {
         activate(ctx);
+        tsr::PoolScope scope(ctx->pool);
         tsr_field *f = new tsr_field();
         f->ctx = ctx;
         f->device = ctx->device;
@@ -380,6 +385,7 @@ tsr_status tsr_dmc_extract(tsr_context *ctx, const tsr_tree *tree, const tsr_fie
     if (field->leafCount != tree->leafCount) return fail(TSR_ERR_INVALID_ARGUMENT, "field does not belong to the tree");
     return guarded([&] {
         activate(ctx);
+        tsr::PoolScope scope(ctx->pool);
         tsr_mesh *m = new tsr_mesh();
         m->ctx = ctx;
         m->device = ctx->device;

@@ -7,6 +7,9 @@
 #include <algorithm>
 #include <hipcub/hipcub.hpp>
 
+#include <map>
+#include <memory>
+#include <mutex>
 #include <string>
 #include <utility>
 
@@ -23,30 +26,146 @@ inline void hipCheck(hipError_t e, const char *what) {
 }
 #define HIPCHECK(x) ::tsr::hipCheck((x), #x)
 
-// Owning device buffer (hipMallocAsync-free: plain hipMalloc / hipFree).
+// Device blocks kept for reuse by one context.  hipFree waits for the whole device and
+// hipMalloc of large blocks maps memory, so the stages' many temporaries would stall the
+// stream at every scope exit; instead a block released while its context's entry point
+// runs goes back to the context's cache and the next allocation of a similar size takes it.
+// All of a context's device work is ordered on its one stream, so a block handed out again
+// is only touched by work queued after the work that used it.  Blocks released outside such
+// a scope (handle destructors, other contexts) are freed as before.
+class DevicePool {
+  public:
+    explicit DevicePool(int device) : device_(device) {}
+    DevicePool(const DevicePool &) = delete;
+    DevicePool &operator=(const DevicePool &) = delete;
+    ~DevicePool() { trim(); }
+    // block size for a request: 256-byte multiples below 2 MiB, 2 MiB multiples above
+    static size_t roundUp(size_t bytes) {
+        const size_t g = bytes >= (2u << 20) ? (2u << 20) : 256u;
+        return (bytes + g - 1) / g * g;
+    }
+    // a cached block of at least `bytes` and at most 25% (+2 MiB) larger, or null
+    void *take(size_t bytes, size_t &blockBytes) {
+        std::lock_guard<std::mutex> g(m_);
+        auto it = cached_.lower_bound(bytes);
+        if (it == cached_.end() || it->first > bytes + bytes / 4 + (2u << 20)) return nullptr;
+        void *q = it->second;
+        blockBytes = it->first;
+        cachedBytes_ -= it->first;
+        cached_.erase(it);
+        return q;
+    }
+    // keep the block, or free it when the cache would grow past its cap
+    void give(void *q, size_t blockBytes) {
+        {
+            std::lock_guard<std::mutex> g(m_);
+            if (cachedBytes_ + blockBytes <= kCapBytes) {
+                cached_.insert({blockBytes, q});
+                cachedBytes_ += blockBytes;
+                return;
+            }
+        }
+        (void)hipFree(q);
+    }
+    // free every cached block (device `device_` made current, then restored)
+    void trim() {
+        std::lock_guard<std::mutex> g(m_);
+        if (cached_.empty()) return;
+        int prev = 0;
+        (void)hipGetDevice(&prev);
+        (void)hipSetDevice(device_);
+        for (auto &kv : cached_) (void)hipFree(kv.second);
+        cached_.clear();
+        cachedBytes_ = 0;
+        (void)hipSetDevice(prev);
+    }
+
+  private:
+    static constexpr size_t kCapBytes = size_t(64) << 30;
+    int device_;
+    std::mutex m_;
+    std::multimap<size_t, void *> cached_;
+    size_t cachedBytes_ = 0;
+};
+
+// the pool of the context whose entry point runs on this thread (null outside one)
+inline std::shared_ptr<DevicePool> &currentPool() {
+    thread_local std::shared_ptr<DevicePool> pool;
+    return pool;
+}
+struct PoolScope {
+    std::shared_ptr<DevicePool> prev;
+    explicit PoolScope(std::shared_ptr<DevicePool> pool) : prev(currentPool()) { currentPool() = std::move(pool); }
+    ~PoolScope() { currentPool() = std::move(prev); }
+    PoolScope(const PoolScope &) = delete;
+    PoolScope &operator=(const PoolScope &) = delete;
+};
+
+// Owning device buffer: hipMalloc / hipFree, through the current context's DevicePool
+// inside an entry point.
 template <typename T>
 struct DBuf {
     T *p = nullptr;
     size_t n = 0;
+    size_t blockBytes = 0;
+    std::shared_ptr<DevicePool> owner;   // pool the block came from (null: plain hipMalloc)
     DBuf() = default;
     explicit DBuf(size_t count) { alloc(count); }
     DBuf(const DBuf &) = delete;
     DBuf &operator=(const DBuf &) = delete;
-    DBuf(DBuf &&o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DBuf(DBuf &&o) noexcept : p(o.p), n(o.n), blockBytes(o.blockBytes), owner(std::move(o.owner)) {
+        o.p = nullptr;
+        o.n = 0;
+        o.blockBytes = 0;
+    }
     DBuf &operator=(DBuf &&o) noexcept {
-        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        if (this != &o) {
+            release();
+            p = o.p;
+            n = o.n;
+            blockBytes = o.blockBytes;
+            owner = std::move(o.owner);
+            o.p = nullptr;
+            o.n = 0;
+            o.blockBytes = 0;
+        }
         return *this;
     }
     ~DBuf() { release(); }
     void alloc(size_t count) {
         release();
         n = count;
-        if (count) HIPCHECK(hipMalloc((void **)&p, sizeof(T) * count));
+        if (!count) return;
+        const std::shared_ptr<DevicePool> &pool = currentPool();
+        if (!pool) {
+            HIPCHECK(hipMalloc((void **)&p, sizeof(T) * count));
+            blockBytes = sizeof(T) * count;
+            return;
+        }
+        const size_t want = DevicePool::roundUp(sizeof(T) * count);
+        p = (T *)pool->take(want, blockBytes);
+        if (!p) {
+            hipError_t e = hipMalloc((void **)&p, want);
+            if (e == hipErrorOutOfMemory) {   // give the cached blocks back and try once more
+                (void)hipGetLastError();
+                pool->trim();
+                e = hipMalloc((void **)&p, want);
+            }
+            if (e != hipSuccess) p = nullptr;
+            HIPCHECK(e);
+            blockBytes = want;
+        }
+        owner = pool;
     }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) {
+            if (owner && owner == currentPool()) owner->give(p, blockBytes);
+            else (void)hipFree(p);
+        }
         p = nullptr;
         n = 0;
+        blockBytes = 0;
+        owner.reset();
     }
     // grow-only: keeps the buffer when it already holds `count` elements
     T *ensure(size_t count) {

@@ -35,6 +35,7 @@ struct tsr_context {
     hipEvent_t kev[2] = {};   // start/end of an instrumented kernel
     float phaseMs[6] = {0, 0, 0, 0, 0, 0};
     tsr_kernel_stat kstat[8];
+    std::shared_ptr<tsr::DevicePool> pool;   // device blocks reused across the stages' temporaries
     tsr::Scratch scratch;
     tsr::ConeArena cone;
     // pinned host staging for large device->host transfers (grow-only, freed with the context)
