@@ -87,7 +87,8 @@ void tsr_context_destroy(tsr_context *ctx) {
     ctx->cone.release();
     if (ctx->pool) ctx->pool->trim();   // blocks still held by live handles are freed by their owners
     ctx->pool.reset();
-    if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    for (void *p : ctx->pinned)
+        if (p) (void)hipHostFree(p);
     for (hipEvent_t &e : ctx->ev) (void)hipEventDestroy(e);
     for (hipEvent_t &e : ctx->kev) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->stream);

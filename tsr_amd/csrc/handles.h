@@ -38,9 +38,11 @@ struct tsr_context {
     std::shared_ptr<tsr::DevicePool> pool;   // device blocks reused across the stages' temporaries
     tsr::Scratch scratch;
     tsr::ConeArena cone;
-    // pinned host staging for large device->host transfers (grow-only, freed with the context)
-    void *pinned = nullptr;
-    size_t pinnedBytes = 0;
+    // pinned host staging for large device->host transfers, one grow-only buffer per use
+    // (slot 0: the sampling visit index, slot 1: the sampling event flags, slot 2: the
+    // containment tree's inner codes), freed with the context
+    void *pinned[3] = {nullptr, nullptr, nullptr};
+    size_t pinnedBytes[3] = {0, 0, 0};
 };
 
 struct tsr_scene {
@@ -112,8 +114,9 @@ struct tsr_mesh {
 };
 
 namespace tsr {
-// pinned host memory of the context, at least `bytes` (contents not preserved on growth)
-void *pinnedHost(tsr_context *ctx, size_t bytes);
+// pinned host memory of the context's staging slot, at least `bytes` (contents not
+// preserved on growth)
+void *pinnedHost(tsr_context *ctx, size_t bytes, int slot = 0);
 // runs fn(begin, end) over [0, n) on up to 16 host threads (OMP_NUM_THREADS caps it),
 // at least `grain` items per thread
 void parallelFor(size_t n, const std::function<void(size_t, size_t)> &fn, size_t grain = 65536);

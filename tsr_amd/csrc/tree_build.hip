@@ -757,11 +757,11 @@ struct InnerSet {
     }
     size_t capacity() const { return map.capacity(); }
     // all keys of a prefix-closed set at once (several threads)
-    void build(const std::vector<u64> &keys) {
-        parallelFor(keys.size(), [&](size_t b, size_t e) {
+    void build(const u64 *keys, size_t n) {
+        parallelFor(n, [&](size_t b, size_t e) {
             for (size_t i = b; i < e; ++i) map.putIfAbsentConcurrent(keys[i], 0u);
         });
-        parallelFor(keys.size(), [&](size_t b, size_t e) {
+        parallelFor(n, [&](size_t b, size_t e) {
             for (size_t i = b; i < e; ++i) {
                 const u64 k = keys[i];
                 if (k == 1ull) continue;
@@ -769,7 +769,7 @@ struct InnerSet {
                 __atomic_fetch_or(&map.slots[slot].val, 1u << (k & 7), __ATOMIC_RELAXED);
             }
         });
-        count = keys.size();
+        count = n;
     }
 };
 
@@ -797,25 +797,24 @@ struct ReplaySets {
     FlatSet t0;
     InnerSet inner;
 };
-ReplaySets buildReplaySets(const std::vector<u64> &t0Inner) {
+ReplaySets buildReplaySets(const u64 *t0Inner, size_t n) {
     ReplaySets r;
-    r.t0.init(t0Inner.size());
-    r.inner.init(t0Inner.size() + 1024);   // grown by the replay when needed
-    parallelFor(t0Inner.size(), [&](size_t b, size_t e) {
+    r.t0.init(n);
+    r.inner.init(n + 1024);   // grown by the replay when needed
+    parallelFor(n, [&](size_t b, size_t e) {
         for (size_t i = b; i < e; ++i) r.t0.insertConcurrent(t0Inner[i]);
     });
-    r.inner.build(t0Inner);
+    r.inner.build(t0Inner, n);
     return r;
 }
 
 void replaySampling(const Frame &f, const tsr_scene *scene, float maxRel, ReplaySets &sets,
-                    const std::vector<uint8_t> &isEvent, VisitIndex &index, std::vector<u64> &created,
+                    uint8_t *pending, u32 units, VisitIndex &index, std::vector<u64> &created,
                     u64 &eventCount) {
     const auto tInit = std::chrono::steady_clock::now();
     FlatSet &t0 = sets.t0;
     InnerSet &inner = sets.inner;
-    const u32 units = (u32)isEvent.size();
-    std::vector<uint8_t> pending(isEvent.begin(), isEvent.end());
+    // pending[u]: unit u is an event (from the speculation pass) or was tainted since
     // the initial events, ascending: listed per chunk in parallel, then heapified at once
     std::vector<u32> initial;
     {
@@ -1135,17 +1134,17 @@ void checkErr(u32 e) {
 
 }  // namespace
 
-void *pinnedHost(tsr_context *ctx, size_t bytes) {
-    if (bytes > ctx->pinnedBytes) {
-        if (ctx->pinned) HIPCHECK(hipHostFree(ctx->pinned));
-        ctx->pinned = nullptr;
-        ctx->pinnedBytes = 0;
+void *pinnedHost(tsr_context *ctx, size_t bytes, int slot) {
+    if (bytes > ctx->pinnedBytes[slot]) {
+        if (ctx->pinned[slot]) HIPCHECK(hipHostFree(ctx->pinned[slot]));
+        ctx->pinned[slot] = nullptr;
+        ctx->pinnedBytes[slot] = 0;
         size_t cap = (size_t)1 << 26;
         while (cap < bytes) cap <<= 1;
-        HIPCHECK(hipHostMalloc(&ctx->pinned, cap, hipHostMallocDefault));
-        ctx->pinnedBytes = cap;
+        HIPCHECK(hipHostMalloc(&ctx->pinned[slot], cap, hipHostMallocDefault));
+        ctx->pinnedBytes[slot] = cap;
     }
-    return ctx->pinned;
+    return ctx->pinned[slot];
 }
 
 namespace {
@@ -1320,16 +1319,13 @@ void buildTree(tsr_context *ctx, const tsr_scene *scene, const tsr_params &prm, 
     HIPCHECK(hipMemsetAsync(t0Counter.p, 0, 4, st));
     compactKeys(sc, inner, innerList.p, t0Counter.p, st);
     const u32 t0Count = readScalar(t0Counter.p, st);
-    std::vector<u64> hInner(t0Count);
-    if (t0Count) {   // ascending codes = ascending depth: the host inserts parents first
-        DBuf<u64> sortedInner(t0Count);
-        sortKeys(sc, innerList.p, sortedInner.p, t0Count, 0, (int)(3 * maxContainerDepth + 1), st);
-        HIPCHECK(hipMemcpyAsync(hInner.data(), sortedInner.p, (size_t)t0Count * 8, hipMemcpyDeviceToHost, st));
-        HIPCHECK(hipStreamSynchronize(st));
-    }
+    // the inner codes in pinned host memory (any order: the host sets insert every key
+    // before they record children in the parents' masks)
+    u64 *hInner = (u64 *)pinnedHost(ctx, std::max<size_t>((size_t)t0Count * 8, 8), 2);
+    if (t0Count) HIPCHECK(hipMemcpyAsync(hInner, innerList.p, (size_t)t0Count * 8, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
     // the replay's host sets are built on host threads while the GPU speculates
-    std::future<ReplaySets> setsFuture = std::async(std::launch::async, buildReplaySets, std::cref(hInner));
+    std::future<ReplaySets> setsFuture = std::async(std::launch::async, buildReplaySets, hInner, (size_t)t0Count);
 
     // ---- sampling refinement: speculative evaluation on T0
     const u32 units = 2 * n;
@@ -1377,9 +1373,11 @@ void buildTree(tsr_context *ctx, const tsr_scene *scene, const tsr_params &prm, 
     }
     checkErr(readScalar(err.p, st));
     mark("hot pairs");
-    std::vector<uint8_t> hEvent(units);
+    // the event flags go to pinned host memory while the index is built; the replay then
+    // keeps its pending flags in place there
+    uint8_t *hEvent = (uint8_t *)pinnedHost(ctx, units, 1);
     VisitIndex index;
-    HIPCHECK(hipMemcpyAsync(hEvent.data(), isEvent.p, units, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(hEvent, isEvent.p, units, hipMemcpyDeviceToHost, st));
     {
         // runs of equal leaves in the sorted hot pairs: unique leaf codes + run offsets + units
         u32 runs = 0;
@@ -1455,7 +1453,7 @@ void buildTree(tsr_context *ctx, const tsr_scene *scene, const tsr_params &prm, 
     u64 events = 0;
     ReplaySets sets = setsFuture.get();
     mark("replay sets (host, overlapped)");
-    replaySampling(f, scene, maxRel, sets, hEvent, index, created, events);
+    replaySampling(f, scene, maxRel, sets, hEvent, units, index, created, events);
     if (trace)
         std::fprintf(stderr, "[tsr tree] units %u pairs %zu hot pairs %zu t0-inner %u events %llu created nodes %zu "
                      "index misses %llu\n", units, pairs, hotPairs, t0Count, (unsigned long long)events,
