@@ -928,14 +928,38 @@ void replaySampling(const Frame &f, const tsr_scene *scene, float maxRel, Replay
             if (f.h(d) == 0.0) throw tsr_error(TSR_ERR_ZERO_SIZE_NODE, "Reached a tree node of size 0");
             double score = -1.0e300;
             u32 best = INVALID;
+            // segmentBox for the 8 children: per axis the children's bounds are the three
+            // planes B(2k), B(2k+1), B(2k+2) at depth d, so their slab parameters
+            // (plane - p) / dir are formed once (the same quotients segmentBox forms per child)
+            u32 kp[3];
+            locCoords(n, kp[0], kp[1], kp[2]);
+            double plane[3][3], q[3][3], dir[3];
+            for (int a = 0; a < 3; ++a) {
+                dir[a] = en[a] - u.p[a];
+                for (u32 j = 0; j < 3; ++j) {
+                    plane[a][j] = f.B(a, 2 * kp[a] + j, d);
+                    if (dir[a] != 0.0) q[a][j] = (plane[a][j] - u.p[a]) / dir[a];
+                }
+            }
+            auto childSegment = [&](u32 c, double &tIn, double &tOut) {
+                tIn = 0.0;
+                tOut = 1.0;
+                for (int a = 0; a < 3; ++a) {
+                    const u32 b = (c >> a) & 1u;
+                    if (dir[a] == 0.0) {
+                        if (u.p[a] < plane[a][b] || u.p[a] > plane[a][b + 1]) return false;
+                        continue;
+                    }
+                    double t1 = q[a][b], t2 = q[a][b + 1];
+                    if (t1 > t2) { const double tmp = t1; t1 = t2; t2 = tmp; }
+                    if (t1 > tIn) tIn = t1;
+                    if (t2 < tOut) tOut = t2;
+                }
+                return tIn <= tOut;
+            };
             for (u32 c = 0; c < 8; ++c) {
-                const u64 ch = (n << 3) | c;
-                u32 k[3];
-                locCoords(ch, k[0], k[1], k[2]);
-                double lo[3], hi[3];
-                for (int a = 0; a < 3; ++a) { lo[a] = f.B(a, k[a], d); hi[a] = f.B(a, (u64)k[a] + 1, d); }
                 double tIn, tOut;
-                if (!segmentBox(tIn, tOut, u.p, en, lo, hi)) continue;
+                if (!childSegment(c, tIn, tOut)) continue;
                 const double overlap = tOut - tIn;
                 if (overlap <= score) continue;
                 score = overlap;
