@@ -120,32 +120,34 @@ __global__ void k_insertAncestors(HashView h, const u64 *__restrict__ containerL
 // Nodes.cpp:433-469 findReliableSamplingNode as a stackless DFS over location
 // codes (parent = loc >> 3, next sibling = loc + 1).
 // ---------------------------------------------------------------------------
+// A leaf is reliable when its centre fits and so does one of its 8 children's
+// (fittingCentre at depth d + 1, with the two centre offsets per axis, their squares and
+// plane terms formed once: the same operations in the same order).
+TSR_HD bool reliableLeafAt(const Frame &f, u64 loc, bool positive, const SamplingUnit &u) {
+    u32 k[3];
+    locCoords(loc, k[0], k[1], k[2]);
+    const u32 d = locDepth(loc);
+    if (!fittingCentre(f, k[0], k[1], k[2], d, positive, u)) return false;
+    double sq[3][2], pl[3][2];
+    for (int a = 0; a < 3; ++a)
+        for (u32 b = 0; b < 2; ++b) {
+            const double dx = f.B(a, 2 * (2 * k[a] + b) + 1, d + 2) - u.p[a];
+            sq[a][b] = dx * dx;
+            pl[a][b] = dx * u.n[a];
+        }
+    const double msd2 = u.msd * u.msd;
+    for (u32 c = 0; c < 8; ++c) {
+        const u32 bx = c & 1, by = (c >> 1) & 1, bz = c >> 2;
+        if ((sq[0][bx] + sq[1][by]) + sq[2][bz] > msd2) continue;
+        if (((pl[0][bx] + pl[1][by]) + pl[2][bz] > 0.0) == positive) return true;
+    }
+    return false;
+}
+
 template <typename IsInner, typename OnLeaf>
 TSR_HD bool findReliable(const Frame &f, u64 start, bool positive, const SamplingUnit &u, IsInner isInner,
                          OnLeaf onLeaf) {
-    // the leaf's centre fits, and so does one of its 8 children's (fittingCentre at depth
-    // d + 1, with the two centre offsets per axis, their squares and plane terms formed once:
-    // the same operations in the same order)
-    auto reliableLeaf = [&](u64 loc) {
-        u32 k[3];
-        locCoords(loc, k[0], k[1], k[2]);
-        const u32 d = locDepth(loc);
-        if (!fittingCentre(f, k[0], k[1], k[2], d, positive, u)) return false;
-        double sq[3][2], pl[3][2];
-        for (int a = 0; a < 3; ++a)
-            for (u32 b = 0; b < 2; ++b) {
-                const double dx = f.B(a, 2 * (2 * k[a] + b) + 1, d + 2) - u.p[a];
-                sq[a][b] = dx * dx;
-                pl[a][b] = dx * u.n[a];
-            }
-        const double msd2 = u.msd * u.msd;
-        for (u32 c = 0; c < 8; ++c) {
-            const u32 bx = c & 1, by = (c >> 1) & 1, bz = c >> 2;
-            if ((sq[0][bx] + sq[1][by]) + sq[2][bz] > msd2) continue;
-            if (((pl[0][bx] + pl[1][by]) + pl[2][bz] > 0.0) == positive) return true;
-        }
-        return false;
-    };
+    auto reliableLeaf = [&](u64 loc) { return reliableLeafAt(f, loc, positive, u); };
     if (!isInner(start)) {
         onLeaf(start);
         return reliableLeaf(start);
@@ -170,6 +172,76 @@ TSR_HD bool findReliable(const Frame &f, u64 start, bool positive, const Samplin
             }
             onLeaf(ch);
             if (reliableLeaf(ch)) return true;
+        }
+        ++next;
+    }
+}
+
+// Host form of findReliable for the in-order replay: the same depth-first search and the
+// same predicates, with each opened node's child planes B(2k + j) (j = 0, 1, 2 per axis)
+// and their plane-distance products formed once for its 8 children (intersectNode forms
+// them per child: the same operations in the same order) and kept on a stack of open nodes.
+template <typename IsInner, typename OnLeaf>
+bool findReliableHost(const Frame &f, u64 start, bool positive, const SamplingUnit &u, IsInner isInner,
+                      OnLeaf onLeaf) {
+    if (!isInner(start)) {
+        onLeaf(start);
+        return reliableLeafAt(f, start, positive, u);
+    }
+    struct Open {
+        u64 loc;
+        double plane[3][3], prod[3][3];
+    };
+    Open stack[MAX_DEPTH + 2];
+    int top = 0;
+    auto open = [&](u64 loc) {
+        Open &o = stack[top++];
+        o.loc = loc;
+        u32 k[3];
+        locCoords(loc, k[0], k[1], k[2]);
+        const u32 d = locDepth(loc) + 1;
+        for (int a = 0; a < 3; ++a)
+            for (u32 j = 0; j < 3; ++j) {
+                o.plane[a][j] = f.B(a, 2 * k[a] + j, d);
+                o.prod[a][j] = (o.plane[a][j] - u.p[a]) * u.n[a];
+            }
+    };
+    // intersectNode for child c of the open node o
+    auto intersectChild = [&](const Open &o, u32 c) {
+        const u32 b[3] = {c & 1u, (c >> 1) & 1u, c >> 2};
+        bool anyRight = false;
+        for (u32 q = 0; q < 8 && !anyRight; ++q) {
+            const double v = (o.prod[0][b[0] + (q & 1)] + o.prod[1][b[1] + ((q >> 1) & 1)]) + o.prod[2][b[2] + (q >> 2)];
+            if ((v > 0.0) == positive) anyRight = true;
+        }
+        if (!anyRight) return false;
+        double d2 = 0.0;
+        for (int a = 0; a < 3; ++a) {
+            const double lo = o.plane[a][b[a]], hi = o.plane[a][b[a] + 1];
+            if (u.p[a] < lo) d2 += (lo - u.p[a]) * (lo - u.p[a]);
+            else if (u.p[a] > hi) d2 += (u.p[a] - hi) * (u.p[a] - hi);
+        }
+        return d2 <= u.msd * u.msd;
+    };
+    open(start);
+    u32 next = 0;
+    for (;;) {
+        Open &o = stack[top - 1];
+        if (next == 8) {
+            if (o.loc == start) return false;
+            next = (u32)(o.loc & 7) + 1;
+            --top;
+            continue;
+        }
+        const u64 ch = (o.loc << 3) | next;
+        if (intersectChild(o, next)) {
+            if (isInner(ch)) {
+                open(ch);
+                next = 0;
+                continue;
+            }
+            onLeaf(ch);
+            if (reliableLeafAt(f, ch, positive, u)) return true;
         }
         ++next;
     }
@@ -880,7 +952,7 @@ void replaySampling(const Frame &f, const tsr_scene *scene, float maxRel, Replay
         const bool positive = (unit & 1) == 0;
         SamplingUnit u;
         loadUnitHost(unit, u);
-        if (!knownToFail && findReliable(f, searchStart(f, u, isInner), positive, u, isInner, noLeaf)) return;
+        if (!knownToFail && findReliableHost(f, searchStart(f, u, isInner), positive, u, isInner, noLeaf)) return;
         ++creating;
         const auto tc = clock::now();
         struct AccC {
@@ -922,7 +994,7 @@ void replaySampling(const Frame &f, const tsr_scene *scene, float maxRel, Replay
                 }
                 if (createdSince.insert(n)) createdList.push_back(n);
                 taint(n, unit);
-                if (findReliable(f, n, positive, u, isInner, noLeaf)) break;
+                if (findReliableHost(f, n, positive, u, isInner, noLeaf)) break;
             }
             const u32 d = locDepth(n) + 1;
             if (f.h(d) == 0.0) throw tsr_error(TSR_ERR_ZERO_SIZE_NODE, "Reached a tree node of size 0");
@@ -1008,7 +1080,7 @@ void replaySampling(const Frame &f, const tsr_scene *scene, float maxRel, Replay
                 SamplingUnit su;
                 loadUnitHost(batch[j], su);
                 const bool positive = (batch[j] & 1) == 0;
-                const bool ok = findReliable(f, searchStart(f, su, inView), positive, su, inView, record);
+                const bool ok = findReliableHost(f, searchStart(f, su, inView), positive, su, inView, record);
                 sp.state = over ? 0u : ok ? 1u : 2u;
             }
         }, 64);
