@@ -973,6 +973,9 @@ void replaySampling(const Frame &f, const tsr_scene *scene, float maxRel, Replay
             s1[a] = std::max(u.p[a], en[a]) + m;
         }
         u64 n = deepestContaining(f, s0, s1, true, isInner);
+        // the nodes one descent creates form a chain below the first, so they all lie in
+        // the same containment-tree leaf: after the first taint the others add no unit
+        bool taintDone = false;
         for (;;) {
             if (!inner.contains(n)) {
                 if (locDepth(n) + 1 > MAX_DEPTH) throw tsr_error(TSR_ERR_TREE_TOO_DEEP, "octree deeper than 19 levels");
@@ -993,7 +996,10 @@ void replaySampling(const Frame &f, const tsr_scene *scene, float maxRel, Replay
                     createdSince = std::move(grown);
                 }
                 if (createdSince.insert(n)) createdList.push_back(n);
-                taint(n, unit);
+                if (!taintDone) {
+                    taint(n, unit);
+                    taintDone = true;
+                }
                 if (findReliableHost(f, n, positive, u, isInner, noLeaf)) break;
             }
             const u32 d = locDepth(n) + 1;
