@@ -784,9 +784,9 @@ __global__ void k_packPriorLinks(const u32 *__restrict__ child, const u32 *__res
 // The priors traversal keeps an LDS stack of inner nodes still to open.  The first
 // pass uses a short stack; a group that would overflow it is abandoned and listed,
 // and a second pass redoes the listed groups with a stack of the proven bound.  The
-// short stack (14 B per entry) leaves room in the 160 KB of LDS per CU for 6 waves per
+// short stack (14 B per entry) leaves room in the 160 KB of LDS per CU for 7 waves per
 // SIMD of one-wave blocks.
-constexpr u32 PRIOR_STACK_SHORT = 384;
+constexpr u32 PRIOR_STACK_SHORT = 320;
 constexpr u32 XCDS = 8;   // MI355X: blocks are dispatched round-robin over 8 XCDs, each with its own L2
 constexpr u32 PRIOR_STACK = 1344;   // >= 64 * (MAX_DEPTH + 1): one step pushes <= 56 more than it pops
 
@@ -1035,12 +1035,12 @@ __device__ __forceinline__ void priorsGroup(PRIORS_GROUP_PARAMS) {
     }
 }
 
-// register use capped for 6 waves per SIMD: the step's memory round trips are hidden
-// better than with 5 (96 VGPRs, no spill), in spite of ~50 bytes of spill; 7 and 8 waves
-// spill more and run slower.  The long-stack instantiation is held to few waves by its LDS
-// stack anyway and is not capped.
+// register use capped for 7 waves per SIMD (72 VGPRs; ~64 bytes of spill: set-up values
+// and one query's S accumulator): the step's memory and LDS round trips are hidden better
+// than with 5 or 6 waves; 8 waves spill more and run slower.  The long-stack instantiation is held to
+// few waves by its LDS stack anyway and is not capped.
 template <u32 GROUP_Q, u32 STACK>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(STACK == PRIOR_STACK_SHORT ? 6 : 1))) void k_priorsGroup(PRIORS_GROUP_PARAMS) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(STACK == PRIOR_STACK_SHORT ? 7 : 1))) void k_priorsGroup(PRIORS_GROUP_PARAMS) {
     priorsGroup<GROUP_Q, STACK>(PRIORS_GROUP_ARGS);
 }
 
