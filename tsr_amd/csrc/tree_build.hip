@@ -886,24 +886,43 @@ void replaySampling(const Frame &f, const tsr_scene *scene, float maxRel, Replay
     const auto tInit = std::chrono::steady_clock::now();
     FlatSet &t0 = sets.t0;
     InnerSet &inner = sets.inner;
-    // pending[u]: unit u is an event (from the speculation pass) or was tainted since
-    // the initial events, ascending: listed per chunk in parallel, then heapified at once
-    std::vector<u32> initial;
+    // pending[u]: unit u is an event (from the speculation pass) or was tainted since.
+    // The units still to run, in ascending order, are the set bits of `queued` (one bit per
+    // unit): the replay only ever asks for the smallest queued unit at or after a position.
+    std::vector<u64> queued(((size_t)units + 63) / 64, 0ull);
+    size_t initialEvents = 0;
     {
-        constexpr size_t CHUNK = 1 << 20;
-        const size_t chunks = (units + CHUNK - 1) / CHUNK;
-        std::vector<std::vector<u32>> part(chunks);
-        parallelFor(chunks, [&](size_t b, size_t e) {
-            for (size_t c = b; c < e; ++c) {
-                const u32 u0 = (u32)(c * CHUNK), u1 = (u32)std::min<size_t>(units, (c + 1) * CHUNK);
-                for (u32 u = u0; u < u1; ++u)
-                    if (pending[u]) part[c].push_back(u);
-            }
+        constexpr size_t WORDS = 1 << 14;   // words per task
+        const size_t words = queued.size();
+        std::vector<size_t> part((words + WORDS - 1) / WORDS, 0);
+        parallelFor(part.size(), [&](size_t b, size_t e) {
+            for (size_t c = b; c < e; ++c)
+                for (size_t w = c * WORDS; w < std::min(words, (c + 1) * WORDS); ++w) {
+                    u64 bits = 0ull;
+                    for (u32 i = 0; i < 64 && w * 64 + i < units; ++i)
+                        if (pending[w * 64 + i]) bits |= 1ull << i;
+                    queued[w] = bits;
+                    part[c] += (size_t)__builtin_popcountll(bits);
+                }
         }, 1);
-        for (auto &v : part) initial.insert(initial.end(), v.begin(), v.end());
+        for (size_t c : part) initialEvents += c;
     }
-    std::priority_queue<u32, std::vector<u32>, std::greater<u32>> heap(std::greater<u32>(), std::move(initial));
-    const size_t initialEvents = heap.size();
+    auto enqueue = [&](u32 v) { queued[v >> 6] |= 1ull << (v & 63); };
+    // the smallest queued unit in [from, end) (removed from the queue), or INVALID
+    auto takeFrom = [&](u32 from, u32 end) -> u32 {
+        if (from >= end) return INVALID;
+        const size_t last = ((size_t)end - 1) >> 6;
+        size_t w = from >> 6;
+        u64 bits = queued[w] & (~0ull << (from & 63));
+        while (!bits) {
+            if (++w > last) return INVALID;
+            bits = queued[w];
+        }
+        const u32 v = (u32)(w * 64 + (size_t)__builtin_ctzll(bits));
+        if (v >= end) return INVALID;
+        queued[w] &= ~(1ull << (v & 63));
+        return v;
+    };
     u64 tainted = 0, creating = 0;
     auto isInner = [&](u64 loc) { return inner.contains(loc); };
     auto noLeaf = [](u64) {};
@@ -931,7 +950,7 @@ void replaySampling(const Frame &f, const tsr_scene *scene, float maxRel, Replay
             if (pending[*it]) continue;
             ++tainted;
             pending[*it] = 1;
-            heap.push(*it);
+            enqueue(*it);
         }
         taintMs += msSince(tt);
     };
@@ -939,7 +958,7 @@ void replaySampling(const Frame &f, const tsr_scene *scene, float maxRel, Replay
     // nodes made inner since the current batch was speculated
     FlatSet createdSince;
     std::vector<u64> createdList;
-    createdSince.init(1u << 15);
+    createdSince.init(1u << 10);   // a batch's creations; grown when needed
     // the sequential semantics: evaluate `unit` on the current tree, refine if needed
     auto processFull = [&](u32 unit, bool knownToFail) {
         ++eventCount;
@@ -1065,12 +1084,16 @@ void replaySampling(const Frame &f, const tsr_scene *scene, float maxRel, Replay
     std::vector<Spec> spec(BATCH);
     u64 speculated = 0;
     const auto tLoop = std::chrono::steady_clock::now();
-    while (!heap.empty()) {
+    u32 cursor = 0;   // every queued unit below it has run
+    for (;;) {
         batch.clear();
-        while (!heap.empty() && batch.size() < BATCH) {
-            batch.push_back(heap.top());
-            heap.pop();
+        for (u32 from = cursor; batch.size() < BATCH;) {
+            const u32 v = takeFrom(from, units);
+            if (v == INVALID) break;
+            batch.push_back(v);
+            from = v + 1;
         }
+        if (batch.empty()) break;
         const auto ts = clock::now();
         parallelFor(batch.size(), [&](size_t b, size_t e) {
             InnerView view{&inner.map};
@@ -1095,13 +1118,16 @@ void replaySampling(const Frame &f, const tsr_scene *scene, float maxRel, Replay
             std::fill(createdSince.keys.begin(), createdSince.keys.end(), 0ull);
             createdList.clear();
         }
+        u32 done = cursor;   // units below `done` have run
         for (size_t j = 0; j < batch.size(); ++j) {
             const u32 u = batch[j];
-            while (!heap.empty() && heap.top() < u) {   // made pending after the batch was formed
-                const u32 v = heap.top();
-                heap.pop();
+            for (;;) {   // made pending after the batch was formed, before u
+                const u32 v = takeFrom(done, u);
+                if (v == INVALID) break;
                 processFull(v, false);
+                done = v + 1;
             }
+            done = u + 1;
             const Spec &sp = spec[j];
             bool valid = sp.state != 0u;
             for (u32 i = 0; valid && i < sp.count && !createdList.empty(); ++i)
@@ -1113,6 +1139,7 @@ void replaySampling(const Frame &f, const tsr_scene *scene, float maxRel, Replay
             }
             processFull(u, valid);   // valid here: the DFS is known to find no reliable leaf
         }
+        cursor = batch.back() + 1;   // units made pending from here on come after the batch
     }
     if (std::getenv("TSR_TRACE"))
         std::fprintf(stderr, "[tsr tree]   replay set-up %.2f ms, in-order loop %.2f ms; initial events %zu, "
