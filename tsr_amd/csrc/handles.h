@@ -40,9 +40,9 @@ struct tsr_context {
     tsr::ConeArena cone;
     // pinned host staging for large device->host transfers, one grow-only buffer per use
     // (slot 0: the sampling visit index, slot 1: the sampling event flags, slot 2: the
-    // containment tree's inner codes), freed with the context
-    void *pinned[3] = {nullptr, nullptr, nullptr};
-    size_t pinnedBytes[3] = {0, 0, 0};
+    // containment tree's inner codes, slot 3: the event flags as bits), freed with the context
+    void *pinned[4] = {nullptr, nullptr, nullptr, nullptr};
+    size_t pinnedBytes[4] = {0, 0, 0, 0};
 };
 
 struct tsr_scene {

@@ -881,37 +881,23 @@ ReplaySets buildReplaySets(const u64 *t0Inner, size_t n) {
 }
 
 void replaySampling(const Frame &f, const tsr_scene *scene, float maxRel, ReplaySets &sets,
-                    uint8_t *pending, u32 units, VisitIndex &index, std::vector<u64> &created,
+                    uint8_t *pending, u64 *queued, u32 units, VisitIndex &index, std::vector<u64> &created,
                     u64 &eventCount) {
     const auto tInit = std::chrono::steady_clock::now();
     FlatSet &t0 = sets.t0;
     InnerSet &inner = sets.inner;
     // pending[u]: unit u is an event (from the speculation pass) or was tainted since.
     // The units still to run, in ascending order, are the set bits of `queued` (one bit per
-    // unit): the replay only ever asks for the smallest queued unit at or after a position.
-    std::vector<u64> queued(((size_t)units + 63) / 64, 0ull);
+    // unit, starting as the event flags): the replay only ever asks for the smallest queued
+    // unit in a range.
+    const size_t queuedWords = ((size_t)units + 63) / 64;
     size_t initialEvents = 0;
-    {
-        constexpr size_t WORDS = 1 << 14;   // words per task
-        const size_t words = queued.size();
-        std::vector<size_t> part((words + WORDS - 1) / WORDS, 0);
-        parallelFor(part.size(), [&](size_t b, size_t e) {
-            for (size_t c = b; c < e; ++c)
-                for (size_t w = c * WORDS; w < std::min(words, (c + 1) * WORDS); ++w) {
-                    u64 bits = 0ull;
-                    for (u32 i = 0; i < 64 && w * 64 + i < units; ++i)
-                        if (pending[w * 64 + i]) bits |= 1ull << i;
-                    queued[w] = bits;
-                    part[c] += (size_t)__builtin_popcountll(bits);
-                }
-        }, 1);
-        for (size_t c : part) initialEvents += c;
-    }
+    for (size_t w = 0; w < queuedWords; ++w) initialEvents += (size_t)__builtin_popcountll(queued[w]);
     auto enqueue = [&](u32 v) { queued[v >> 6] |= 1ull << (v & 63); };
     // the smallest queued unit in [from, end) (removed from the queue), or INVALID
     auto takeFrom = [&](u32 from, u32 end) -> u32 {
         if (from >= end) return INVALID;
-        const size_t last = ((size_t)end - 1) >> 6;
+        const size_t last = std::min(queuedWords - 1, ((size_t)end - 1) >> 6);
         size_t w = from >> 6;
         u64 bits = queued[w] & (~0ull << (from & 63));
         while (!bits) {
@@ -1222,6 +1208,13 @@ __global__ void k_hotVisits(HashView hot, const u64 *__restrict__ staged, const 
     if (mode == 0) hotCount[u] = k;
 }
 
+// the event flags as one bit per unit (bit u & 63 of word u >> 6)
+__global__ void k_eventBits(const uint8_t *__restrict__ isEvent, u32 units, u64 *__restrict__ bits) {
+    const u32 u = blockIdx.x * BLOCK + threadIdx.x;
+    const u64 b = __ballot(u < units && isEvent[u] != 0);
+    if ((threadIdx.x & 63) == 0 && u < units) bits[u >> 6] = b;
+}
+
 // units (ascending) whose staged visits include `key`
 __global__ void k_matchStaged(const u64 *__restrict__ staged, const u32 *__restrict__ visitCount,
                               const uint8_t *__restrict__ isEvent, u32 units, u64 key, uint8_t *__restrict__ flag) {
@@ -1507,6 +1500,14 @@ void buildTree(tsr_context *ctx, const tsr_scene *scene, const tsr_params &prm, 
     uint8_t *hEvent = (uint8_t *)pinnedHost(ctx, units, 1);
     VisitIndex index;
     HIPCHECK(hipMemcpyAsync(hEvent, isEvent.p, units, hipMemcpyDeviceToHost, st));
+    // the same flags as one bit per unit: the replay's initial queue
+    const size_t eventWords = ((size_t)units + 63) / 64;
+    u64 *hEventBits = (u64 *)pinnedHost(ctx, eventWords * 8, 3);
+    {
+        DBuf<u64> bits(eventWords);
+        k_eventBits<<<gridFor(units, BLOCK), BLOCK, 0, st>>>(isEvent.p, units, bits.p);
+        HIPCHECK(hipMemcpyAsync(hEventBits, bits.p, eventWords * 8, hipMemcpyDeviceToHost, st));
+    }
     {
         // runs of equal leaves in the sorted hot pairs: unique leaf codes + run offsets + units
         u32 runs = 0;
@@ -1582,7 +1583,7 @@ void buildTree(tsr_context *ctx, const tsr_scene *scene, const tsr_params &prm, 
     u64 events = 0;
     ReplaySets sets = setsFuture.get();
     mark("replay sets (host, overlapped)");
-    replaySampling(f, scene, maxRel, sets, hEvent, units, index, created, events);
+    replaySampling(f, scene, maxRel, sets, hEvent, hEventBits, units, index, created, events);
     if (trace)
         std::fprintf(stderr, "[tsr tree] units %u pairs %zu hot pairs %zu t0-inner %u events %llu created nodes %zu "
                      "index misses %llu\n", units, pairs, hotPairs, t0Count, (unsigned long long)events,
