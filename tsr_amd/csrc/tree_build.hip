@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <chrono>
 #include <functional>
+#include <atomic>
 #include <future>
 #include <cstdio>
 #include <cstdlib>
@@ -880,6 +881,8 @@ ReplaySets buildReplaySets(const u64 *t0Inner, size_t n) {
     return r;
 }
 
+unsigned hostThreads();   // worker threads of parallelFor (defined with its pool below)
+
 void replaySampling(const Frame &f, const tsr_scene *scene, float maxRel, ReplaySets &sets,
                     uint8_t *pending, u64 *queued, u32 units, VisitIndex &index, std::vector<u64> &created,
                     u64 &eventCount) {
@@ -1081,10 +1084,15 @@ void replaySampling(const Frame &f, const tsr_scene *scene, float maxRel, Replay
         }
         if (batch.empty()) break;
         const auto ts = clock::now();
-        parallelFor(batch.size(), [&](size_t b, size_t e) {
+        // units are taken in runs of 16 from a shared counter: their searches differ widely
+        // in length, so a static split leaves threads idle
+        std::atomic<size_t> nextUnit{0};
+        const size_t batchSize = batch.size();
+        parallelFor(hostThreads(), [&](size_t, size_t) {
             InnerView view{&inner.map};
             auto inView = [&](u64 loc) { return view.contains(loc); };
-            for (size_t j = b; j < e; ++j) {
+            for (size_t j0; (j0 = nextUnit.fetch_add(16, std::memory_order_relaxed)) < batchSize;)
+            for (size_t j = j0; j < std::min(batchSize, j0 + 16); ++j) {
                 Spec &sp = spec[j];
                 sp.count = 0;
                 bool over = false;
@@ -1098,7 +1106,7 @@ void replaySampling(const Frame &f, const tsr_scene *scene, float maxRel, Replay
                 const bool ok = findReliableHost(f, searchStart(f, su, inView), positive, su, inView, record);
                 sp.state = over ? 0u : ok ? 1u : 2u;
             }
-        }, 64);
+        }, 1);
         specMs += msSince(ts);
         if (!createdList.empty()) {   // a new batch: nothing created since it was speculated
             std::fill(createdSince.keys.begin(), createdSince.keys.end(), 0ull);
