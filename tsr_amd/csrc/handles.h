@@ -39,10 +39,10 @@ struct tsr_context {
     tsr::Scratch scratch;
     tsr::ConeArena cone;
     // pinned host staging for large device->host transfers, one grow-only buffer per use
-    // (slot 0: the sampling visit index, slot 1: the sampling event flags, slot 2: the
-    // containment tree's inner codes, slot 3: the event flags as bits), freed with the context
-    void *pinned[4] = {nullptr, nullptr, nullptr, nullptr};
-    size_t pinnedBytes[4] = {0, 0, 0, 0};
+    // (slot 0: the sampling visit index, slot 1: the containment tree's inner codes, slot 2:
+    // the sampling event flags as bits), freed with the context
+    void *pinned[3] = {nullptr, nullptr, nullptr};
+    size_t pinnedBytes[3] = {0, 0, 0};
 };
 
 struct tsr_scene {

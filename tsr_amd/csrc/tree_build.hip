@@ -884,16 +884,17 @@ ReplaySets buildReplaySets(const u64 *t0Inner, size_t n) {
 unsigned hostThreads();   // worker threads of parallelFor (defined with its pool below)
 
 void replaySampling(const Frame &f, const tsr_scene *scene, float maxRel, ReplaySets &sets,
-                    uint8_t *pending, u64 *queued, u32 units, VisitIndex &index, std::vector<u64> &created,
+                    u64 *queued, u32 units, VisitIndex &index, std::vector<u64> &created,
                     u64 &eventCount) {
     const auto tInit = std::chrono::steady_clock::now();
     FlatSet &t0 = sets.t0;
     InnerSet &inner = sets.inner;
-    // pending[u]: unit u is an event (from the speculation pass) or was tainted since.
     // The units still to run, in ascending order, are the set bits of `queued` (one bit per
     // unit, starting as the event flags): the replay only ever asks for the smallest queued
-    // unit in a range.
+    // unit in a range.  `pendingBits` marks the units that were ever queued (events, then
+    // tainted units); one bit per unit keeps it cache-resident for the taints' random probes.
     const size_t queuedWords = ((size_t)units + 63) / 64;
+    std::vector<u64> pendingBits(queued, queued + queuedWords);
     size_t initialEvents = 0;
     for (size_t w = 0; w < queuedWords; ++w) initialEvents += (size_t)__builtin_popcountll(queued[w]);
     auto enqueue = [&](u32 v) { queued[v >> 6] |= 1ull << (v & 63); };
@@ -936,9 +937,11 @@ void replaySampling(const Frame &f, const tsr_scene *scene, float maxRel, Replay
         const u32 *from = std::upper_bound(first, last, writer);
         candidates += (u64)(last - from);
         for (const u32 *it = from; it != last; ++it) {
-            if (pending[*it]) continue;
+            u64 &word = pendingBits[*it >> 6];
+            const u64 bit = 1ull << (*it & 63);
+            if (word & bit) continue;
             ++tainted;
-            pending[*it] = 1;
+            word |= bit;
             enqueue(*it);
         }
         taintMs += msSince(tt);
@@ -1451,7 +1454,7 @@ void buildTree(tsr_context *ctx, const tsr_scene *scene, const tsr_params &prm, 
     const u32 t0Count = readScalar(t0Counter.p, st);
     // the inner codes in pinned host memory (any order: the host sets insert every key
     // before they record children in the parents' masks)
-    u64 *hInner = (u64 *)pinnedHost(ctx, std::max<size_t>((size_t)t0Count * 8, 8), 2);
+    u64 *hInner = (u64 *)pinnedHost(ctx, std::max<size_t>((size_t)t0Count * 8, 8), 1);
     if (t0Count) HIPCHECK(hipMemcpyAsync(hInner, innerList.p, (size_t)t0Count * 8, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
     // the replay's host sets are built on host threads while the GPU speculates
@@ -1503,14 +1506,11 @@ void buildTree(tsr_context *ctx, const tsr_scene *scene, const tsr_params &prm, 
     }
     checkErr(readScalar(err.p, st));
     mark("hot pairs");
-    // the event flags go to pinned host memory while the index is built; the replay then
-    // keeps its pending flags in place there
-    uint8_t *hEvent = (uint8_t *)pinnedHost(ctx, units, 1);
+    // the event flags, one bit per unit, go to pinned host memory while the index is built:
+    // the replay's initial queue
     VisitIndex index;
-    HIPCHECK(hipMemcpyAsync(hEvent, isEvent.p, units, hipMemcpyDeviceToHost, st));
-    // the same flags as one bit per unit: the replay's initial queue
     const size_t eventWords = ((size_t)units + 63) / 64;
-    u64 *hEventBits = (u64 *)pinnedHost(ctx, eventWords * 8, 3);
+    u64 *hEventBits = (u64 *)pinnedHost(ctx, eventWords * 8, 2);
     {
         DBuf<u64> bits(eventWords);
         k_eventBits<<<gridFor(units, BLOCK), BLOCK, 0, st>>>(isEvent.p, units, bits.p);
@@ -1591,7 +1591,7 @@ void buildTree(tsr_context *ctx, const tsr_scene *scene, const tsr_params &prm, 
     u64 events = 0;
     ReplaySets sets = setsFuture.get();
     mark("replay sets (host, overlapped)");
-    replaySampling(f, scene, maxRel, sets, hEvent, hEventBits, units, index, created, events);
+    replaySampling(f, scene, maxRel, sets, hEventBits, units, index, created, events);
     if (trace)
         std::fprintf(stderr, "[tsr tree] units %u pairs %zu hot pairs %zu t0-inner %u events %llu created nodes %zu "
                      "index misses %llu\n", units, pairs, hotPairs, t0Count, (unsigned long long)events,
