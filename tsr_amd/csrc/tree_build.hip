@@ -752,28 +752,30 @@ struct FlatMap {
     }
 };
 
-// read-only host view of an open-addressing u64 -> u32 table laid out like the
-// device HashView (same hash, linear probing, key 0 = empty), e.g. one built on the
-// GPU and copied to host memory
-struct FlatMapView {
-    const u64 *keys = nullptr;
-    const u32 *vals = nullptr;
+// host view of the (T0 leaf -> units that visited it) index for the "hot" leaves
+// (those near units the speculation could not settle); other leaves are looked
+// up on the device on demand
+// One slot of the hot-leaf table, laid out like the device HashView (same hash, linear
+// probing, key 0 = empty) with the leaf's run of visiting units in the same 16 bytes, so a
+// lookup costs one cache line.
+struct RunSlot {
+    u64 key;
+    u32 first, count;   // units [first, first + count) of the unit list; count 0: no visits
+};
+struct RunSlotView {
+    const RunSlot *slots = nullptr;
     u64 mask = 0;
-    const u32 *find(u64 k) const {
+    const RunSlot *find(u64 k) const {
         for (u64 s = hashMix(k) & mask;; s = (s + 1) & mask) {
-            const u64 v = keys[s];
-            if (v == k) return &vals[s];
+            const u64 v = slots[s].key;
+            if (v == k) return &slots[s];
             if (v == 0ull) return nullptr;
         }
     }
 };
 
-// host view of the (T0 leaf -> units that visited it) index for the "hot" leaves
-// (those near units the speculation could not settle); other leaves are looked
-// up on the device on demand
 struct VisitIndex {
-    FlatMapView run;              // hot leaf -> run index into runStart, INVALID if no unit visited it
-    const u32 *runStart = nullptr;   // runs + 1 offsets into unit
+    RunSlotView run;                 // hot leaf -> its run of visiting units
     const u32 *unit = nullptr;       // visiting units, ascending within a run
     std::function<void(u64, std::vector<u32> &)> fetch;   // exact lookup on the device
     u64 misses = 0;
@@ -781,10 +783,10 @@ struct VisitIndex {
     // holds the list of a leaf outside the hot part
     void units(u64 loc, const u32 *&first, const u32 *&last, std::vector<u32> &spill) {
         first = last = nullptr;
-        if (const u32 *r = run.find(loc)) {
-            if (*r != INVALID) {
-                first = unit + runStart[*r];
-                last = unit + runStart[*r + 1];
+        if (const RunSlot *r = run.find(loc)) {
+            if (r->count) {
+                first = unit + r->first;
+                last = first + r->count;
             }
             return;
         }
@@ -1219,6 +1221,22 @@ __global__ void k_hotVisits(HashView hot, const u64 *__restrict__ staged, const 
     if (mode == 0) hotCount[u] = k;
 }
 
+// the hot-leaf table as RunSlots: each occupied slot's key with its run of units (a run
+// index of INVALID: no unit visited the leaf)
+__global__ void k_packRunSlots(const u64 *__restrict__ keys, const u32 *__restrict__ vals, size_t cap,
+                               const u32 *__restrict__ runOff, const u32 *__restrict__ runLen,
+                               RunSlot *__restrict__ out) {
+    const size_t s = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (s >= cap) return;
+    const u64 k = keys[s];
+    const u32 r = k ? vals[s] : INVALID;
+    RunSlot o;
+    o.key = k;
+    o.first = r != INVALID ? runOff[r] : 0u;
+    o.count = r != INVALID ? runLen[r] : 0u;
+    out[s] = o;
+}
+
 // the event flags as one bit per unit (bit u & 63 of word u >> 6)
 __global__ void k_eventBits(const uint8_t *__restrict__ isEvent, u32 units, u64 *__restrict__ bits) {
     const u32 u = blockIdx.x * BLOCK + threadIdx.x;
@@ -1542,24 +1560,21 @@ void buildTree(tsr_context *ctx, const tsr_scene *scene, const tsr_params &prm, 
         if (runs) k_insertIndexed<<<gridFor(runs, BLOCK), BLOCK, 0, st>>>(runMap.view(), runLeaf.p, runs, err.p);
         if (nh) k_insertConst<<<gridFor(nh, BLOCK), BLOCK, 0, st>>>(runMap.view(), hotKeys.p, nh, INVALID, err.p);
         const size_t cap = runMap.capacity;
-        const size_t bytes = cap * 8 + cap * 4 + ((size_t)runs + 1) * 4 + hotPairs * 4;
+        const size_t bytes = cap * sizeof(RunSlot) + hotPairs * 4;
         char *pin = (char *)pinnedHost(ctx, bytes + 64);
-        u64 *hKeys = (u64 *)pin;
-        u32 *hVals = (u32 *)(hKeys + cap);
-        u32 *hRunStart = hVals + cap;
-        u32 *hUnit = hRunStart + runs + 1;
-        HIPCHECK(hipMemcpyAsync(hKeys, runMap.keys.p, cap * 8, hipMemcpyDeviceToHost, st));
-        HIPCHECK(hipMemcpyAsync(hVals, runMap.vals.p, cap * 4, hipMemcpyDeviceToHost, st));
-        if (runs) {
-            HIPCHECK(hipMemcpyAsync(hRunStart, runOff.p, (size_t)runs * 4, hipMemcpyDeviceToHost, st));
-            HIPCHECK(hipMemcpyAsync(hUnit, hotUnitSorted.p, hotPairs * 4, hipMemcpyDeviceToHost, st));
+        RunSlot *hSlots = (RunSlot *)pin;
+        u32 *hUnit = (u32 *)(hSlots + cap);
+        {   // (key, run) slots packed on the device, then one copy
+            DBuf<RunSlot> slots(cap);
+            k_packRunSlots<<<gridFor(cap, BLOCK), BLOCK, 0, st>>>(runMap.keys.p, runMap.vals.p, cap, runOff.p,
+                                                                 runLen.p, slots.p);
+            HIPCHECK(hipMemcpyAsync(hSlots, slots.p, cap * sizeof(RunSlot), hipMemcpyDeviceToHost, st));
+            if (runs) HIPCHECK(hipMemcpyAsync(hUnit, hotUnitSorted.p, hotPairs * 4, hipMemcpyDeviceToHost, st));
+            HIPCHECK(hipStreamSynchronize(st));
         }
-        HIPCHECK(hipStreamSynchronize(st));
         checkErr(readScalar(err.p, st));
-        hRunStart[runs] = (u32)hotPairs;
-        index.runStart = hRunStart;
         index.unit = hUnit;
-        index.run = FlatMapView{hKeys, hVals, cap - 1};
+        index.run = RunSlotView{hSlots, cap - 1};
         mark("hot index transfer");
         if (trace) std::fprintf(stderr, "[tsr tree]   hot leaves %u, runs %u\n", nh, runs);
     }
