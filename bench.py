@@ -1,13 +1,17 @@
 """Benchmark: input samples/s from resident SoA samples to occupancy field + DMC mesh.
 
 One step = Tree() + Occupancy(tree) + DualMarchingCells::extractMesh on one MI355X
-(HIP), for the thin-wedge + sphere scene (64 cameras, 640x480 depth maps, 0.3 %
-depth noise, 1 % outliers; samples resident in HBM before timing starts).
+(HIP), with the samples resident in HBM before timing starts.  The default workload
+is the largest single-GPU scene: 30 random primitives including thin sheets in
+[-4,4]^3, 120 cameras at 3..20, 1024x768 depth maps, 2 % outliers (36.5M samples
+after miss filtering).  `--config 1` runs the thin wedge + sphere scene.
 Prints one JSON line.  `--gpus N` with N > 1 exits: this build runs on one GPU.
 """
 import argparse
 import json
 import os
+
+import numpy as np
 import sys
 import time
 
@@ -28,6 +32,10 @@ WORKLOADS = {
 }
 
 
+# workload names of the committed counter profiles (profiles/pmc_traffic_<kernel>_<name>.txt)
+WORKLOAD_TAGS = {1: "wedge_sphere_9.2M", 4: "primitives30_36.5M"}
+
+
 def cpu_info():
     model, cores = "unknown", os.cpu_count() or 1
     try:
@@ -45,10 +53,19 @@ def cpu_info():
     return model, cores
 
 
-def cpu_baseline(config, res, threads):
-    """The oracle in its OpenMP float mode (reference parallel structure) on a bounded sample."""
+# samples of the CPU leg: about 0.57M samples of the same scene (10-30 s of host work)
+CPU_RES = {0: 1.0, 1: 0.25, 3: 0.5, 4: 0.125}
+
+
+def cpu_baseline_and_precision(ctx, config, res, threads):
+    """CPU leg.  (1) The oracle in its OpenMP float mode (the reference's Real = float and
+    its parallel structure) timed on a bounded sample of the workload: the same scene at a
+    reduced resolution.  (2) Precision of the GPU path on that same sample against the
+    oracle in FP64: per-leaf field error, sign flips and mesh Hausdorff distance, for the
+    default accumulation and for plain FP32 atomics."""
     from oracle.oracle import Oracle
-    from tsr_amd import SyntheticScene
+    from tsr_amd import DualMarchingCells, Occupancy, Scene, SyntheticScene, Tree, _lib
+    from tsr_amd.metrics import bbox_diagonal, field_error, hausdorff
 
     syn = SyntheticScene(config, seed=0, resolution_scale=res)
     o = Oracle(syn.samples, syn.cameras)
@@ -58,7 +75,34 @@ def cpu_baseline(config, res, threads):
     o.dmc()
     dt = time.perf_counter() - t0
     ph = o.timings()
-    return syn.count, dt, ph
+    # FP64 reference field and mesh on the same tree (OpenMP FP64)
+    o.occupancy(threads=threads, fp64=True)
+    o.dmc()
+    ref_tree, ref_field, ref_mesh = o.tree(), o.field(), o.mesh()
+    diag = bbox_diagonal(syn.samples)
+    prec = {"samples": syn.count, "resolution_scale": res, "reference": "oracle FP64 (OpenMP)",
+            "bbox_diagonal": diag}
+    scene = Scene(ctx, syn.samples, syn.cameras)
+    tree = Tree(ctx, scene)
+    t = tree.download()
+    prec["tree_tables_equal"] = bool(all(np.array_equal(t[k], ref_tree[k]) for k in
+                                         ("children", "sample_target", "cells")))
+    for name, flags in (("fp32_kahan_segmented", _lib.TSR_ACCUM_KAHAN_SEGMENTED),
+                        ("fp32_atomic", _lib.TSR_ACCUM_FP32_ATOMIC)):
+        occ = Occupancy(ctx, tree, scene, flags=flags | _lib.TSR_PRIORS_ALL_LEAVES)
+        mesh = DualMarchingCells(ctx, tree, occ)
+        f, m = occ.download(), mesh.download()
+        rec = field_error(f["occupancy"], f["kernel_sums"], ref_field) if prec["tree_tables_equal"] else {}
+        h, how = hausdorff(m, ref_mesh, diag)
+        rec["hausdorff_over_bbox_diagonal"] = h
+        rec["hausdorff_method"] = how
+        rec["mesh_identical"] = bool(how.startswith("same topology") and h == 0.0)
+        prec[name] = rec
+        mesh.close()
+        occ.close()
+    tree.close()
+    scene.close()
+    return syn.count, dt, ph, prec
 
 
 def main():
@@ -66,9 +110,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", type=int, default=1)
+    ap.add_argument("--config", type=int, default=4)
     ap.add_argument("--res", type=float, default=1.0)
-    ap.add_argument("--cpu-res", type=float, default=0.25, help="resolution scale of the CPU baseline sample")
+    ap.add_argument("--cpu-res", type=float, default=None,
+                    help="resolution scale of the CPU baseline / precision sample (default: ~0.57M samples)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned-compare", action="store_true",
                     help="skip the learned-cull vs plain comparison after the timed steps")
@@ -82,7 +127,7 @@ def main():
               "(the multi-GPU sharding plan is design-only)", file=sys.stderr)
         sys.exit(2)
 
-    from tsr_amd import Context, DualMarchingCells, Occupancy, Scene, SyntheticScene, Tree
+    from tsr_amd import Context, DualMarchingCells, Occupancy, Scene, SyntheticScene, Tree, device_memory
 
     t0 = time.perf_counter()
     syn = SyntheticScene(args.config, seed=0, resolution_scale=args.res)
@@ -119,6 +164,7 @@ def main():
         print(f"[bench] warmup {w} done", file=sys.stderr, flush=True)
     ctx.synchronize()
     ctx.reset_kernel_stats()
+    mem_before, _ = device_memory(reset_peak=True)
     phases = []
     t0 = time.perf_counter()
     for k in range(args.steps):
@@ -157,6 +203,8 @@ def main():
             "hits_emptiness": int(hits[0]), "hits_sampleness": int(hits[1]),
             "phase_ms": {"tree": ph[0], "sampleness": ph[1], "emptiness": ph[2], "priors": ph[3],
                          "occupancy": ph[4], "dmc": ph[5]},
+            "device_memory_peak_gib": device_memory()[1] / 2**30,
+            "device_memory_before_steps_gib": mem_before / 2**30,
         },
         "roofline": None,
         "cpu_baseline": None,
@@ -171,9 +219,10 @@ def main():
         dom = max(out["kernels"], key=lambda k: out["kernels"][k]["ms_per_launch"])
         kd = out["kernels"][dom]
         traffic = None
-        tfile = os.path.join(ROOT, "profiles", f"pmc_traffic_{dom}.txt")
-        # the committed counter profile covers the default workload only
-        if os.path.exists(tfile) and args.config == 1 and args.res == 1.0 and not args.learned:
+        tag = WORKLOAD_TAGS.get(args.config)
+        tfile = os.path.join(ROOT, "profiles", f"pmc_traffic_{dom}_{tag}.txt")
+        # committed counter profiles cover the full-size plain workloads
+        if tag and os.path.exists(tfile) and args.res == 1.0 and not args.learned:
             try:
                 with open(tfile) as f:
                     traffic = json.loads(f.readline())["hbm_bytes_per_launch"]
@@ -188,9 +237,10 @@ def main():
     if not args.no_learned_compare:
         from tsr_amd import _lib
         from tsr_amd.culling import load_culler
+        from tsr_amd.metrics import bbox_diagonal, hausdorff
         cul = culler if culler is not None else load_culler(ctx)
         tree = Tree(ctx, scene)
-        times = {}
+        times, meshes = {}, {}
         for mode in ("plain", "learned", "plain", "learned"):   # second round counts
             ctx.synchronize()
             t1 = time.perf_counter()
@@ -203,26 +253,35 @@ def main():
             times[mode] = 1000.0 * (time.perf_counter() - t1)
             if mode == "learned":
                 culled = o.cull_counts()[1]
+            meshes[mode] = m.download()
             m.close()
             o.close()
         tree.close()
         out["learned_vs_plain"] = {
             "plain_ms": times["plain"], "learned_ms": times["learned"],
             "speedup": times["plain"] / times["learned"], "culled_leaves": culled,
+            "hausdorff_over_bbox_diagonal": hausdorff(meshes["learned"], meshes["plain"],
+                                                      bbox_diagonal(syn.samples))[0],
             "scope": "occupancy field + DMC on the same tree, one run each after a warm-up run; "
                      "culler trained on config-3 scenes (tsr_amd/culler_weights.npz)",
         }
     if not args.no_cpu_baseline:
         model, cores = cpu_info()
         threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(cores, 64)
-        cn, cdt, cph = cpu_baseline(args.config, args.cpu_res, threads)
+        cres = args.cpu_res if args.cpu_res is not None else CPU_RES.get(args.config, 0.25) * args.res
+        cn, cdt, cph, prec = cpu_baseline_and_precision(ctx, args.config, cres, threads)
+        print(f"[bench] cpu leg done ({cn} samples, {cdt:.1f}s)", file=sys.stderr, flush=True)
         out["cpu_baseline"] = {
             "value": cn / cdt, "unit": "samples/s", "cores": threads, "kind": "port",
-            "sample": f"same scene at resolution scale {args.cpu_res} ({cn} samples), full path "
-                      f"(tree serial as in the reference, OpenMP float occupancy/priors, serial DMC); {model}",
+            "sample": f"same scene at resolution scale {cres:g} ({cn} samples, {cn / n:.4f} of the workload), "
+                      f"full path (tree serial as in the reference, OpenMP float occupancy/priors, serial DMC); "
+                      f"{model}; value extrapolated to the full workload as the sample's samples/s "
+                      f"(linear in samples)",
+            "extrapolated": True,
             "phase_s": {"tree": cph[0], "sampleness": cph[1], "emptiness": cph[2], "priors": cph[3],
                         "occupancy": cph[4], "dmc": cph[5]},
         }
+        out["precision"] = prec
     print(json.dumps(out), flush=True)
 
 

@@ -123,6 +123,11 @@ tsr_status tsr_context_reset_kernel_stats(tsr_context *ctx);
 /* Plain device copy kernel of `bytes` bytes (float4 loads and stores) on the
  * context stream: the reference point for scaling HBM traffic counters. */
 tsr_status tsr_calibration_copy(tsr_context *ctx, uint64_t bytes);
+/* Device memory held by the library in this process (all contexts and handles,
+ * cached blocks included) and its high-water mark; reset_peak != 0 restarts the
+ * mark at the current value after reading it.  No reference counterpart: it
+ * backs the footprint figures of large scenes. */
+tsr_status tsr_device_memory(uint64_t *current_bytes, uint64_t *peak_bytes, int reset_peak);
 
 /* Scene = device copy of samples + cameras (the Scene singleton's data). */
 tsr_status tsr_scene_upload(tsr_context *ctx, const tsr_samples_view *samples, const tsr_cameras_view *cameras,

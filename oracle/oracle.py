@@ -76,8 +76,10 @@ class Oracle:
         self._check(lib().oracle_build_tree(self.h))
         return self
 
-    def occupancy(self, threads=1, brute_force=False):
-        self._check(lib().oracle_occupancy(self.h, (int(threads) << 8) | int(bool(brute_force))))
+    def occupancy(self, threads=1, brute_force=False, fp64=False):
+        """threads > 1: OpenMP float mode (the reference's Real = float), or OpenMP FP64 with fp64=True."""
+        mode = (int(threads) << 8) | int(bool(brute_force)) | (2 if fp64 else 0)
+        self._check(lib().oracle_occupancy(self.h, mode))
         return self
 
     def dmc(self, occupancy=None):

@@ -953,15 +953,18 @@ int oracle_build_tree(void *hp) {
     }
 }
 
-// mode bit0: brute force traversal (no cone culling); bits 8..15: thread count
-// (0/1 = serial FP64 parity mode; >1 = OpenMP float baseline mode)
+// mode bit0: brute force traversal (no cone culling); bit1: FP64 even with
+// threads; bits 8..15: thread count (0/1 = serial FP64 parity mode; >1 =
+// OpenMP float baseline mode, or OpenMP FP64 with bit1 -- the FP64 sums then
+// accumulate in a thread-dependent order, ~1e-15 relative)
 int oracle_occupancy(void *hp, int mode) {
     Handle *h = (Handle *)hp;
     try {
         const int threads = (mode >> 8) & 0xff;
+        const bool fp64 = (mode & 2) != 0;
         const u32 L = (u32)h->t.leafNode.size();
         h->ksum.assign(2 * (size_t)L, 0); h->clen.assign(2 * (size_t)L, 0); h->prior.assign(L, 0);
-        if (threads > 1) {
+        if (threads > 1 && !fp64) {
             delete h->occf;
             h->occf = new OccupancyT<float>(h->s, h->t, h->prm);
             auto &o = *h->occf;
@@ -985,6 +988,7 @@ int oracle_occupancy(void *hp, int mode) {
         delete h->occ;
         h->occ = new OccupancyT<double>(h->s, h->t, h->prm);
         auto &o = *h->occ;
+        o.threads = threads > 1 ? threads : 1;
         o.bruteForce = mode & 1;
         double t0 = now(); o.pass(true);
         double t1 = now(); o.pass(false);

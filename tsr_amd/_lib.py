@@ -93,6 +93,7 @@ SIGNATURES = [
     ("tsr_context_kernel_stats", ctypes.c_int, [vp, ctypes.c_int, f64p, u64p, u64p, u64p]),
     ("tsr_context_reset_kernel_stats", ctypes.c_int, [vp]),
     ("tsr_calibration_copy", ctypes.c_int, [vp, u64]),
+    ("tsr_device_memory", ctypes.c_int, [ctypes.POINTER(u64), ctypes.POINTER(u64), ctypes.c_int]),
     ("tsr_scene_upload", ctypes.c_int, [vp, ctypes.POINTER(SamplesView), ctypes.POINTER(CamerasView), ctypes.POINTER(vp)]),
     ("tsr_scene_destroy", None, [vp]),
     ("tsr_scene_download", ctypes.c_int, [vp, vp, f32p, f32p, f32p, f32p, f32p, f32p, f32p, f32p, u32p]),

@@ -70,6 +70,13 @@ class Context:
         L.check(L.lib().tsr_calibration_copy(self.h, int(nbytes)))
 
 
+def device_memory(reset_peak=False):
+    """(current, peak) device bytes held by the library in this process (tsr_device_memory)."""
+    cur, peak = ctypes.c_uint64(0), ctypes.c_uint64(0)
+    L.check(L.lib().tsr_device_memory(ctypes.byref(cur), ctypes.byref(peak), int(bool(reset_peak))))
+    return cur.value, peak.value
+
+
 class SyntheticScene:
     """Seeded analytic scene (config 0..4) generated on the host (tsr_synthetic_*)."""
 
@@ -183,21 +190,24 @@ class Tree:
         except Exception:
             pass
 
-    def download(self):
-        N, Lc, C, S = self.node_count, self.leaf_count, self.cell_count, self.sample_count
-        o = {
-            "children": np.zeros(N, np.uint32), "parents": np.zeros(N, np.uint32),
-            "sample_counts": np.zeros(N, np.uint32), "sample_starts": np.zeros(N, np.uint32),
-            "node_to_leaf": np.zeros(N, np.uint32), "leaf_nodes": np.zeros(Lc, np.uint32),
-            "leaf_scopes": np.zeros(4 * Lc, np.float32), "cells": np.zeros(8 * C, np.uint32),
-            "sample_target": np.zeros(S, np.uint32), "node_depth": np.zeros(N, np.uint32),
-        }
-        L.check(L.lib().tsr_tree_download(self.ctx.h, self.h, _up(o["children"]), _up(o["parents"]),
-                                          _up(o["sample_counts"]), _up(o["sample_starts"]), _up(o["node_to_leaf"]),
-                                          _up(o["leaf_nodes"]), _fp(o["leaf_scopes"]), _up(o["cells"]),
-                                          _up(o["sample_target"]), _up(o["node_depth"])))
-        return o
+    TABLES = ("children", "parents", "sample_counts", "sample_starts", "node_to_leaf", "leaf_nodes",
+              "leaf_scopes", "cells", "sample_target", "node_depth")
 
+    def download(self, keys=None):
+        """Host copies of the tables; `keys` limits the download to those tables."""
+        N, Lc, C, S = self.node_count, self.leaf_count, self.cell_count, self.sample_count
+        shape = {"children": (N, np.uint32), "parents": (N, np.uint32), "sample_counts": (N, np.uint32),
+                 "sample_starts": (N, np.uint32), "node_to_leaf": (N, np.uint32), "leaf_nodes": (Lc, np.uint32),
+                 "leaf_scopes": (4 * Lc, np.float32), "cells": (8 * C, np.uint32), "sample_target": (S, np.uint32),
+                 "node_depth": (N, np.uint32)}
+        want = self.TABLES if keys is None else tuple(keys)
+        o = {k: np.zeros(shape[k][0], shape[k][1]) for k in want}
+        g = lambda k, f: f(o[k]) if k in o else None  # noqa: E731
+        L.check(L.lib().tsr_tree_download(self.ctx.h, self.h, g("children", _up), g("parents", _up),
+                                          g("sample_counts", _up), g("sample_starts", _up), g("node_to_leaf", _up),
+                                          g("leaf_nodes", _up), g("leaf_scopes", _fp), g("cells", _up),
+                                          g("sample_target", _up), g("node_depth", _up)))
+        return o
 
     def download_samples(self):
         """The samples in node order (Samples::reorder)."""

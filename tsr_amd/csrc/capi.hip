@@ -138,6 +138,14 @@ tsr_status tsr_calibration_copy(tsr_context *ctx, uint64_t bytes) {
     });
 }
 
+tsr_status tsr_device_memory(uint64_t *current_bytes, uint64_t *peak_bytes, int reset_peak) {
+    MemCounter &m = memCounter();
+    if (current_bytes) *current_bytes = m.current.load();
+    if (peak_bytes) *peak_bytes = m.peak.load();
+    if (reset_peak) m.peak.store(m.current.load());
+    return TSR_OK;
+}
+
 tsr_status tsr_scene_upload(tsr_context *ctx, const tsr_samples_view *s, const tsr_cameras_view *c,
                             tsr_scene **out) {
     if (!ctx || !s || !c || !out) return fail(TSR_ERR_INVALID_ARGUMENT, "null argument");

@@ -50,7 +50,15 @@ TSR_HD u64 makeLoc(u32 kx, u32 ky, u32 kz, u32 d) {
     return (1ull << (3 * d)) | spread3(kx) | (spread3(ky) << 1) | (spread3(kz) << 2);
 }
 TSR_HD u32 locDepth(u64 loc) {   // position of the leading marker bit / 3
-    return (63u - (u32)__builtin_clzll(loc)) / 3u;
+    u32 b = 63u - (u32)__builtin_clzll(loc);
+#ifdef __HIP_DEVICE_COMPILE__
+    // Hide the bit position's small range from the optimiser: knowing b < 64 it divides
+    // with 16-bit multiplies and byte selects, and on gfx950 that form gave sporadic wrong
+    // results (whole wavefronts of wrong pre-order keys, a few per million, varying from
+    // run to run); the 32-bit multiply-high form has given none.
+    asm volatile("" : "+v"(b));
+#endif
+    return b / 3u;
 }
 TSR_HD void locCoords(u64 loc, u32 &kx, u32 &ky, u32 &kz) {
     const u32 d = locDepth(loc);

@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <hipcub/hipcub.hpp>
 
+#include <atomic>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -25,6 +26,33 @@ inline void hipCheck(hipError_t e, const char *what) {
     }
 }
 #define HIPCHECK(x) ::tsr::hipCheck((x), #x)
+
+// Device bytes this process holds through the library's buffers (hipMalloc'd blocks,
+// cached ones included) and their high-water mark since the last reset.
+struct MemCounter {
+    std::atomic<size_t> current{0}, peak{0};
+    void add(size_t b) {
+        const size_t now = current.fetch_add(b) + b;
+        size_t p = peak.load();
+        while (now > p && !peak.compare_exchange_weak(p, now)) {
+        }
+    }
+    void sub(size_t b) { current.fetch_sub(b); }
+};
+inline MemCounter &memCounter() {
+    static MemCounter m;
+    return m;
+}
+inline hipError_t trackedMalloc(void **p, size_t bytes) {
+    const hipError_t e = hipMalloc(p, bytes);
+    if (e == hipSuccess) memCounter().add(bytes);
+    return e;
+}
+inline void trackedFree(void *p, size_t bytes) {
+    if (!p) return;
+    (void)hipFree(p);
+    memCounter().sub(bytes);
+}
 
 // Device blocks kept for reuse by one context.  hipFree waits for the whole device and
 // hipMalloc of large blocks maps memory, so the stages' many temporaries would stall the
@@ -65,7 +93,7 @@ class DevicePool {
                 return;
             }
         }
-        (void)hipFree(q);
+        trackedFree(q, blockBytes);
     }
     // free every cached block (device `device_` made current, then restored)
     void trim() {
@@ -74,7 +102,7 @@ class DevicePool {
         int prev = 0;
         (void)hipGetDevice(&prev);
         (void)hipSetDevice(device_);
-        for (auto &kv : cached_) (void)hipFree(kv.second);
+        for (auto &kv : cached_) trackedFree(kv.second, kv.first);
         cached_.clear();
         cachedBytes_ = 0;
         (void)hipSetDevice(prev);
@@ -138,18 +166,18 @@ struct DBuf {
         if (!count) return;
         const std::shared_ptr<DevicePool> &pool = currentPool();
         if (!pool) {
-            HIPCHECK(hipMalloc((void **)&p, sizeof(T) * count));
+            HIPCHECK(trackedMalloc((void **)&p, sizeof(T) * count));
             blockBytes = sizeof(T) * count;
             return;
         }
         const size_t want = DevicePool::roundUp(sizeof(T) * count);
         p = (T *)pool->take(want, blockBytes);
         if (!p) {
-            hipError_t e = hipMalloc((void **)&p, want);
+            hipError_t e = trackedMalloc((void **)&p, want);
             if (e == hipErrorOutOfMemory) {   // give the cached blocks back and try once more
                 (void)hipGetLastError();
                 pool->trim();
-                e = hipMalloc((void **)&p, want);
+                e = trackedMalloc((void **)&p, want);
             }
             if (e != hipSuccess) p = nullptr;
             HIPCHECK(e);
@@ -160,7 +188,7 @@ struct DBuf {
     void release() {
         if (p) {
             if (owner && owner == currentPool()) owner->give(p, blockBytes);
-            else (void)hipFree(p);
+            else trackedFree(p, blockBytes);
         }
         p = nullptr;
         n = 0;
