@@ -210,6 +210,22 @@ def uniform(depth):
     return o
 
 
+def split_tree(codes):
+    """Oracle handle holding the tree over [0,1)^3 whose inner nodes are the given location
+    codes ((1 << 3d) | Morton code; parents before children), with leaves and dual cells."""
+    one = np.ones(1, np.float32)
+    samples = {k: one.copy() * 0.5 for k in ("px", "py", "pz")}
+    samples.update({"nx": one * 0, "ny": one * 0, "nz": one.copy(), "scale": one.copy() * 0.1,
+                    "confidence": one.copy(), "parent_cameras": np.zeros(1, np.uint32)})
+    o = Oracle(samples, {"x": one * 3, "y": one * 0, "z": one * 0})
+    L = lib()
+    L.oracle_split_tree.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64), ctypes.c_uint32]
+    c = np.ascontiguousarray(codes, np.uint64)
+    if L.oracle_split_tree(o.h, c.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), len(c)) != 0:
+        raise OracleError(L.oracle_error(o.h).decode())
+    return o
+
+
 def leaf_centres(o):
     L = lib()
     L.oracle_leaf_centre.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double)]

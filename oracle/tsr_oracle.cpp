@@ -1156,6 +1156,41 @@ int oracle_uniform_tree(void *hp, u32 depth) {
     }
 }
 
+// Replace the handle's tree by the tree over the unit cube whose inner nodes are the given
+// location codes ((1 << 3d) | Morton code, parents listed before children; the root, code
+// 1, is implied) and build leaves + dual cells; used by the dual-cell order known answers.
+int oracle_split_tree(void *hp, const u64 *codes, u32 n) {
+    Handle *h = (Handle *)hp;
+    try {
+        h->t = Tree();
+        Tree &t = h->t;
+        t.rootSizeF = 1.0f; t.rootSize = 1.0;
+        t.newNode(INVALID, 0, 0, 0, 0);
+        t.createChildren(0);
+        for (u32 i = 0; i < n; ++i) {
+            const u64 code = codes[i];
+            u32 d = 0;
+            while ((code >> (3 * (d + 1))) != 0) ++d;
+            u32 nd = 0;
+            for (u32 lvl = 0; lvl < d; ++lvl) {
+                if (t.isLeaf(nd)) throw std::runtime_error("split code below a leaf");
+                nd = t.children[nd] + (u32)((code >> (3 * (d - 1 - lvl))) & 7);
+            }
+            if (t.isLeaf(nd)) t.createChildren(nd);
+        }
+        Builder b(h->s, t);
+        b.reorder();
+        t.nodeToLeaf.assign(t.count(), INVALID);
+        b.gather(0);
+        b.byNode(0);
+        h->timings.assign(6, 0.0);
+        return 0;
+    } catch (const std::exception &e) {
+        h->err = e.what();
+        return 1;
+    }
+}
+
 // leaf centre of leaf l (3 doubles)
 void oracle_leaf_centre(void *hp, u32 l, double *out) {
     Handle *h = (Handle *)hp;
