@@ -184,6 +184,11 @@ void tsr_field_destroy(tsr_field *field);
  * removal / hole filling).  Fails with TSR_ERR_NON_MANIFOLD_EDGE like
  * FlexibleMesh::addEdgeConflict. */
 tsr_status tsr_dmc_extract(tsr_context *ctx, const tsr_tree *tree, const tsr_field *field, tsr_mesh **out);
+/* DualMarchingCells over an explicit per-leaf occupancy (leaf_count doubles in host
+ * memory) instead of a computed field: the crust of any scalar field on the tree's dual
+ * cells (DualMarchingCells.cpp:41-111 with Occupancy::getOccupancy replaced by the given
+ * values).  Fails with TSR_ERR_NON_MANIFOLD_EDGE like tsr_dmc_extract. */
+tsr_status tsr_dmc_extract_values(tsr_context *ctx, const tsr_tree *tree, const double *occupancy, tsr_mesh **out);
 tsr_status tsr_mesh_get_info(const tsr_mesh *mesh, tsr_mesh_info *info);
 /* positions: 3 doubles per vertex; scales: 1 double per vertex; edge_keys: 2
  * leaf indices per vertex (EdgeVertexIndex); triangles: 3 vertex ids each. */

@@ -109,6 +109,7 @@ SIGNATURES = [
     ("tsr_field_culled_leaves", ctypes.c_int, [vp, vp, u8p]),
     ("tsr_field_destroy", None, [vp]),
     ("tsr_dmc_extract", ctypes.c_int, [vp, vp, vp, ctypes.POINTER(vp)]),
+    ("tsr_dmc_extract_values", ctypes.c_int, [vp, vp, f64p, ctypes.POINTER(vp)]),
     ("tsr_mesh_get_info", ctypes.c_int, [vp, ctypes.POINTER(MeshInfo)]),
     ("tsr_mesh_download", ctypes.c_int, [vp, vp, f64p, f64p, u32p, u32p]),
     ("tsr_mesh_destroy", None, [vp]),

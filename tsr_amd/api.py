@@ -306,6 +306,25 @@ class DualMarchingCells:
         return o
 
 
+class _MeshHandle(DualMarchingCells):
+    def __init__(self, ctx, h):   # noqa: super().__init__ is not called: the handle exists
+        self.ctx = ctx
+        self.h = h
+        info = L.MeshInfo()
+        L.check(L.lib().tsr_mesh_get_info(h, ctypes.byref(info)))
+        self.vertex_count, self.triangle_count = info.vertex_count, info.triangle_count
+
+
+def dmc_from_values(ctx, tree, occupancy):
+    """DualMarchingCells over an explicit per-leaf occupancy (tsr_dmc_extract_values)."""
+    o = np.ascontiguousarray(occupancy, np.float64)
+    if len(o) != tree.leaf_count:
+        raise ValueError(f"expected {tree.leaf_count} values, got {len(o)}")
+    h = L.vp()
+    L.check(L.lib().tsr_dmc_extract_values(ctx.h, tree.h, _dp(o), ctypes.byref(h)))
+    return _MeshHandle(ctx, h)
+
+
 MLP_PARAM_COUNT = 64 * 16 + 64 + 64 * 64 + 64 + 64 * 64 + 64 + 64 + 1
 NODE_FEATURES = 16
 

@@ -400,8 +400,30 @@ tsr_status tsr_dmc_extract(tsr_context *ctx, const tsr_tree *tree, const tsr_fie
         m->device = ctx->device;
         try {
             recordPhase(ctx, 5, true);
-            extractDMC(ctx, tree, field, m);
+            extractDMC(ctx, tree, field->occ.p, m);
             recordPhase(ctx, 5, false);
+            HIPCHECK(hipStreamSynchronize(ctx->stream));
+        } catch (...) {
+            delete m;
+            throw;
+        }
+        *out = m;
+    });
+}
+
+tsr_status tsr_dmc_extract_values(tsr_context *ctx, const tsr_tree *tree, const double *occupancy, tsr_mesh **out) {
+    if (!ctx || !tree || !occupancy || !out) return fail(TSR_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    return guarded([&] {
+        activate(ctx);
+        tsr::PoolScope scope(ctx->pool);
+        DBuf<double> occ;
+        upload(occ, occupancy, tree->leafCount, ctx->stream);
+        tsr_mesh *m = new tsr_mesh();
+        m->ctx = ctx;
+        m->device = ctx->device;
+        try {
+            extractDMC(ctx, tree, occ.p, m);
             HIPCHECK(hipStreamSynchronize(ctx->stream));
         } catch (...) {
             delete m;

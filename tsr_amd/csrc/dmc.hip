@@ -13,6 +13,8 @@
 //   manifold   undirected edges used by more than two triangles -> TSR_ERR_NON_MANIFOLD_EDGE
 //              (FlexibleMesh.cpp:1184-1260)
 #include <algorithm>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "handles.h"
@@ -268,58 +270,164 @@ __global__ void k_edgeConflicts(const u64 *__restrict__ sorted, u32 n, u32 *__re
 
 }  // namespace
 
-void extractDMC(tsr_context *ctx, const tsr_tree *tree, const tsr_field *field, tsr_mesh *mesh) {
+namespace {
+
+__global__ void k_zeroU32(u32 *p, u32 n) {
+    const u32 i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+// Device temporaries of one extraction: either blocks from the context's pool (kept in
+// `bufs` until the extraction ends) or consecutive 256-byte aligned pieces of a fixed
+// region [base, base + cap) -- the no-allocation entry point's scratch array.
+struct DmcArena {
+    std::vector<DBuf<unsigned char>> *bufs = nullptr;
+    unsigned char *base = nullptr;
+    size_t cap = 0, used = 0;
+    template <typename T>
+    T *get(size_t n) {
+        const size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+        if (bufs) {
+            bufs->emplace_back(bytes);
+            return (T *)bufs->back().p;
+        }
+        const size_t at = (used + 255) & ~size_t(255);
+        if (at + bytes > cap)
+            throw tsr_error(TSR_ERR_INVALID_ARGUMENT,
+                            "the mesh needs more device scratch than the no-allocation maximum of " +
+                                std::to_string(cap >> 20) + " MiB (use tsr_dmc_extract)");
+        used = at + bytes;
+        return (T *)(base + at);
+    }
+};
+
+template <typename K, typename V>
+void sortPairsIn(DmcArena &ar, const K *kin, K *kout, const V *vin, V *vout, u32 n, int endBit, hipStream_t st) {
+    if (n == 0) return;
+    size_t tmp = 0;
+    HIPCHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, kin, kout, vin, vout, (int)n, 0, endBit, st));
+    void *d = ar.get<unsigned char>(tmp);
+    HIPCHECK(hipcub::DeviceRadixSort::SortPairs(d, tmp, kin, kout, vin, vout, (int)n, 0, endBit, st));
+}
+
+template <typename K>
+void sortKeysIn(DmcArena &ar, const K *kin, K *kout, u32 n, int endBit, hipStream_t st) {
+    if (n == 0) return;
+    size_t tmp = 0;
+    HIPCHECK(hipcub::DeviceRadixSort::SortKeys(nullptr, tmp, kin, kout, (int)n, 0, endBit, st));
+    void *d = ar.get<unsigned char>(tmp);
+    HIPCHECK(hipcub::DeviceRadixSort::SortKeys(d, tmp, kin, kout, (int)n, 0, endBit, st));
+}
+
+template <bool Inclusive>
+void scanIn(DmcArena &ar, const u32 *in, u32 *out, u32 n, hipStream_t st) {
+    if (n == 0) return;
+    size_t tmp = 0;
+    if (Inclusive) HIPCHECK(hipcub::DeviceScan::InclusiveSum(nullptr, tmp, in, out, (int)n, st));
+    else HIPCHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, in, out, (int)n, st));
+    void *d = ar.get<unsigned char>(tmp);
+    if (Inclusive) HIPCHECK(hipcub::DeviceScan::InclusiveSum(d, tmp, in, out, (int)n, st));
+    else HIPCHECK(hipcub::DeviceScan::ExclusiveSum(d, tmp, in, out, (int)n, st));
+}
+
+// Where the mesh goes: the mesh handle's own buffers (allocated once the counts are
+// known), or caller-provided device buffers of stated capacities.
+struct DmcOutput {
+    tsr_mesh *mesh = nullptr;
+    double *pos = nullptr, *scale = nullptr;
+    u32 *keys = nullptr, *tris = nullptr;
+    u32 maxVertices = 0, maxTriangles = 0;
+    u32 vertexCount = 0, triangleCount = 0;
+    void reserveTriangles(u32 T) {
+        triangleCount = T;
+        if (mesh) {
+            mesh->triangleCount = T;
+            mesh->tris.alloc(3 * (size_t)T);
+            tris = mesh->tris.p;
+        } else if (T > maxTriangles) {
+            throw tsr_error(TSR_ERR_INVALID_ARGUMENT, "the mesh has " + std::to_string(T) +
+                                                          " triangles, more than the given capacity " +
+                                                          std::to_string(maxTriangles));
+        }
+    }
+    void reserveVertices(u32 V) {
+        vertexCount = V;
+        if (mesh) {
+            mesh->vertexCount = V;
+            mesh->pos.alloc(3 * (size_t)V);
+            mesh->scale.alloc(V);
+            mesh->keys.alloc(2 * (size_t)V);
+            pos = mesh->pos.p;
+            scale = mesh->scale.p;
+            keys = mesh->keys.p;
+        } else if (V > maxVertices) {
+            throw tsr_error(TSR_ERR_INVALID_ARGUMENT, "the mesh has " + std::to_string(V) +
+                                                          " vertices, more than the given capacity " +
+                                                          std::to_string(maxVertices));
+        }
+    }
+};
+
+}  // namespace
+
+// DualMarchingCells::extractMesh over the per-leaf occupancy `occ` (device, FP64)
+static void dmcCore(tsr_context *ctx, const tsr_tree *tree, const double *occ, DmcArena &ar, DmcOutput &out) {
     hipStream_t st = ctx->stream;
-    Scratch &sc = ctx->scratch;
     const u32 C = tree->cellCount;
-    mesh->vertexCount = 0;
-    mesh->triangleCount = 0;
+    out.reserveTriangles(0);
+    out.reserveVertices(0);
     if (C == 0) return;
-    DBuf<u32> counts(C), offsets(C);
-    k_countTriangles<<<gridFor(C, BLOCK), BLOCK, 0, st>>>(tree->cells.p, field->occ.p, C, counts.p);
-    exclusiveScan(sc, counts.p, offsets.p, C, st);
-    const u64 T64 = (u64)readScalar(offsets.p + C - 1, st) + readScalar(counts.p + C - 1, st);
+    u32 *counts = ar.get<u32>(C), *offsets = ar.get<u32>(C);
+    k_countTriangles<<<gridFor(C, BLOCK), BLOCK, 0, st>>>(tree->cells.p, occ, C, counts);
+    scanIn<false>(ar, counts, offsets, C, st);
+    const u64 T64 = (u64)readScalar(offsets + C - 1, st) + readScalar(counts + C - 1, st);
     if (3 * T64 >= (u64)INVALID) throw tsr_error(TSR_ERR_OUT_OF_MEMORY, "too many triangles");
     const u32 T = (u32)T64;
-    mesh->triangleCount = T;
+    out.reserveTriangles(T);
     if (T == 0) return;
     const u32 n3 = 3 * T;
-    DBuf<u64> cornerKey(n3), keySorted(n3);
-    DBuf<u32> triCell(T), triEdges(T), occ(n3), occSorted(n3), head(n3), runIdx(n3);
-    k_emitTriangles<<<gridFor(C, BLOCK), BLOCK, 0, st>>>(tree->cells.p, field->occ.p, C, offsets.p, cornerKey.p,
-                                                         triCell.p, triEdges.p);
-    k_iota32<<<gridFor(n3, BLOCK), BLOCK, 0, st>>>(occ.p, n3);
-    sortPairs(sc, cornerKey.p, keySorted.p, occ.p, occSorted.p, n3, 0, 64, st);
-    k_runHeads<<<gridFor(n3, BLOCK), BLOCK, 0, st>>>(keySorted.p, n3, head.p);
-    inclusiveScan(sc, head.p, runIdx.p, n3, st);
-    const u32 V = readScalar(runIdx.p + n3 - 1, st);
-    k_minusOne<<<gridFor(n3, BLOCK), BLOCK, 0, st>>>(runIdx.p, n3);
-    DBuf<u32> runFirstOcc(V), runOrder(V), firstOccSorted(V), runSortedByOcc(V), vidOfRun(V);
-    k_runFirst<<<gridFor(n3, BLOCK), BLOCK, 0, st>>>(head.p, runIdx.p, occSorted.p, n3, runFirstOcc.p);
-    k_iota32<<<gridFor(V, BLOCK), BLOCK, 0, st>>>(runOrder.p, V);
-    sortPairs(sc, runFirstOcc.p, firstOccSorted.p, runOrder.p, runSortedByOcc.p, V, 0, 32, st);
-    k_scatterRank<<<gridFor(V, BLOCK), BLOCK, 0, st>>>(runSortedByOcc.p, V, vidOfRun.p);
-    mesh->tris.alloc(n3);
-    k_triIndices<<<gridFor(n3, BLOCK), BLOCK, 0, st>>>(runIdx.p, occSorted.p, vidOfRun.p, n3, mesh->tris.p);
-    mesh->vertexCount = V;
-    mesh->pos.alloc(3 * (size_t)V);
-    mesh->scale.alloc(V);
-    mesh->keys.alloc(2 * (size_t)V);
-    k_vertexData<<<gridFor(V, BLOCK), BLOCK, 0, st>>>(tree->frame, tree->cells.p, field->occ.p, tree->leafLoc.p,
-                                                      firstOccSorted.p, triCell.p, triEdges.p, V, mesh->pos.p,
-                                                      mesh->scale.p, mesh->keys.p);
+    u64 *cornerKey = ar.get<u64>(n3), *keySorted = ar.get<u64>(n3);
+    u32 *triCell = ar.get<u32>(T), *triEdges = ar.get<u32>(T), *occIdx = ar.get<u32>(n3);
+    u32 *occSorted = ar.get<u32>(n3), *head = ar.get<u32>(n3), *runIdx = ar.get<u32>(n3);
+    k_emitTriangles<<<gridFor(C, BLOCK), BLOCK, 0, st>>>(tree->cells.p, occ, C, offsets, cornerKey, triCell,
+                                                         triEdges);
+    k_iota32<<<gridFor(n3, BLOCK), BLOCK, 0, st>>>(occIdx, n3);
+    sortPairsIn(ar, cornerKey, keySorted, occIdx, occSorted, n3, 64, st);
+    k_runHeads<<<gridFor(n3, BLOCK), BLOCK, 0, st>>>(keySorted, n3, head);
+    scanIn<true>(ar, head, runIdx, n3, st);
+    const u32 V = readScalar(runIdx + n3 - 1, st);
+    k_minusOne<<<gridFor(n3, BLOCK), BLOCK, 0, st>>>(runIdx, n3);
+    out.reserveVertices(V);
+    u32 *runFirstOcc = ar.get<u32>(V), *runOrder = ar.get<u32>(V), *firstOccSorted = ar.get<u32>(V);
+    u32 *runSortedByOcc = ar.get<u32>(V), *vidOfRun = ar.get<u32>(V);
+    k_runFirst<<<gridFor(n3, BLOCK), BLOCK, 0, st>>>(head, runIdx, occSorted, n3, runFirstOcc);
+    k_iota32<<<gridFor(V, BLOCK), BLOCK, 0, st>>>(runOrder, V);
+    sortPairsIn(ar, runFirstOcc, firstOccSorted, runOrder, runSortedByOcc, V, 32, st);
+    k_scatterRank<<<gridFor(V, BLOCK), BLOCK, 0, st>>>(runSortedByOcc, V, vidOfRun);
+    k_triIndices<<<gridFor(n3, BLOCK), BLOCK, 0, st>>>(runIdx, occSorted, vidOfRun, n3, out.tris);
+    k_vertexData<<<gridFor(V, BLOCK), BLOCK, 0, st>>>(tree->frame, tree->cells.p, occ, tree->leafLoc.p,
+                                                      firstOccSorted, triCell, triEdges, V, out.pos, out.scale,
+                                                      out.keys);
     // FlexibleMesh::addEdge: an undirected edge may border at most two triangles
-    {
-        DBuf<u64> ek(n3), ekSorted(n3);
-        DBuf<u32> conflict(1);
-        HIPCHECK(hipMemsetAsync(conflict.p, 0, 4, st));
-        k_edgeKeys<<<gridFor(T, BLOCK), BLOCK, 0, st>>>(mesh->tris.p, T, ek.p);
-        sortKeys(sc, ek.p, ekSorted.p, n3, 0, 64, st);
-        k_edgeConflicts<<<gridFor(n3, BLOCK), BLOCK, 0, st>>>(ekSorted.p, n3, conflict.p);
-        if (readScalar(conflict.p, st))
-            throw tsr_error(TSR_ERR_NON_MANIFOLD_EDGE, "BOOM: a DMC edge is shared by more than two triangles");
-    }
+    u64 *ek = cornerKey, *ekSorted = keySorted;   // the corner keys are no longer needed
+    u32 *conflict = ar.get<u32>(1);
+    k_zeroU32<<<1, BLOCK, 0, st>>>(conflict, 1);
+    k_edgeKeys<<<gridFor(T, BLOCK), BLOCK, 0, st>>>(out.tris, T, ek);
+    sortKeysIn(ar, ek, ekSorted, n3, 64, st);
+    k_edgeConflicts<<<gridFor(n3, BLOCK), BLOCK, 0, st>>>(ekSorted, n3, conflict);
+    if (readScalar(conflict, st))
+        throw tsr_error(TSR_ERR_NON_MANIFOLD_EDGE, "BOOM: a DMC edge is shared by more than two triangles");
     HIPCHECK(hipStreamSynchronize(st));
+}
+
+void extractDMC(tsr_context *ctx, const tsr_tree *tree, const double *occ, tsr_mesh *mesh) {
+    std::vector<DBuf<unsigned char>> bufs;
+    bufs.reserve(32);
+    DmcArena ar;
+    ar.bufs = &bufs;
+    DmcOutput out;
+    out.mesh = mesh;
+    dmcCore(ctx, tree, occ, ar, out);
 }
 
 }  // namespace tsr

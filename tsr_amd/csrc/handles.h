@@ -148,7 +148,8 @@ void nodeFeatures(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *scene
 void cullerMask(tsr_context *ctx, const tsr_culler *culler, const tsr_tree *tree, const tsr_scene *scene,
                 const tsr_params &prm, const u32 *states, const u32 *hitsS, const float *ksumS,
                 DBuf<uint8_t> &mask);
-void extractDMC(tsr_context *ctx, const tsr_tree *tree, const tsr_field *field, tsr_mesh *mesh);
+// DualMarchingCells::extractMesh over the per-leaf occupancy `occ` (device, FP64)
+void extractDMC(tsr_context *ctx, const tsr_tree *tree, const double *occ, tsr_mesh *mesh);
 void calibrationCopy(tsr_context *ctx, u64 bytes);
 }  // namespace tsr
 
