@@ -189,6 +189,22 @@ tsr_status tsr_dmc_extract(tsr_context *ctx, const tsr_tree *tree, const tsr_fie
  * cells (DualMarchingCells.cpp:41-111 with Occupancy::getOccupancy replaced by the given
  * values).  Fails with TSR_ERR_NON_MANIFOLD_EDGE like tsr_dmc_extract. */
 tsr_status tsr_dmc_extract_values(tsr_context *ctx, const tsr_tree *tree, const double *occupancy, tsr_mesh **out);
+/* tsr_dmc_extract into caller-provided DEVICE buffers, allocating no device memory.  No
+ * counterpart in the reference: it is for callers that cannot allocate (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ *   positions: 3 * max_vertices doubles, scales: max_vertices doubles, edge_keys:
+ *   2 * max_vertices, triangles: 3 * max_triangles (same layouts as tsr_mesh_download).
+ *   vertex_count / triangle_count (host) receive the mesh size, also when the call fails
+ *   because a capacity is too small (TSR_ERR_INVALID_ARGUMENT; the buffers are then
+ *   partly written).
+ * The temporaries live in one fixed device scratch array of the library of 256 MiB per
+ * GPU (held until the process ends), enough for meshes of about two million triangles; a
+ * mesh that needs more fails with TSR_ERR_INVALID_ARGUMENT.  Every call on a GPU uses that
+ * GPU's array, so calls that overlap on one GPU (several contexts or threads) race. */
+tsr_status tsr_dmc_extract_noalloc(tsr_context *ctx, const tsr_tree *tree, const tsr_field *field,
+                                   uint32_t max_vertices, uint32_t max_triangles, double *positions, double *scales,
+                                   uint32_t *edge_keys, uint32_t *triangles, uint32_t *vertex_count,
+                                   uint32_t *triangle_count);
 tsr_status tsr_mesh_get_info(const tsr_mesh *mesh, tsr_mesh_info *info);
 /* positions: 3 doubles per vertex; scales: 1 double per vertex; edge_keys: 2
  * leaf indices per vertex (EdgeVertexIndex); triangles: 3 vertex ids each. */

@@ -72,6 +72,7 @@ tsr_status tsr_context_create(int device, tsr_context **out) {
         tsr_context *c = new tsr_context();
         c->device = device;
         c->pool = std::make_shared<tsr::DevicePool>(device);
+        (void)tsr::noAllocScratch(device);   // this GPU's copy of the no-allocation scratch array
         HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         for (hipEvent_t &e : c->ev) HIPCHECK(hipEventCreate(&e));
         for (hipEvent_t &e : c->kev) HIPCHECK(hipEventCreate(&e));
@@ -430,6 +431,24 @@ tsr_status tsr_dmc_extract_values(tsr_context *ctx, const tsr_tree *tree, const 
             throw;
         }
         *out = m;
+    });
+}
+
+tsr_status tsr_dmc_extract_noalloc(tsr_context *ctx, const tsr_tree *tree, const tsr_field *field,
+                                   uint32_t max_vertices, uint32_t max_triangles, double *positions, double *scales,
+                                   uint32_t *edge_keys, uint32_t *triangles, uint32_t *vertex_count,
+                                   uint32_t *triangle_count) {
+    if (!ctx || !tree || !field || !vertex_count || !triangle_count)
+        return fail(TSR_ERR_INVALID_ARGUMENT, "null argument");
+    *vertex_count = 0;
+    *triangle_count = 0;
+    if ((max_vertices && (!positions || !scales || !edge_keys)) || (max_triangles && !triangles))
+        return fail(TSR_ERR_INVALID_ARGUMENT, "null output buffer with a non-zero capacity");
+    if (field->leafCount != tree->leafCount) return fail(TSR_ERR_INVALID_ARGUMENT, "field does not belong to the tree");
+    return guarded([&] {
+        activate(ctx);
+        extractDMCNoAlloc(ctx, tree, field->occ.p, max_vertices, max_triangles, positions, scales, edge_keys,
+                          triangles, vertex_count, triangle_count);
     });
 }
 

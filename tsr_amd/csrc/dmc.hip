@@ -420,6 +420,10 @@ static void dmcCore(tsr_context *ctx, const tsr_tree *tree, const double *occ, D
     HIPCHECK(hipStreamSynchronize(st));
 }
 
+// the no-allocation entry point's temporaries: one array per GPU, alive until the process ends
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(256) << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
 void extractDMC(tsr_context *ctx, const tsr_tree *tree, const double *occ, tsr_mesh *mesh) {
     std::vector<DBuf<unsigned char>> bufs;
     bufs.reserve(32);
@@ -428,6 +432,45 @@ void extractDMC(tsr_context *ctx, const tsr_tree *tree, const double *occ, tsr_m
     DmcOutput out;
     out.mesh = mesh;
     dmcCore(ctx, tree, occ, ar, out);
+}
+
+// The device address of the scratch array on the current device, looked up once per device
+// (each GPU holds its own copy of the array).
+void *noAllocScratch(int device) {
+    static std::mutex m;
+    static std::vector<void *> ptrs;
+    std::lock_guard<std::mutex> g(m);
+    if (device < 0) throw tsr_error(TSR_ERR_INVALID_ARGUMENT, "bad device");
+    if ((size_t)device >= ptrs.size()) ptrs.resize(device + 1, nullptr);
+    if (!ptrs[device]) {
+        void *p = nullptr;
+        HIPCHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        ptrs[device] = p;
+    }
+    return ptrs[device];
+}
+
+void extractDMCNoAlloc(tsr_context *ctx, const tsr_tree *tree, const double *occ, u32 maxVertices, u32 maxTriangles,
+                       double *pos, double *scale, u32 *keys, u32 *tris, u32 *vertexCount, u32 *triangleCount) {
+    DmcArena ar;
+    ar.base = (unsigned char *)noAllocScratch(ctx->device);
+    ar.cap = NOALLOC_SCRATCH_BYTES;
+    DmcOutput out;
+    out.pos = pos;
+    out.scale = scale;
+    out.keys = keys;
+    out.tris = tris;
+    out.maxVertices = maxVertices;
+    out.maxTriangles = maxTriangles;
+    try {
+        dmcCore(ctx, tree, occ, ar, out);
+    } catch (...) {
+        *vertexCount = out.vertexCount;
+        *triangleCount = out.triangleCount;
+        throw;
+    }
+    *vertexCount = out.vertexCount;
+    *triangleCount = out.triangleCount;
 }
 
 }  // namespace tsr

@@ -150,6 +150,12 @@ void cullerMask(tsr_context *ctx, const tsr_culler *culler, const tsr_tree *tree
                 DBuf<uint8_t> &mask);
 // DualMarchingCells::extractMesh over the per-leaf occupancy `occ` (device, FP64)
 void extractDMC(tsr_context *ctx, const tsr_tree *tree, const double *occ, tsr_mesh *mesh);
+// the same into caller-provided device buffers, temporaries in the no-allocation scratch
+void extractDMCNoAlloc(tsr_context *ctx, const tsr_tree *tree, const double *occ, u32 maxVertices, u32 maxTriangles,
+                       double *pos, double *scale, u32 *keys, u32 *tris, u32 *vertexCount, u32 *triangleCount);
+// device address of the no-allocation scratch array on `device` (looked up once per device;
+// that device must be current)
+void *noAllocScratch(int device);
 void calibrationCopy(tsr_context *ctx, u64 bytes);
 }  // namespace tsr
 
