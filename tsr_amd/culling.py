@@ -249,7 +249,9 @@ def main(argv=None):
         if worst <= args.max_hausdorff:
             chosen = thr
     if chosen is None:
-        chosen = min(float(t) for t in args.thresholds.split(","))
+        print(f"[culling] no threshold keeps the tuning scenes within {args.max_hausdorff:.1e} of the diagonal; "
+              "no weights written", file=sys.stderr, flush=True)
+        return 1
     L.check(L.lib().tsr_culler_set_weights(ctx.h, culler.h, weights.ctypes.data_as(L.f32p), ctypes_float(chosen)))
     held = [evaluate(ctx, culler, s, args.res) for s in _seeds(args.holdout)]
     summary = {
