@@ -40,7 +40,8 @@ typedef enum tsr_status {
     TSR_ERR_OUT_OF_MEMORY = 9,
     TSR_ERR_ZERO_SIZE_NODE = 10,       /* Nodes.cpp:303-304 node of size 0                          */
     TSR_ERR_SAMPLE_NOT_INSERTED = 11,  /* Nodes.cpp:615-616                                          */
-    TSR_ERR_INVALID_SCALE = 12         /* Samples.cpp:280-288 non-positive sample scale             */
+    TSR_ERR_INVALID_SCALE = 12,        /* Samples.cpp:280-288 non-positive sample scale             */
+    TSR_ERR_FILE = 13                  /* FileCorruptionException: missing, short or invalid file   */
 } tsr_status;
 
 /* Parameters of the path, names as in App.cfg:63-73 (defaults = App.cfg values). */
@@ -160,6 +161,32 @@ tsr_status tsr_tree_samples_download(tsr_context *ctx, const tsr_tree *tree, flo
                                      uint32_t *parent_cameras);
 void tsr_tree_destroy(tsr_tree *tree);
 
+/* Leaves::createNeighborsEdges / getNeighbors (Leaves.cpp:82-180, Leaves.h:36-67): the
+ * face-neighbour graph of the leaves as CSR -- offsets: leaf_count + 1, neighbors: total;
+ * leaf l's neighbours are neighbors[offsets[l] .. offsets[l + 1]) in the reference's order.
+ * Built on the GPU on first request (or read from a leaves file) and kept with the tree. */
+tsr_status tsr_tree_neighbor_count(tsr_context *ctx, tsr_tree *tree, uint64_t *total);
+tsr_status tsr_tree_neighbors_download(tsr_context *ctx, tsr_tree *tree, uint32_t *offsets, uint32_t *neighbors);
+
+/* On-disk formats (Tree::saveToFiles / Tree(const Path&), Nodes.cpp:770-807,
+ * Leaves.cpp:230-292, DualCells.cpp:359-385; Occupancy.cpp:663-766; Samples.cpp:693-765):
+ * the reference's payloads in its order behind a 16-byte header ("TSRF", kind, version 0,
+ * bytes per Real).  NULL paths are skipped.  Loading fails with TSR_ERR_FILE on missing,
+ * short, inconsistent or foreign files.
+ * tsr_tree_load: the scene must hold the tree's samples in node order (as samples files
+ * store them, tsr_tree_samples_save); nodes must be in depth-first child-block order.
+ * tsr_field_save: real_bytes 4 (the reference's Real = float) or 8 (lossless for this
+ * library's FP64 priors). */
+tsr_status tsr_tree_save(tsr_context *ctx, tsr_tree *tree, const char *nodes_path, const char *leaves_path,
+                         const char *cells_path);
+tsr_status tsr_tree_load(tsr_context *ctx, const tsr_scene *scene, const tsr_params *params, const char *nodes_path,
+                         const char *leaves_path, const char *cells_path, tsr_tree **out);
+tsr_status tsr_tree_samples_save(tsr_context *ctx, const tsr_tree *tree, const char *path);
+tsr_status tsr_samples_file_info(const char *path, uint32_t *count, uint32_t *max_cameras_per_sample);
+/* host arrays of count (parent_cameras: count * max_cameras_per_sample); NULL = skip */
+tsr_status tsr_samples_file_read(const char *path, float *px, float *py, float *pz, float *nx, float *ny, float *nz,
+                                 float *scale, float *confidence, uint32_t *parent_cameras);
+
 /* Occupancy::Occupancy(tree): S pass, flag propagation, E pass, priors and the
  * FP64 occupancy ratio.  flags: TSR_ACCUM_* | optional TSR_CULL_LEARNED (then
  * `culler` must be non-NULL). */
@@ -179,6 +206,11 @@ tsr_status tsr_field_cull_counts(const tsr_field *field, uint32_t *culled_nodes,
 /* Per leaf (leaf_count bytes): 1 if learned culling removed it, else 0. */
 tsr_status tsr_field_culled_leaves(tsr_context *ctx, const tsr_field *field, uint8_t *culled);
 void tsr_field_destroy(tsr_field *field);
+/* Occupancy::saveToFile / Occupancy(const Path&) for a tree (see the on-disk formats above);
+ * loading recomputes the occupancy ratio from the loaded sums and priors. */
+tsr_status tsr_field_save(tsr_context *ctx, const tsr_field *field, const char *path, int real_bytes);
+tsr_status tsr_field_load(tsr_context *ctx, const tsr_tree *tree, const tsr_params *params, const char *path,
+                          tsr_field **out);
 
 /* DualMarchingCells::extractMesh on the GPU (raw crust mesh, before isle
  * removal / hole filling).  Fails with TSR_ERR_NON_MANIFOLD_EDGE like

@@ -152,11 +152,20 @@ public:
     const std::vector<uint32> &getNodeToLeafLinks() const { return mNodeToLeaf; }
     const Scope &getScope(uint32 leafIdx) const { return mScopes[leafIdx]; }
     const std::vector<Scope> &getScopes() const { return mScopes; }
+    // Leaves.h:49-55,132-140: the face-neighbour graph (Leaves.cpp:82-180) as CSR
+    const uint32 *getNeighbors(uint32 &size, uint32 leafIdx) const {
+        const uint32 start = mNeighborsOffsets[leafIdx];
+        size = mNeighborsOffsets[leafIdx + 1] - start;
+        return mNeighbors.data() + start;
+    }
+    const uint32 *getNeighborsOffsets() const { return mNeighborsOffsets.data(); }
+    const uint32 *getNeighbors() const { return mNeighbors.data(); }
 
 private:
     friend class Tree;
     std::vector<Scope> mScopes;
     std::vector<uint32> mNodeToLeaf;
+    std::vector<uint32> mNeighborsOffsets, mNeighbors;
 };
 
 // DualCells.h:33-35: 8 leaf indices per interior dual cell, in child-slot order
@@ -171,16 +180,46 @@ private:
     std::vector<uint32> mIndices;
 };
 
-// Tree.h:26-75: Tree::Tree() = octree nodes, leaves and dual cells, built on the GPU
+// Tree.h:26-75: Tree::Tree() = octree nodes, leaves and dual cells, built on the GPU;
+// Tree(const Path&) loads them from <filesBeginning>.nodes / .leaves / .dualCells
 class Tree {
 public:
     explicit Tree(const Scene &scene, const tsr_params *params = nullptr) : mScene(&scene) {
+        setParams(params);
+        check(tsr_tree_build(scene.getContext().get(), scene.get(), &mParams, &mTree));
+        mirror();
+    }
+    // the scene must hold the tree's samples in node order (a samples file of the tree)
+    Tree(const Scene &scene, const std::string &filesBeginning, const tsr_params *params = nullptr) : mScene(&scene) {
+        setParams(params);
+        const std::string n = filesBeginning + ".nodes", l = filesBeginning + ".leaves",
+                          c = filesBeginning + ".dualCells";
+        check(tsr_tree_load(scene.getContext().get(), scene.get(), &mParams, n.c_str(), l.c_str(), c.c_str(), &mTree));
+        mirror();
+    }
+    ~Tree() { tsr_tree_destroy(mTree); }
+    Tree(const Tree &) = delete;
+    Tree &operator=(const Tree &) = delete;
+
+    // Tree.cpp:129-134 (the samples in node order go to <filesBeginning>.samples)
+    void saveToFiles(const std::string &filesBeginning) const {
+        const std::string n = filesBeginning + ".nodes", l = filesBeginning + ".leaves",
+                          c = filesBeginning + ".dualCells", s = filesBeginning + ".samples";
+        tsr_context *ctx = mScene->getContext().get();
+        check(tsr_tree_save(ctx, mTree, n.c_str(), l.c_str(), c.c_str()));
+        check(tsr_tree_samples_save(ctx, mTree, s.c_str()));
+    }
+
+private:
+    void setParams(const tsr_params *params) {
         tsr_params p;
         tsr_params_default(&p);
         if (params) p = *params;
         mParams = p;
-        tsr_context *ctx = scene.getContext().get();
-        check(tsr_tree_build(ctx, scene.get(), &p, &mTree));
+    }
+    void mirror() {
+        tsr_context *ctx = mScene->getContext().get();
+        const Scene &scene = *mScene;
         tsr_tree_info info;
         check(tsr_tree_get_info(mTree, &info));
         mInfo = info;
@@ -207,11 +246,14 @@ public:
         for (uint32 l = 0; l < L; ++l)
             mLeaves.mScopes[l] = Scope(Vector3(scopes[4 * l], scopes[4 * l + 1], scopes[4 * l + 2]), scopes[4 * l + 3],
                                        mLeafNodes[l]);
+        uint64_t edges = 0;   // Leaves.cpp:22-28: the neighbour graph comes with the leaves
+        check(tsr_tree_neighbor_count(ctx, mTree, &edges));
+        mLeaves.mNeighborsOffsets.resize((size_t)L + 1);
+        mLeaves.mNeighbors.resize(edges);
+        check(tsr_tree_neighbors_download(ctx, mTree, mLeaves.mNeighborsOffsets.data(), mLeaves.mNeighbors.data()));
     }
-    ~Tree() { tsr_tree_destroy(mTree); }
-    Tree(const Tree &) = delete;
-    Tree &operator=(const Tree &) = delete;
 
+public:
     const Nodes &getNodes() const { return mNodes; }
     const Leaves &getLeaves() const { return mLeaves; }
     const DualCells &getDualCells() const { return mDualCells; }
@@ -303,12 +345,31 @@ public:
     enum DataType { EMPTINESS, SAMPLENESS, DATA_TYPE_COUNT };
     enum NodeStateFlag { NODE_FLAG_SAMPLENESS = 0x1 << 0, NODE_FLAG_EMPTINESS = 0x1 << 1 };
 
-    explicit Occupancy(const Tree *tree, int flags = TSR_ACCUM_KAHAN_SEGMENTED, const tsr_culler *culler = nullptr)
+    // Priors for every leaf, as Occupancy.cpp:530-568 computes them; TSR_ACCUM_KAHAN_SEGMENTED
+    // alone is the faster form whose priors are NaN where the occupancy ratio does not read them.
+    explicit Occupancy(const Tree *tree, int flags = TSR_ACCUM_KAHAN_SEGMENTED | TSR_PRIORS_ALL_LEAVES,
+                       const tsr_culler *culler = nullptr)
         : mTree(tree) {
         tsr_context *ctx = tree->getScene().getContext().get();
         check(tsr_occupancy_compute(ctx, tree->get(), tree->getScene().get(), &tree->getParams(), flags, culler,
                                     &mField));
-        const uint32 L = tree->getLeaves().getCount(), N = tree->getNodes().getCount();
+        mirror();
+    }
+    // Occupancy(const Path&): Occupancy.cpp:663-720
+    Occupancy(const Tree *tree, const std::string &fileName) : mTree(tree) {
+        check(tsr_field_load(tree->getScene().getContext().get(), tree->get(), &tree->getParams(), fileName.c_str(),
+                             &mField));
+        mirror();
+    }
+    // Occupancy.cpp:722-766 (Real = float unless realBytes = 8)
+    void saveToFile(const std::string &fileName, int realBytes = 4) const {
+        check(tsr_field_save(mTree->getScene().getContext().get(), mField, fileName.c_str(), realBytes));
+    }
+
+private:
+    void mirror() {
+        tsr_context *ctx = mTree->getScene().getContext().get();
+        const uint32 L = mTree->getLeaves().getCount(), N = mTree->getNodes().getCount();
         mKernelSums.resize((size_t)2 * L);
         mConeLengths.resize((size_t)2 * L);
         mPriors.resize(L);
@@ -317,6 +378,8 @@ public:
         check(tsr_field_download(ctx, mField, mKernelSums.data(), mConeLengths.data(), mPriors.data(),
                                  mNodeStates.data(), mOccupancy.data(), &mConeCount));
     }
+
+public:
     ~Occupancy() { tsr_field_destroy(mField); }
     Occupancy(const Occupancy &) = delete;
     Occupancy &operator=(const Occupancy &) = delete;

@@ -137,6 +137,15 @@ class Oracle:
         return {"positions": pos, "scales": self.array("vscale", np.float64, V), "edge_keys": keys,
                 "triangles": self.array("triangles", np.uint32, 3 * T), "conflicts": self.count("conflicts")}
 
+    def leaf_neighbours(self):
+        """Leaves::createNeighborsEdges: (offsets[L + 1], neighbours) CSR of the current tree"""
+        L = lib()
+        L.oracle_leaf_neighbours.restype = ctypes.c_uint64
+        L.oracle_leaf_neighbours.argtypes = [ctypes.c_void_p]
+        m = int(L.oracle_leaf_neighbours(self.h))
+        n = self.count("leaves")
+        return self.array("neighbour_offsets", np.uint32, n + 1), self.array("neighbours", np.uint32, m)
+
     def timings(self):
         return self.array("timings", np.float64, 6)
 

@@ -878,6 +878,46 @@ void extractMesh(const Tree &t, const std::vector<double> &occ, Mesh &m) {
 // ---------------------------------------------------------------------------
 // Handle + C ABI for the test harness
 // ---------------------------------------------------------------------------
+// Leaves.cpp:82-180 createNeighborsEdges: for every leaf (in leaf order) and side 0..5
+// (Nodes.cpp:60-84: -x, +x, -y, +y, -z, +z) the deepest node, at most as deep as the leaf,
+// holding the same-size neighbour cell (getUnresponsibleEdgeNeighbor).  An inner node (finer
+// leaves there) or a same-depth leaf with a smaller node index is skipped; otherwise the pair
+// is an edge, appended to both leaves' lists in this loop order (insertNeighborsEdge).
+void leafNeighbours(const Tree &t, std::vector<u32> &offsets, std::vector<u32> &nbrs) {
+    const u32 L = (u32)t.leafNode.size();
+    struct E { u32 a, b; };
+    std::vector<E> edges;
+    for (u32 l = 0; l < L; ++l) {
+        const u32 nd = t.leafNode[l];
+        const u32 d = t.depth[nd];
+        for (u32 side = 0; side < 6; ++side) {
+            const int axis = (int)(side / 2);
+            const long long step = (side & 1) ? 1 : -1;
+            long long q[3] = {(long long)t.kx[nd], (long long)t.ky[nd], (long long)t.kz[nd]};
+            q[axis] += step;
+            if (q[axis] < 0 || q[axis] >= (1ll << d)) continue;   // outside the root: invalid node
+            u32 n = 0, nd2 = 0;
+            for (;;) {   // Nodes.cpp:145-184 getNode with maxDepth = d
+                if (t.isLeaf(n) || nd2 == d) break;
+                const u32 sh = d - nd2 - 1;
+                const u32 c = (u32)(((q[0] >> sh) & 1) | (((q[1] >> sh) & 1) << 1) | (((q[2] >> sh) & 1) << 2));
+                n = t.children[n] + c;
+                ++nd2;
+            }
+            if (!t.isLeaf(n)) continue;
+            if (nd2 == d && n < nd) continue;
+            edges.push_back({l, t.nodeToLeaf[n]});
+        }
+    }
+    offsets.assign(L + 1, 0);
+    for (const E &e : edges) { ++offsets[e.a]; ++offsets[e.b]; }
+    u32 acc = 0;
+    for (u32 i = 0; i <= L; ++i) { const u32 c = offsets[i]; offsets[i] = acc; acc += c; }
+    nbrs.assign(acc, INVALID);
+    std::vector<u32> fill(offsets.begin(), offsets.end() - 1);
+    for (const E &e : edges) { nbrs[fill[e.a]++] = e.b; nbrs[fill[e.b]++] = e.a; }
+}
+
 struct Handle {
     Samples s;
     Params prm{2, 100, 25, 1, 2};
@@ -891,6 +931,7 @@ struct Handle {
     std::vector<float> leafMin;    // leaf scopes as float (min xyz, size)
     std::vector<u32> leafDepth;
     std::vector<double> ksum, clen, prior;   // exported per-leaf field arrays (double)
+    std::vector<u32> nbrOffsets, nbrs;       // leaf neighbour CSR (on request)
     ~Handle() { delete occ; delete occf; }
 };
 
@@ -1079,7 +1120,16 @@ const void *oracle_data(void *hp, const char *what) {
     if (w == "vkey0") return h->mesh.vkey0.data();
     if (w == "vkey1") return h->mesh.vkey1.data();
     if (w == "triangles") return h->mesh.tri.data();
+    if (w == "neighbour_offsets") return h->nbrOffsets.data();
+    if (w == "neighbours") return h->nbrs.data();
     return nullptr;
+}
+
+// Leaves::createNeighborsEdges on the current tree; returns the edge list length
+u64 oracle_leaf_neighbours(void *hp) {
+    Handle *h = (Handle *)hp;
+    leafNeighbours(h->t, h->nbrOffsets, h->nbrs);
+    return h->nbrs.size();
 }
 
 // MC tables for the table-invariant tests

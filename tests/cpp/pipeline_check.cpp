@@ -2,7 +2,7 @@
 // Tree -> Occupancy(tree) -> extractCrust().getSurface(), on a seeded synthetic
 // scene.  Checks the accessors against each other and prints one JSON line of
 // counts and checksums that the Python binding of the same run must reproduce.
-//   usage: pipeline_check <config> <resolution_scale> [seed]
+//   usage: pipeline_check <config> <resolution_scale> [seed] [files beginning: save + reload]
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -94,6 +94,56 @@ int main(int argc, char **argv) {
                 posSum += o;
             }
         }
+        // all-leaf priors (Occupancy.cpp:530-568): every leaf whose occupancy reads a prior has one
+        uint32 finitePriors = 0;
+        for (uint32 l = 0; l < leaves.getCount(); ++l) {
+            const bool finite = std::isfinite(occupancy.getPriorForEmptiness(l));
+            finitePriors += finite;
+            if (occupancy.getKernelSum(Occupancy::EMPTINESS, l) + occupancy.getKernelSum(Occupancy::SAMPLENESS, l) >=
+                occupancy.getConfidenceThreshold())
+                EXPECT(finite && std::isfinite(occupancy.getObservationPD(l)));
+        }
+        // Leaves::getNeighbors: symmetric lists
+        uint64_t neighbourSum = 0;
+        for (uint32 l = 0; l < leaves.getCount(); ++l) {
+            uint32 n = 0;
+            const uint32 *nb = leaves.getNeighbors(n, l);
+            neighbourSum += n;
+            for (uint32 j = 0; j < n; ++j) {
+                uint32 m = 0;
+                const uint32 *back = leaves.getNeighbors(m, nb[j]);
+                bool found = false;
+                for (uint32 k = 0; k < m; ++k) found |= back[k] == l;
+                EXPECT(found);
+            }
+        }
+        // Tree::saveToFiles -> samples file -> Tree(const Path&); Occupancy::saveToFile -> Occupancy(const Path&)
+        if (argc > 4) {
+            const std::string base = argv[4];
+            tree.saveToFiles(base);
+            occupancy.saveToFile(base + ".occupancy", 8);
+            const std::string sp = base + ".samples";
+            uint32 n = 0, mc = 0;
+            check(tsr_samples_file_info(sp.c_str(), &n, &mc));
+            std::vector<float> a[8];
+            for (auto &v : a) v.resize(n);
+            std::vector<uint32> cams((size_t)n * mc);
+            check(tsr_samples_file_read(sp.c_str(), a[0].data(), a[1].data(), a[2].data(), a[3].data(), a[4].data(),
+                                        a[5].data(), a[6].data(), a[7].data(), cams.data()));
+            tsr_samples_view lv{n, mc, a[0].data(), a[1].data(), a[2].data(), a[3].data(), a[4].data(), a[5].data(),
+                                a[6].data(), a[7].data(), cams.data()};
+            Scene loadedScene(ctx, lv, cv);
+            Tree loaded(loadedScene, base);
+            EXPECT(loaded.getNodes().getCount() == nodes.getCount());
+            EXPECT(loaded.getDualCells().getIndexCount() == cells.getIndexCount());
+            for (uint32 i = 0; i < cells.getIndexCount(); ++i)
+                EXPECT(loaded.getDualCells().getIndices()[i] == cells.getIndices()[i]);
+            for (uint32 nd = 0; nd < nodes.getCount(); ++nd)
+                EXPECT(loaded.getNodes().getChildBlock(nd) == nodes.getChildBlock(nd));
+            Occupancy reloaded(&loaded, base + ".occupancy");
+            for (uint32 l = 0; l < leaves.getCount(); ++l)
+                EXPECT(reloaded.getOccupancy(l) == occupancy.getOccupancy(l));
+        }
         const DualMarchingCells &crust = occupancy.extractCrust(1000);
         EXPECT(occupancy.getCrust() == &crust);
         const FlexibleMesh &mesh = crust.getSurface();
@@ -102,13 +152,15 @@ int main(int argc, char **argv) {
         std::printf("{\"samples\": %u, \"nodes\": %u, \"leaves\": %u, \"cells\": %u, \"max_depth\": %u, "
                     "\"sample_depth_sum\": %llu, \"positive_leaves\": %u, \"positive_sum\": %.17g, "
                     "\"cone_count\": %.17g, \"vertices\": %u, \"triangles\": %u, \"cells_hash\": %llu, "
-                    "\"triangles_hash\": %llu, \"positions_hash\": %llu, \"failures\": %d}\n",
+                    "\"triangles_hash\": %llu, \"positions_hash\": %llu, \"finite_priors\": %u, "
+                    "\"neighbour_entries\": %llu, \"failures\": %d}\n",
                     sv.count, nodes.getCount(), leaves.getCount(), cells.getCellCount(), tree.getMaxDepth(),
                     (unsigned long long)depthSum, positive, posSum, occupancy.getConeCount(), mesh.getVertexCount(),
                     mesh.getTriangleCount(),
                     (unsigned long long)fnv(cells.getIndices(), (size_t)cells.getIndexCount() * 4),
                     (unsigned long long)fnv(mesh.getIndices(), (size_t)mesh.getIndexCount() * 4),
-                    (unsigned long long)fnv(mesh.getPositionsWS(), (size_t)mesh.getVertexCount() * 24), failures);
+                    (unsigned long long)fnv(mesh.getPositionsWS(), (size_t)mesh.getVertexCount() * 24), finitePriors,
+                    (unsigned long long)neighbourSum, failures);
         tsr_synthetic_destroy(syn);
     } catch (const Exception &e) {
         std::fprintf(stderr, "pipeline_check: %s (status %d)\n", e.what(), (int)e.getStatus());

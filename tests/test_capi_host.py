@@ -90,3 +90,22 @@ def test_synthetic_is_seeded_and_has_outliers():
     n = np.stack([a.samples["nx"], a.samples["ny"], a.samples["nz"]], 1)
     facing = np.sum(n * (cams[a.samples["parent_cameras"]] - p), axis=1) > 0
     assert 0.001 < 1.0 - facing.mean() < 0.02
+
+
+def test_samples_file_errors_without_device(tmp_path):
+    """Samples::loadFromFile's FileCorruptionException cases map to TSR_ERR_FILE (host only)."""
+    lib = L.lib()
+    n, mc = ctypes.c_uint32(0), ctypes.c_uint32(0)
+    missing = str(tmp_path / "missing.samples").encode()
+    assert lib.tsr_samples_file_info(missing, ctypes.byref(n), ctypes.byref(mc)) == 13
+    bad = tmp_path / "bad.samples"
+    bad.write_bytes(b"XXXX" + bytes(12))
+    assert lib.tsr_samples_file_info(str(bad).encode(), ctypes.byref(n), ctypes.byref(mc)) == 13
+    assert b"not a file of this kind" in lib.tsr_last_error()
+    short = tmp_path / "short.samples"
+    short.write_bytes(b"TSRF" + np.array([5, 0, 4, 10, 1], np.uint32).tobytes())   # 10 samples promised
+    assert lib.tsr_samples_file_info(str(short).encode(), ctypes.byref(n), ctypes.byref(mc)) == 0
+    assert (n.value, mc.value) == (10, 1)
+    buf = np.zeros(10, np.float32)
+    assert lib.tsr_samples_file_read(str(short).encode(), *([buf.ctypes.data_as(L.f32p)] * 8), None) == 13
+    assert b"too short" in lib.tsr_last_error()

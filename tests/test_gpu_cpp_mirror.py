@@ -30,13 +30,15 @@ def test_cpp_mirror_compiles_and_links():
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("config,res", [(0, 0.5), (3, 0.125)])
-def test_cpp_mirror_matches_python_binding(gpu_ctx, config, res):
-    r = subprocess.run([BIN, str(config), str(res), "0"], capture_output=True, text=True, timeout=120)
+def test_cpp_mirror_matches_python_binding(gpu_ctx, config, res, tmp_path):
+    r = subprocess.run([BIN, str(config), str(res), "0", str(tmp_path / "scene")], capture_output=True, text=True,
+                       timeout=120)
     assert r.returncode == 0, r.stderr
     got = json.loads(r.stdout.strip().splitlines()[-1])
     assert got["failures"] == 0
     syn = tsr_amd.SyntheticScene(config, seed=0, resolution_scale=res)
-    scene, tree, occ, mesh = tsr_amd.reconstruct(gpu_ctx, syn.samples, syn.cameras)
+    scene, tree, occ, mesh = tsr_amd.reconstruct(gpu_ctx, syn.samples, syn.cameras,
+                                                 flags=tsr_amd._lib.TSR_PRIORS_ALL_LEAVES)
     t = tree.download()
     f = occ.download()
     m = mesh.download()
@@ -48,7 +50,8 @@ def test_cpp_mirror_matches_python_binding(gpu_ctx, config, res):
         "max_depth": tree.max_depth, "sample_depth_sum": depth_sum, "positive_leaves": int(len(pos)),
         "cone_count": f["cone_count"], "vertices": mesh.vertex_count, "triangles": mesh.triangle_count,
         "cells_hash": _fnv(t["cells"]), "triangles_hash": _fnv(m["triangles"]),
-        "positions_hash": _fnv(m["positions"]),
+        "positions_hash": _fnv(m["positions"]), "finite_priors": int(np.isfinite(f["priors"]).sum()),
+        "neighbour_entries": int(len(tree.neighbors()[1])),
     }
     for k, v in expect.items():
         assert got[k] == v, k

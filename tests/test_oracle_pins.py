@@ -281,3 +281,62 @@ def test_every_sample_has_reliable_sampling_leaves(cfg, res):
             kside = (kids - p[i]) @ nrm[i] > 0
             ok[positive] = bool(np.any((kd2 <= msd[i] ** 2) & (kside == positive)))
         assert ok[True] and ok[False], f"sample {i}: reliable leaf missing ({ok})"
+
+
+def _expected_neighbour_csr(t):
+    """Leaves.cpp:82-180 derived from geometry alone: leaves A, B are neighbours iff their
+    boxes share part of a face.  The pair is inserted by the finer leaf (the same-size
+    one with the smaller node index when equal) at its loop step (leaf index, side), and
+    each leaf's list is in the order of those steps."""
+    ch = t["children"]
+    leaves = t["leaf_nodes"].astype(np.int64)
+    n = len(ch)
+    depth = np.zeros(n, np.int64)
+    k = np.zeros((n, 3), np.int64)
+    for nd in range(n):
+        c = ch[nd]
+        if c == INVALID:
+            continue
+        for s in range(8):
+            depth[c + s] = depth[nd] + 1
+            k[c + s] = 2 * k[nd] + [(s >> 0) & 1, (s >> 1) & 1, (s >> 2) & 1]
+    D = int(depth.max())
+    d = depth[leaves]
+    lo = k[leaves] << (D - d)[:, None]
+    sz = (1 << (D - d))
+    hi = lo + sz[:, None]
+    L = len(leaves)
+    recs = {i: [] for i in range(L)}
+    for a in range(L):
+        for axis in range(3):
+            o1, o2 = [x for x in range(3) if x != axis]
+            for side_sign in (-1, 1):
+                touch = (hi[:, axis] == lo[a, axis]) if side_sign < 0 else (lo[:, axis] == hi[a, axis])
+                ov = (np.minimum(hi[:, o1], hi[a, o1]) > np.maximum(lo[:, o1], lo[a, o1])) & \
+                     (np.minimum(hi[:, o2], hi[a, o2]) > np.maximum(lo[:, o2], lo[a, o2]))
+                for b in np.flatnonzero(touch & ov):
+                    # a is responsible if it is finer, or the same size with the smaller node index
+                    if sz[a] < sz[b] or (sz[a] == sz[b] and leaves[a] < leaves[b]):
+                        side = 2 * axis + (1 if side_sign > 0 else 0)
+                        key = a * 6 + side
+                        recs[a].append((key, b))
+                        recs[b].append((key, a))
+    offs = [0]
+    nb = []
+    for i in range(L):
+        nb.extend(b for _, b in sorted(recs[i]))
+        offs.append(len(nb))
+    return np.array(offs, np.uint32), np.array(nb, np.uint32)
+
+
+@pytest.mark.parametrize("which", ["sphere", "two-samples"])
+def test_leaf_neighbour_csr_matches_geometry(which):
+    if which == "sphere":
+        s, c = scene(0, 0.125)
+    else:
+        s, c = _scene([[0, 0, 0], [0, 0, 0]], [[1, 0, 0], [1, 0, 0]], [0.05, 0.5])
+    o = Oracle(s, c, PARAMS).build_tree()
+    offs, nb = o.leaf_neighbours()
+    eo, en = _expected_neighbour_csr(o.tree())
+    assert np.array_equal(offs, eo)
+    assert np.array_equal(nb, en)

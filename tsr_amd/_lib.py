@@ -74,6 +74,7 @@ STATUS_NAMES = {
     10: "TSR_ERR_ZERO_SIZE_NODE",
     11: "TSR_ERR_SAMPLE_NOT_INSERTED",
     12: "TSR_ERR_INVALID_SCALE",
+    13: "TSR_ERR_FILE",
 }
 
 TSR_ACCUM_KAHAN_SEGMENTED = 0
@@ -102,6 +103,17 @@ SIGNATURES = [
     ("tsr_tree_download", ctypes.c_int, [vp, vp, u32p, u32p, u32p, u32p, u32p, u32p, f32p, u32p, u32p, u32p]),
     ("tsr_tree_samples_download", ctypes.c_int, [vp, vp, f32p, f32p, f32p, f32p, f32p, f32p, f32p, f32p, u32p]),
     ("tsr_tree_destroy", None, [vp]),
+    ("tsr_tree_neighbor_count", ctypes.c_int, [vp, vp, u64p]),
+    ("tsr_tree_neighbors_download", ctypes.c_int, [vp, vp, u32p, u32p]),
+    ("tsr_tree_save", ctypes.c_int, [vp, vp, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p]),
+    ("tsr_tree_load", ctypes.c_int, [vp, vp, ctypes.POINTER(Params), ctypes.c_char_p, ctypes.c_char_p,
+                                     ctypes.c_char_p, ctypes.POINTER(vp)]),
+    ("tsr_tree_samples_save", ctypes.c_int, [vp, vp, ctypes.c_char_p]),
+    ("tsr_samples_file_info", ctypes.c_int, [ctypes.c_char_p, u32p, u32p]),
+    ("tsr_samples_file_read", ctypes.c_int, [ctypes.c_char_p, f32p, f32p, f32p, f32p, f32p, f32p, f32p, f32p,
+                                             u32p]),
+    ("tsr_field_save", ctypes.c_int, [vp, vp, ctypes.c_char_p, ctypes.c_int]),
+    ("tsr_field_load", ctypes.c_int, [vp, vp, ctypes.POINTER(Params), ctypes.c_char_p, ctypes.POINTER(vp)]),
     ("tsr_occupancy_compute", ctypes.c_int, [vp, vp, vp, ctypes.POINTER(Params), ctypes.c_int, vp, ctypes.POINTER(vp)]),
     ("tsr_field_download", ctypes.c_int, [vp, vp, f32p, f32p, f64p, u32p, f64p, f64p]),
     ("tsr_field_hit_counts", ctypes.c_int, [vp, u64p]),

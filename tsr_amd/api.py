@@ -70,6 +70,18 @@ class Context:
         L.check(L.lib().tsr_calibration_copy(self.h, int(nbytes)))
 
 
+def read_samples_file(path):
+    """Samples::loadFromFile: SoA arrays (dict) and cameras per sample."""
+    n, mc = ctypes.c_uint32(0), ctypes.c_uint32(0)
+    L.check(L.lib().tsr_samples_file_info(str(path).encode(), ctypes.byref(n), ctypes.byref(mc)))
+    out = {k: np.zeros(n.value, np.float32) for k in ("px", "py", "pz", "nx", "ny", "nz", "scale", "confidence")}
+    out["parent_cameras"] = np.zeros(n.value * mc.value, np.uint32)
+    L.check(L.lib().tsr_samples_file_read(str(path).encode(), *[_fp(out[k]) for k in
+                                                                ("px", "py", "pz", "nx", "ny", "nz", "scale",
+                                                                 "confidence")], _up(out["parent_cameras"])))
+    return out, mc.value
+
+
 def device_memory(reset_peak=False):
     """(current, peak) device bytes held by the library in this process (tsr_device_memory)."""
     cur, peak = ctypes.c_uint64(0), ctypes.c_uint64(0)
@@ -209,6 +221,43 @@ class Tree:
                                           g("sample_target", _up), g("node_depth", _up)))
         return o
 
+    def neighbors(self):
+        """Leaves::getNeighbors for all leaves: (offsets[leaf_count + 1], neighbours) CSR."""
+        total = ctypes.c_uint64(0)
+        L.check(L.lib().tsr_tree_neighbor_count(self.ctx.h, self.h, ctypes.byref(total)))
+        offs = np.zeros(self.leaf_count + 1, np.uint32)
+        nb = np.zeros(total.value, np.uint32)
+        L.check(L.lib().tsr_tree_neighbors_download(self.ctx.h, self.h, _up(offs), _up(nb)))
+        return offs, nb
+
+    def save(self, nodes_path=None, leaves_path=None, cells_path=None, samples_path=None):
+        """Tree::saveToFiles (.nodes, .leaves, .dualCells) and the samples in node order."""
+        enc = lambda p: None if p is None else str(p).encode()  # noqa: E731
+        L.check(L.lib().tsr_tree_save(self.ctx.h, self.h, enc(nodes_path), enc(leaves_path), enc(cells_path)))
+        if samples_path is not None:
+            L.check(L.lib().tsr_tree_samples_save(self.ctx.h, self.h, enc(samples_path)))
+
+    @classmethod
+    def load(cls, ctx, scene, nodes_path, leaves_path=None, cells_path=None, params=None):
+        """Tree::Tree(const Path&) over a scene holding the tree's samples in node order."""
+        t = cls.__new__(cls)
+        t.ctx = ctx
+        p = params or L.default_params()
+        h = L.vp()
+        enc = lambda q: None if q is None else str(q).encode()  # noqa: E731
+        L.check(L.lib().tsr_tree_load(ctx.h, scene.h, ctypes.byref(p), enc(nodes_path), enc(leaves_path),
+                                      enc(cells_path), ctypes.byref(h)))
+        t.h = h
+        info = L.TreeInfo()
+        L.check(L.lib().tsr_tree_get_info(h, ctypes.byref(info)))
+        t.node_count, t.leaf_count, t.cell_count = info.node_count, info.leaf_count, info.cell_count
+        t.max_depth = info.max_depth
+        t.root_min = np.array(info.root_min[:], np.float32)
+        t.root_size = np.float32(info.root_size)
+        t.sample_count = scene.count
+        t.max_cams = scene.max_cams
+        return t
+
     def download_samples(self):
         """The samples in node order (Samples::reorder)."""
         n = self.sample_count
@@ -243,6 +292,23 @@ class Occupancy:
             self.close()
         except Exception:
             pass
+
+    def save(self, path, real_bytes=8):
+        """Occupancy::saveToFile; real_bytes 4 is the reference's Real = float layout."""
+        L.check(L.lib().tsr_field_save(self.ctx.h, self.h, str(path).encode(), int(real_bytes)))
+
+    @classmethod
+    def load(cls, ctx, tree, path, params=None):
+        """Occupancy(const Path&) for a tree; the occupancy ratio is recomputed."""
+        o = cls.__new__(cls)
+        o.ctx = ctx
+        p = params or L.default_params()
+        h = L.vp()
+        L.check(L.lib().tsr_field_load(ctx.h, tree.h, ctypes.byref(p), str(path).encode(), ctypes.byref(h)))
+        o.h = h
+        o.leaf_count = tree.leaf_count
+        o.node_count = tree.node_count
+        return o
 
     def cull_counts(self):
         """(nodes, leaves) removed by learned culling (0, 0 without it)"""

@@ -310,6 +310,112 @@ void tsr_tree_destroy(tsr_tree *t) {
     delete t;
 }
 
+tsr_status tsr_tree_neighbor_count(tsr_context *ctx, tsr_tree *tree, uint64_t *total) {
+    if (!ctx || !tree || !total) return fail(TSR_ERR_INVALID_ARGUMENT, "null argument");
+    return guarded([&] {
+        activate(ctx);
+        tsr::PoolScope scope(ctx->pool);
+        leafNeighbours(ctx, tree);
+        *total = tree->nbrCount;
+    });
+}
+
+tsr_status tsr_tree_neighbors_download(tsr_context *ctx, tsr_tree *tree, uint32_t *offsets, uint32_t *neighbors) {
+    if (!ctx || !tree) return fail(TSR_ERR_INVALID_ARGUMENT, "null argument");
+    return guarded([&] {
+        activate(ctx);
+        tsr::PoolScope scope(ctx->pool);
+        leafNeighbours(ctx, tree);
+        download(offsets, tree->nbrOffsets, (size_t)tree->leafCount + 1, ctx->stream);
+        download(neighbors, tree->nbrs, tree->nbrCount, ctx->stream);
+        HIPCHECK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+tsr_status tsr_tree_save(tsr_context *ctx, tsr_tree *tree, const char *nodes_path, const char *leaves_path,
+                         const char *cells_path) {
+    if (!ctx || !tree) return fail(TSR_ERR_INVALID_ARGUMENT, "null argument");
+    return guarded([&] {
+        activate(ctx);
+        tsr::PoolScope scope(ctx->pool);
+        saveTree(ctx, tree, nodes_path, leaves_path, cells_path);
+    });
+}
+
+tsr_status tsr_tree_load(tsr_context *ctx, const tsr_scene *scene, const tsr_params *params, const char *nodes_path,
+                         const char *leaves_path, const char *cells_path, tsr_tree **out) {
+    if (!ctx || !scene || !nodes_path || !out) return fail(TSR_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    tsr_params prm;
+    tsr_params_default(&prm);
+    if (params) prm = *params;
+    return guarded([&] {
+        activate(ctx);
+        tsr::PoolScope scope(ctx->pool);
+        tsr_tree *t = new tsr_tree();
+        t->ctx = ctx;
+        t->device = ctx->device;
+        try {
+            loadTree(ctx, scene, prm, nodes_path, leaves_path, cells_path, t);
+        } catch (...) {
+            delete t;
+            throw;
+        }
+        *out = t;
+    });
+}
+
+tsr_status tsr_tree_samples_save(tsr_context *ctx, const tsr_tree *tree, const char *path) {
+    if (!ctx || !tree || !path) return fail(TSR_ERR_INVALID_ARGUMENT, "null argument");
+    return guarded([&] {
+        activate(ctx);
+        saveTreeSamples(ctx, tree, path);
+    });
+}
+
+tsr_status tsr_samples_file_info(const char *path, uint32_t *count, uint32_t *max_cameras_per_sample) {
+    if (!path || !count || !max_cameras_per_sample) return fail(TSR_ERR_INVALID_ARGUMENT, "null argument");
+    return guarded([&] { samplesFileInfo(path, *count, *max_cameras_per_sample); });
+}
+
+tsr_status tsr_samples_file_read(const char *path, float *px, float *py, float *pz, float *nx, float *ny, float *nz,
+                                 float *scale, float *confidence, uint32_t *parent_cameras) {
+    if (!path) return fail(TSR_ERR_INVALID_ARGUMENT, "null argument");
+    return guarded([&] { samplesFileRead(path, px, py, pz, nx, ny, nz, scale, confidence, parent_cameras); });
+}
+
+tsr_status tsr_field_save(tsr_context *ctx, const tsr_field *field, const char *path, int real_bytes) {
+    if (!ctx || !field || !path) return fail(TSR_ERR_INVALID_ARGUMENT, "null argument");
+    if (real_bytes != 4 && real_bytes != 8) return fail(TSR_ERR_INVALID_ARGUMENT, "real_bytes must be 4 or 8");
+    return guarded([&] {
+        activate(ctx);
+        saveField(ctx, field, path, real_bytes);
+    });
+}
+
+tsr_status tsr_field_load(tsr_context *ctx, const tsr_tree *tree, const tsr_params *params, const char *path,
+                          tsr_field **out) {
+    if (!ctx || !tree || !path || !out) return fail(TSR_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    tsr_params prm;
+    tsr_params_default(&prm);
+    if (params) prm = *params;
+    return guarded([&] {
+        activate(ctx);
+        tsr::PoolScope scope(ctx->pool);
+        tsr_field *f = new tsr_field();
+        f->ctx = ctx;
+        f->device = ctx->device;
+        try {
+            loadField(ctx, tree, prm, path, f);
+        } catch (...) {
+            delete f;
+            throw;
+        }
+        *out = f;
+    });
+}
+
 tsr_status tsr_occupancy_compute(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *scene,
                                  const tsr_params *params, int flags, const tsr_culler *culler, tsr_field **out) {
     if (!ctx || !tree || !scene || !out) return fail(TSR_ERR_INVALID_ARGUMENT, "null argument");

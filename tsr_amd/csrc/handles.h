@@ -81,6 +81,10 @@ struct tsr_tree {
     tsr::DBuf<tsr::u32> cams;
     tsr::DHash nodeHash;               // location code -> node id (all nodes)
     tsr::u64 samplingEvents = 0, samplingCreated = 0, samplingUnits = 0;
+    // leaf face-neighbour graph (Leaves.cpp:82-180), CSR built on request or loaded
+    bool hasNeighbours = false;
+    tsr::DBuf<tsr::u32> nbrOffsets, nbrs;
+    tsr::u64 nbrCount = 0;
 };
 
 struct tsr_field {
@@ -138,6 +142,26 @@ inline void recordPhase(tsr_context *ctx, int phase, bool start) {
     HIPCHECK(hipEventRecord(ctx->ev[2 * phase + (start ? 0 : 1)], ctx->stream));
 }
 void buildTree(tsr_context *ctx, const tsr_scene *scene, const tsr_params &prm, tsr_tree *tree);
+// root scope of a sample set (Tree.cpp:70-101); checks the scales (Samples.cpp:280-288)
+Frame rootFrame(tsr_context *ctx, const float *px, const float *py, const float *pz, const float *scale, u32 n,
+                float maxRel);
+// container location code and depth per sample (device arrays of n)
+void sampleContainers(tsr_context *ctx, const Frame &f, const float *px, const float *py, const float *pz,
+                      const float *scale, u32 n, float maxRel, u64 *loc, uint8_t *depth);
+// FP64 occupancy ratio of every leaf from the field's sums and priors (Occupancy.cpp:570-587)
+void occupancyValues(tsr_context *ctx, const tsr_field *field, double confidenceThreshold);
+// Leaves::createNeighborsEdges on the GPU, kept on the tree (no-op when present)
+void leafNeighbours(tsr_context *ctx, tsr_tree *tree);
+// on-disk formats (storage.hip)
+void saveTree(tsr_context *ctx, tsr_tree *tree, const char *nodesPath, const char *leavesPath, const char *cellsPath);
+void loadTree(tsr_context *ctx, const tsr_scene *scene, const tsr_params &prm, const char *nodesPath,
+              const char *leavesPath, const char *cellsPath, tsr_tree *tree);
+void saveField(tsr_context *ctx, const tsr_field *field, const char *path, int realBytes);
+void loadField(tsr_context *ctx, const tsr_tree *tree, const tsr_params &prm, const char *path, tsr_field *field);
+void saveTreeSamples(tsr_context *ctx, const tsr_tree *tree, const char *path);
+void samplesFileInfo(const char *path, u32 &count, u32 &maxCams);
+void samplesFileRead(const char *path, float *px, float *py, float *pz, float *nx, float *ny, float *nz,
+                     float *scale, float *conf, u32 *cams);
 void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *scene, const tsr_params &prm,
                       int flags, const tsr_culler *culler, tsr_field *field);
 // per-leaf emptiness hit counts of every `stride`-th link (gated by the S flags in `states`)

@@ -1474,6 +1474,16 @@ void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *s
     HIPCHECK(hipStreamSynchronize(st));
 }
 
+void occupancyValues(tsr_context *ctx, const tsr_field *field, double confidenceThreshold) {
+    const u32 L = field->leafCount;
+    if (L == 0) return;
+    tsr_field *fw = const_cast<tsr_field *>(field);
+    if (fw->occ.n < L) fw->occ.alloc(L);
+    k_occupancy<<<gridFor(L, BLOCK), BLOCK, 0, ctx->stream>>>(field->ksum.p, field->prior.p, L, field->coneCount,
+                                                              confidenceThreshold, nullptr, fw->occ.p);
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+}
+
 void emptinessSubsample(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *scene, const tsr_params &prm,
                         const u32 *states, u32 stride, u32 *leafHits) {
     hipStream_t st = ctx->stream;

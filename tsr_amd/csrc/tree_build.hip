@@ -1392,6 +1392,53 @@ void parallelFor(size_t n, const std::function<void(size_t, size_t)> &fn, size_t
     for (auto &t : extra) t.join();
 }
 
+// Tree.cpp:70-101 getRootScope over Samples.cpp:290-298,442-449 (AABB of the samples
+// inflated by their max sampling distance + EPSILON, float arithmetic as Real = float);
+// sample scales are checked first (Samples.cpp:280-288).
+Frame rootFrame(tsr_context *ctx, const float *px, const float *py, const float *pz, const float *scale, u32 n,
+                float maxRel) {
+    hipStream_t st = ctx->stream;
+    if (n == 0) throw tsr_error(TSR_ERR_NO_SAMPLES, "no samples");
+    DBuf<u32> err(1);
+    HIPCHECK(hipMemsetAsync(err.p, 0, 4, st));
+    k_checkScales<<<gridFor(n, BLOCK), BLOCK, 0, st>>>(scale, n, err.p);
+    checkErr(readScalar(err.p, st));
+    const u32 nb = std::min<u32>(1024, gridFor(n, BLOCK));
+    DBuf<float> partial(6 * (size_t)nb);
+    k_aabb<<<nb, BLOCK, 0, st>>>(px, py, pz, scale, maxRel, n, partial.p);
+    std::vector<float> hp(6 * (size_t)nb);
+    HIPCHECK(hipMemcpyAsync(hp.data(), partial.p, partial.bytes(), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    float mn[3] = {3.4028235e38f, 3.4028235e38f, 3.4028235e38f}, mx[3] = {-3.4028235e38f, -3.4028235e38f, -3.4028235e38f};
+    for (u32 b = 0; b < nb; ++b)
+        for (int a = 0; a < 3; ++a) {
+            mn[a] = std::min(mn[a], hp[6 * b + a]);
+            mx[a] = std::max(mx[a], hp[6 * b + 3 + a]);
+        }
+    Frame f{};
+    float c[3], e[3];
+    for (int a = 0; a < 3; ++a) { c[a] = (mn[a] + mx[a]) * 0.5f; e[a] = mx[a] - mn[a]; }
+    const float size = std::max(std::max(e[0], e[1]), e[2]);
+    const float half = size * 0.5f;
+    f.rootSizeF = size;
+    f.rootSize = (double)size;
+    for (int a = 0; a < 3; ++a) {
+        f.rootMinF[a] = c[a] - half;
+        f.rootMin[a] = (double)f.rootMinF[a];
+    }
+    return f;
+}
+
+// container location code and depth of every sample (Nodes.cpp:231-269 findContainer's stop)
+void sampleContainers(tsr_context *ctx, const Frame &f, const float *px, const float *py, const float *pz,
+                      const float *scale, u32 n, float maxRel, u64 *loc, uint8_t *depth) {
+    hipStream_t st = ctx->stream;
+    DBuf<u32> err(1);
+    HIPCHECK(hipMemsetAsync(err.p, 0, 4, st));
+    k_containers<<<gridFor(n, BLOCK), BLOCK, 0, st>>>(f, px, py, pz, scale, maxRel, n, loc, depth, err.p);
+    checkErr(readScalar(err.p, st));
+}
+
 void buildTree(tsr_context *ctx, const tsr_scene *scene, const tsr_params &prm, tsr_tree *tree) {
     hipStream_t st = ctx->stream;
     // TSR_TRACE=1: per-stage wall times on stderr (synchronises after each stage)
@@ -1412,35 +1459,7 @@ void buildTree(tsr_context *ctx, const tsr_scene *scene, const tsr_params &prm, 
     tree->sampleCount = n;
     DBuf<u32> err(1);
     HIPCHECK(hipMemsetAsync(err.p, 0, 4, st));
-    k_checkScales<<<gridFor(n, BLOCK), BLOCK, 0, st>>>(scene->scale.p, n, err.p);
-    checkErr(readScalar(err.p, st));
-
-    // ---- root scope (float, Real = float semantics of the reference)
-    const u32 nb = std::min<u32>(1024, gridFor(n, BLOCK));
-    DBuf<float> partial(6 * (size_t)nb);
-    k_aabb<<<nb, BLOCK, 0, st>>>(scene->px.p, scene->py.p, scene->pz.p, scene->scale.p, maxRel, n, partial.p);
-    std::vector<float> hp(6 * (size_t)nb);
-    HIPCHECK(hipMemcpyAsync(hp.data(), partial.p, partial.bytes(), hipMemcpyDeviceToHost, st));
-    HIPCHECK(hipStreamSynchronize(st));
-    float mn[3] = {3.4028235e38f, 3.4028235e38f, 3.4028235e38f}, mx[3] = {-3.4028235e38f, -3.4028235e38f, -3.4028235e38f};
-    for (u32 b = 0; b < nb; ++b)
-        for (int a = 0; a < 3; ++a) {
-            mn[a] = std::min(mn[a], hp[6 * b + a]);
-            mx[a] = std::max(mx[a], hp[6 * b + 3 + a]);
-        }
-    Frame f{};
-    {
-        float c[3], e[3];
-        for (int a = 0; a < 3; ++a) { c[a] = (mn[a] + mx[a]) * 0.5f; e[a] = mx[a] - mn[a]; }
-        const float size = std::max(std::max(e[0], e[1]), e[2]);
-        const float half = size * 0.5f;
-        f.rootSizeF = size;
-        f.rootSize = (double)size;
-        for (int a = 0; a < 3; ++a) {
-            f.rootMinF[a] = c[a] - half;
-            f.rootMin[a] = (double)f.rootMinF[a];
-        }
-    }
+    const Frame f = rootFrame(ctx, scene->px.p, scene->py.p, scene->pz.p, scene->scale.p, n, maxRel);
     tree->frame = f;
     mark("root scope");
 
