@@ -154,6 +154,9 @@ tsr_status tsr_scene_upload(tsr_context *ctx, const tsr_samples_view *s, const t
     if (s->count == 0) return fail(TSR_ERR_NO_SAMPLES, "no samples");
     if (c->count == 0) return fail(TSR_ERR_NO_SAMPLES, "no cameras");
     if (s->max_cameras_per_sample == 0) return fail(TSR_ERR_INVALID_ARGUMENT, "max_cameras_per_sample must be >= 1");
+    // Samples.h:395-402: camera-to-sample links are indexed by uint32 (checked before any array is read)
+    if ((uint64_t)s->count * s->max_cameras_per_sample >= 0xffffffffull)
+        return fail(TSR_ERR_LINK_OVERFLOW, "Number of traced rays exceeds maximum count. Supported maximum is 2^32-1.");
     if (!s->px || !s->py || !s->pz || !s->nx || !s->ny || !s->nz || !s->scale || !s->confidence || !s->parent_cameras ||
         !c->x || !c->y || !c->z)
         return fail(TSR_ERR_INVALID_ARGUMENT, "null array in view");
