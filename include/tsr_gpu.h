@@ -237,6 +237,15 @@ tsr_status tsr_dmc_extract_noalloc(tsr_context *ctx, const tsr_tree *tree, const
                                    uint32_t max_vertices, uint32_t max_triangles, double *positions, double *scales,
                                    uint32_t *edge_keys, uint32_t *triangles, uint32_t *vertex_count,
                                    uint32_t *triangle_count);
+/* The rest of DualMarchingCells' constructor (DualMarchingCells.cpp:26-39) on a mesh in
+ * place: isle removal (FlexibleMesh::deleteIsolatedGeometry, FlexibleMesh.cpp:1411-1426,
+ * IslesEraser.cpp: vertex components below min_triangle_isle_size vertices go), hole filling
+ * (FlexibleMesh::fillHoles, FlexibleMesh.cpp:1580-1932; new vertices have scale 0 and edge
+ * keys 0xffffffff) and angle-weighted vertex normals (Mesh.cpp:145-202).  An empty crust
+ * after isle removal is left without holes filled or normals, as in the reference. */
+tsr_status tsr_crust_postprocess(tsr_context *ctx, tsr_mesh *mesh, uint32_t min_triangle_isle_size);
+/* normals: 3 doubles per vertex (zeros before tsr_crust_postprocess) */
+tsr_status tsr_mesh_download_normals(tsr_context *ctx, const tsr_mesh *mesh, double *normals);
 tsr_status tsr_mesh_get_info(const tsr_mesh *mesh, tsr_mesh_info *info);
 /* positions: 3 doubles per vertex; scales: 1 double per vertex; edge_keys: 2
  * leaf indices per vertex (EdgeVertexIndex); triangles: 3 vertex ids each. */

@@ -317,17 +317,18 @@ public:
     const double *getScales() const { return mScales.data(); }
     const uint32 *getIndices() const { return mIndices.data(); }
     const uint32 *getEdgeKeys() const { return mEdgeKeys.data(); }
+    const double *getNormals() const { return mNormals.data(); }
 
 private:
     friend class DualMarchingCells;
-    std::vector<double> mPositions, mScales;
+    std::vector<double> mPositions, mScales, mNormals;
     std::vector<uint32> mIndices, mEdgeKeys;
 };
 
 class Occupancy;
 
-// DualMarchingCells.h:34-37 (the raw crust: minTriangleIsleSize is accepted for
-// interface parity; isle removal and hole filling are not part of this library)
+// DualMarchingCells.h:34-37 / DualMarchingCells.cpp:26-39: extractMesh, then isle removal
+// (vertex components below minTriangleIsleSize), hole filling and vertex normals
 class DualMarchingCells {
 public:
     DualMarchingCells(const Occupancy &occupancy, uint32 minTriangleIsleSize);
@@ -432,6 +433,7 @@ inline DualMarchingCells::DualMarchingCells(const Occupancy &occupancy, uint32 m
     tsr_mesh *mesh = nullptr;
     check(tsr_dmc_extract(ctx, tree.get(), occupancy.get(), &mesh));
     std::unique_ptr<tsr_mesh, void (*)(tsr_mesh *)> owner(mesh, tsr_mesh_destroy);
+    check(tsr_crust_postprocess(ctx, mesh, minTriangleIsleSize));
     tsr_mesh_info info;
     check(tsr_mesh_get_info(mesh, &info));
     mSurface.mPositions.resize((size_t)3 * info.vertex_count);
@@ -440,6 +442,8 @@ inline DualMarchingCells::DualMarchingCells(const Occupancy &occupancy, uint32 m
     mSurface.mIndices.resize((size_t)3 * info.triangle_count);
     check(tsr_mesh_download(ctx, mesh, mSurface.mPositions.data(), mSurface.mScales.data(), mSurface.mEdgeKeys.data(),
                             mSurface.mIndices.data()));
+    mSurface.mNormals.resize((size_t)3 * info.vertex_count);
+    check(tsr_mesh_download_normals(ctx, mesh, mSurface.mNormals.data()));
 }
 
 }  // namespace tsrgpu

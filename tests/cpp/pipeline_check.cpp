@@ -148,6 +148,11 @@ int main(int argc, char **argv) {
         EXPECT(occupancy.getCrust() == &crust);
         const FlexibleMesh &mesh = crust.getSurface();
         for (uint32 i = 0; i < mesh.getIndexCount(); ++i) EXPECT(mesh.getIndices()[i] < mesh.getVertexCount());
+        for (uint32 v = 0; v < mesh.getVertexCount(); ++v) {   // unit normals (or zero on collapsed triangles)
+            const double *n = mesh.getNormals() + 3 * (size_t)v;
+            const double l = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+            EXPECT(l == 0.0 || std::fabs(l - 1.0) < 1e-9);
+        }
 
         std::printf("{\"samples\": %u, \"nodes\": %u, \"leaves\": %u, \"cells\": %u, \"max_depth\": %u, "
                     "\"sample_depth_sum\": %llu, \"positive_leaves\": %u, \"positive_sum\": %.17g, "

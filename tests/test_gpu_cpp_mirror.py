@@ -41,7 +41,7 @@ def test_cpp_mirror_matches_python_binding(gpu_ctx, config, res, tmp_path):
                                                  flags=tsr_amd._lib.TSR_PRIORS_ALL_LEAVES)
     t = tree.download()
     f = occ.download()
-    m = mesh.download()
+    m = mesh.postprocess(1000).download()   # DualMarchingCells(occupancy, 1000)
     owner = np.repeat(np.arange(tree.node_count), t["sample_counts"])
     depth_sum = int(t["node_depth"][owner[t["sample_target"]]].sum())
     pos = f["occupancy"][f["occupancy"] > 0]

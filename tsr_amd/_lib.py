@@ -124,6 +124,8 @@ SIGNATURES = [
     ("tsr_dmc_extract_values", ctypes.c_int, [vp, vp, f64p, ctypes.POINTER(vp)]),
     ("tsr_dmc_extract_noalloc", ctypes.c_int, [vp, vp, vp, ctypes.c_uint32, ctypes.c_uint32, vp, vp, vp, vp,
                                                u32p, u32p]),
+    ("tsr_crust_postprocess", ctypes.c_int, [vp, vp, ctypes.c_uint32]),
+    ("tsr_mesh_download_normals", ctypes.c_int, [vp, vp, f64p]),
     ("tsr_mesh_get_info", ctypes.c_int, [vp, ctypes.POINTER(MeshInfo)]),
     ("tsr_mesh_download", ctypes.c_int, [vp, vp, f64p, f64p, u32p, u32p]),
     ("tsr_mesh_destroy", None, [vp]),

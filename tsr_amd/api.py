@@ -363,6 +363,19 @@ class DualMarchingCells:
         except Exception:
             pass
 
+    def postprocess(self, min_triangle_isle_size=1000):
+        """Isle removal, hole filling and vertex normals (the rest of DualMarchingCells' constructor)."""
+        L.check(L.lib().tsr_crust_postprocess(self.ctx.h, self.h, int(min_triangle_isle_size)))
+        info = L.MeshInfo()
+        L.check(L.lib().tsr_mesh_get_info(self.h, ctypes.byref(info)))
+        self.vertex_count, self.triangle_count = info.vertex_count, info.triangle_count
+        return self
+
+    def normals(self):
+        n = np.zeros(3 * self.vertex_count, np.float64)
+        L.check(L.lib().tsr_mesh_download_normals(self.ctx.h, self.h, _dp(n)))
+        return n
+
     def download(self):
         V, T = self.vertex_count, self.triangle_count
         o = {"positions": np.zeros(3 * V, np.float64), "scales": np.zeros(V, np.float64),

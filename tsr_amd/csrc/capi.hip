@@ -561,6 +561,26 @@ tsr_status tsr_dmc_extract_noalloc(tsr_context *ctx, const tsr_tree *tree, const
     });
 }
 
+tsr_status tsr_crust_postprocess(tsr_context *ctx, tsr_mesh *mesh, uint32_t min_triangle_isle_size) {
+    if (!ctx || !mesh) return fail(TSR_ERR_INVALID_ARGUMENT, "null argument");
+    return guarded([&] {
+        activate(ctx);
+        tsr::PoolScope scope(ctx->pool);
+        postprocessCrust(ctx, mesh, min_triangle_isle_size, true, true);
+    });
+}
+
+tsr_status tsr_mesh_download_normals(tsr_context *ctx, const tsr_mesh *m, double *normals) {
+    if (!ctx || !m || !normals) return fail(TSR_ERR_INVALID_ARGUMENT, "null argument");
+    return guarded([&] {
+        activate(ctx);
+        const size_t n = 3 * (size_t)m->vertexCount;
+        if (m->normals.p && m->normals.n >= n) download(normals, m->normals, n, ctx->stream);
+        else std::memset(normals, 0, n * sizeof(double));
+        HIPCHECK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
 tsr_status tsr_mesh_get_info(const tsr_mesh *m, tsr_mesh_info *info) {
     if (!m || !info) return fail(TSR_ERR_INVALID_ARGUMENT, "null argument");
     info->vertex_count = m->vertexCount;

@@ -115,6 +115,7 @@ struct tsr_mesh {
     tsr::DBuf<double> scale;    // 1 per vertex
     tsr::DBuf<tsr::u32> keys;   // 2 per vertex
     tsr::DBuf<tsr::u32> tris;   // 3 per triangle
+    tsr::DBuf<double> normals;  // 3 per vertex once post-processed
 };
 
 namespace tsr {
@@ -181,6 +182,8 @@ void extractDMCNoAlloc(tsr_context *ctx, const tsr_tree *tree, const double *occ
 // that device must be current)
 void *noAllocScratch(int device);
 void calibrationCopy(tsr_context *ctx, u64 bytes);
+// DualMarchingCells.cpp:26-39 after extractMesh: isle removal, hole filling, normals
+void postprocessCrust(tsr_context *ctx, tsr_mesh *mesh, u32 minIsleSize, bool fill, bool normals);
 }  // namespace tsr
 
 
