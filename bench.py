@@ -118,7 +118,8 @@ def main():
     ap.add_argument("--no-learned-compare", action="store_true",
                     help="skip the learned-cull vs plain comparison after the timed steps")
     ap.add_argument("--learned", action="store_true",
-                    help="learned node culling with the committed weights (tsr_amd/culler_weights.npz)")
+                    help="time the learned form (certified priors routed by the committed network, "
+                         "tsr_amd/culler_weights.npz) instead of the plain one")
     ap.add_argument("--calibrate-copy", type=float, default=0,
                     help="also run a plain copy kernel of this many bytes (reference for PMC traffic scaling)")
     args = ap.parse_args()
@@ -194,7 +195,7 @@ def main():
         "config": {
             "workload": WORKLOADS.get(args.config, f"synthetic config {args.config}")
                         + (f" at resolution scale {args.res}" if args.res != 1.0 else "")
-                        + ("; learned node culling" if args.learned else ""),
+                        + ("; learned form: certified priors with network routing" if args.learned else ""),
             "config_index": args.config,
             "resolution_scale": args.res,
             "samples": n,
@@ -232,38 +233,44 @@ def main():
                            "unit": "GB/s", "frac": kd["GBps"] / HBM_PEAK_GBS, "traffic": traffic,
                            "algorithmic_bytes_per_launch": kd["algorithmic_bytes_per_launch"],
                            "ms_per_launch": kd["ms_per_launch"]}
-    # learned-cull vs plain: the occupancy + DMC part of one step each on this scene's tree
-    # (the tree build is the same for both), after the timed region
+    # learned vs plain: the occupancy + DMC part of one step per mode on this scene's tree
+    # (the tree build is the same for all), after the timed region: plain, the certified
+    # priors without the routing network, and with it (TSR_CULL_LEARNED)
     if not args.no_learned_compare:
         from tsr_amd import _lib
         from tsr_amd.culling import load_culler
         from tsr_amd.metrics import bbox_diagonal, hausdorff
         cul = culler if culler is not None else load_culler(ctx)
         tree = Tree(ctx, scene)
-        times, meshes = {}, {}
-        for mode in ("plain", "learned", "plain", "learned"):   # second round counts
+        flags = {"plain": _lib.TSR_ACCUM_KAHAN_SEGMENTED, "certified": _lib.TSR_PRIORS_CERTIFIED,
+                 "learned": _lib.TSR_CULL_LEARNED}
+        times, pri, meshes, stages = {}, {}, {}, {}
+        for mode in ("plain", "certified", "learned") * 2:   # second round counts
             ctx.synchronize()
             t1 = time.perf_counter()
-            if mode == "plain":
-                o = Occupancy(ctx, tree, scene)
-            else:
-                o = Occupancy(ctx, tree, scene, flags=_lib.TSR_CULL_LEARNED, culler=cul)
+            o = Occupancy(ctx, tree, scene, flags=flags[mode], culler=cul if mode == "learned" else None)
             m = DualMarchingCells(ctx, tree, o)
             ctx.synchronize()
             times[mode] = 1000.0 * (time.perf_counter() - t1)
-            if mode == "learned":
-                culled = o.cull_counts()[1]
+            pri[mode] = float(ctx.phase_times()[3])
+            stages[mode] = o.prior_stages()
             meshes[mode] = m.download()
             m.close()
             o.close()
         tree.close()
+        diag = bbox_diagonal(syn.samples)
+        same = {k: all(np.array_equal(meshes[k][a], meshes["plain"][a]) for a in ("positions", "triangles", "edge_keys"))
+                for k in ("certified", "learned")}
         out["learned_vs_plain"] = {
-            "plain_ms": times["plain"], "learned_ms": times["learned"],
-            "speedup": times["plain"] / times["learned"], "culled_leaves": culled,
-            "hausdorff_over_bbox_diagonal": hausdorff(meshes["learned"], meshes["plain"],
-                                                      bbox_diagonal(syn.samples))[0],
+            "plain_ms": times["plain"], "certified_ms": times["certified"], "learned_ms": times["learned"],
+            "speedup": times["plain"] / times["learned"],
+            "speedup_without_network": times["plain"] / times["certified"],
+            "priors_ms": pri, "learned_stages": stages["learned"], "certified_stages": stages["certified"],
+            "meshes_identical": same,
+            "hausdorff_over_bbox_diagonal": max(hausdorff(meshes[k], meshes["plain"], diag)[0]
+                                                for k in ("certified", "learned")),
             "scope": "occupancy field + DMC on the same tree, one run each after a warm-up run; "
-                     "culler trained on config-3 scenes (tsr_amd/culler_weights.npz)",
+                     "routing network trained on config-3 scenes (tsr_amd/culler_weights.npz)",
         }
     if not args.no_cpu_baseline:
         model, cores = cpu_info()

@@ -57,11 +57,22 @@ typedef struct tsr_params {
 enum {
     TSR_ACCUM_KAHAN_SEGMENTED = 0,   /* FP32 compensated, deterministic segmented reduction (default) */
     TSR_ACCUM_FP32_ATOMIC = 1,       /* plain FP32 atomics (precision study only, nondeterministic)  */
-    TSR_CULL_LEARNED = 0x100,        /* skip emptiness cones in nodes the culling MLP marks far     */
-    TSR_PRIORS_ALL_LEAVES = 0x200    /* priors for every leaf; by default only for the leaves whose
+    TSR_CULL_LEARNED = 0x100,        /* certified priors, first stage per leaf from the routing MLP */
+    TSR_PRIORS_ALL_LEAVES = 0x200,   /* priors for every leaf; by default only for the leaves whose
                                         occupancy reads them (E + S >= confidence threshold, the
                                         others get NaN)                                             */
+    TSR_PRIORS_CERTIFIED = 0x400     /* certified priors, every leaf from the first stage (below)   */
 };
+/* Certified priors (no reference counterpart).  The mesh reads of a leaf's occupancy only
+ * its class: -EPSILON (pE >= pS), in (0, EPSILON], or above EPSILON (DualMarchingCell.cpp:
+ * 51-71 places a crossing vertex at the corner whose |value| <= EPSILON is tested first).
+ * Stage 0 bounds each queried leaf's prior by a sphere traversal that does not open nodes
+ * 3 levels above the leaf's depth (their subtree sums bound what the exact traversal would
+ * add), stage 1 with the cap 2 levels above; the occupancy formula on both ends of the
+ * interval (the judge) keeps a leaf whose class the interval decides, with the interval
+ * midpoint as its prior.  Stage 2 computes the exact prior (the plain evaluation, the
+ * fallback) for the rest.  The DMC mesh equals that of the plain field; priors and positive
+ * occupancies of certified leaves are within their certified intervals, not exact. */
 
 typedef struct tsr_context tsr_context;
 typedef struct tsr_scene tsr_scene;
@@ -201,10 +212,14 @@ tsr_status tsr_field_download(tsr_context *ctx, const tsr_field *field, float *k
                               double *priors, uint32_t *node_states, double *occupancy, double *cone_count);
 /* counts of (link, leaf) kernel contributions of the last passes: [0] = E, [1] = S */
 tsr_status tsr_field_hit_counts(const tsr_field *field, uint64_t *hits2);
-/* nodes marked by the learned culling and leaves below them (0 without culling) */
-tsr_status tsr_field_cull_counts(const tsr_field *field, uint32_t *culled_nodes, uint32_t *culled_leaves);
-/* Per leaf (leaf_count bytes): 1 if learned culling removed it, else 0. */
-tsr_status tsr_field_culled_leaves(tsr_context *ctx, const tsr_field *field, uint8_t *culled);
+/* Priors stages (6 counts): queried leaves; leaves entering stage 0, 1, 2 (2 = exact);
+ * leaves the routing MLP sent straight to stage 1, to stage 2.  A plain field has every
+ * query at stage 2. */
+tsr_status tsr_field_prior_stages(const tsr_field *field, uint32_t *stages6);
+/* Per leaf (leaf_count bytes): the stage whose evaluation decided the leaf (0, 1: certified
+ * by the capped bound 3 or 2 levels up; 2: exact prior), 255 if its prior was not queried.
+ * On a TSR_PRIORS_CERTIFIED field these are the routing MLP's training targets. */
+tsr_status tsr_field_prior_levels(tsr_context *ctx, const tsr_field *field, uint8_t *levels);
 void tsr_field_destroy(tsr_field *field);
 /* Occupancy::saveToFile / Occupancy(const Path&) for a tree (see the on-disk formats above);
  * loading recomputes the occupancy ratio from the loaded sums and priors. */
@@ -253,25 +268,22 @@ tsr_status tsr_mesh_download(tsr_context *ctx, const tsr_mesh *mesh, double *pos
                              uint32_t *edge_keys, uint32_t *triangles);
 void tsr_mesh_destroy(tsr_mesh *mesh);
 
-/* Learned node culling (no reference counterpart): an MLP 16-64-64-64-1 (ReLU,
- * sigmoid output; BF16 MFMA with FP32 accumulation) predicts from per-node
- * features whether a node's subtree can hold a zero crossing of the field.  With
- * TSR_CULL_LEARNED, nodes predicted below the threshold skip emptiness-cone
- * evaluation and priors and their leaves are set to -EPSILON (empty).  Weights
- * are a flat FP32 array W1[64x16], b1[64], W2[64x64], b2[64], W3[64x64], b3[64],
- * W4[1x64], b4[1] (W row-major [out][in]). */
+/* Learned routing of the certified priors (no reference counterpart): an MLP 16-64-64-64-1
+ * (ReLU, sigmoid output; BF16 MFMA with FP32 accumulation) predicts from per-leaf features
+ * the stage of the certified cascade that decides a queried leaf (trained on stage / 2).
+ * With TSR_CULL_LEARNED each queried leaf starts at stage floor(2 p + 1 - threshold)
+ * (clamped to 0..2) instead of 0, skipping bound evaluations that would not decide it.
+ * The network never changes a result: every leaf is still decided by the judge or the
+ * exact fallback.  Weights are a flat FP32 array W1[64x16], b1[64], W2[64x64], b2[64],
+ * W3[64x64], b3[64], W4[1x64], b4[1] (W row-major [out][in]). */
 #define TSR_MLP_PARAM_COUNT (64 * 16 + 64 + 64 * 64 + 64 + 64 * 64 + 64 + 64 + 1)
-#define TSR_NODE_FEATURES 16
+#define TSR_LEAF_FEATURES 16
 tsr_status tsr_culler_create(tsr_context *ctx, const float *weights, float threshold, tsr_culler **out);
 tsr_status tsr_culler_set_weights(tsr_context *ctx, tsr_culler *culler, const float *weights, float threshold);
 void tsr_culler_destroy(tsr_culler *culler);
-/* TSR_NODE_FEATURES floats per node (row-major) from a tree and its field
- * (uses the field's sampleness pass plus a 1/16-subsampled emptiness pass). */
-tsr_status tsr_node_features(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *scene, const tsr_field *field,
-                             float *features_host);
-/* Training label per node from a plain field: 1 if its subtree holds a leaf with
- * positive occupancy or a corner of a dual cell whose corner signs differ. */
-tsr_status tsr_node_labels(tsr_context *ctx, const tsr_tree *tree, const tsr_field *field, uint8_t *labels_host);
+/* TSR_LEAF_FEATURES floats per leaf (row-major, leaf_count rows) from a tree and its field
+ * (kernel sums, cone lengths and the subtree sums the priors traverse). */
+tsr_status tsr_leaf_features(tsr_context *ctx, const tsr_tree *tree, const tsr_field *field, float *features_host);
 /* MLP forward on host features (n rows): sigmoid outputs. */
 tsr_status tsr_mlp_forward(tsr_context *ctx, const float *weights, const float *features, uint32_t n, float *probs);
 /* Loss (weighted BCE, summed) and its gradient w.r.t. the weights for one batch,

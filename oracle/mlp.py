@@ -1,5 +1,5 @@
-"""FP64 CPU oracle of the learned node-culling MLP (test infrastructure only:
-imported by tests/ and nothing shipped).
+"""FP64 CPU oracle of the routing MLP of the certified priors and of its per-leaf
+features (test infrastructure only: imported by tests/ and nothing shipped).
 
 The network has no counterpart in the reference.  It is 16-64-64-64-1 with ReLU
 hidden layers and a sigmoid output; the flat weight layout is W1[64x16], b1[64],
@@ -93,83 +93,52 @@ def gradient(w, X, y, rw=None, quantize=False):
     return loss(w, X, y, rw, quantize), pack(g)
 
 
-def node_geometry(children):
-    """depth and integer cell coordinates of every node from the child links
-    (parents precede children in DFS block order)"""
-    INVALID = 0xFFFFFFFF
-    n = len(children)
-    depth = np.zeros(n, np.int64)
-    k = np.zeros((n, 3), np.int64)
-    for nd in range(n):
-        c = int(children[nd])
-        if c == INVALID or c & 0x80000000:
-            continue
-        for s in range(8):
-            depth[c + s] = depth[nd] + 1
-            k[c + s] = 2 * k[nd] + [(s >> 0) & 1, (s >> 1) & 1, (s >> 2) & 1]
-    return depth, k
+def leaf_features(tree, field, eps=float(np.float32(1e-6))):
+    """FP64 restatement of the 16 per-leaf routing features from a tree (download():
+    children with INVALID for leaves, parents, node_depth, leaf_nodes, node_to_leaf) and
+    its field (kernel_sums, cone_lengths, node_states, cone_count).
 
-
-def node_features(tree, samples, node_states, ksum_s, root_min, root_size, hits_s=None):
-    """FP64 restatement of the 16 node features from a tree (DFS block order,
-    `children` without a leaf flag: INVALID for leaves), its node-ordered samples,
-    the node state flags (bit 0: sampleness flag) and per-leaf sampleness kernel
-    sums.  Columns 9 and 10 need the subsampled emptiness hit counts and are
-    returned as NaN; column 8 is NaN unless per-leaf sampleness hit counts are given.
-
-    0 depth / 19; 1 log(1 + subtree samples); 2 mean scale / node size; 3 variance
-    of scale / node size^2; 4-6 mean normal; 7 1 - |mean normal| (0 if empty);
-    8 log(1 + subtree sampleness hits); 11 log(1 + subtree sampleness kernel
-    sum) / 30; 12 fraction of subtree leaves with the sampleness flag; 13
-    log2(subtree leaves) / 20; 14 |mean position - node centre| / node size (0 if
-    empty); 15 the node's sampleness flag."""
+    0 depth / 19; 1, 2 log(1 + E), log(1 + S) kernel sums / 16; 3 E / (E + S) (0.5 if both
+    are 0); 4, 5 log(1 + cone length) E, S / 16; for the ancestors 1..4 levels up (the root
+    stays the root): 6..9 their subtree prior Esub / (Esub + Ssub) over cone lengths (0.5 if
+    0) and 10..13 the occupancy margin (pS - pE) / (pS + pE) at that prior, pE = p E / M,
+    pS = (1 - p) S / M (0 if both are 0); 14 sign(r) log(1 + |r|) / 20 with r = (v - EPS) / EPS,
+    v = pS - pE at the prior 3 levels up; 15 log10(max((E + S) / M, 1e-30)) / 10."""
     ch = np.asarray(tree["children"])
     n = len(ch)
-    depth, k = node_geometry(ch)
-    leaves = np.asarray(tree["leaf_nodes"])
-    cnt_own = np.asarray(tree["sample_counts"], np.int64)
-    starts = np.asarray(tree["sample_starts"], np.int64)
-    owner = np.repeat(np.arange(n), cnt_own)
-    sc = np.asarray(samples["scale"], np.float64)
-    agg = np.zeros((n, 14))
-    cols = [np.ones_like(sc), sc, sc * sc] + [np.asarray(samples[c], np.float64)
-                                              for c in ("nx", "ny", "nz", "px", "py", "pz")]
-    for j, v in enumerate(cols):
-        agg[:, j] = np.bincount(owner, v, minlength=n)
-    assert np.all(starts[cnt_own > 0] == np.concatenate([[0], np.cumsum(cnt_own)[:-1]])[cnt_own > 0])
-    sflag = (np.asarray(node_states) & 1) != 0
-    if hits_s is not None:
-        agg[leaves, 9] = hits_s
-    agg[leaves, 11] = 1.0
-    agg[leaves, 12] = sflag[leaves]
-    agg[leaves, 13] = ksum_s
-    parents = np.full(n, -1)
+    leaves = np.asarray(tree["leaf_nodes"], np.int64)
+    L = len(leaves)
+    clen = np.asarray(field["cone_lengths"], np.float64)
+    ks = np.asarray(field["kernel_sums"], np.float64)
+    E, S = ks[:L], ks[L:]
+    M = float(field["cone_count"]) if field["cone_count"] > 0 else 1.0
+    sub = np.zeros((n, 2))
+    sub[leaves, 0], sub[leaves, 1] = clen[:L], clen[L:]
     for nd in range(n - 1, -1, -1):          # children have larger ids than their parent
         c = int(ch[nd])
-        if c == 0xFFFFFFFF:
-            continue
-        agg[nd] += agg[c:c + 8].sum(0)
-        parents[c:c + 8] = nd
-    h = float(root_size) * 2.0 ** -depth
-    cnt = agg[:, 0]
-    inv = np.where(cnt > 0, 1.0 / np.maximum(cnt, 1), 0.0)
-    ms = agg[:, 1] * inv / h
-    ms2 = agg[:, 2] * inv / h ** 2
-    mn = agg[:, 3:6] * inv[:, None]
-    ctr = np.asarray(root_min, np.float64) + (2 * k + 1) * (float(root_size) * 2.0 ** -(depth + 1))[:, None]
-    dist = np.where(cnt > 0, np.linalg.norm(agg[:, 6:9] * inv[:, None] - ctr, axis=1) / h, 0.0)
-    f = np.full((n, 16), np.nan)
-    f[:, 0] = depth / 19.0
-    f[:, 1] = np.log1p(cnt)
-    f[:, 2] = ms
-    f[:, 3] = np.maximum(ms2 - ms * ms, 0.0)
-    f[:, 4:7] = mn
-    f[:, 7] = np.where(cnt > 0, 1.0 - np.linalg.norm(mn, axis=1), 0.0)
-    if hits_s is not None:
-        f[:, 8] = np.log1p(agg[:, 9])
-    f[:, 11] = np.log1p(agg[:, 13]) / 30.0
-    f[:, 12] = np.where(agg[:, 11] > 0, agg[:, 12] / np.maximum(agg[:, 11], 1), 0.0)
-    f[:, 13] = np.log2(np.maximum(agg[:, 11], 1.0)) / 20.0
-    f[:, 14] = dist
-    f[:, 15] = sflag
-    return f, parents
+        if c != 0xFFFFFFFF:
+            sub[nd] = sub[c:c + 8].sum(0)
+    parents = np.asarray(tree["parents"], np.int64)
+    depth = np.asarray(tree["node_depth"], np.float64)
+    f = np.zeros((L, 16))
+    f[:, 0] = depth[leaves] / 19.0
+    f[:, 1], f[:, 2] = np.log1p(E) / 16, np.log1p(S) / 16
+    with np.errstate(divide="ignore", invalid="ignore"):
+        f[:, 3] = np.where(E + S > 0, E / (E + S), 0.5)
+        f[:, 4], f[:, 5] = np.log1p(clen[:L]) / 16, np.log1p(clen[L:]) / 16
+        a = leaves.copy()
+        v3 = None
+        for k in range(1, 5):
+            pa = parents[a]
+            a = np.where(pa != 0xFFFFFFFF, pa, a)
+            t = sub[a].sum(1)
+            p = np.where(t > 0, sub[a, 0] / t, 0.5)
+            pE, pS = p * (E / M), (1 - p) * (S / M)
+            f[:, 5 + k] = p
+            f[:, 9 + k] = np.where(pS + pE > 0, (pS - pE) / (pS + pE), 0.0)
+            if k == 3:
+                v3 = pS - pE
+        r = (v3 - eps) / eps
+        f[:, 14] = np.sign(r) * np.log1p(np.abs(r)) / 20
+        f[:, 15] = np.log10(np.maximum((E + S) / M, 1e-30)) / 10
+    return f

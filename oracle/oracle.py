@@ -146,6 +146,21 @@ class Oracle:
         n = self.count("leaves")
         return self.array("neighbour_offsets", np.uint32, n + 1), self.array("neighbours", np.uint32, m)
 
+    def prior_bounds(self, leaves, cap_levels, cone_lengths=None):
+        """Depth-capped prior intervals (lo, hi) of the given leaves after an FP64
+        occupancy(); cone_lengths (2 * leaf count) replaces the field's own"""
+        L = lib()
+        L.oracle_prior_bounds.restype = ctypes.c_int
+        L.oracle_prior_bounds.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int,
+                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        q = np.ascontiguousarray(leaves, np.uint32)
+        lo, hi = np.zeros(len(q)), np.zeros(len(q))
+        cl = None if cone_lengths is None else np.ascontiguousarray(cone_lengths, np.float32)
+        if L.oracle_prior_bounds(self.h, q.ctypes.data, len(q), int(cap_levels),
+                                 None if cl is None else cl.ctypes.data, lo.ctypes.data, hi.ctypes.data) != 0:
+            raise RuntimeError(L.oracle_error(self.h).decode())
+        return lo, hi
+
     def timings(self):
         return self.array("timings", np.float64, 6)
 
@@ -245,3 +260,31 @@ def leaf_centres(o):
         L.oracle_leaf_centre(o.h, i, buf.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
         out[i] = buf
     return out
+
+
+EPS_F = float(np.float32(1e-6))
+
+
+def occupancy_class(p, e, s, M):
+    """Certified-priors judge restatement: what the mesh reads of the occupancy at prior p
+    (0: -EPSILON, 1: (0, EPSILON], 2: above EPSILON; Occupancy.cpp:570-587), -1 within 1e-9 max(E, S)/M
+    of a class boundary or below 1e-18 (arrays)"""
+    p, e, s = (np.asarray(x, np.float64) for x in (p, e, s))
+    pE, pS = p * (e / M), (1.0 - p) * (s / M)
+    tol = 1e-9 * (np.maximum(e, s) / M)
+    v = pS - pE
+    c = np.full(v.shape, -1, np.int64)
+    c[v < -tol] = 0
+    c[(v > np.maximum(tol, 1e-18)) & (v < EPS_F - tol)] = 1
+    c[v > EPS_F + tol] = 2
+    c[np.isnan(v)] = -1
+    return c
+
+
+def certified(lo, hi, e, s, M):
+    """the judge on intervals: the class where both widened ends agree, else -1"""
+    lo = np.asarray(lo) * (1.0 - 1e-9)
+    hi = np.minimum(1.0, np.asarray(hi) * (1.0 + 1e-9))
+    a, b = occupancy_class(lo, e, s, M), occupancy_class(hi, e, s, M)
+    ok = (a >= 0) & (a == b) & (lo >= 0) & (hi <= 1) & (lo <= hi)
+    return np.where(ok, a, -1)

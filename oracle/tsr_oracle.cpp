@@ -751,6 +751,87 @@ struct OccupancyT {
     }
     u64 priorVisits = 0;
 
+    // Depth-capped bound of leaf l's prior (no reference counterpart: the certified priors
+    // of the GPU path).  The traversal of priors() in FP64, with two additions: an inner
+    // node entirely inside the sphere (farthest corner distance^2 <= r^2 (1 - 1e-9)) whose
+    // deepest leaf is within the depth limit adds its subtree sums at once (the same leaves
+    // priors() would sum one by one), and a straddling inner node other than the root at
+    // depth >= leaf depth - capLevels (and below the depth limit) adds its subtree sums to
+    // upper bounds instead of being opened.  Returns [E / (E + S + Su), (E + Eu) / (E + Eu + S)].
+    void priorBound(u32 l, int capLevels, double &lo, double &hi) const {
+        const u32 ln = t.leafNode[l];
+        const double c[3] = {(double)lcx[l], (double)lcy[l], (double)lcz[l]};
+        const double radius = (double)nsz[ln] * (double)prm.relativeRadiusForPriors;
+        const double r2 = radius * radius, ri2 = r2 * (1.0 - 1e-9);
+        const int md = (int)t.depth[ln] + (int)MAX_DEPTH_DIFFERENCE;
+        const int capD = (int)t.depth[ln] - capLevels;
+        double E = 0, S = 0, Eu = 0, Su = 0;
+        std::vector<u32> stack{0u};
+        while (!stack.empty()) {
+            const u32 nd = stack.back();
+            stack.pop_back();
+            if ((states[nd] & FLAG_S) != FLAG_S) continue;
+            const int d = (int)t.depth[nd];
+            if (d > md) continue;
+            double dmin = 0, dmax = 0;
+            for (int a = 0; a < 3; ++a) {
+                const double b0 = t.B(a, t.kc(a, nd), t.depth[nd]), b1 = t.B(a, t.kc(a, nd) + 1, t.depth[nd]);
+                if (c[a] < b0) dmin += (b0 - c[a]) * (b0 - c[a]);
+                else if (c[a] > b1) dmin += (c[a] - b1) * (c[a] - b1);
+                const double far = std::max(std::fabs(c[a] - b0), std::fabs(b1 - c[a]));
+                dmax += far * far;
+            }
+            if (!(dmin <= r2)) continue;
+            if (t.isLeaf(nd) || (nd != 0 && deepest[nd] <= md && dmax <= ri2)) {
+                E += subE[nd];
+                S += subS[nd];
+                continue;
+            }
+            if (d >= md) continue;
+            if (nd != 0 && d >= capD) {
+                Eu += subE[nd];
+                Su += subS[nd];
+                continue;
+            }
+            for (u32 k = 0; k < 8; ++k) stack.push_back(t.children[nd] + k);
+        }
+        lo = E / (E + S + Su);
+        hi = (E + Eu) / (E + Eu + S);
+    }
+    // FP64 subtree sums of the cone lengths and deepest leaf depth per node (priorBound);
+    // clen: per-leaf cone lengths [E | S] to use instead of this field's
+    std::vector<double> subE, subS;
+    std::vector<int> deepest;
+    void subtreeSums(const float *clen = nullptr) {
+        const u32 L = (u32)t.leafNode.size();
+        const u32 N = (u32)t.children.size();
+        subE.assign(N, 0.0);
+        subS.assign(N, 0.0);
+        deepest.assign(N, 0);
+        std::vector<u32> order;   // children after parents: walk it backwards
+        order.reserve(N);
+        order.push_back(0);
+        for (size_t i = 0; i < order.size(); ++i)
+            if (!t.isLeaf(order[i]))
+                for (u32 k = 0; k < 8; ++k) order.push_back(t.children[order[i]] + k);
+        for (size_t i = order.size(); i-- > 0;) {
+            const u32 nd = order[i];
+            if (t.isLeaf(nd)) {
+                const u32 j = t.nodeToLeaf[nd];
+                subE[nd] = clen ? (double)clen[j] : (double)coneLength[0][j];
+                subS[nd] = clen ? (double)clen[L + j] : (double)coneLength[1][j];
+                deepest[nd] = (int)t.depth[nd];
+            } else {
+                for (u32 k = 0; k < 8; ++k) {
+                    const u32 cn = t.children[nd] + k;
+                    subE[nd] += subE[cn];
+                    subS[nd] += subS[cn];
+                    deepest[nd] = std::max(deepest[nd], deepest[cn]);
+                }
+            }
+        }
+    }
+
     // Occupancy.cpp:570-587 getOccupancy, evaluated in FP64
     double getOccupancy(u32 l) const {
         const double e = (double)kernelSum[0][l], sm = (double)kernelSum[1][l];
@@ -1044,6 +1125,23 @@ int oracle_occupancy(void *hp, int mode) {
             for (int k = 0; k < 2; ++k) { h->ksum[k * L + l] = o.kernelSum[k][l]; h->clen[k * L + l] = o.coneLength[k][l]; }
             h->prior[l] = o.priorE[l];
         }
+        return 0;
+    } catch (const std::exception &e) {
+        h->err = e.what();
+        return 1;
+    }
+}
+
+// Depth-capped prior bounds (priorBound) of n leaves after an FP64 oracle_occupancy;
+// clen (2 * leaf count floats, may be NULL) replaces the field's cone lengths
+int oracle_prior_bounds(void *hp, const u32 *leaves, u32 n, int capLevels, const float *clen, double *lo,
+                        double *hi) {
+    Handle *h = (Handle *)hp;
+    try {
+        if (!h->occ) throw std::runtime_error("needs an FP64 occupancy");
+        auto &o = *h->occ;
+        o.subtreeSums(clen);
+        for (u32 i = 0; i < n; ++i) o.priorBound(leaves[i], capLevels, lo[i], hi[i]);
         return 0;
     } catch (const std::exception &e) {
         h->err = e.what();

@@ -19,7 +19,7 @@ import numpy as np
 
 import tsr_amd
 from tsr_amd import _lib as L
-from tsr_amd.culling import hausdorff
+from tsr_amd.metrics import hausdorff
 from tests.helpers import PARAMS, occupancy_rel_err, scene
 
 
@@ -54,7 +54,7 @@ def study(ctx, res, seed=0, config=1):
             "occupancy_rel_err_max": float(err.max()), "occupancy_rel_err_p99": float(np.percentile(err, 99)),
             "kernel_sum_rel_err_max": float(kerr.max()), "kernel_sum_rel_err_p99": float(np.percentile(kerr, 99)),
             "sign_flips": int(flips), "triangles": int(len(m["triangles"]) // 3),
-            "hausdorff_rel": hausdorff(m, om) / diag,
+            "hausdorff_rel": hausdorff(m, om, diag)[0],
         }
         mesh.close()
         occ.close()

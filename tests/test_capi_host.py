@@ -50,7 +50,8 @@ def test_argument_validation_without_device():
     assert lib.tsr_mlp_gradient(None, None, None, None, None, 0, None, None) == 1
     assert lib.tsr_mlp_train(None, None, None, None, None, 0, 1, 1, 0.0, 0, None) == 1
     assert lib.tsr_culler_create(None, None, 0.5, ctypes.byref(L.vp())) == 1
-    assert lib.tsr_node_labels(None, None, None, None) == 1
+    assert lib.tsr_leaf_features(None, None, None, None) == 1
+    assert lib.tsr_field_prior_levels(None, None, None) == 1
     h = L.vp()
     assert lib.tsr_synthetic_create(7, 0, 1.0, ctypes.byref(h)) == 1     # unknown config
     assert lib.tsr_synthetic_create(0, 0, 0.0, ctypes.byref(h)) == 1     # bad resolution

@@ -1,64 +1,41 @@
-"""What the trained culler (tsr_amd/culler_weights.npz) gains over the plain
-method on a held-out scene (config 3, seed 1000, 32 cameras at 320x240): the
-emptiness work it removes, what it costs in mesh change, and the wall-clock
-ratio (printed, not asserted: the feature pass costs more than the culled work
-on these scenes)."""
-import time
-
+"""What the trained routing network (tsr_amd/culler_weights.npz) gains on a held-out scene
+(config 3, seed 1000, 32 cameras at 320x240), against the certified cascade without it and
+against the plain evaluation: the stages it routes and the wall-clock ratios (printed; the
+test asserts the learned form is no slower than plain -- on these scenes the network's
+routing gains nothing over the cascade without it, see profiles/culler_report.json).  The mesh of every form is
+asserted equal to the plain one, so the Hausdorff distance is 0, within the 0.1% bound."""
 import numpy as np
 import pytest
 
 import tsr_amd
-from tsr_amd import _lib as L
-from tsr_amd.culling import hausdorff, load_culler
+from tsr_amd.culling import load_culler, run_modes
+from tsr_amd.metrics import bbox_diagonal, hausdorff
 
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.xfail(reason="the node culler moves the mesh by more than 0.1% of the bbox diagonal "
-                          "on held-out scenes (see tsr_amd/culler_report.json)", strict=False)
-def test_trained_culler_on_held_out_scene(gpu_ctx):
+def test_trained_router_on_held_out_scene(gpu_ctx):
     syn = tsr_amd.SyntheticScene(3, seed=1000, resolution_scale=1.0)
     sc = tsr_amd.Scene(gpu_ctx, syn.samples, syn.cameras)
     tree = tsr_amd.Tree(gpu_ctx, sc)
     culler = load_culler(gpu_ctx)
-    runs = {}
-    for mode in ("plain", "learned", "plain", "learned"):   # second round timed warm
-        gpu_ctx.synchronize()
-        t0 = time.perf_counter()
-        if mode == "plain":
-            occ = tsr_amd.Occupancy(gpu_ctx, tree, sc)
-        else:
-            occ = tsr_amd.Occupancy(gpu_ctx, tree, sc, flags=L.TSR_CULL_LEARNED, culler=culler)
-        mesh = tsr_amd.DualMarchingCells(gpu_ctx, tree, occ)
-        gpu_ctx.synchronize()
-        dt = time.perf_counter() - t0
-        if mode in runs:
-            runs[mode][0].close()
-        runs[mode] = (occ, mesh.download(), dt, occ.hit_counts())
-        mesh.close()
-    (po, pm, pdt, ph), (lo, lm, ldt, lh) = runs["plain"], runs["learned"]
-    labels = tsr_amd.node_labels(gpu_ctx, tree, po)
-    leaves = tree.download()["leaf_nodes"]
-    culled = lo.culled_leaves() == 1
-    fp, fl = po.download(), lo.download()
-    missed = int(np.sum(culled & (labels[leaves] == 1)))
-    flips = int(np.sum((fp["occupancy"] > 0) != (fl["occupancy"] > 0)))
-    P = pm["positions"].reshape(-1, 3)
-    diag = float(np.linalg.norm(P.max(0) - P.min(0)))
-    h = hausdorff(pm, lm) / diag
-    print(f"\nculled leaves {culled.sum()} of {tree.leaf_count}, emptiness hits {lh[0]} vs {ph[0]} "
-          f"({1 - lh[0] / ph[0]:.4%} removed), missed surface leaves {missed}, sign flips {flips}, "
-          f"Hausdorff {h:.2e} of the diagonal, plain {1e3 * pdt:.1f} ms, learned {1e3 * ldt:.1f} ms "
-          f"(speed ratio {pdt / ldt:.2f})")
-    assert culled.sum() > 0 and lh[0] < ph[0]
-    # the culled leaves are (almost) all far-empty; the few sign flips come from priors
-    # reading the emptiness cone lengths that culled leaves no longer receive
-    assert missed <= 0.001 * np.sum(labels[leaves] == 1)
-    assert flips <= 1e-4 * tree.leaf_count
-    assert h <= 1e-3   # 0.1 % of the bbox diagonal
-    for o in (po, lo):
-        o.close()
+    r = run_modes(gpu_ctx, tree, sc, culler, reps=3)
+    diag = bbox_diagonal(syn.samples)
+    pm = r["plain"][1]
+    for mode in ("certified", "learned"):
+        m = r[mode][1]
+        for k in ("positions", "triangles", "edge_keys"):
+            assert np.array_equal(m[k], pm[k]), (mode, k)
+        assert hausdorff(m, pm, diag)[0] <= 1e-3
+    st = r["learned"][4]
+    pri = {k: r[k][3][3] for k in r}
+    ms = {k: 1e3 * r[k][2] for k in r}
+    print(f"\nstages with the network {st}, without {r['certified'][4]}; priors ms {pri}; "
+          f"occupancy + DMC ms {ms}; learned vs plain {ms['plain'] / ms['learned']:.2f}x, "
+          f"network vs none (priors) {pri['certified'] / pri['learned']:.2f}x")
+    assert st["stage0"] == st["queries"] - st["routed_stage1"] - st["routed_exact"]
+    assert pri["learned"] < pri["plain"] and pri["certified"] < pri["plain"]
+    assert ms["learned"] <= ms["plain"] * 1.02
     culler.close()
     tree.close()
     sc.close()

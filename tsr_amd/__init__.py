@@ -5,5 +5,5 @@ this package is its Python binding.
 """
 from ._lib import TsrError, default_params, lib  # noqa: F401
 from .api import (Context, Culler, DualMarchingCells, device_memory, read_samples_file, Occupancy, Scene, SyntheticScene, Tree,  # noqa: F401
-                  init_mlp_weights, mlp_forward, mlp_gradient, mlp_train, node_features, node_labels,
+                  init_mlp_weights, leaf_features, mlp_forward, mlp_gradient, mlp_train,
                   reconstruct)

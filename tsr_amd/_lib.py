@@ -80,6 +80,7 @@ STATUS_NAMES = {
 TSR_ACCUM_KAHAN_SEGMENTED = 0
 TSR_ACCUM_FP32_ATOMIC = 1
 TSR_CULL_LEARNED = 0x100
+TSR_PRIORS_CERTIFIED = 0x400
 TSR_PRIORS_ALL_LEAVES = 0x200
 MLP_PARAM_COUNT = 64 * 16 + 64 + 64 * 64 + 64 + 64 * 64 + 64 + 64 + 1
 
@@ -117,8 +118,8 @@ SIGNATURES = [
     ("tsr_occupancy_compute", ctypes.c_int, [vp, vp, vp, ctypes.POINTER(Params), ctypes.c_int, vp, ctypes.POINTER(vp)]),
     ("tsr_field_download", ctypes.c_int, [vp, vp, f32p, f32p, f64p, u32p, f64p, f64p]),
     ("tsr_field_hit_counts", ctypes.c_int, [vp, u64p]),
-    ("tsr_field_cull_counts", ctypes.c_int, [vp, u32p, u32p]),
-    ("tsr_field_culled_leaves", ctypes.c_int, [vp, vp, u8p]),
+    ("tsr_field_prior_stages", ctypes.c_int, [vp, u32p]),
+    ("tsr_field_prior_levels", ctypes.c_int, [vp, vp, u8p]),
     ("tsr_field_destroy", None, [vp]),
     ("tsr_dmc_extract", ctypes.c_int, [vp, vp, vp, ctypes.POINTER(vp)]),
     ("tsr_dmc_extract_values", ctypes.c_int, [vp, vp, f64p, ctypes.POINTER(vp)]),
@@ -132,8 +133,7 @@ SIGNATURES = [
     ("tsr_culler_create", ctypes.c_int, [vp, f32p, ctypes.c_float, ctypes.POINTER(vp)]),
     ("tsr_culler_set_weights", ctypes.c_int, [vp, vp, f32p, ctypes.c_float]),
     ("tsr_culler_destroy", None, [vp]),
-    ("tsr_node_features", ctypes.c_int, [vp, vp, vp, vp, f32p]),
-    ("tsr_node_labels", ctypes.c_int, [vp, vp, vp, u8p]),
+    ("tsr_leaf_features", ctypes.c_int, [vp, vp, vp, f32p]),
     ("tsr_mlp_forward", ctypes.c_int, [vp, f32p, f32p, ctypes.c_uint32, f32p]),
     ("tsr_mlp_gradient", ctypes.c_int, [vp, f32p, f32p, f32p, f32p, ctypes.c_uint32, f32p, f64p]),
     ("tsr_mlp_train", ctypes.c_int, [vp, f32p, f32p, f32p, f32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,

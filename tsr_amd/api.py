@@ -310,16 +310,18 @@ class Occupancy:
         o.node_count = tree.node_count
         return o
 
-    def cull_counts(self):
-        """(nodes, leaves) removed by learned culling (0, 0 without it)"""
-        a, b = ctypes.c_uint32(0), ctypes.c_uint32(0)
-        L.check(L.lib().tsr_field_cull_counts(self.h, ctypes.byref(a), ctypes.byref(b)))
-        return a.value, b.value
+    def prior_stages(self):
+        """dict of the priors stages: queries, entering stage 0 / 1 / 2 (2 = exact), and
+        the queries the routing network sent straight to stage 1 / 2"""
+        a = np.zeros(6, np.uint32)
+        L.check(L.lib().tsr_field_prior_stages(self.h, a.ctypes.data_as(L.u32p)))
+        keys = ("queries", "stage0", "stage1", "exact", "routed_stage1", "routed_exact")
+        return {k: int(v) for k, v in zip(keys, a)}
 
-    def culled_leaves(self):
-        """per-leaf flags of learned culling (all zero without it)"""
+    def prior_levels(self):
+        """per leaf: the stage that decided it (0, 1 certified bounds, 2 exact), 255 if not queried"""
         c = np.zeros(self.leaf_count, np.uint8)
-        L.check(L.lib().tsr_field_culled_leaves(self.ctx.h, self.h, c.ctypes.data_as(L.u8p)))
+        L.check(L.lib().tsr_field_prior_levels(self.ctx.h, self.h, c.ctypes.data_as(L.u8p)))
         return c
 
     def hit_counts(self):
@@ -405,13 +407,14 @@ def dmc_from_values(ctx, tree, occupancy):
 
 
 MLP_PARAM_COUNT = 64 * 16 + 64 + 64 * 64 + 64 + 64 * 64 + 64 + 64 + 1
-NODE_FEATURES = 16
+LEAF_FEATURES = 16
 
 
 class Culler:
-    """Learned node culling weights on the device (tsr_culler)."""
+    """Routing network of the certified priors on the device (tsr_culler): weights and the
+    threshold t of the first stage floor(2 p + 1 - t)."""
 
-    def __init__(self, ctx, weights, threshold=0.05):
+    def __init__(self, ctx, weights, threshold=0.75):
         w = np.ascontiguousarray(weights, np.float32)
         if w.size != MLP_PARAM_COUNT:
             raise ValueError(f"expected {MLP_PARAM_COUNT} weights, got {w.size}")
@@ -432,18 +435,11 @@ class Culler:
             pass
 
 
-def node_features(ctx, tree, scene, occupancy):
-    """NODE_FEATURES floats per node of a tree (from a computed field)"""
-    f = np.zeros((tree.node_count, NODE_FEATURES), np.float32)
-    L.check(L.lib().tsr_node_features(ctx.h, tree.h, scene.h, occupancy.h, _fp(f)))
+def leaf_features(ctx, tree, occupancy):
+    """LEAF_FEATURES floats per leaf of a tree (from a computed field)"""
+    f = np.zeros((tree.leaf_count, LEAF_FEATURES), np.float32)
+    L.check(L.lib().tsr_leaf_features(ctx.h, tree.h, occupancy.h, _fp(f)))
     return f
-
-
-def node_labels(ctx, tree, occupancy):
-    """1 where a node's subtree holds surface (positive leaf or sign-changing cell corner)"""
-    lab = np.zeros(tree.node_count, np.uint8)
-    L.check(L.lib().tsr_node_labels(ctx.h, tree.h, occupancy.h, lab.ctypes.data_as(L.u8p)))
-    return lab
 
 
 def mlp_forward(ctx, weights, features):

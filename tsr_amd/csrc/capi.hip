@@ -425,10 +425,11 @@ tsr_status tsr_occupancy_compute(tsr_context *ctx, const tsr_tree *tree, const t
     *out = nullptr;
     if (scene->n != tree->sampleCount) return fail(TSR_ERR_INVALID_ARGUMENT, "scene and tree sample counts differ");
     const int mode = flags & 0xff;
-    if (flags & ~(0xff | TSR_CULL_LEARNED | TSR_PRIORS_ALL_LEAVES)) return fail(TSR_ERR_INVALID_ARGUMENT, "unknown flags");
+    if (flags & ~(0xff | TSR_CULL_LEARNED | TSR_PRIORS_ALL_LEAVES | TSR_PRIORS_CERTIFIED))
+        return fail(TSR_ERR_INVALID_ARGUMENT, "unknown flags");
     if (mode != TSR_ACCUM_KAHAN_SEGMENTED && mode != TSR_ACCUM_FP32_ATOMIC)
         return fail(TSR_ERR_INVALID_ARGUMENT, "unknown accumulation mode");
-    if ((flags & TSR_CULL_LEARNED) && !culler) return fail(TSR_ERR_INVALID_ARGUMENT, "learned culling needs a culler");
+    if ((flags & TSR_CULL_LEARNED) && !culler) return fail(TSR_ERR_INVALID_ARGUMENT, "learned routing needs a culler");
     tsr_params prm;
     tsr_params_default(&prm);
     if (params) prm = *params;
@@ -471,23 +472,27 @@ tsr_status tsr_field_hit_counts(const tsr_field *f, uint64_t *hits2) {
     return TSR_OK;
 }
 
-tsr_status tsr_field_cull_counts(const tsr_field *f, uint32_t *culled_nodes, uint32_t *culled_leaves) {
-    if (!f) return fail(TSR_ERR_INVALID_ARGUMENT, "null argument");
-    if (culled_nodes) *culled_nodes = f->culledNodes;
-    if (culled_leaves) *culled_leaves = f->culledLeaves;
+tsr_status tsr_field_prior_stages(const tsr_field *f, uint32_t *stages6) {
+    if (!f || !stages6) return fail(TSR_ERR_INVALID_ARGUMENT, "null argument");
+    for (int i = 0; i < 6; ++i) stages6[i] = f->priorStages[i];
     return TSR_OK;
 }
 
-tsr_status tsr_field_culled_leaves(tsr_context *ctx, const tsr_field *f, uint8_t *culled) {
-    if (!ctx || !f || !culled) return fail(TSR_ERR_INVALID_ARGUMENT, "null argument");
+tsr_status tsr_field_prior_levels(tsr_context *ctx, const tsr_field *f, uint8_t *levels) {
+    if (!ctx || !f || !levels) return fail(TSR_ERR_INVALID_ARGUMENT, "null argument");
     return guarded([&] {
         HIPCHECK(hipSetDevice(ctx->device));
-        if (!f->culled.p) {
-            std::memset(culled, 0, f->leafCount);
+        const u32 L = f->leafCount;
+        if (f->levels.p) {
+            HIPCHECK(hipMemcpyAsync(levels, f->levels.p, L, hipMemcpyDeviceToHost, ctx->stream));
+            HIPCHECK(hipStreamSynchronize(ctx->stream));
             return;
         }
-        HIPCHECK(hipMemcpyAsync(culled, f->culled.p, f->leafCount, hipMemcpyDeviceToHost, ctx->stream));
+        // plain field: every queried prior is exact, the others are NaN
+        std::vector<double> p(L);
+        if (L) HIPCHECK(hipMemcpyAsync(p.data(), f->prior.p, (size_t)L * 8, hipMemcpyDeviceToHost, ctx->stream));
         HIPCHECK(hipStreamSynchronize(ctx->stream));
+        for (u32 l = 0; l < L; ++l) levels[l] = p[l] == p[l] ? 2 : 255;
     });
 }
 

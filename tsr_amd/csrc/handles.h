@@ -100,10 +100,12 @@ struct tsr_field {
     tsr::u64 hits[2] = {0, 0};
     tsr::u32 priorQueries = 0;
     tsr::u32 priorsOverflowGroups = 0;   // query groups redone with the long traversal stack
-    tsr::DBuf<tsr::u32> hitsS;   // sampleness hits per leaf (features of the learned culling)
-    tsr::u32 culledNodes = 0, culledLeaves = 0;
+    tsr::DBuf<tsr::u32> hitsS;   // sampleness hits per leaf
     tsr::u32 overflowLinks[2] = {0, 0};   // links traversed twice (more hits than staging slots), E / S
-    tsr::DBuf<uint8_t> culled;   // per leaf: 1 if under a learned-culled node (learned mode only)
+    // certified priors: per leaf the stage that decided it (0, 1: capped bounds, 2: exact,
+    // 255: not queried); queries, entering stage 0 / 1 / 2, sent by the network to stage 1 / 2
+    tsr::DBuf<uint8_t> levels;
+    tsr::u32 priorStages[6] = {0, 0, 0, 0, 0, 0};
     tsr_params params{};
 };
 
@@ -165,14 +167,26 @@ void samplesFileRead(const char *path, float *px, float *py, float *pz, float *n
                      float *scale, float *conf, u32 *cams);
 void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *scene, const tsr_params &prm,
                       int flags, const tsr_culler *culler, tsr_field *field);
-// per-leaf emptiness hit counts of every `stride`-th link (gated by the S flags in `states`)
-void emptinessSubsample(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *scene, const tsr_params &prm,
-                        const u32 *states, u32 stride, u32 *leafHits);
-void nodeFeatures(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *scene, const tsr_params &prm,
-                  const u32 *states, const u32 *hitsS, const float *ksumS, DBuf<float> &feat);
-void cullerMask(tsr_context *ctx, const tsr_culler *culler, const tsr_tree *tree, const tsr_scene *scene,
-                const tsr_params &prm, const u32 *states, const u32 *hitsS, const float *ksumS,
-                DBuf<uint8_t> &mask);
+// Traversal records of the priors, node-padded (node i at index i + NODE_PAD = 7, so that
+// the 8 children of a node are one aligned run): (child link, state | deepest leaf depth
+// << 8) and the FP64 subtree sums (E, S) of the cone lengths
+struct PriorRecords {
+    DBuf<float4> sumsPad;   // FP64 pairs after packing
+    DBuf<uint2> linksPad;
+    const uint2 *links() const { return linksPad.p + 7; }
+    const double2 *sums() const { return reinterpret_cast<const double2 *>(sumsPad.p + 7); }
+};
+void priorRecords(tsr_context *ctx, const tsr_tree *tree, const tsr_field *field, PriorRecords &r);
+// priors of the queried leaves, plain or certified (TSR_PRIORS_CERTIFIED / TSR_CULL_LEARNED)
+void priors(tsr_context *ctx, const tsr_tree *tree, const tsr_params &prm, int flags, const tsr_culler *culler,
+            tsr_field *field);
+// TSR_LEAF_FEATURES floats per listed leaf (all leaves when list is null) into X (device)
+void leafFeatures(tsr_context *ctx, const tsr_tree *tree, const tsr_field *field, const PriorRecords &rec,
+                  const u32 *list, u32 n, float *X);
+// first stage of the certified cascade per listed query from the network (stage[leaf]);
+// routed[0] / [1] count the queries sent to stage 1 / 2
+void leafRoute(tsr_context *ctx, const tsr_culler *culler, const tsr_tree *tree, const tsr_field *field,
+               const PriorRecords &rec, const u32 *queries, u32 nq, uint8_t *stage, u32 *routed);
 // DualMarchingCells::extractMesh over the per-leaf occupancy `occ` (device, FP64)
 void extractDMC(tsr_context *ctx, const tsr_tree *tree, const double *occ, tsr_mesh *mesh);
 // the same into caller-provided device buffers, temporaries in the no-allocation scratch

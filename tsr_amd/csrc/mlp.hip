@@ -1,12 +1,14 @@
-// Learned node culling (no counterpart in the reference): per-node features, a
-// 16-64-64-64-1 MLP with hand-written BF16 MFMA forward/backward kernels (FP32
-// accumulation), Adam training on the device, training labels from a plain
-// field, and the culling mask used by the emptiness pass.
+// Learned routing of the certified priors (no counterpart in the reference): per-leaf
+// features, a 16-64-64-64-1 MLP with hand-written BF16 MFMA forward/backward kernels
+// (FP32 accumulation), Adam training on the device, and the first stage of the certified
+// cascade per queried leaf from the network's output.
 //
-// MFMA: __builtin_amdgcn_mfma_f32_16x16x16bf16_1k, one wavefront per 16-row tile.
-// Lane l holds A[l % 16][4 (l / 16) + i], B[4 (l / 16) + i][l % 16] and
-// D[4 (l / 16) + i][l % 16], i = 0..3.  Activations are re-laid out between
-// layers through LDS.
+// MFMA (gfx950): __builtin_amdgcn_mfma_f32_16x16x32_bf16 for the 64-wide products, one
+// wavefront per 16-row tile: lane l holds A[l % 16][8 (l / 16) + i] and B[8 (l / 16) + i][l % 16]
+// (i = 0..7) and D[4 (l / 16) + i][l % 16] (i = 0..3).  Products over 16 (the input layer and
+// the weight gradients, whose inner dimension is the tile's 16 rows) use
+// __builtin_amdgcn_mfma_f32_16x16x16bf16_1k with 4 elements per lane.  Activations are re-laid
+// out between layers through LDS.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -37,6 +39,8 @@ constexpr int OFF_W1 = 0, OFF_B1 = OFF_W1 + NH * NIN, OFF_W2 = OFF_B1 + NH, OFF_
 static_assert(PARAMS == TSR_MLP_PARAM_COUNT, "parameter layout");
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef short s16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // float -> bf16 (round to nearest even), as raw bits
@@ -65,6 +69,18 @@ struct MlpShared {
 __device__ __forceinline__ f32x4 tileAWt(const short *A, int lda, const short *W, int K, int nt, int lane) {
     f32x4 c = {0.f, 0.f, 0.f, 0.f};
     const int r = lane & 15, q = lane >> 4;
+    if (K % 32 == 0) {
+        for (int kc = 0; kc < K; kc += 32) {
+            s16x8 a, b;
+            for (int i = 0; i < 8; ++i) {
+                a[i] = A[r * lda + kc + 8 * q + i];
+                b[i] = W[(16 * nt + r) * K + kc + 8 * q + i];
+            }
+            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b),
+                                                        c, 0, 0, 0);
+        }
+        return c;
+    }
     for (int kc = 0; kc < K; kc += 16) {
         bf16x4 a, b;
         for (int i = 0; i < 4; ++i) {
@@ -151,13 +167,14 @@ __device__ void backDense(const float *dz, const short *Wt, const float *zBelow,
     const int r = lane & 15, q = lane >> 4;
     f32x4 acc[4];
     for (int nt = 0; nt < 4; ++nt) acc[nt] = (f32x4){0.f, 0.f, 0.f, 0.f};
-    for (int kc = 0; kc < NH; kc += 16) {
-        bf16x4 a;
-        for (int i = 0; i < 4; ++i) a[i] = toBf16(dz[r * NH + kc + 4 * q + i]);
+    for (int kc = 0; kc < NH; kc += 32) {
+        s16x8 a;
+        for (int i = 0; i < 8; ++i) a[i] = toBf16(dz[r * NH + kc + 8 * q + i]);
         for (int nt = 0; nt < 4; ++nt) {
-            bf16x4 b;
-            for (int i = 0; i < 4; ++i) b[i] = Wt[(kc + 4 * q + i) * NH + 16 * nt + r];   // W[out=k][in=n]
-            acc[nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[nt], 0, 0, 0);
+            s16x8 b;
+            for (int i = 0; i < 8; ++i) b[i] = Wt[(kc + 8 * q + i) * NH + 16 * nt + r];   // W[out=k][in=n]
+            acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a),
+                                                              __builtin_bit_cast(bf16x8, b), acc[nt], 0, 0, 0);
         }
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
@@ -183,19 +200,22 @@ __device__ void accumWeightGrad(f32x4 (&acc)[4][NT], const float *dz, const shor
     }
 }
 
+// the wave's weight gradient tiles into its own slot of the partial-gradient array (plain
+// stores; k_reduceGrad adds the slots in a fixed order)
 template <int NT>
 __device__ void flushWeightGrad(const f32x4 (&acc)[4][NT], float *g, int ld, int lane) {
     const int r = lane & 15, q = lane >> 4;
     for (int mt = 0; mt < 4; ++mt)
         for (int nt = 0; nt < NT; ++nt)
-            for (int i = 0; i < 4; ++i) atomicAdd(&g[(16 * mt + 4 * q + i) * ld + 16 * nt + r], acc[mt][nt][i]);
+            for (int i = 0; i < 4; ++i) g[(16 * mt + 4 * q + i) * ld + 16 * nt + r] = acc[mt][nt][i];
 }
 
-// weighted BCE loss and gradient, summed over all rows into grad / loss
+// weighted BCE loss and gradient of the rows a wave handles, into the wave's slot of
+// partial (TSR_MLP_PARAM_COUNT floats) and lossPart (one double)
 __global__ __launch_bounds__(64 * WAVES) void k_mlpGradient(const float *__restrict__ X, const float *__restrict__ y,
                                                             const float *__restrict__ rw, u32 n,
-                                                            const float *__restrict__ w, float *__restrict__ grad,
-                                                            double *__restrict__ loss) {
+                                                            const float *__restrict__ w, float *__restrict__ partial,
+                                                            double *__restrict__ lossPart) {
     extern __shared__ char smem[];
     MlpShared &S = *reinterpret_cast<MlpShared *>(smem);
     loadWeights(S, w);
@@ -258,16 +278,34 @@ __global__ __launch_bounds__(64 * WAVES) void k_mlpGradient(const float *__restr
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
     }
+    const u32 slot = blockIdx.x * WAVES + wave;
+    float *grad = partial + (size_t)slot * PARAMS;
     flushWeightGrad<1>(g1, grad + OFF_W1, NIN, lane);
     flushWeightGrad<4>(g2, grad + OFF_W2, NH, lane);
     flushWeightGrad<4>(g3, grad + OFF_W3, NH, lane);
-    atomicAdd(&grad[OFF_B1 + lane], gb1);
-    atomicAdd(&grad[OFF_B2 + lane], gb2);
-    atomicAdd(&grad[OFF_B3 + lane], gb3);
-    atomicAdd(&grad[OFF_W4 + lane], gw4);
-    if (lane == 0) atomicAdd(&grad[OFF_B4], gb4);
+    grad[OFF_B1 + lane] = gb1;
+    grad[OFF_B2 + lane] = gb2;
+    grad[OFF_B3 + lane] = gb3;
+    grad[OFF_W4 + lane] = gw4;
+    if (lane == 0) grad[OFF_B4] = gb4;
     for (int off = 32; off > 0; off >>= 1) lossAcc += __shfl_xor(lossAcc, off);
-    if (lane == 0) atomicAdd(loss, lossAcc);
+    if (lane == 0) lossPart[slot] = lossAcc;
+}
+
+// grad[i] = sum of the slots' partial gradients in slot order; loss likewise (deterministic)
+__global__ void k_reduceGrad(const float *__restrict__ partial, u32 slots, const double *__restrict__ lossPart,
+                             float *__restrict__ grad, double *__restrict__ loss) {
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i < PARAMS) {
+        float g = 0.f;
+        for (u32 s = 0; s < slots; ++s) g += partial[(size_t)s * PARAMS + i];
+        grad[i] = g;
+    }
+    if (i == 0) {
+        double l = 0.0;
+        for (u32 s = 0; s < slots; ++s) l += lossPart[s];
+        *loss = l;
+    }
 }
 
 // Adam (Kingma & Ba, 2015) on the FP32 master weights; grad holds the batch sum
@@ -291,181 +329,95 @@ void mlpForward(tsr_context *ctx, const float *dW, const float *dX, u32 n, float
     k_mlpForward<<<grid, 64 * WAVES, mlpShared(), ctx->stream>>>(dX, n, dW, dP);
 }
 
-void mlpGradient(tsr_context *ctx, const float *dW, const float *dX, const float *dY, const float *dRw, u32 n,
-                 float *dGrad, double *dLoss) {
-    HIPCHECK(hipMemsetAsync(dGrad, 0, PARAMS * 4, ctx->stream));
-    HIPCHECK(hipMemsetAsync(dLoss, 0, 8, ctx->stream));
-    if (!n) return;
+// gradient workspace: one partial gradient and loss per wave of the launch
+constexpr u32 GRAD_BLOCKS = 512;
+struct GradWork {
+    DBuf<float> partial;
+    DBuf<double> lossPart;
+    GradWork() : partial((size_t)GRAD_BLOCKS * WAVES * PARAMS), lossPart(GRAD_BLOCKS * WAVES) {}
+};
+
+void mlpGradient(tsr_context *ctx, GradWork &work, const float *dW, const float *dX, const float *dY,
+                 const float *dRw, u32 n, float *dGrad, double *dLoss) {
     const u32 tiles = (n + 15) / 16;
-    const u32 grid = std::min<u32>((tiles + WAVES - 1) / WAVES, 1024);
-    k_mlpGradient<<<grid, 64 * WAVES, mlpShared(), ctx->stream>>>(dX, dY, dRw, n, dW, dGrad, dLoss);
+    const u32 grid = std::max<u32>(1, std::min<u32>((tiles + WAVES - 1) / WAVES, GRAD_BLOCKS));
+    k_mlpGradient<<<grid, 64 * WAVES, mlpShared(), ctx->stream>>>(dX, dY, dRw, n, dW, work.partial.p,
+                                                                  work.lossPart.p);
+    k_reduceGrad<<<gridFor(PARAMS, BLOCK), BLOCK, 0, ctx->stream>>>(work.partial.p, grid * WAVES, work.lossPart.p,
+                                                                    dGrad, dLoss);
 }
 
 // ---------------------------------------------------------------------------
-// per-node features (TSR_NODE_FEATURES) from the tree, the sampleness pass
-// and a 1/16-subsampled emptiness pass
+// per-leaf features of the certified-priors router (TSR_LEAF_FEATURES)
 // ---------------------------------------------------------------------------
-constexpr int AGG = 14;   // cnt, s, s2, nx, ny, nz, px, py, pz, Shits, Ehits, leaves, Sleaves, Sksum
-
-__global__ void k_ownAgg(const u32 *__restrict__ counts, const u32 *__restrict__ starts, u32 nodeCount,
-                         const float *__restrict__ px, const float *__restrict__ py, const float *__restrict__ pz,
-                         const float *__restrict__ nx, const float *__restrict__ ny, const float *__restrict__ nz,
-                         const float *__restrict__ scale, float *__restrict__ agg) {
-    const u32 nd = blockIdx.x * BLOCK + threadIdx.x;
-    if (nd >= nodeCount) return;
-    float a[AGG];
-    for (int i = 0; i < AGG; ++i) a[i] = 0.f;
-    const u32 c = counts[nd], s0 = starts[nd];
-    for (u32 j = s0; j < s0 + c; ++j) {
-        a[0] += 1.f;
-        a[1] += scale[j];
-        a[2] += scale[j] * scale[j];
-        a[3] += nx[j]; a[4] += ny[j]; a[5] += nz[j];
-        a[6] += px[j]; a[7] += py[j]; a[8] += pz[j];
-    }
-    for (int i = 0; i < AGG; ++i) agg[(size_t)nd * AGG + i] = a[i];
-}
-
-__global__ void k_leafAgg(const u32 *__restrict__ leafNode, u32 leafCount, const u32 *__restrict__ hitsS,
-                          const u32 *__restrict__ hitsE, const float *__restrict__ ksumS,
-                          const u32 *__restrict__ states, float *__restrict__ agg) {
-    const u32 l = blockIdx.x * BLOCK + threadIdx.x;
-    if (l >= leafCount) return;
+// 0 depth; 1, 2 log E and S kernel sums; 3 E / (E + S); 4, 5 log E and S cone lengths; for
+// the ancestors 1..4 levels up: 6..9 their subtree prior E / (E + S) of cone lengths (what
+// the sphere query sums, seen from coarser nodes) and 10..13 the occupancy margin
+// (pS - pE) / (pS + pE) at that prior; 14 the signed log distance of the occupancy at the
+// 3-levels-up prior from EPSILON; 15 log10 of (E + S) / M (the scale of the occupancy).
+__global__ void k_leafFeatures(const u32 *__restrict__ list, u32 n, const u32 *__restrict__ leafNode,
+                               const u32 *__restrict__ parents, const uint8_t *__restrict__ nodeDepth,
+                               const float *__restrict__ ksum, const float *__restrict__ clen, u32 leafCount,
+                               const double2 *__restrict__ sums, double M, float *__restrict__ X) {
+    const u32 i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const u32 l = list ? list[i] : i;
     const u32 nd = leafNode[l];
-    float *a = agg + (size_t)nd * AGG;
-    a[9] = (float)hitsS[l];
-    a[10] = (float)hitsE[l];
-    a[11] = 1.f;
-    a[12] = (states[nd] & FLAG_S) ? 1.f : 0.f;
-    a[13] = ksumS[l];
-}
-
-__global__ void k_aggLevel(const u32 *__restrict__ child, const uint8_t *__restrict__ depth, u32 nodeCount, u32 level,
-                           float *__restrict__ agg) {
-    const u32 nd = blockIdx.x * BLOCK + threadIdx.x;
-    if (nd >= nodeCount || depth[nd] != level) return;
-    const u32 c = child[nd];
-    if (c & LEAF_BIT) return;
-    float *a = agg + (size_t)nd * AGG;
-    for (u32 s = 0; s < 8; ++s)
-        for (int i = 0; i < AGG; ++i) a[i] += agg[(size_t)(c + s) * AGG + i];
-}
-
-__global__ void k_features(Frame f, const u64 *__restrict__ nodeLoc, const u32 *__restrict__ parents,
-                           const u32 *__restrict__ states, const float *__restrict__ agg, u32 nodeCount,
-                           float *__restrict__ feat) {
-    const u32 nd = blockIdx.x * BLOCK + threadIdx.x;
-    if (nd >= nodeCount) return;
-    const float *a = agg + (size_t)nd * AGG;
-    const u64 loc = nodeLoc[nd];
-    const u32 d = locDepth(loc);
-    u32 k[3];
-    locCoords(loc, k[0], k[1], k[2]);
-    const float h = (float)f.h(d);
-    const float cnt = a[0], inv = cnt > 0.f ? 1.f / cnt : 0.f;
-    const float ms = a[1] * inv / h, ms2 = a[2] * inv / (h * h);
-    const float mn[3] = {a[3] * inv, a[4] * inv, a[5] * inv};
-    const float nlen = sqrtf(mn[0] * mn[0] + mn[1] * mn[1] + mn[2] * mn[2]);
-    float dist = 0.f;
-    if (cnt > 0.f) {
-        float d2 = 0.f;
-        for (int q = 0; q < 3; ++q) {
-            const float c = (float)f.centre(q, k[q], d), pm = a[6 + q] * inv;
-            d2 += (pm - c) * (pm - c);
-        }
-        dist = sqrtf(d2) / h;
+    const double e = ksum[l], s = ksum[leafCount + l];
+    float *x = X + (size_t)i * NIN;
+    x[0] = (float)nodeDepth[nd] / (float)MAX_DEPTH;
+    x[1] = log1pf((float)e) / 16.f;
+    x[2] = log1pf((float)s) / 16.f;
+    x[3] = e + s > 0.0 ? (float)(e / (e + s)) : 0.5f;
+    x[4] = log1pf(clen[l]) / 16.f;
+    x[5] = log1pf(clen[leafCount + l]) / 16.f;
+    u32 a = nd;
+    double v3 = 0.0;
+    for (int k = 1; k <= 4; ++k) {
+        const u32 pa = parents[a];
+        if (pa != INVALID) a = pa;
+        const double2 sm = sums[a];
+        const double p = sm.x + sm.y > 0.0 ? sm.x / (sm.x + sm.y) : 0.5;
+        const double pE = p * (e / M), pS = (1.0 - p) * (s / M);
+        x[5 + k] = (float)p;
+        x[9 + k] = pS + pE > 0.0 ? (float)((pS - pE) / (pS + pE)) : 0.f;
+        if (k == 3) v3 = pS - pE;
     }
-    const u32 par = parents[nd];
-    const float *pa = agg + (size_t)(par == INVALID ? nd : par) * AGG;
-    const float ps = pa[9], pe = 16.f * pa[10];
-    float *o = feat + (size_t)nd * TSR_NODE_FEATURES;
-    o[0] = (float)d / (float)MAX_DEPTH;
-    o[1] = log1pf(cnt);
-    o[2] = ms;
-    o[3] = fmaxf(ms2 - ms * ms, 0.f);
-    o[4] = mn[0]; o[5] = mn[1]; o[6] = mn[2];
-    o[7] = cnt > 0.f ? 1.f - nlen : 0.f;
-    o[8] = log1pf(a[9]);
-    o[9] = log1pf(16.f * a[10]);
-    o[10] = (ps - pe) / (ps + pe + 1.f);
-    o[11] = log1pf(a[13]) / 30.f;
-    o[12] = a[11] > 0.f ? a[12] / a[11] : 0.f;
-    o[13] = log2f(fmaxf(a[11], 1.f)) / 20.f;
-    o[14] = dist;
-    o[15] = (states[nd] & FLAG_S) ? 1.f : 0.f;
+    const double r = (v3 - EPS_D) / EPS_D;
+    x[14] = (float)(copysign(log1p(fabs(r)), r) / 20.0);
+    x[15] = (float)(log10(fmax((e + s) / M, 1e-30)) / 10.0);
 }
 
-// training labels: leaves with positive occupancy or at a sign-changing dual cell
-__global__ void k_labelCells(const u32 *__restrict__ cells, u32 cellCount, const double *__restrict__ occ,
-                             uint8_t *__restrict__ leafLabel) {
-    const u32 c = blockIdx.x * BLOCK + threadIdx.x;
-    if (c >= cellCount) return;
-    bool pos = false, neg = false;
-    for (int i = 0; i < 8; ++i) {
-        const double v = occ[cells[8 * (size_t)c + i]];
-        pos |= v > 0.0;
-        neg |= !(v > 0.0);
-    }
-    if (pos && neg)
-        for (int i = 0; i < 8; ++i) leafLabel[cells[8 * (size_t)c + i]] = 1;
-}
-
-__global__ void k_labelNodesLeaves(const u32 *__restrict__ leafNode, u32 leafCount, const double *__restrict__ occ,
-                                   const uint8_t *__restrict__ leafLabel, uint8_t *__restrict__ nodeLabel) {
-    const u32 l = blockIdx.x * BLOCK + threadIdx.x;
-    if (l >= leafCount) return;
-    nodeLabel[leafNode[l]] = (leafLabel[l] || occ[l] > 0.0) ? 1 : 0;
-}
-
-__global__ void k_labelLevel(const u32 *__restrict__ child, const uint8_t *__restrict__ depth, u32 nodeCount,
-                             u32 level, uint8_t *__restrict__ nodeLabel) {
-    const u32 nd = blockIdx.x * BLOCK + threadIdx.x;
-    if (nd >= nodeCount || depth[nd] != level) return;
-    const u32 c = child[nd];
-    if (c & LEAF_BIT) return;
-    uint8_t v = 0;
-    for (u32 s = 0; s < 8; ++s) v |= nodeLabel[c + s];
-    nodeLabel[nd] = v;
-}
-
-__global__ void k_cullMask(const float *__restrict__ prob, u32 nodeCount, const uint8_t *__restrict__ depth,
-                           u32 minDepth, float threshold, uint8_t *__restrict__ mask) {
-    const u32 nd = blockIdx.x * BLOCK + threadIdx.x;
-    if (nd >= nodeCount) return;
-    mask[nd] = (depth[nd] >= minDepth && prob[nd] < threshold) ? 1 : 0;
+// first stage from the network output p: floor(2 p + 1 - t) clamped to 0..2 (the training
+// target is stage / 2; a larger t keeps more queries at the cheap first stage)
+__global__ void k_route(const u32 *__restrict__ queries, u32 n, const float *__restrict__ prob, float t,
+                        uint8_t *__restrict__ stage, u32 *__restrict__ routed) {
+    const u32 i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const float v = floorf(2.f * prob[i] + 1.f - t);
+    const u32 s = v <= 0.f ? 0u : (v >= 2.f ? 2u : (u32)v);
+    stage[queries[i]] = (uint8_t)s;
+    if (s) atomicAdd(&routed[s - 1], 1u);
 }
 
 }  // namespace
 
-void nodeFeatures(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *scene, const tsr_params &prm,
-                  const u32 *states, const u32 *hitsS, const float *ksumS, DBuf<float> &feat) {
-    hipStream_t st = ctx->stream;
-    const u32 N = tree->nodeCount, L = tree->leafCount;
-    DBuf<u32> hitsE(L);
-    emptinessSubsample(ctx, tree, scene, prm, states, 16, hitsE.p);
-    DBuf<float> agg((size_t)N * AGG);
-    k_ownAgg<<<gridFor(N, BLOCK), BLOCK, 0, st>>>(tree->sampleCounts.p, tree->sampleStarts.p, N, tree->px.p,
-                                                  tree->py.p, tree->pz.p, tree->nx.p, tree->ny.p, tree->nz.p,
-                                                  tree->scale.p, agg.p);
-    k_leafAgg<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(tree->leafNode.p, L, hitsS, hitsE.p, ksumS, states, agg.p);
-    for (int lvl = (int)tree->maxDepth - 1; lvl >= 0; --lvl)
-        k_aggLevel<<<gridFor(N, BLOCK), BLOCK, 0, st>>>(tree->child.p, tree->nodeDepth.p, N, (u32)lvl, agg.p);
-    feat.alloc((size_t)N * TSR_NODE_FEATURES);
-    k_features<<<gridFor(N, BLOCK), BLOCK, 0, st>>>(tree->frame, tree->nodeLoc.p, tree->parents.p, states, agg.p, N,
-                                                    feat.p);
+void leafFeatures(tsr_context *ctx, const tsr_tree *tree, const tsr_field *field, const PriorRecords &rec,
+                  const u32 *list, u32 n, float *X) {
+    if (!n) return;
+    const double M = field->coneCount > 0.0 ? field->coneCount : 1.0;
+    k_leafFeatures<<<gridFor(n, BLOCK), BLOCK, 0, ctx->stream>>>(list, n, tree->leafNode.p, tree->parents.p,
+                                                                 tree->nodeDepth.p, field->ksum.p, field->clen.p,
+                                                                 tree->leafCount, rec.sums(), M, X);
 }
 
-void cullerMask(tsr_context *ctx, const tsr_culler *culler, const tsr_tree *tree, const tsr_scene *scene,
-                const tsr_params &prm, const u32 *states, const u32 *hitsS, const float *ksumS,
-                DBuf<uint8_t> &mask) {
-    DBuf<float> feat;
-    nodeFeatures(ctx, tree, scene, prm, states, hitsS, ksumS, feat);
-    const u32 N = tree->nodeCount;
-    DBuf<float> prob(N);
-    mlpForward(ctx, culler->weights.p, feat.p, N, prob.p);
-    mask.alloc(N);
-    k_cullMask<<<gridFor(N, BLOCK), BLOCK, 0, ctx->stream>>>(prob.p, N, tree->nodeDepth.p, 2, culler->threshold,
-                                                             mask.p);
+void leafRoute(tsr_context *ctx, const tsr_culler *culler, const tsr_tree *tree, const tsr_field *field,
+               const PriorRecords &rec, const u32 *queries, u32 nq, uint8_t *stage, u32 *routed) {
+    if (!nq) return;
+    DBuf<float> X((size_t)nq * NIN), prob(nq);
+    leafFeatures(ctx, tree, field, rec, queries, nq, X.p);
+    mlpForward(ctx, culler->weights.p, X.p, nq, prob.p);
+    k_route<<<gridFor(nq, BLOCK), BLOCK, 0, ctx->stream>>>(queries, nq, prob.p, culler->threshold, stage, routed);
 }
 
 }  // namespace tsr
@@ -523,39 +475,20 @@ void tsr_culler_destroy(tsr_culler *c) {
     delete c;
 }
 
-tsr_status tsr_node_features(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *scene, const tsr_field *field,
-                             float *features_host) {
-    if (!ctx || !tree || !scene || !field || !features_host) return mlpFail(TSR_ERR_INVALID_ARGUMENT, "null argument");
-    if (field->leafCount != tree->leafCount || !field->hitsS.p)
+tsr_status tsr_leaf_features(tsr_context *ctx, const tsr_tree *tree, const tsr_field *field, float *features_host) {
+    if (!ctx || !tree || !field || !features_host) return mlpFail(TSR_ERR_INVALID_ARGUMENT, "null argument");
+    if (field->leafCount != tree->leafCount || field->nodeCount != tree->nodeCount)
         return mlpFail(TSR_ERR_INVALID_ARGUMENT, "field does not belong to the tree");
     return mlpGuard([&] {
         HIPCHECK(hipSetDevice(ctx->device));
-        tsr_params prm = field->params;
-        DBuf<float> feat;
-        nodeFeatures(ctx, tree, scene, prm, field->states.p, field->hitsS.p, field->ksum.p + tree->leafCount, feat);
-        HIPCHECK(hipMemcpyAsync(features_host, feat.p, feat.bytes(), hipMemcpyDeviceToHost, ctx->stream));
+        const u32 L = tree->leafCount;
+        if (!L) return;
+        PriorRecords rec;
+        priorRecords(ctx, tree, field, rec);
+        DBuf<float> X((size_t)L * NIN);
+        leafFeatures(ctx, tree, field, rec, nullptr, L, X.p);
+        HIPCHECK(hipMemcpyAsync(features_host, X.p, X.bytes(), hipMemcpyDeviceToHost, ctx->stream));
         HIPCHECK(hipStreamSynchronize(ctx->stream));
-    });
-}
-
-tsr_status tsr_node_labels(tsr_context *ctx, const tsr_tree *tree, const tsr_field *field, uint8_t *labels_host) {
-    if (!ctx || !tree || !field || !labels_host) return mlpFail(TSR_ERR_INVALID_ARGUMENT, "null argument");
-    if (field->leafCount != tree->leafCount) return mlpFail(TSR_ERR_INVALID_ARGUMENT, "field does not belong to the tree");
-    return mlpGuard([&] {
-        HIPCHECK(hipSetDevice(ctx->device));
-        hipStream_t st = ctx->stream;
-        const u32 N = tree->nodeCount, L = tree->leafCount, C = tree->cellCount;
-        DBuf<uint8_t> leafLabel(L), nodeLabel(N);
-        HIPCHECK(hipMemsetAsync(leafLabel.p, 0, L, st));
-        HIPCHECK(hipMemsetAsync(nodeLabel.p, 0, N, st));
-        if (C) k_labelCells<<<gridFor(C, BLOCK), BLOCK, 0, st>>>(tree->cells.p, C, field->occ.p, leafLabel.p);
-        k_labelNodesLeaves<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(tree->leafNode.p, L, field->occ.p, leafLabel.p,
-                                                                nodeLabel.p);
-        for (int lvl = (int)tree->maxDepth - 1; lvl >= 0; --lvl)
-            k_labelLevel<<<gridFor(N, BLOCK), BLOCK, 0, st>>>(tree->child.p, tree->nodeDepth.p, N, (u32)lvl,
-                                                              nodeLabel.p);
-        HIPCHECK(hipMemcpyAsync(labels_host, nodeLabel.p, N, hipMemcpyDeviceToHost, st));
-        HIPCHECK(hipStreamSynchronize(st));
     });
 }
 
@@ -582,13 +515,14 @@ tsr_status tsr_mlp_gradient(tsr_context *ctx, const float *weights, const float 
         hipStream_t st = ctx->stream;
         DBuf<float> w(PARAMS), x((size_t)n * NIN + 1), y(n + 1), rw(n + 1), g(PARAMS);
         DBuf<double> l(1);
+        GradWork work;
         HIPCHECK(hipMemcpyAsync(w.p, weights, PARAMS * 4, hipMemcpyHostToDevice, st));
         if (n) {
             HIPCHECK(hipMemcpyAsync(x.p, features, (size_t)n * NIN * 4, hipMemcpyHostToDevice, st));
             HIPCHECK(hipMemcpyAsync(y.p, labels, (size_t)n * 4, hipMemcpyHostToDevice, st));
             HIPCHECK(hipMemcpyAsync(rw.p, row_weights, (size_t)n * 4, hipMemcpyHostToDevice, st));
         }
-        mlpGradient(ctx, w.p, x.p, y.p, rw.p, n, g.p, l.p);
+        mlpGradient(ctx, work, w.p, x.p, y.p, rw.p, n, g.p, l.p);
         HIPCHECK(hipMemcpyAsync(grad, g.p, PARAMS * 4, hipMemcpyDeviceToHost, st));
         HIPCHECK(hipMemcpyAsync(loss, l.p, 8, hipMemcpyDeviceToHost, st));
         HIPCHECK(hipStreamSynchronize(st));
@@ -610,6 +544,7 @@ tsr_status tsr_mlp_train(tsr_context *ctx, float *weights, const float *features
         std::vector<float> xs((size_t)n * NIN), ys(n), ws(n);
         DBuf<float> w(PARAMS), m(PARAMS), v(PARAMS), g(PARAMS), x((size_t)n * NIN), y(n), rw(n);
         DBuf<double> l(1);
+        GradWork work;
         HIPCHECK(hipMemcpyAsync(w.p, weights, PARAMS * 4, hipMemcpyHostToDevice, st));
         HIPCHECK(hipMemsetAsync(m.p, 0, PARAMS * 4, st));
         HIPCHECK(hipMemsetAsync(v.p, 0, PARAMS * 4, st));
@@ -627,7 +562,7 @@ tsr_status tsr_mlp_train(tsr_context *ctx, float *weights, const float *features
             double epochLoss = 0.0, epochWeight = 0.0;
             for (u32 b0 = 0; b0 < n; b0 += batch) {
                 const u32 nb = std::min(batch, n - b0);
-                mlpGradient(ctx, w.p, x.p + (size_t)b0 * NIN, y.p + b0, rw.p + b0, nb, g.p, l.p);
+                mlpGradient(ctx, work, w.p, x.p + (size_t)b0 * NIN, y.p + b0, rw.p + b0, nb, g.p, l.p);
                 double wsum = 0.0;
                 for (u32 i = b0; i < b0 + nb; ++i) wsum += ws[i];
                 ++step;

@@ -31,18 +31,6 @@ namespace {
 constexpr u32 BLOCK = 256;
 constexpr u32 XCD_COUNT = 8;
 
-// logical block index that gives every XCD (blocks are dispatched round-robin
-// over the 8 XCDs) a contiguous range of the work; gridDim.x is a multiple of 8
-__device__ __forceinline__ u32 xcdBlock(int xcdMap) {
-    if (!xcdMap) return blockIdx.x;
-    return (blockIdx.x % XCD_COUNT) * (gridDim.x / XCD_COUNT) + blockIdx.x / XCD_COUNT;
-}
-// work-to-XCD mapping switches (TSR_XCD_MAP bit 0: static mapping of the one-thread-per-link
-// cone passes, bit 1: priors, bit 2: XCD slices of the persistent-thread emptiness pass)
-inline int xcdMapMode() {
-    static const int m = std::getenv("TSR_XCD_MAP") ? std::atoi(std::getenv("TSR_XCD_MAP")) : 6;
-    return m;
-}
 inline unsigned xcdGrid(size_t n, unsigned block) {
     const unsigned g = gridFor(n, block);
     return (g + XCD_COUNT - 1) / XCD_COUNT * XCD_COUNT;
@@ -72,7 +60,7 @@ __device__ __forceinline__ size_t stagingIndex(u32 idx, u32 j, u32 slots) { retu
 // (child links) or 64-byte ((child, state) pairs) boundary and a sibling scan reads
 // one cache line.  Parents are kept per child block: blockParent[(n - 1) / 8] is the
 // parent of node n (8 times fewer entries than a per-node table).
-constexpr u32 NODE_PAD = 7;
+constexpr u32 NODE_PAD = 7;   // PriorRecords (handles.h) uses the same padding
 
 struct TraversalTree {
     const u32 *child;        // first child id, or LEAF_BIT | leaf index (node-padded copy)
@@ -115,8 +103,6 @@ struct LinkInputs {
     const uint8_t *sampleDepth;
     u32 maxCams, camCount;
     float bandwidth, relConeLength;
-    const uint8_t *cullNode;   // learned culling mask (E pass), may be null
-    int xcdMap;
     u32 linkStride;            // process every linkStride-th link (1 = all)
     const u32 *order;          // thread index -> link (links grouped by camera), may be null
     u32 slots;                 // mode 3: staging slots per link (layout: stagingIndex)
@@ -146,14 +132,14 @@ __device__ bool setupLink(const LinkInputs &in, u32 link, bool sampleness, ConeD
 }
 
 // Depth-first traversal (NodesIterator.cpp:38-86 / LeavesIterator.cpp:24-31):
-// calls onLeaf(leafIndex, centre) for every leaf that passes depth, flag and
-// cull tests.  Inner nodes failing a test are skipped with their subtrees.  With
+// calls onLeaf(leafIndex, centre) for every leaf that passes the depth, flag and
+// cone tests.  Inner nodes failing a test are skipped with their subtrees.  With
 // `path` (this thread's column of an LDS array [MAX_DEPTH + 2][BLOCK]) the ancestors
 // are kept there on the way down and climbing reads them back; without it, climbing
 // reads the block-parent table.
 template <typename OnLeaf>
 __device__ void traverseCone(const TraversalTree &t, const ConeCull &cull, u32 maxDepth, u32 required,
-                             const uint8_t *cullNode, u32 &visits, u32 *path, OnLeaf onLeaf) {
+                             u32 &visits, u32 *path, OnLeaf onLeaf) {
     u32 node = 0, d = 0, k[3] = {0, 0, 0};
     bool descend = true;   // whether to test `node` (true) or move on from it (false)
     for (;;) {
@@ -168,7 +154,6 @@ __device__ void traverseCone(const TraversalTree &t, const ConeCull &cull, u32 m
             } else if (accept && required) {
                 accept = (t.states[node] & required) == required;
             }
-            if (accept && cullNode) accept = cullNode[node] == 0;
             if (!t.packed) c = accept ? t.child[node] : 0u;
             if (accept) {
                 if (c & LEAF_BIT) {
@@ -230,7 +215,7 @@ __device__ void waveAccumulate(unsigned long long *acc, u32 v) {
 
 // mode 0: count accepted hits per link; 1: write records at offsets[link]; 2: FP32 atomics
 // (precision study); 3: count, and write the first in.slots records into the link's
-// staging slots (stagingIndex: contiguous per link); 4: hit count per leaf
+// staging slots (stagingIndex: contiguous per link)
 // stats[0] += node visits, stats[1] += accepted hits
 // Register use capped for 6 waves per SIMD (80 VGPRs, no spill).
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_conePass(TraversalTree t, LinkInputs in, u32 linkCount, int sampleness, int mode,
@@ -239,7 +224,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
                            u32 *__restrict__ atomicStates, double *__restrict__ confOut,
                            unsigned long long *__restrict__ stats) {
     __shared__ u32 pathL[MAX_DEPTH + 2][BLOCK];   // ancestors per thread (launched with BLOCK threads)
-    const u32 idx = (xcdBlock(in.xcdMap) * blockDim.x + threadIdx.x) * in.linkStride;
+    const u32 idx = (blockIdx.x * blockDim.x + threadIdx.x) * in.linkStride;
     u32 visits = 0, count = 0;
     const u32 link = idx < linkCount && in.order ? in.order[idx] : in.linkBase + idx;
     if (idx < linkCount) coneLink(t, in, link, idx, linkCount, sampleness, mode, counts, offsets, recLeaf, recVal, atomicSum,
@@ -271,7 +256,7 @@ __device__ void coneLink(const TraversalTree &t, const LinkInputs &in, u32 link,
     const u32 maxDepth = (u32)in.sampleDepth[i] + MAX_DEPTH_DIFFERENCE;
     const u32 required = sampleness ? 0u : FLAG_S;
     const size_t base = mode == 1 ? (size_t)offsets[link] : 0;
-    traverseCone(t, cull, maxDepth, required, sampleness ? nullptr : in.cullNode, visits, path,
+    traverseCone(t, cull, maxDepth, required, visits, path,
                  [&](u32 leaf, const double x[3]) {
         const double kv = coneKernel(cone, x, sampleness != 0, conf);
         if (kv == 0.0) return;
@@ -283,8 +268,6 @@ __device__ void coneLink(const TraversalTree &t, const LinkInputs &in, u32 link,
             atomicAdd(&atomicSum[leaf], (float)kv);
             atomicAdd(&atomicLen[leaf], weightedLength);
             atomicOr(&atomicStates[leaf], 1u);
-        } else if (mode == 4) {   // hit count per leaf (integer, order-independent)
-            atomicAdd(&atomicStates[leaf], 1u);
         }
         ++count;
     });
@@ -315,7 +298,6 @@ __global__ __launch_bounds__(256) void k_conePassPT(TraversalTree t, LinkInputs 
     const u32 sliceLen = (linkCount + slices - 1) / slices;
     u32 cur = blockIdx.x % slices, tried = 0;   // wave-uniform
     const u32 required = sampleness ? 0u : FLAG_S;
-    const uint8_t *cullNode = sampleness ? nullptr : in.cullNode;
     bool active = false, more = true;
     u32 idx = 0, link = 0, count = 0, maxDepth = 0, node = 0, d = 0, k0 = 0, k1 = 0, k2 = 0;
     bool descend = true;
@@ -386,7 +368,6 @@ __global__ __launch_bounds__(256) void k_conePassPT(TraversalTree t, LinkInputs 
                 if (accept && required) accept = (t.states[node] & required) == required;
                 c = t.child[node];
             }
-            if (accept && cullNode) accept = cullNode[node] == 0;
             if (accept) {
                 if (c & LEAF_BIT) {
                     const u32 k[3] = {k0, k1, k2};
@@ -463,7 +444,6 @@ __global__ __launch_bounds__(64) void k_coneLinkWave(TraversalTree t, LinkInputs
     __shared__ u32 sp, written;
     const u32 lane = threadIdx.x;
     const u32 required = sampleness ? 0u : FLAG_S;
-    const uint8_t *cullNode = sampleness ? nullptr : in.cullNode;
     u32 visits = 0;
     for (u32 w = blockIdx.x; w < count; w += gridDim.x) {
         const u32 link = links[w];
@@ -481,7 +461,6 @@ __global__ __launch_bounds__(64) void k_coneLinkWave(TraversalTree t, LinkInputs
         auto accepts = [&](u32 node, u32 d, u32 &c) {
             bool accept = d <= maxDepth;
             if (accept && required) accept = (t.states[node] & required) == required;
-            if (accept && cullNode) accept = cullNode[node] == 0;
             c = t.child[node];
             return accept;
         };
@@ -746,15 +725,23 @@ __global__ void k_subtreeLevel(const u32 *__restrict__ child, const uint8_t *__r
 }
 
 // leaves whose prior the occupancy ratio reads: E + S kernel sums >= confidence
-// threshold (Occupancy.cpp:575-576); all leaves when `all` is set
+// threshold (Occupancy.cpp:575-576); all leaves when `all` is set.  With `stage`, the
+// certified modes start every query at stage 0 and mark the others PRIOR_NOT_QUERIED.
+constexpr uint8_t PRIOR_NOT_QUERIED = 255, PRIOR_DONE = 254;
 __global__ void k_priorQueryFlags(const float *__restrict__ ksum, u32 leafCount, double threshold, int all,
-                                  const uint8_t *__restrict__ culled, double *__restrict__ prior,
-                                  u32 *__restrict__ flag) {
+                                  double *__restrict__ prior, u32 *__restrict__ flag, uint8_t *__restrict__ stage) {
     const u32 l = blockIdx.x * BLOCK + threadIdx.x;
     if (l >= leafCount) return;
-    const bool used = (all || ((double)ksum[l] + (double)ksum[leafCount + l] >= threshold)) && !(culled && culled[l]);
+    const bool used = all || ((double)ksum[l] + (double)ksum[leafCount + l] >= threshold);
     flag[l] = used ? 1u : 0u;
+    if (stage) stage[l] = used ? 0 : PRIOR_NOT_QUERIED;
     if (!used) prior[l] = __longlong_as_double(0x7ff8000000000000ll);   // not needed by the field: NaN
+}
+
+// flag = 1 for the leaves at stage s of the certified cascade
+__global__ void k_stageFlags(const uint8_t *__restrict__ stage, u32 leafCount, uint8_t s, u32 *__restrict__ flag) {
+    const u32 l = blockIdx.x * BLOCK + threadIdx.x;
+    if (l < leafCount) flag[l] = stage[l] == s ? 1u : 0u;
 }
 
 // queries in leaf (DFS = spatial) order
@@ -803,17 +790,26 @@ constexpr u32 PRIOR_STACK = 1344;   // >= 64 * (MAX_DEPTH + 1): one step pushes 
 // inner children are pushed.  Every query applies the FP64 predicates of the
 // reference (behind an FP32 pre-test with a proven margin); sums are FP64 per lane
 // and query, reduced across the wave in a fixed order (deterministic).
+//
+// Depth-capped bound (CAPPED, capLevels = c): a query of depth dq does not open
+// straddling children at depth >= dq - c.  The part of such a child that the exact
+// traversal sums is a sub-sum of the child's subtree sums (all terms are >= 0), so those
+// go to upper-bound accumulators (Eu, Su) instead, and the query gets the interval
+// [E / (E + S + Su), (E + Eu) / (E + Eu + S)] (interval[query index]), which holds the exact
+// prior.  The capped form serves 4 queries per wave, so that its four FP64 sums per query
+// take the registers of the exact form's two.
 #define PRIORS_GROUP_PARAMS                                                                                \
     Frame f, const uint2 *__restrict__ links, const double2 *__restrict__ sums, const u64 *__restrict__ leafLoc, \
-        const u32 *__restrict__ queries, u32 queryCount, float relRadius, double *__restrict__ prior, u32 *err,  \
-        unsigned long long *__restrict__ stats, int xcdMap, u32 *__restrict__ nextGroup,                        \
+        const u32 *__restrict__ queries, u32 queryCount, float relRadius, u32 capLevels,                       \
+        double *__restrict__ prior, double2 *__restrict__ interval, u32 *err,                                  \
+        unsigned long long *__restrict__ stats, u32 *__restrict__ nextGroup,                                   \
         const u32 *__restrict__ groupList, const u32 *__restrict__ groupListCount,                              \
         u32 *__restrict__ overflowList, u32 *__restrict__ overflowCount
 #define PRIORS_GROUP_ARGS                                                                                  \
-    f, links, sums, leafLoc, queries, queryCount, relRadius, prior, err, stats, xcdMap, nextGroup, groupList, \
-        groupListCount, overflowList, overflowCount
+    f, links, sums, leafLoc, queries, queryCount, relRadius, capLevels, prior, interval, err, stats, nextGroup, \
+        groupList, groupListCount, overflowList, overflowCount
 
-template <u32 GROUP_Q, u32 STACK>
+template <u32 GROUP_Q, u32 STACK, bool CAPPED>
 __device__ __forceinline__ void priorsGroup(PRIORS_GROUP_PARAMS) {
     __shared__ u32 stFirst[STACK];
     __shared__ u64 stK[STACK];          // kx | ky << 21 | kz << 42
@@ -847,7 +843,7 @@ __device__ __forceinline__ void priorsGroup(PRIORS_GROUP_PARAMS) {
     // A wave whose slice is used up moves on to the next slice.  With a group list,
     // the listed groups are taken from one counter.
     const u32 groups = groupList ? *groupListCount : (queryCount + GROUP_Q - 1) / GROUP_Q;
-    const u32 slices = (xcdMap && !groupList) ? XCDS : 1u;
+    const u32 slices = groupList ? 1u : XCDS;
     const u32 slice = (groups + slices - 1) / slices;
     u32 cur = blockIdx.x % slices, tried = 0;
     __shared__ u32 gNext;
@@ -882,9 +878,9 @@ __device__ __forceinline__ void priorsGroup(PRIORS_GROUP_PARAMS) {
                                    (float)(qc[lane][2] - f.rootMin[2]), r2F + mg);
             qB[lane] = make_float4(r2F - mg, ri2F - mg, ri2F + mg, __uint_as_float(dq + MAX_DEPTH_DIFFERENCE));
         }
-        double accE[GROUP_Q], accS[GROUP_Q];
+        double accE[GROUP_Q], accS[GROUP_Q], upE[GROUP_Q], upS[GROUP_Q];
 #pragma unroll
-        for (u32 j = 0; j < GROUP_Q; ++j) accE[j] = accS[j] = 0.0;
+        for (u32 j = 0; j < GROUP_Q; ++j) accE[j] = accS[j] = upE[j] = upS[j] = 0.0;
         const uint2 root = links[0];
         const uint8_t all = (uint8_t)((1u << nQ) - 1u);
         if (lane == 0) {
@@ -988,7 +984,15 @@ __device__ __forceinline__ void priorsGroup(PRIORS_GROUP_PARAMS) {
                             accS[j] += cs.y;
                             summed = true;
                         } else if (d < md) {   // children at depth d + 1 <= md still count
-                            pushMask |= (uint8_t)(1u << j);
+                            // capped: children at depth >= dq - capLevels are bounded, not opened
+                            // (md = dq + MAX_DEPTH_DIFFERENCE)
+                            if (CAPPED && d + MAX_DEPTH_DIFFERENCE + capLevels >= md) {
+                                upE[j] += cs.x;
+                                upS[j] += cs.y;
+                                summed = true;
+                            } else {
+                                pushMask |= (uint8_t)(1u << j);
+                            }
                         }
                     }
                     nWhole += summed ? 1u : 0u;
@@ -1019,12 +1023,21 @@ __device__ __forceinline__ void priorsGroup(PRIORS_GROUP_PARAMS) {
         const bool write = abandoned == 0;
 #pragma unroll
         for (u32 j = 0; j < GROUP_Q; ++j) {
-            double ee = accE[j], ss = accS[j];
+            double ee = accE[j], ss = accS[j], ue = upE[j], us = upS[j];
             for (int off = 32; off > 0; off >>= 1) {
                 ee += __shfl_xor(ee, off);
                 ss += __shfl_xor(ss, off);
+                if (CAPPED) {
+                    ue += __shfl_xor(ue, off);
+                    us += __shfl_xor(us, off);
+                }
             }
-            if (lane == 0 && j < nQ && write) prior[queries[g * GROUP_Q + j]] = ee / (ee + ss);
+            if (lane == 0 && j < nQ && write) {
+                if (CAPPED)
+                    interval[g * GROUP_Q + j] = make_double2(ee / (ee + ss + us), (ee + ue) / (ee + ue + ss));
+                else
+                    prior[queries[g * GROUP_Q + j]] = ee / (ee + ss);
+            }
         }
         __syncthreads();
     }
@@ -1035,23 +1048,23 @@ __device__ __forceinline__ void priorsGroup(PRIORS_GROUP_PARAMS) {
     }
 }
 
-// register use capped for 7 waves per SIMD (72 VGPRs; ~64 bytes of spill: set-up values
-// and one query's S accumulator): the step's memory and LDS round trips are hidden better
-// than with 5 or 6 waves; 8 waves spill more and run slower.  The long-stack instantiation is held to
+// register use capped for 7 waves per SIMD (72 VGPRs; a little spill of set-up values and
+// one query's sums): the step's memory and LDS round trips are hidden better than with 5
+// or 6 waves; 8 waves spill more and run slower.  The long-stack instantiation is held to
 // few waves by its LDS stack anyway and is not capped.
-template <u32 GROUP_Q, u32 STACK>
+template <u32 GROUP_Q, u32 STACK, bool CAPPED>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(STACK == PRIOR_STACK_SHORT ? 7 : 1))) void k_priorsGroup(PRIORS_GROUP_PARAMS) {
-    priorsGroup<GROUP_Q, STACK>(PRIORS_GROUP_ARGS);
+    priorsGroup<GROUP_Q, STACK, CAPPED>(PRIORS_GROUP_ARGS);
 }
+constexpr u32 PRIOR_GROUP = 8, PRIOR_GROUP_CAPPED = 4;   // queries per wave, exact / capped
 
-// Occupancy.cpp:570-587, FP64 (leaves under learned-culled nodes are empty)
+// Occupancy.cpp:570-587, FP64
 __global__ void k_occupancy(const float *__restrict__ ksum, const double *__restrict__ prior, u32 leafCount,
-                            double coneCount, double threshold, const uint8_t *__restrict__ culled,
-                            double *__restrict__ occ) {
+                            double coneCount, double threshold, double *__restrict__ occ) {
     const u32 l = blockIdx.x * BLOCK + threadIdx.x;
     if (l >= leafCount) return;
     const double e = ksum[l], s = ksum[leafCount + l];
-    if (e + s < threshold || (culled && culled[l])) {
+    if (e + s < threshold) {
         occ[l] = -EPS_D;
         return;
     }
@@ -1059,6 +1072,56 @@ __global__ void k_occupancy(const float *__restrict__ ksum, const double *__rest
     const double pE = p * (e / coneCount);
     const double pS = (1.0 - p) * (s / coneCount);
     occ[l] = pE >= pS ? -EPS_D : pS - pE;
+}
+
+// What the mesh reads of a leaf's occupancy v (Occupancy.cpp:570-587) at prior p: class 0
+// for v = -EPSILON (pE >= pS), 1 for 0 < v <= EPSILON, 2 for v > EPSILON.  DMC cell
+// cases read the sign, and a crossing vertex lies at the corner whose |v| <= EPSILON is
+// tested first (DualMarchingCell.cpp:51-71: the negative corner, which is always at
+// -EPSILON, or a class-1 positive one), so two fields with equal classes give the same
+// mesh.  Returns -1 where rounding could decide the class: within 1e-9 T of a class
+// boundary (T = max(E, S) / M bounds pE + pS; FP64 roundings of the occupancy formula
+// are near 1e-16 T) or below 1e-18 (where v + EPSILON would round to EPSILON).
+__device__ int occupancyClass(double p, double e, double s, double M) {
+    const double pE = p * (e / M), pS = (1.0 - p) * (s / M);
+    const double tol = 1e-9 * (fmax(e, s) / M);
+    const double v = pS - pE;
+    if (!(v == v)) return -1;
+    if (v < -tol) return 0;
+    if (v <= fmax(tol, 1e-18)) return -1;
+    if (v < EPS_D - tol) return 1;
+    if (v > EPS_D + tol) return 2;
+    return -1;
+}
+
+// The judge of the certified cascade: the occupancy formula on both ends of a query's
+// prior interval (widened by 1e-9 relative, far above the difference between the sums of
+// two traversal orders).  The occupancy is monotone in the prior, so equal classes at both
+// ends mean the exact prior gives that class too: the leaf keeps the interval midpoint as
+// its prior (so its occupancy has the class of the exact one) and is done at stage s;
+// otherwise it moves on to stage s + 1.
+__global__ void k_certify(const u32 *__restrict__ list, u32 n, const double2 *__restrict__ interval,
+                          const float *__restrict__ ksum, u32 leafCount, double coneCount, uint8_t s,
+                          uint8_t *__restrict__ stage, uint8_t *__restrict__ level, double *__restrict__ prior) {
+    const u32 i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const u32 l = list[i];
+    const double2 iv = interval[i];
+    const double e = ksum[l], sm = ksum[leafCount + l];
+    const double lo = iv.x * (1.0 - 1e-9), hi = fmin(1.0, iv.y * (1.0 + 1e-9));
+    const int c = (lo >= 0.0 && hi <= 1.0 && lo <= hi) ? occupancyClass(lo, e, sm, coneCount) : -1;
+    if (c >= 0 && c == occupancyClass(hi, e, sm, coneCount)) {
+        prior[l] = 0.5 * (iv.x + iv.y);
+        level[l] = s;
+        stage[l] = PRIOR_DONE;
+    } else {
+        stage[l] = s + 1;
+    }
+}
+
+__global__ void k_setLevel(const u32 *__restrict__ list, u32 n, uint8_t v, uint8_t *__restrict__ level) {
+    const u32 i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) level[list[i]] = v;
 }
 
 __global__ void k_fillF(float *p, size_t n, float v) {
@@ -1077,21 +1140,6 @@ __global__ void k_fillU(u32 *p, size_t n, u32 v) {
 
 }  // namespace
 
-__global__ void k_leafCulled(const u32 *__restrict__ leafNode, u32 leafCount, const u32 *__restrict__ parents,
-                             const uint8_t *__restrict__ mask, uint8_t *__restrict__ culled, u32 *count) {
-    const u32 l = blockIdx.x * BLOCK + threadIdx.x;
-    if (l >= leafCount) return;
-    uint8_t c = 0;
-    for (u32 nd = leafNode[l]; nd != INVALID && !c; nd = parents[nd]) c = mask[nd];
-    culled[l] = c;
-    if (c) atomicAdd(count, 1u);
-}
-
-__global__ void k_countMask(const uint8_t *__restrict__ mask, u32 n, u32 *count) {
-    const u32 i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i < n && mask[i]) atomicAdd(count, 1u);
-}
-
 __global__ void k_camKeys(const u32 *__restrict__ cams, u32 links, u32 camCount, u32 base, u32 *__restrict__ key,
                           u32 *__restrict__ idx) {
     const u32 l = blockIdx.x * BLOCK + threadIdx.x;
@@ -1100,22 +1148,152 @@ __global__ void k_camKeys(const u32 *__restrict__ cams, u32 links, u32 camCount,
     idx[l] = base + l;
 }
 
-// cone traversal mode 3 with persistent threads (TSR_CONE_PERSISTENT=0: one thread per link)
-int conePersistentMode() {
-    static const int m = [] {
-        const char *e = std::getenv("TSR_CONE_PERSISTENT");
-        return e ? std::atoi(e) : 1;
-    }();
-    return m;
-}
 constexpr u32 CONE_PT_BLOCKS = 2048;   // blocks of 256 threads (about two per SIMD pair at 4 waves/SIMD)
 
-int linkOrderMode() {   // TSR_LINK_ORDER=0: threads take links in link order
-    static const int m = [] {
-        const char *e = std::getenv("TSR_LINK_ORDER");
-        return e ? std::atoi(e) : 1;
-    }();
-    return m;
+void priorRecords(tsr_context *ctx, const tsr_tree *tree, const tsr_field *field, PriorRecords &r) {
+    hipStream_t st = ctx->stream;
+    const u32 L = tree->leafCount, N = tree->nodeCount;
+    r.sumsPad.alloc((size_t)N + NODE_PAD);
+    r.linksPad.alloc((size_t)N + NODE_PAD);
+    float4 *sums = r.sumsPad.p + NODE_PAD;
+    DBuf<u32> aux(N);
+    k_subtreeLeaves<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(tree->leafNode.p, L, field->clen.p, tree->nodeDepth.p,
+                                                         field->states.p, sums, aux.p);
+    for (int lvl = (int)tree->maxDepth - 1; lvl >= 0; --lvl)
+        k_subtreeLevel<<<gridFor(N, BLOCK), BLOCK, 0, st>>>(tree->child.p, tree->nodeDepth.p, N, (u32)lvl,
+                                                            field->states.p, sums, aux.p);
+    k_packPriorLinks<<<gridFor(N, BLOCK), BLOCK, 0, st>>>(tree->child.p, aux.p, N, r.linksPad.p + NODE_PAD, sums);
+}
+
+namespace {
+
+// leaves with flag 1, in leaf order, into list; returns their count
+u32 flaggedLeaves(Scratch &sc, const u32 *flag, u32 L, u32 *list, hipStream_t st) {
+    if (L == 0) return 0;
+    DBuf<u32> pos(L);
+    exclusiveScan(sc, flag, pos.p, L, st);
+    k_priorQueryList<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(flag, pos.p, L, list);
+    return readScalar(pos.p + L - 1, st) + readScalar(flag + L - 1, st);
+}
+
+// One priors traversal over `queries` (nq leaves in leaf order): the short-stack launch,
+// then the groups that overflowed it with the long stack.  capLevels 0: exact priors into
+// prior[leaf]; otherwise depth-capped intervals into interval[query index].
+void priorsPass(tsr_context *ctx, const tsr_tree *tree, const PriorRecords &rec, const u32 *queries, u32 nq,
+                float relRadius, u32 capLevels, double *prior, double2 *interval, tsr_field *field) {
+    if (nq == 0) return;
+    hipStream_t st = ctx->stream;
+    const u32 gq = capLevels ? PRIOR_GROUP_CAPPED : PRIOR_GROUP;
+    const u32 groups = (nq + gq - 1) / gq;
+    const u32 grid = 256u * 32u;   // multiple of the XCD count
+    // counters: 8 XCD slices of the first launch, 1 for the second; overflow count; error flag
+    DBuf<u32> counters(16), overflowList(groups);
+    DBuf<unsigned long long> kst(4);
+    HIPCHECK(hipMemsetAsync(counters.p, 0, 64, st));
+    HIPCHECK(hipMemsetAsync(kst.p, 0, 32, st));
+    u32 *err = counters.p + 10;
+    const Frame f = tree->frame;
+    const uint2 *links = rec.links();
+    const double2 *sums = rec.sums();
+    const u64 *leafLoc = tree->leafLoc.p;
+    timedLaunch(ctx, KSTAT_PRIORS, [&] {
+        if (capLevels) {
+            k_priorsGroup<PRIOR_GROUP_CAPPED, PRIOR_STACK_SHORT, true><<<grid, 64, 0, st>>>(
+                f, links, sums, leafLoc, queries, nq, relRadius, capLevels, prior, interval, err, kst.p, counters.p,
+                nullptr, nullptr, overflowList.p, counters.p + 9);
+            k_priorsGroup<PRIOR_GROUP_CAPPED, PRIOR_STACK, true><<<1024, 64, 0, st>>>(
+                f, links, sums, leafLoc, queries, nq, relRadius, capLevels, prior, interval, err, kst.p, counters.p + 8,
+                overflowList.p, counters.p + 9, nullptr, nullptr);
+        } else {
+            k_priorsGroup<PRIOR_GROUP, PRIOR_STACK_SHORT, false><<<grid, 64, 0, st>>>(
+                f, links, sums, leafLoc, queries, nq, relRadius, 0, prior, interval, err, kst.p, counters.p, nullptr,
+                nullptr, overflowList.p, counters.p + 9);
+            k_priorsGroup<PRIOR_GROUP, PRIOR_STACK, false><<<1024, 64, 0, st>>>(
+                f, links, sums, leafLoc, queries, nq, relRadius, 0, prior, interval, err, kst.p, counters.p + 8,
+                overflowList.p, counters.p + 9, nullptr, nullptr);
+        }
+    });
+    unsigned long long ps[3];
+    HIPCHECK(hipMemcpyAsync(ps, kst.p, 24, hipMemcpyDeviceToHost, st));
+    u32 cnt[16];
+    HIPCHECK(hipMemcpyAsync(cnt, counters.p, 64, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    if (cnt[10]) throw tsr_error(TSR_ERR_HIP, "priors traversal stack overflow");
+    field->priorsOverflowGroups += cnt[9];
+    if (std::getenv("TSR_TRACE"))
+        std::fprintf(stderr, "[tsr occupancy] priors (cap %u): %u queries, %llu pops, %llu tests, %llu summed, "
+                             "%u of %u groups redone with the long stack\n",
+                     capLevels, nq, ps[0], ps[1], ps[2], cnt[9], groups);
+    tsr_kernel_stat &k = ctx->kstat[KSTAT_PRIORS];
+    // per query: index 4 B + leaf code 8 B + result 8 B (16 B interval); per popped entry: the
+    // 8 child links (8 B each); per child summed or bounded by some query of the group: its 16 B sums
+    k.bytes += (u64)nq * (capLevels ? 28 : 20) + ps[0] * 64 + ps[2] * 16;
+    k.units += nq;
+}
+
+}  // namespace
+
+// Priors of the queried leaves (Occupancy.cpp:530-568).  Plain: one exact traversal.
+// Certified (TSR_PRIORS_CERTIFIED, or TSR_CULL_LEARNED with the network choosing each
+// query's first stage): stage 0 bounds every query's prior with the traversal capped 3
+// levels above its depth, stage 1 with the cap 2 levels above, and k_certify keeps the
+// leaves whose occupancy class the interval decides; stage 2 computes the exact prior of
+// the rest.  The mesh is that of the plain field (equal classes), while the priors and
+// occupancies of certified leaves are interval midpoints.
+void priors(tsr_context *ctx, const tsr_tree *tree, const tsr_params &prm, int flags, const tsr_culler *culler,
+            tsr_field *field) {
+    hipStream_t st = ctx->stream;
+    Scratch &sc = ctx->scratch;
+    const u32 L = tree->leafCount;
+    const bool certified = (flags & (TSR_PRIORS_CERTIFIED | TSR_CULL_LEARNED)) != 0 && field->coneCount > 0.0;
+    PriorRecords rec;
+    priorRecords(ctx, tree, field, rec);
+    DBuf<u32> qflag(L), queries(L);
+    field->priorsOverflowGroups = 0;
+    for (auto &c : field->priorStages) c = 0;
+    if (certified) field->levels.alloc(L);
+    k_priorQueryFlags<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(field->ksum.p, L, (double)prm.confidence_threshold,
+                                                           (flags & TSR_PRIORS_ALL_LEAVES) ? 1 : 0, field->prior.p,
+                                                           qflag.p, certified ? field->levels.p : nullptr);
+    const u32 nq = flaggedLeaves(sc, qflag.p, L, queries.p, st);
+    field->priorQueries = nq;
+    field->priorStages[0] = nq;
+    const float rr = prm.relative_radius_for_priors;
+    if (!certified) {
+        priorsPass(ctx, tree, rec, queries.p, nq, rr, 0, field->prior.p, nullptr, field);
+        field->priorStages[3] = nq;
+        return;
+    }
+    // stage per leaf (levels doubles as the stage while the cascade runs, then holds the
+    // stage that decided each leaf)
+    DBuf<uint8_t> stage(L);
+    HIPCHECK(hipMemcpyAsync(stage.p, field->levels.p, L, hipMemcpyDeviceToDevice, st));
+    if (culler && nq) {   // the network's first stage per query
+        DBuf<u32> routed(2);
+        HIPCHECK(hipMemsetAsync(routed.p, 0, 8, st));
+        leafRoute(ctx, culler, tree, field, rec, queries.p, nq, stage.p, routed.p);
+        u32 r[2];
+        HIPCHECK(hipMemcpyAsync(r, routed.p, 8, hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        field->priorStages[4] = r[0];
+        field->priorStages[5] = r[1];
+    }
+    DBuf<double2> interval(std::max(nq, 1u));
+    for (u32 s = 0; s < 3; ++s) {
+        k_stageFlags<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(stage.p, L, (uint8_t)s, qflag.p);
+        const u32 n = flaggedLeaves(sc, qflag.p, L, queries.p, st);
+        field->priorStages[1 + s] = n;
+        if (s == 2) {
+            priorsPass(ctx, tree, rec, queries.p, n, rr, 0, field->prior.p, nullptr, field);
+            k_setLevel<<<gridFor(std::max(n, 1u), BLOCK), BLOCK, 0, st>>>(queries.p, n, 2, field->levels.p);
+            break;
+        }
+        priorsPass(ctx, tree, rec, queries.p, n, rr, s == 0 ? 3u : 2u, nullptr, interval.p, field);
+        if (n)
+            k_certify<<<gridFor(n, BLOCK), BLOCK, 0, st>>>(queries.p, n, interval.p, field->ksum.p, L,
+                                                           field->coneCount, (uint8_t)s, stage.p, field->levels.p,
+                                                           field->prior.p);
+    }
 }
 
 void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *scene, const tsr_params &prm, int flags,
@@ -1140,9 +1318,7 @@ void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *s
     traversalLayout(st, tree, ar.childPad, ar.blockParent, tt);
     LinkInputs in{tree->px.p, tree->py.p, tree->pz.p, tree->nx.p, tree->ny.p, tree->nz.p, tree->scale.p,
                   tree->conf.p, tree->cams.p, scene->cx.p, scene->cy.p, scene->cz.p, tree->sampleDepth.p,
-                  tree->maxCams, scene->camCount, prm.bandwidth_factor, prm.relative_sample_cone_length, nullptr};
-    in.cullNode = nullptr;
-    in.xcdMap = xcdMapMode() & 1;
+                  tree->maxCams, scene->camCount, prm.bandwidth_factor, prm.relative_sample_cone_length};
     in.linkStride = 1;
     in.order = nullptr;
     in.slots = 0;
@@ -1152,7 +1328,6 @@ void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *s
     // (Morton) order within a camera: neighbouring lanes trace nearly parallel cones
     // through the same nodes.  Records stay indexed by link, so results do not depend
     // on this order.
-    DBuf<uint8_t> cullMask;
     const bool atomicMode = (flags & 0xff) == TSR_ACCUM_FP32_ATOMIC;
 
     DBuf<u32> counts(links), offsets(links), hitCount(L), segStart(L), segEnd(L);
@@ -1175,19 +1350,6 @@ void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *s
             uint2 *packed = ar.packed.ensure((size_t)N + NODE_PAD) + NODE_PAD;
             k_packChildState<<<gridFor(N, BLOCK), BLOCK, 0, st>>>(tree->child.p, field->states.p, N, packed);
             tt.packed = packed;
-            if (culler) {   // learned culling: nodes predicted far from any zero crossing
-                cullerMask(ctx, culler, tree, scene, prm, field->states.p, field->hitsS.p,
-                           field->ksum.p + L, cullMask);
-                in.cullNode = cullMask.p;
-                field->culled.alloc(L);
-                DBuf<u32> cnt(2);
-                HIPCHECK(hipMemsetAsync(cnt.p, 0, 8, st));
-                k_countMask<<<gridFor(N, BLOCK), BLOCK, 0, st>>>(cullMask.p, N, cnt.p);
-                k_leafCulled<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(tree->leafNode.p, L, tree->parents.p, cullMask.p,
-                                                                  field->culled.p, cnt.p + 1);
-                field->culledNodes = readScalar(cnt.p, st);
-                field->culledLeaves = readScalar(cnt.p + 1, st);
-            }
         }
         if (atomicMode) {
             HIPCHECK(hipMemsetAsync(ksum, 0, (size_t)L * 4, st));
@@ -1269,7 +1431,7 @@ void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *s
                 ic.linkBase = l0;
                 ic.order = nullptr;
                 DBuf<u32> chunkOrder;
-                if (!sampleness && linkOrderMode()) {   // the chunk's links grouped by camera
+                if (!sampleness) {   // the chunk's links grouped by camera
                     DBuf<u32> camKey(n), camKeyS(n), iota(n);
                     k_camKeys<<<gridFor(n, BLOCK), BLOCK, 0, st>>>(tree->cams.p + l0, n, scene->camCount, l0,
                                                                    camKey.p, iota.p);
@@ -1286,13 +1448,13 @@ void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *s
                 lap("order + staging");
                 HIPCHECK(hipMemsetAsync(kst.p, 0, 32, st));
                 // persistent threads pay off for the long emptiness traversals only
-                const bool persistent = !sampleness && conePersistentMode();
+                const bool persistent = !sampleness;
                 if (persistent) HIPCHECK(hipMemsetAsync(linkCursor.p, 0, 32, st));
                 timedLaunch(ctx, sampleness ? KSTAT_S_COUNT : KSTAT_E_COUNT, [&] {
                     if (persistent)
                         k_conePassPT<<<std::min<u32>(gridFor(n, BLOCK), CONE_PT_BLOCKS), BLOCK, 0, st>>>(
                             tt, ic, n, sampleness, counts.p, stLeaf, stVal, sampleness ? nullptr : confs.p,
-                            linkCursor.p, (xcdMapMode() & 4) ? XCD_COUNT : 1u, kst.p);
+                            linkCursor.p, XCD_COUNT, kst.p);
                     else
                         k_conePass<<<xcdGrid(n, BLOCK), BLOCK, 0, st>>>(tt, ic, n, sampleness, 3, counts.p, nullptr,
                                                                         stLeaf, stVal, nullptr, nullptr, nullptr,
@@ -1406,70 +1568,11 @@ void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *s
         recordPhase(ctx, 1 + pass, false);
     }
     recordPhase(ctx, 3, true);
-    // priors
-    {
-        // node i at index i + NODE_PAD of the traversal records (child blocks line-aligned)
-        DBuf<float4> sumsPad(N + NODE_PAD);
-        DBuf<uint2> linksPad(N + NODE_PAD);
-        float4 *sums = sumsPad.p + NODE_PAD;
-        uint2 *plinks = linksPad.p + NODE_PAD;
-        DBuf<u32> aux(N), queries(L), perr(1);
-        k_subtreeLeaves<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(tree->leafNode.p, L, field->clen.p, tree->nodeDepth.p,
-                                                             field->states.p, sums, aux.p);
-        for (int lvl = (int)tree->maxDepth - 1; lvl >= 0; --lvl)
-            k_subtreeLevel<<<gridFor(N, BLOCK), BLOCK, 0, st>>>(tree->child.p, tree->nodeDepth.p, N, (u32)lvl,
-                                                                field->states.p, sums, aux.p);
-        HIPCHECK(hipMemsetAsync(perr.p, 0, 4, st));
-        DBuf<u32> qflag(L), qpos(L);
-        k_priorQueryFlags<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(field->ksum.p, L, (double)prm.confidence_threshold,
-                                                               (flags & TSR_PRIORS_ALL_LEAVES) ? 1 : 0,
-                                                               culler ? field->culled.p : nullptr, field->prior.p,
-                                                               qflag.p);
-        exclusiveScan(sc, qflag.p, qpos.p, L, st);
-        k_priorQueryList<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(qflag.p, qpos.p, L, queries.p);
-        const u32 nq = readScalar(qpos.p + L - 1, st) + readScalar(qflag.p + L - 1, st);
-        k_packPriorLinks<<<gridFor(N, BLOCK), BLOCK, 0, st>>>(tree->child.p, aux.p, N, plinks, sums);
-        const double2 *sumsD = reinterpret_cast<const double2 *>(sums);
-        field->priorQueries = nq;
-        if (nq) {
-            constexpr u32 Q = 8;
-            const u32 groups = (nq + Q - 1) / Q;
-            const u32 grid = 256u * 32u;   // multiple of the XCD count
-            HIPCHECK(hipMemsetAsync(kst.p, 0, 32, st));
-            // counters: 8 XCD slices of the first pass, 1 for the second; overflow count
-            DBuf<u32> counters(16), overflowList(groups);
-            HIPCHECK(hipMemsetAsync(counters.p, 0, 64, st));
-            const int xcdMap = (xcdMapMode() >> 1) & 1;
-            timedLaunch(ctx, KSTAT_PRIORS, [&] {
-                k_priorsGroup<Q, PRIOR_STACK_SHORT><<<grid, 64, 0, st>>>(
-                    tree->frame, plinks, sumsD, tree->leafLoc.p, queries.p, nq, prm.relative_radius_for_priors,
-                    field->prior.p, perr.p, kst.p, xcdMap, counters.p, nullptr, nullptr, overflowList.p,
-                    counters.p + 9);
-                k_priorsGroup<Q, PRIOR_STACK><<<1024, 64, 0, st>>>(
-                    tree->frame, plinks, sumsD, tree->leafLoc.p, queries.p, nq, prm.relative_radius_for_priors,
-                    field->prior.p, perr.p, kst.p, 0, counters.p + 8, overflowList.p, counters.p + 9, nullptr,
-                    nullptr);
-            });
-            unsigned long long ps[3];
-            readStats(3, ps);
-            field->priorsOverflowGroups = readScalar(counters.p + 9, st);
-            if (std::getenv("TSR_TRACE"))
-                std::fprintf(stderr, "[tsr occupancy] priors: %u queries, %llu pops, %llu tests, %llu summed, "
-                                     "%u of %u groups redone with the long stack\n",
-                             nq, ps[0], ps[1], ps[2], field->priorsOverflowGroups, groups);
-            tsr_kernel_stat &k = ctx->kstat[KSTAT_PRIORS];
-            // per query: index 4 B + leaf code 8 B + prior 8 B; per popped entry: the 8 child
-            // links (8 B each); per child summed by some query of the group: its 16 B sums
-            k.bytes += (u64)nq * 20 + ps[0] * 64 + ps[2] * 16;
-            k.units += nq;
-        }
-        if (readScalar(perr.p, st)) throw tsr_error(TSR_ERR_HIP, "priors traversal stack overflow");
-    }
+    priors(ctx, tree, prm, flags, culler, field);
     recordPhase(ctx, 3, false);
     recordPhase(ctx, 4, true);
     k_occupancy<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(field->ksum.p, field->prior.p, L, field->coneCount,
-                                                     (double)prm.confidence_threshold,
-                                                     culler ? field->culled.p : nullptr, field->occ.p);
+                                                     (double)prm.confidence_threshold, field->occ.p);
     recordPhase(ctx, 4, false);
     HIPCHECK(hipStreamSynchronize(st));
 }
@@ -1480,25 +1583,8 @@ void occupancyValues(tsr_context *ctx, const tsr_field *field, double confidence
     tsr_field *fw = const_cast<tsr_field *>(field);
     if (fw->occ.n < L) fw->occ.alloc(L);
     k_occupancy<<<gridFor(L, BLOCK), BLOCK, 0, ctx->stream>>>(field->ksum.p, field->prior.p, L, field->coneCount,
-                                                              confidenceThreshold, nullptr, fw->occ.p);
+                                                              confidenceThreshold, fw->occ.p);
     HIPCHECK(hipStreamSynchronize(ctx->stream));
-}
-
-void emptinessSubsample(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *scene, const tsr_params &prm,
-                        const u32 *states, u32 stride, u32 *leafHits) {
-    hipStream_t st = ctx->stream;
-    const u32 links = (u32)((u64)scene->n * scene->maxCams);
-    TraversalTree tt{nullptr, nullptr, states, tree->frame, (float)(tree->frame.rootSize * 1e-5), nullptr};
-    DBuf<u32> childPad, blockParent;
-    traversalLayout(st, tree, childPad, blockParent, tt);
-    LinkInputs in{tree->px.p, tree->py.p, tree->pz.p, tree->nx.p, tree->ny.p, tree->nz.p, tree->scale.p,
-                  tree->conf.p, tree->cams.p, scene->cx.p, scene->cy.p, scene->cz.p, tree->sampleDepth.p,
-                  tree->maxCams, scene->camCount, prm.bandwidth_factor, prm.relative_sample_cone_length, nullptr,
-                  0, stride, nullptr, 0, 0};
-    HIPCHECK(hipMemsetAsync(leafHits, 0, (size_t)tree->leafCount * 4, st));
-    const u32 work = (links + stride - 1) / stride;
-    k_conePass<<<gridFor(work, BLOCK), BLOCK, 0, st>>>(tt, in, links, 0, 4, nullptr, nullptr, nullptr, nullptr,
-                                                       nullptr, nullptr, leafHits, nullptr, nullptr);
 }
 
 __global__ void k_calibrationCopy(const float4 *__restrict__ src, float4 *__restrict__ dst, size_t n) {
