@@ -155,6 +155,11 @@ tsr_status tsr_scene_download(tsr_context *ctx, const tsr_scene *scene, float *p
  * from the same scene gives the same tree. */
 tsr_status tsr_tree_build(tsr_context *ctx, const tsr_scene *scene, const tsr_params *params, tsr_tree **out);
 tsr_status tsr_tree_get_info(const tsr_tree *tree, tsr_tree_info *info);
+/* Statistics of the sampling refinement (Nodes.cpp:433-517; no reference counterpart):
+ * [0] sampling units (2 per sample), [1] units that refined, [2] nodes they created,
+ * [3] visited-leaf lookups answered on the device (leaves outside the transferred part of
+ * the visit index; TSR_HOT_RADIUS, in max sampling distances, sizes that part). */
+tsr_status tsr_tree_build_stats(const tsr_tree *tree, uint64_t *stats4);
 /* Host copies of the tables (NULL pointers are skipped):
  *   children/parents/sample_counts/sample_starts: node_count (Nodes.h:143-147)
  *   node_to_leaf: node_count (Leaves.h:90); leaf_nodes: leaf_count

@@ -144,3 +144,32 @@ def test_link_chunks_match_single_pass(gpu_ctx, monkeypatch):
     assert np.array_equal(m1["triangles"], m2["triangles"])
     for h in (many, one, tree, sc):
         h.close()
+
+
+@pytest.mark.parametrize("radius", ["0", "0.3"])
+def test_device_lookups_of_the_visit_index(gpu_ctx, monkeypatch, radius):
+    """TSR_HOT_RADIUS shrinks the part of the sampling visit index sent to the host, so the
+    in-order replay asks the device for the visits of the leaves outside it: the tree is the
+    oracle's, bit for bit, and the device lookups did run."""
+    import tsr_amd
+    from oracle.oracle import Oracle
+    s, c = scene(3, 0.125, seed=5)
+    print(f"\n[device lookups] config=3 res=0.125 seed=5 TSR_HOT_RADIUS={radius}", flush=True)
+    monkeypatch.setenv("TSR_HOT_RADIUS", radius)
+    sc = tsr_amd.Scene(gpu_ctx, s, c)
+    tree = tsr_amd.Tree(gpu_ctx, sc)
+    monkeypatch.delenv("TSR_HOT_RADIUS")
+    st = tree.build_stats()
+    print(f"build stats {st}", flush=True)
+    assert st["events"] > 0 and st["device_lookups"] > 0
+    t, o = tree.download(), Oracle(s, c, PARAMS).build_tree().tree()
+    for k in ("children", "parents", "sample_counts", "sample_starts", "node_to_leaf", "leaf_nodes",
+              "sample_target", "cells"):
+        assert np.array_equal(t[k], o[k]), k
+    # the default radius answers every lookup from the transferred part
+    tree2 = tsr_amd.Tree(gpu_ctx, sc)
+    assert tree2.build_stats()["device_lookups"] == 0
+    assert np.array_equal(tree2.download()["children"], t["children"])
+    tree2.close()
+    tree.close()
+    sc.close()

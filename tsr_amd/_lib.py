@@ -101,6 +101,7 @@ SIGNATURES = [
     ("tsr_scene_download", ctypes.c_int, [vp, vp, f32p, f32p, f32p, f32p, f32p, f32p, f32p, f32p, u32p]),
     ("tsr_tree_build", ctypes.c_int, [vp, vp, ctypes.POINTER(Params), ctypes.POINTER(vp)]),
     ("tsr_tree_get_info", ctypes.c_int, [vp, ctypes.POINTER(TreeInfo)]),
+    ("tsr_tree_build_stats", ctypes.c_int, [vp, u64p]),
     ("tsr_tree_download", ctypes.c_int, [vp, vp, u32p, u32p, u32p, u32p, u32p, u32p, f32p, u32p, u32p, u32p]),
     ("tsr_tree_samples_download", ctypes.c_int, [vp, vp, f32p, f32p, f32p, f32p, f32p, f32p, f32p, f32p, u32p]),
     ("tsr_tree_destroy", None, [vp]),

@@ -221,6 +221,12 @@ class Tree:
                                           g("sample_target", _up), g("node_depth", _up)))
         return o
 
+    def build_stats(self):
+        """sampling refinement statistics: units, refining units, created nodes, device lookups"""
+        a = np.zeros(4, np.uint64)
+        L.check(L.lib().tsr_tree_build_stats(self.h, a.ctypes.data_as(L.u64p)))
+        return {k: int(v) for k, v in zip(("units", "events", "created", "device_lookups"), a)}
+
     def neighbors(self):
         """Leaves::getNeighbors for all leaves: (offsets[leaf_count + 1], neighbours) CSR."""
         total = ctypes.c_uint64(0)

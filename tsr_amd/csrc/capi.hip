@@ -247,6 +247,15 @@ tsr_status tsr_tree_get_info(const tsr_tree *t, tsr_tree_info *info) {
     return TSR_OK;
 }
 
+tsr_status tsr_tree_build_stats(const tsr_tree *t, uint64_t *stats4) {
+    if (!t || !stats4) return fail(TSR_ERR_INVALID_ARGUMENT, "null argument");
+    stats4[0] = t->samplingUnits;
+    stats4[1] = t->samplingEvents;
+    stats4[2] = t->samplingCreated;
+    stats4[3] = t->samplingMisses;
+    return TSR_OK;
+}
+
 tsr_status tsr_tree_download(tsr_context *ctx, const tsr_tree *t, uint32_t *children, uint32_t *parents,
                              uint32_t *sample_counts, uint32_t *sample_starts, uint32_t *node_to_leaf,
                              uint32_t *leaf_nodes, float *leaf_scopes, uint32_t *cells, uint32_t *sample_target,

@@ -81,6 +81,7 @@ struct tsr_tree {
     tsr::DBuf<tsr::u32> cams;
     tsr::DHash nodeHash;               // location code -> node id (all nodes)
     tsr::u64 samplingEvents = 0, samplingCreated = 0, samplingUnits = 0;
+    tsr::u64 samplingMisses = 0;   // visit-index lookups answered by the device (leaves outside the hot part)
     // leaf face-neighbour graph (Leaves.cpp:82-180), CSR built on request or loaded
     bool hasNeighbours = false;
     tsr::DBuf<tsr::u32> nbrOffsets, nbrs;

@@ -40,8 +40,13 @@ constexpr u32 VISIT_CAP = 48;   // visited-leaf records per sampling unit before
 // transferred part of the visit index, in max sampling distances: a refinement subdivides
 // only nodes that meet its unit's segment [p, p +- n msd], so the T0 leaves holding them lie
 // within msd of p (5% more covers float rounding of p, n and msd); leaves outside are
-// looked up exactly on the device when a cascade reaches them
-constexpr double HOT_RADIUS = 1.05;
+// looked up exactly on the device when a cascade reaches them.  TSR_HOT_RADIUS overrides
+// the radius (read at every build; a smaller one sends more lookups to the device path,
+// with the same tree).
+double hotRadius() {
+    const char *e = std::getenv("TSR_HOT_RADIUS");
+    return e ? std::atof(e) : 1.05;
+}
 
 // error bits written by kernels
 enum : u32 { ERR_OUTSIDE = 1, ERR_TOO_DEEP = 2, ERR_HASH_FULL = 4, ERR_NOT_INSERTED = 8 };
@@ -1156,12 +1161,12 @@ void replaySampling(const Frame &f, const tsr_scene *scene, float maxRel, Replay
 // settle: the writes of the in-order replay (and of short cascades) land there.
 __global__ void k_markHot(Frame f, HashView inner, HashView hot, const float *__restrict__ px,
                           const float *__restrict__ py, const float *__restrict__ pz, const float *__restrict__ scale,
-                          float maxRel, const uint8_t *__restrict__ isEvent, u32 units, u32 *err) {
+                          float maxRel, double hotRadius, const uint8_t *__restrict__ isEvent, u32 units, u32 *err) {
     const u32 unit = blockIdx.x * BLOCK + threadIdx.x;
     if (unit >= units || !isEvent[unit] || ((unit & 1) && isEvent[unit - 1])) return;
     const u32 i = unit >> 1;
     const double p[3] = {px[i], py[i], pz[i]};
-    const double r = HOT_RADIUS * (double)(scale[i] * maxRel);
+    const double r = hotRadius * (double)(scale[i] * maxRel);
     auto overlaps = [&](u64 loc) {
         u32 k[3];
         locCoords(loc, k[0], k[1], k[2]);
@@ -1268,11 +1273,6 @@ __global__ void k_insertIndexed(HashView h, const u64 *__restrict__ keys, u32 n,
 __global__ void k_insertConst(HashView h, const u64 *__restrict__ keys, u32 n, u32 val, u32 *err) {
     const u32 i = blockIdx.x * BLOCK + threadIdx.x;
     if (i < n) h.insert(keys[i], val, err);
-}
-
-__global__ void k_matchLeaf(const u64 *__restrict__ leaf, size_t n, u64 key, uint8_t *__restrict__ flag) {
-    const size_t j = (size_t)blockIdx.x * BLOCK + threadIdx.x;
-    if (j < n) flag[j] = leaf[j] == key ? 1 : 0;
 }
 
 void checkErr(u32 e) {
@@ -1512,7 +1512,8 @@ void buildTree(tsr_context *ctx, const tsr_scene *scene, const tsr_params &prm, 
     DHash hot;
     hot.alloc(std::max<u64>(4096, (u64)units / 2), st);
     k_markHot<<<gridFor(units, BLOCK), BLOCK, 0, st>>>(f, inner.view(), hot.view(), scene->px.p, scene->py.p,
-                                                       scene->pz.p, scene->scale.p, maxRel, isEvent.p, units, err.p);
+                                                       scene->pz.p, scene->scale.p, maxRel, hotRadius(), isEvent.p,
+                                                       units, err.p);
     DBuf<u32> counter(1);
     size_t hotPairs = 0;
     DBuf<u64> hotLeafSorted;
@@ -1635,6 +1636,7 @@ void buildTree(tsr_context *ctx, const tsr_scene *scene, const tsr_params &prm, 
     tree->samplingEvents = events;
     tree->samplingCreated = 8 * (u64)created.size();
     tree->samplingUnits = units;
+    tree->samplingMisses = index.misses;
 
     // re-size the inner hash with room for balancing, insert T0 + created
     {
