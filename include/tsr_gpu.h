@@ -165,7 +165,8 @@ tsr_status tsr_tree_build_stats(const tsr_tree *tree, uint64_t *stats4);
  *   node_to_leaf: node_count (Leaves.h:90); leaf_nodes: leaf_count
  *   leaf_scopes: 4 floats per leaf = min x,y,z, size (Scope, Scope.h:36-38)
  *   cells: 8 * cell_count leaf indices (DualCells.h:67)
- *   sample_target: new index of each original sample (Nodes.cpp:625-626)
+ *   sample_target: new index of each original sample (Nodes.cpp:625-626), one per sample the
+ *   tree was built from (0xFFFFFFFF for samples erased since)
  *   node_depth: node_count */
 tsr_status tsr_tree_download(tsr_context *ctx, const tsr_tree *tree, uint32_t *children, uint32_t *parents,
                              uint32_t *sample_counts, uint32_t *sample_starts, uint32_t *node_to_leaf,
@@ -217,6 +218,23 @@ tsr_status tsr_field_download(tsr_context *ctx, const tsr_field *field, float *k
                               double *priors, uint32_t *node_states, double *occupancy, double *cone_count);
 /* counts of (link, leaf) kernel contributions of the last passes: [0] = E, [1] = S */
 tsr_status tsr_field_hit_counts(const tsr_field *field, uint64_t *hits2);
+/* Occupancy::addSamples / eraseSamples (Occupancy.h:57-59,167-175; Occupancy.cpp:129-136,
+ * 138-248): the field follows a change of samples.  The S pass, the S flags up the tree and
+ * the gated E pass run over the camera links of the chosen samples only, and their per-leaf
+ * kernel sums, cone lengths and cone count M are added to (erase: subtracted from) the
+ * field's; node flags are only ever set, as in the reference.  Priors (in the field's priors
+ * mode) and the occupancy ratio are then recomputed.  samples: indices into the tree's
+ * samples (node order, as Samples after Nodes::reorderSamples).  Scene::eraseSamples
+ * (Scene.cpp:270-296) erases from the field first, then from the tree. */
+tsr_status tsr_occupancy_add_samples(tsr_context *ctx, tsr_field *field, const tsr_tree *tree, const tsr_scene *scene,
+                                     const uint32_t *samples, uint32_t count);
+tsr_status tsr_occupancy_erase_samples(tsr_context *ctx, tsr_field *field, const tsr_tree *tree,
+                                       const tsr_scene *scene, const uint32_t *samples, uint32_t count);
+/* Tree::eraseSamples (Tree.cpp:50-53, Nodes.cpp:644-671) with Samples::compact: drops the
+ * tree's samples whose inliers[i] == 0 (node order, one byte per sample) and moves every
+ * node's sample count and start accordingly; the input-to-node sample map maps erased samples
+ * to 0xFFFFFFFF.  The scene (input-order samples, cameras) is not changed. */
+tsr_status tsr_tree_erase_samples(tsr_context *ctx, tsr_tree *tree, const uint8_t *inliers);
 /* Priors stages (6 counts): queried leaves; leaves entering stage 0, 1, 2 (2 = exact);
  * leaves the routing MLP sent straight to stage 1, to stage 2.  A plain field has every
  * query at stage 2. */

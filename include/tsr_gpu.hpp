@@ -210,6 +210,17 @@ public:
         check(tsr_tree_samples_save(ctx, mTree, s.c_str()));
     }
 
+    // Tree.cpp:50-53 / Nodes.cpp:644-671 with Samples::compact: sampleOffsets[i] = samples
+    // erased before sample i (node order, sample count + 1 entries, as Scene.cpp:270-289 forms them)
+    void eraseSamples(const uint32 *sampleOffsets) {
+        uint32 S = 0;
+        for (uint32 c : mNodes.mSampleCounts) S += c;
+        std::vector<uint8_t> inliers(S);
+        for (uint32 i = 0; i < S; ++i) inliers[i] = sampleOffsets[i + 1] == sampleOffsets[i] ? 1 : 0;
+        check(tsr_tree_erase_samples(mScene->getContext().get(), mTree, inliers.data()));
+        mirror();
+    }
+
 private:
     void setParams(const tsr_params *params) {
         tsr_params p;
@@ -365,6 +376,18 @@ public:
     // Occupancy.cpp:722-766 (Real = float unless realBytes = 8)
     void saveToFile(const std::string &fileName, int realBytes = 4) const {
         check(tsr_field_save(mTree->getScene().getContext().get(), mField, fileName.c_str(), realBytes));
+    }
+    // Occupancy.h:57-59,167-175 (Occupancy.cpp:129-136): the field follows added or erased
+    // samples (indices into the tree's node-ordered samples); erase before Tree::eraseSamples
+    void addSamples(const uint32 *chosenSamples, const uint32 count) {
+        check(tsr_occupancy_add_samples(mTree->getScene().getContext().get(), mField, mTree->get(),
+                                        mTree->getScene().get(), chosenSamples, count));
+        mirror();
+    }
+    void eraseSamples(const uint32 *doomedSamples, const uint32 count) {
+        check(tsr_occupancy_erase_samples(mTree->getScene().getContext().get(), mField, mTree->get(),
+                                          mTree->getScene().get(), doomedSamples, count));
+        mirror();
     }
 
 private:

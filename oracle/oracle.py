@@ -146,6 +146,17 @@ class Oracle:
         n = self.count("leaves")
         return self.array("neighbour_offsets", np.uint32, n + 1), self.array("neighbours", np.uint32, m)
 
+    def update(self, removal, samples):
+        """Occupancy::addSamples / eraseSamples after an FP64 occupancy(): samples are
+        ORIGINAL (input order) indices"""
+        L = lib()
+        L.oracle_update.restype = ctypes.c_int
+        L.oracle_update.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint32]
+        q = np.ascontiguousarray(samples, np.uint32)
+        if L.oracle_update(self.h, int(bool(removal)), q.ctypes.data, len(q)) != 0:
+            raise RuntimeError(L.oracle_error(self.h).decode())
+        return self
+
     def prior_bounds(self, leaves, cap_levels, cone_lengths=None):
         """Depth-capped prior intervals (lo, hi) of the given leaves after an FP64
         occupancy(); cone_lengths (2 * leaf count) replaces the field's own"""
@@ -288,3 +299,18 @@ def certified(lo, hi, e, s, M):
     a, b = occupancy_class(lo, e, s, M), occupancy_class(hi, e, s, M)
     ok = (a >= 0) & (a == b) & (lo >= 0) & (hi <= 1) & (lo <= hi)
     return np.where(ok, a, -1)
+
+
+def erase_samples_in_nodes(counts, starts, inliers):
+    """Nodes::eraseSamples (Nodes.cpp:644-671) with the sample offsets Scene::eraseSamples forms
+    (Scene.cpp:270-289): offsets[i] = erased samples before i; per node the erased samples of
+    its range leave its count, and its start moves back by the erased samples before it.
+    Returns (counts, starts) after the erasure."""
+    keep = np.asarray(inliers, bool)
+    offsets = np.zeros(len(keep) + 1, np.int64)
+    offsets[1:] = np.cumsum(~keep)
+    counts = np.asarray(counts, np.int64)
+    starts = np.asarray(starts, np.int64)
+    ends = starts + counts
+    deleted = offsets[ends] - offsets[starts]
+    return (counts - deleted).astype(np.uint32), (starts - offsets[starts]).astype(np.uint32)

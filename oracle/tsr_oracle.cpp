@@ -631,18 +631,22 @@ struct OccupancyT {
         return states[n];
     }
 
-    // Occupancy.cpp:138-248 updateKernels
-    void pass(bool sampleness) {
+    // Occupancy.cpp:138-248 updateKernels; with `chosen`, over the links of those samples only
+    // (getCameraToSampleLinkData, Occupancy.cpp:250-277), kernels and lengths negated and no
+    // flags set when `removal` (addKernel's `negative`, Occupancy.cpp:279-346)
+    void pass(bool sampleness, const u32 *chosen = nullptr, u32 chosenCount = 0, bool removal = false) {
         const int type = sampleness ? 1 : 0;
         if (!sampleness) forward(0, FLAG_S);
         const u32 leafFlag = sampleness ? FLAG_S : FLAG_E;
         const u32 required = sampleness ? 0u : FLAG_S;
-        const long long linkCount = (long long)s.n * s.maxCams;
+        const Real sign = removal ? Real(-1) : Real(1);
+        const long long linkCount = (long long)(chosen ? chosenCount : s.n) * s.maxCams;
         double mLocal = 0;
         u64 hitsLocal = 0, visitsLocal = 0;
 #pragma omp parallel for schedule(dynamic, 128) reduction(+ : mLocal, hitsLocal, visitsLocal) num_threads(threads) if (threads > 1)
-        for (long long link = 0; link < linkCount; ++link) {
-            const u32 i = (u32)(link / s.maxCams);   // Samples.h:419-437
+        for (long long pos = 0; pos < linkCount; ++pos) {
+            const u32 i = chosen ? chosen[pos / s.maxCams] : (u32)(pos / s.maxCams);   // Samples.h:419-437
+            const long long link = (long long)i * s.maxCams + pos % s.maxCams;
             const u32 cam = s.cams[link];
             if (cam >= s.camCount) continue;          // Cameras.h:171-176 isValid
             const Real camWS[3] = {(Real)s.cx[cam], (Real)s.cy[cam], (Real)s.cz[cam]};
@@ -661,8 +665,8 @@ struct OccupancyT {
             cone.init(camWS, nrm, end, R, L);
             if (cone.degenerate) continue;
             const Real conf = (Real)s.conf[i];
-            const Real weightedLength = conf * cone.length;
-            if (!sampleness) mLocal += (double)conf;
+            const Real weightedLength = sign * conf * cone.length;
+            if (!sampleness) mLocal += (double)(sign * conf);
             const u32 maxDepth = t.sampleDepth[i] + MAX_DEPTH_DIFFERENCE;   // Occupancy.cpp:518-521
             // LeavesIterator over ConeNodesChecker (NodesIterator.cpp:38-86, ConeNodesChecker.cpp:50-66)
             u32 stack[8 * 24];
@@ -691,14 +695,14 @@ struct OccupancyT {
                 ++hitsLocal;
                 if (threads > 1) {
 #pragma omp atomic
-                    kernelSum[type][l] += k;
+                    kernelSum[type][l] += sign * k;
 #pragma omp atomic
                     coneLength[type][l] += weightedLength;
-                    states[nd] |= leafFlag;   // benign race as in the reference (Occupancy.cpp:233)
+                    if (!removal) states[nd] |= leafFlag;   // benign race as in the reference (Occupancy.cpp:233)
                 } else {
-                    kernelSum[type][l] += k;
+                    kernelSum[type][l] += sign * k;
                     coneLength[type][l] += weightedLength;
-                    states[nd] |= leafFlag;
+                    if (!removal) states[nd] |= leafFlag;
                 }
             }
         }
@@ -1120,6 +1124,31 @@ int oracle_occupancy(void *hp, int mode) {
         for (u32 l = 0; l < L; ++l) o.occupancy[l] = o.getOccupancy(l);
         double t4 = now();
         h->timings[1] = t1 - t0; h->timings[2] = t2 - t1; h->timings[3] = t3 - t2; h->timings[4] = t4 - t3;
+        h->occValues = o.occupancy;
+        for (u32 l = 0; l < L; ++l) {
+            for (int k = 0; k < 2; ++k) { h->ksum[k * L + l] = o.kernelSum[k][l]; h->clen[k * L + l] = o.coneLength[k][l]; }
+            h->prior[l] = o.priorE[l];
+        }
+        return 0;
+    } catch (const std::exception &e) {
+        h->err = e.what();
+        return 1;
+    }
+}
+
+// Occupancy::addSamples / eraseSamples (Occupancy.h:167-175, 222-227) after an FP64
+// oracle_occupancy: the S and E passes over the chosen samples (ORIGINAL sample indices),
+// then priors and the occupancy ratio of every leaf
+int oracle_update(void *hp, int removal, const u32 *chosen, u32 n) {
+    Handle *h = (Handle *)hp;
+    try {
+        if (!h->occ) throw std::runtime_error("needs an FP64 occupancy");
+        auto &o = *h->occ;
+        o.pass(true, chosen, n, removal != 0);
+        o.pass(false, chosen, n, removal != 0);
+        o.priors();
+        const u32 L = (u32)h->t.leafNode.size();
+        for (u32 l = 0; l < L; ++l) o.occupancy[l] = o.getOccupancy(l);
         h->occValues = o.occupancy;
         for (u32 l = 0; l < L; ++l) {
             for (int k = 0; k < 2; ++k) { h->ksum[k * L + l] = o.kernelSum[k][l]; h->clen[k * L + l] = o.coneLength[k][l]; }

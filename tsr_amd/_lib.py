@@ -119,6 +119,9 @@ SIGNATURES = [
     ("tsr_occupancy_compute", ctypes.c_int, [vp, vp, vp, ctypes.POINTER(Params), ctypes.c_int, vp, ctypes.POINTER(vp)]),
     ("tsr_field_download", ctypes.c_int, [vp, vp, f32p, f32p, f64p, u32p, f64p, f64p]),
     ("tsr_field_hit_counts", ctypes.c_int, [vp, u64p]),
+    ("tsr_occupancy_add_samples", ctypes.c_int, [vp, vp, vp, vp, u32p, ctypes.c_uint32]),
+    ("tsr_occupancy_erase_samples", ctypes.c_int, [vp, vp, vp, vp, u32p, ctypes.c_uint32]),
+    ("tsr_tree_erase_samples", ctypes.c_int, [vp, vp, u8p]),
     ("tsr_field_prior_stages", ctypes.c_int, [vp, u32p]),
     ("tsr_field_prior_levels", ctypes.c_int, [vp, vp, u8p]),
     ("tsr_field_destroy", None, [vp]),

@@ -189,6 +189,7 @@ class Tree:
         self.root_min = np.array(info.root_min[:], np.float32)
         self.root_size = np.float32(info.root_size)
         self.sample_count = scene.count
+        self.input_count = scene.count   # samples the tree was built from
         self.max_cams = scene.max_cams
 
     def close(self):
@@ -207,7 +208,7 @@ class Tree:
 
     def download(self, keys=None):
         """Host copies of the tables; `keys` limits the download to those tables."""
-        N, Lc, C, S = self.node_count, self.leaf_count, self.cell_count, self.sample_count
+        N, Lc, C, S = self.node_count, self.leaf_count, self.cell_count, self.input_count
         shape = {"children": (N, np.uint32), "parents": (N, np.uint32), "sample_counts": (N, np.uint32),
                  "sample_starts": (N, np.uint32), "node_to_leaf": (N, np.uint32), "leaf_nodes": (Lc, np.uint32),
                  "leaf_scopes": (4 * Lc, np.float32), "cells": (8 * C, np.uint32), "sample_target": (S, np.uint32),
@@ -220,6 +221,12 @@ class Tree:
                                           g("leaf_nodes", _up), g("leaf_scopes", _fp), g("cells", _up),
                                           g("sample_target", _up), g("node_depth", _up)))
         return o
+
+    def erase_samples(self, inliers):
+        """Tree::eraseSamples + Samples::compact: keep the samples with inliers[i] != 0 (node order)"""
+        m = np.ascontiguousarray(np.asarray(inliers) != 0, np.uint8)
+        L.check(L.lib().tsr_tree_erase_samples(self.ctx.h, self.h, m.ctypes.data_as(L.u8p)))
+        self.sample_count = int(m.sum())
 
     def build_stats(self):
         """sampling refinement statistics: units, refining units, created nodes, device lookups"""
@@ -261,6 +268,7 @@ class Tree:
         t.root_min = np.array(info.root_min[:], np.float32)
         t.root_size = np.float32(info.root_size)
         t.sample_count = scene.count
+        t.input_count = scene.count
         t.max_cams = scene.max_cams
         return t
 
@@ -315,6 +323,18 @@ class Occupancy:
         o.leaf_count = tree.leaf_count
         o.node_count = tree.node_count
         return o
+
+    def add_samples(self, tree, scene, samples):
+        """Occupancy::addSamples: the field follows added samples (node-order indices)"""
+        idx = np.ascontiguousarray(samples, np.uint32)
+        L.check(L.lib().tsr_occupancy_add_samples(self.ctx.h, self.h, tree.h, scene.h, idx.ctypes.data_as(L.u32p),
+                                                  len(idx)))
+
+    def erase_samples(self, tree, scene, samples):
+        """Occupancy::eraseSamples: the field follows erased samples (node-order indices)"""
+        idx = np.ascontiguousarray(samples, np.uint32)
+        L.check(L.lib().tsr_occupancy_erase_samples(self.ctx.h, self.h, tree.h, scene.h, idx.ctypes.data_as(L.u32p),
+                                                    len(idx)))
 
     def prior_stages(self):
         """dict of the priors stages: queries, entering stage 0 / 1 / 2 (2 = exact), and

@@ -277,7 +277,7 @@ tsr_status tsr_tree_download(tsr_context *ctx, const tsr_tree *t, uint32_t *chil
         download(node_to_leaf, t->nodeToLeaf, N, st);
         download(leaf_nodes, t->leafNode, L, st);
         download(cells, t->cells, 8 * (size_t)t->cellCount, st);
-        download(sample_target, t->sampleTarget, t->sampleCount, st);
+        download(sample_target, t->sampleTarget, t->sampleTarget.n, st);
         if (node_depth) {
             std::vector<uint8_t> d(N);
             HIPCHECK(hipMemcpyAsync(d.data(), t->nodeDepth.p, N, hipMemcpyDeviceToHost, st));
@@ -432,7 +432,7 @@ tsr_status tsr_occupancy_compute(tsr_context *ctx, const tsr_tree *tree, const t
                                  const tsr_params *params, int flags, const tsr_culler *culler, tsr_field **out) {
     if (!ctx || !tree || !scene || !out) return fail(TSR_ERR_INVALID_ARGUMENT, "null argument");
     *out = nullptr;
-    if (scene->n != tree->sampleCount) return fail(TSR_ERR_INVALID_ARGUMENT, "scene and tree sample counts differ");
+    if (tree->maxCams != scene->maxCams) return fail(TSR_ERR_INVALID_ARGUMENT, "scene and tree camera links differ");
     const int mode = flags & 0xff;
     if (flags & ~(0xff | TSR_CULL_LEARNED | TSR_PRIORS_ALL_LEAVES | TSR_PRIORS_CERTIFIED))
         return fail(TSR_ERR_INVALID_ARGUMENT, "unknown flags");
@@ -479,6 +479,41 @@ tsr_status tsr_field_hit_counts(const tsr_field *f, uint64_t *hits2) {
     hits2[0] = f->hits[0];
     hits2[1] = f->hits[1];
     return TSR_OK;
+}
+
+namespace {
+tsr_status samplesChanged(tsr_context *ctx, tsr_field *f, const tsr_tree *tree, const tsr_scene *scene,
+                          const uint32_t *samples, uint32_t count, bool removal) {
+    if (!ctx || !f || !tree || !scene || (count && !samples)) return fail(TSR_ERR_INVALID_ARGUMENT, "null argument");
+    if (f->leafCount != tree->leafCount || f->nodeCount != tree->nodeCount)
+        return fail(TSR_ERR_INVALID_ARGUMENT, "field does not belong to the tree");
+    if (tree->maxCams != scene->maxCams) return fail(TSR_ERR_INVALID_ARGUMENT, "scene and tree camera links differ");
+    if (!f->states.p || !f->clen.p) return fail(TSR_ERR_INVALID_ARGUMENT, "field has no passes to update");
+    return guarded([&] {
+        activate(ctx);
+        tsr::PoolScope scope(ctx->pool);
+        updateOccupancy(ctx, tree, scene, samples, count, removal, f);
+    });
+}
+}  // namespace
+
+tsr_status tsr_occupancy_add_samples(tsr_context *ctx, tsr_field *field, const tsr_tree *tree, const tsr_scene *scene,
+                                     const uint32_t *samples, uint32_t count) {
+    return samplesChanged(ctx, field, tree, scene, samples, count, false);
+}
+
+tsr_status tsr_occupancy_erase_samples(tsr_context *ctx, tsr_field *field, const tsr_tree *tree,
+                                       const tsr_scene *scene, const uint32_t *samples, uint32_t count) {
+    return samplesChanged(ctx, field, tree, scene, samples, count, true);
+}
+
+tsr_status tsr_tree_erase_samples(tsr_context *ctx, tsr_tree *tree, const uint8_t *inliers) {
+    if (!ctx || !tree || (tree->sampleCount && !inliers)) return fail(TSR_ERR_INVALID_ARGUMENT, "null argument");
+    return guarded([&] {
+        activate(ctx);
+        tsr::PoolScope scope(ctx->pool);
+        eraseTreeSamples(ctx, tree, inliers);
+    });
 }
 
 tsr_status tsr_field_prior_stages(const tsr_field *f, uint32_t *stages6) {

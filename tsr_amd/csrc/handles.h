@@ -108,6 +108,7 @@ struct tsr_field {
     tsr::DBuf<uint8_t> levels;
     tsr::u32 priorStages[6] = {0, 0, 0, 0, 0, 0};
     tsr_params params{};
+    int flags = 0;   // tsr_occupancy_compute flags of the field
 };
 
 struct tsr_mesh {
@@ -168,6 +169,13 @@ void samplesFileRead(const char *path, float *px, float *py, float *pz, float *n
                      float *scale, float *conf, u32 *cams);
 void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *scene, const tsr_params &prm,
                       int flags, const tsr_culler *culler, tsr_field *field);
+// Occupancy::addSamples / eraseSamples: the field follows a change of samples (indices into
+// the tree's node-ordered samples, host array of n)
+void updateOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *scene, const u32 *chosenHost, u32 n,
+                     bool removal, tsr_field *field);
+// Tree::eraseSamples + Samples::compact: drops the samples with inliers[i] == 0 (node order,
+// host array of the tree's sample count)
+void eraseTreeSamples(tsr_context *ctx, tsr_tree *tree, const uint8_t *inliersHost);
 // Traversal records of the priors, node-padded (node i at index i + NODE_PAD = 7, so that
 // the 8 children of a node are one aligned run): (child link, state | deepest leaf depth
 // << 8) and the FP64 subtree sums (E, S) of the cone lengths

@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "handles.h"
+#include "link_ranges.h"
 
 namespace tsr {
 
@@ -1296,14 +1297,214 @@ void priors(tsr_context *ctx, const tsr_tree *tree, const tsr_params &prm, int f
     }
 }
 
+// The record pipeline of one cone pass over links [0, links) of `in` (Occupancy.cpp:138-248
+// updateKernels): per-leaf kernel sums and cone lengths into ksum / clen (Kahan sums over
+// records sorted stably by leaf: deterministic), accepted hits per leaf into hitCount, and
+// per link the confidence of its cone if not degenerate into confs (E pass, may be null).
+void conePassRecords(tsr_context *ctx, const tsr_tree *tree, TraversalTree &tt,
+                     const LinkInputs &in, u32 links, int sampleness, float *ksum, float *clen, u32 *hitCount,
+                     double *confs, u64 &hitsOut, u32 &overOut) {
+    hipStream_t st = ctx->stream;
+    Scratch &sc = ctx->scratch;
+    ConeArena &ar = ctx->cone;
+    const u32 L = tree->leafCount;
+    DBuf<u32> counts(links), offsets(links), segStart(L), segEnd(L);
+    DBuf<unsigned long long> kst(4);
+    auto readStats = [&](int n, unsigned long long *out) {
+        HIPCHECK(hipMemcpyAsync(out, kst.p, 8 * (size_t)n, hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+    };
+    // Links are processed in chunks of at most coneChunkLinks() (one chunk below that).
+    // Per chunk, one traversal counts every link's accepted hits and stages its first
+    // in.slots records; links with more are traversed again, writing at their final
+    // offsets; the records are sorted by leaf (stable: link order within a leaf) and
+    // reduced with Kahan sums.  Chunk results are folded in chunk order.
+    // Algorithmic bytes per link: sample SoA 37 B (position, normal, scale, confidence,
+    // camera link, depth) + 4 B count; per visited node 4 B child link (+4 B state in
+    // the E pass); 12 B per staged record.
+    const u64 nodeBytes = sampleness ? 4 : 8;
+    const u32 chunkLinks = coneChunkLinks();
+    DBuf<float> part, acc;
+    DBuf<u32> partHits;
+    bool merging = false;
+    auto startMerging = [&] {
+        merging = true;
+        part.alloc(4 * (size_t)L);
+        partHits.alloc(L);
+        acc.alloc(4 * (size_t)L);
+        HIPCHECK(hipMemsetAsync(acc.p, 0, 16 * (size_t)L, st));
+        HIPCHECK(hipMemsetAsync(hitCount, 0, 4 * (size_t)L, st));
+    };
+    u64 totalHits = 0;
+    u32 totalOver = 0;
+    // link ranges in ascending order; the range length adapts to the records per link
+    // seen so far, and a range whose records would exceed what one sort handles is
+    // halved and traversed again
+    const bool trace = std::getenv("TSR_TRACE") != nullptr;
+    auto tmark = std::chrono::steady_clock::now();
+    auto lap = [&](const char *what) {   // TSR_TRACE: synchronised sub-step times
+        if (!trace) return;
+        HIPCHECK(hipStreamSynchronize(st));
+        const auto t = std::chrono::steady_clock::now();
+        std::fprintf(stderr, "[tsr occupancy]   %-28s %8.2f ms\n", what,
+                     std::chrono::duration<double, std::milli>(t - tmark).count());
+        tmark = t;
+    };
+    DBuf<u32> linkCursor(8);
+    double perLinkEstimate = 0.0;
+    if ((u64)links * 64 > CONE_MAX_RECORDS) {
+        // records per link estimated on every 64th link (counting traversal), so that the
+        // first range already fits one sort
+        LinkInputs ie = in;
+        ie.linkStride = 64;
+        const u32 sampled = (links + 63) / 64;
+        HIPCHECK(hipMemsetAsync(kst.p, 0, 32, st));
+        k_conePass<<<gridFor(sampled, BLOCK), BLOCK, 0, st>>>(tt, ie, links, sampleness, 0, counts.p, nullptr,
+                                                              nullptr, nullptr, nullptr, nullptr, nullptr,
+                                                              nullptr, kst.p);
+        unsigned long long es[2];
+        readStats(2, es);
+        perLinkEstimate = std::max(1.0, 1.25 * (double)es[1] / sampled);
+    }
+    LinkRanges ranges(links, chunkLinks, CONE_MAX_RECORDS, perLinkEstimate);
+    if (trace && perLinkEstimate > 0.0)
+        std::fprintf(stderr, "[tsr occupancy] %s pass: ~%.1f records per link (1/64 sample), ranges of %u links\n",
+                     sampleness ? "sampleness" : "emptiness", perLinkEstimate / 1.25, ranges.rangeLinks);
+    while (!ranges.done()) {
+        const u32 l0 = ranges.begin;
+        const u32 n = ranges.next();
+        if (ranges.merging && !merging) startMerging();
+        lap("(chunk start)");
+        LinkInputs ic = in;
+        ic.linkBase = l0;
+        ic.order = nullptr;
+        DBuf<u32> chunkOrder;
+        if (!sampleness) {   // the chunk's links grouped by camera
+            DBuf<u32> camKey(n), camKeyS(n), iota(n);
+            k_camKeys<<<gridFor(n, BLOCK), BLOCK, 0, st>>>(in.cams + l0, n, in.camCount, l0,
+                                                           camKey.p, iota.p);
+            chunkOrder.alloc(n);
+            int bits = 1;
+            while (bits < 32 && (1ull << bits) <= in.camCount) ++bits;
+            sortPairs(sc, camKey.p, camKeyS.p, iota.p, chunkOrder.p, n, 0, bits, st);
+            ic.order = chunkOrder.p;
+        }
+        ic.slots = (u32)std::min<size_t>(CONE_SLOTS_MAX, CONE_STAGING_BUDGET / (12 * (size_t)std::max(n, 1u)));
+        const size_t stageCount = ((size_t)n + STAGE_GROUP - 1) / STAGE_GROUP * STAGE_GROUP * ic.slots;
+        u32 *stLeaf = ar.stLeaf.ensure(stageCount);
+        uint2 *stVal = ar.stVal.ensure(stageCount);
+        lap("order + staging");
+        HIPCHECK(hipMemsetAsync(kst.p, 0, 32, st));
+        // persistent threads pay off for the long emptiness traversals only
+        const bool persistent = !sampleness;
+        if (persistent) HIPCHECK(hipMemsetAsync(linkCursor.p, 0, 32, st));
+        timedLaunch(ctx, sampleness ? KSTAT_S_COUNT : KSTAT_E_COUNT, [&] {
+            if (persistent)
+                k_conePassPT<<<std::min<u32>(gridFor(n, BLOCK), CONE_PT_BLOCKS), BLOCK, 0, st>>>(
+                    tt, ic, n, sampleness, counts.p, stLeaf, stVal, sampleness ? nullptr : confs,
+                    linkCursor.p, XCD_COUNT, kst.p);
+            else
+                k_conePass<<<xcdGrid(n, BLOCK), BLOCK, 0, st>>>(tt, ic, n, sampleness, 3, counts.p, nullptr,
+                                                                stLeaf, stVal, nullptr, nullptr, nullptr,
+                                                                sampleness ? nullptr : confs, kst.p);
+        });
+        unsigned long long cs[2];
+        readStats(2, cs);
+        {
+            tsr_kernel_stat &k = ctx->kstat[sampleness ? KSTAT_S_COUNT : KSTAT_E_COUNT];
+            k.bytes += (u64)n * 41 + cs[0] * nodeBytes + cs[1] * 12;
+            k.units += n;
+        }
+        lap("traversal (mode 3)");
+        // cs[1] is the range's record total in 64 bits: a range over the cap is split
+        // before the 32-bit offsets could wrap
+        bool impossible = false;
+        if (!ranges.accept(n, cs[1], impossible)) {
+            if (impossible) throw tsr_error(TSR_ERR_OUT_OF_MEMORY, "too many kernel contributions for one link");
+            continue;
+        }
+        exclusiveScan(sc, counts.p + l0, offsets.p + l0, n, st);   // chunk-relative offsets
+        const u64 total = (u64)readScalar(offsets.p + l0 + n - 1, st) + readScalar(counts.p + l0 + n - 1, st);
+        if (total != cs[1]) throw tsr_error(TSR_ERR_HIP, "cone pass record counts disagree with the hit total");
+        totalHits += total;
+        u32 *recLeaf = ar.recLeaf.ensure(total), *recLeafS = ar.recLeafS.ensure(total);
+        uint2 *recVal = ar.recVal.ensure(total), *recValS = ar.recValS.ensure(total);
+        k_gatherSlots<<<gridFor(n, BLOCK), BLOCK, 0, st>>>(counts.p, offsets.p, ic.order, l0, n, ic.slots,
+                                                          stLeaf, stVal, recLeaf, recVal);
+        lap("scan + gather");
+        // links beyond their slots: list them and traverse those again
+        DBuf<u32> ovFlag(n), ovList(n), ovCount(1);
+        k_overflowFlags<<<gridFor(n, BLOCK), BLOCK, 0, st>>>(counts.p + l0, n, ic.slots, ovFlag.p);
+        {
+            DBuf<u32> iota(n);
+            k_iotaU<<<gridFor(n, BLOCK), BLOCK, 0, st>>>(iota.p, n, l0);
+            size_t tmp = 0;
+            HIPCHECK(hipcub::DeviceSelect::Flagged(nullptr, tmp, iota.p, ovFlag.p, ovList.p, ovCount.p, (int)n, st));
+            HIPCHECK(hipcub::DeviceSelect::Flagged(sc.get(tmp), tmp, iota.p, ovFlag.p, ovList.p, ovCount.p, (int)n, st));
+        }
+        const u32 nOver = readScalar(ovCount.p, st);
+        totalOver += nOver;
+        if (std::getenv("TSR_TRACE"))
+            std::fprintf(stderr, "[tsr occupancy] %s pass, links %u..%u: %llu hits, %u links over %u slots\n",
+                         sampleness ? "sampleness" : "emptiness", l0, l0 + n - 1,
+                         (unsigned long long)total, nOver, ic.slots);
+        if (nOver) {
+            LinkInputs ino = ic;
+            ino.order = ovList.p;
+            HIPCHECK(hipMemsetAsync(kst.p, 0, 32, st));
+            // few, long traversals: one wavefront per link
+            DBuf<u32> werr(1);
+            HIPCHECK(hipMemsetAsync(werr.p, 0, 4, st));
+            timedLaunch(ctx, sampleness ? KSTAT_S_WRITE : KSTAT_E_WRITE, [&] {
+                k_coneLinkWave<<<std::min<u32>(nOver, 65536), 64, 0, st>>>(
+                    tt, ic, ovList.p, nOver, sampleness, tree->nodeLoc.p, offsets.p, counts.p, recLeaf,
+                    recVal, werr.p, kst.p);
+            });
+            if (const u32 we = readScalar(werr.p, st))
+                throw tsr_error(TSR_ERR_HIP, we & 1u ? "cone traversal stack overflow"
+                                                     : "cone re-traversal disagrees with the counting pass");
+            readStats(2, cs);
+            tsr_kernel_stat &k = ctx->kstat[sampleness ? KSTAT_S_WRITE : KSTAT_E_WRITE];
+            k.bytes += (u64)nOver * 45 + cs[0] * nodeBytes + cs[1] * 12;
+            k.units += nOver;
+        }
+        lap("overflow list + re-traversal");
+        int bits = 1;
+        while (bits < 32 && (1ull << bits) < L) ++bits;
+        sortPairs(sc, recLeaf, recLeafS, recVal, recValS, total, 0, bits, st);
+        lap("sort by leaf");
+        k_fillU<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(segStart.p, L, INVALID);
+        if (total)
+            k_segmentStarts<<<gridFor(total, BLOCK), BLOCK, 0, st>>>(recLeafS, total, segStart.p, segEnd.p);
+        if (!merging) {
+            k_segmentReduce<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(recValS, segStart.p, segEnd.p, L,
+                                                                 ksum, clen, hitCount, nullptr, nullptr);
+        } else {
+            float *pk = part.p, *pl = part.p + L, *pkc = part.p + 2 * (size_t)L, *plc = part.p + 3 * (size_t)L;
+            k_segmentReduce<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(recValS, segStart.p, segEnd.p, L,
+                                                                 pk, pl, partHits.p, pkc, plc);
+            k_mergeChunk<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(L, pk, pkc, pl, plc, partHits.p, acc.p,
+                                                              acc.p + L, acc.p + 2 * (size_t)L,
+                                                              acc.p + 3 * (size_t)L, hitCount);
+        }
+        ranges.finished(n, total);
+    }
+    if (merging)
+        k_finishChunks<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(L, acc.p, acc.p + L, acc.p + 2 * (size_t)L,
+                                                            acc.p + 3 * (size_t)L, ksum, clen);
+    hitsOut = totalHits;
+    overOut = totalOver;
+}
+
 void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *scene, const tsr_params &prm, int flags,
                       const tsr_culler *culler, tsr_field *field) {
     hipStream_t st = ctx->stream;
     Scratch &sc = ctx->scratch;
     const u32 L = tree->leafCount, N = tree->nodeCount;
-    const u64 links64 = (u64)scene->n * scene->maxCams;
+    const u64 links64 = (u64)tree->sampleCount * tree->maxCams;
     if (links64 >= (u64)INVALID) throw tsr_error(TSR_ERR_LINK_OVERFLOW, "Number of traced rays exceeds maximum count. Supported maximum is 2^32-1.");
     const u32 links = (u32)links64;
+    field->flags = flags;
     field->leafCount = L;
     field->nodeCount = N;
     field->ksum.alloc(2 * (size_t)L);
@@ -1330,13 +1531,8 @@ void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *s
     // on this order.
     const bool atomicMode = (flags & 0xff) == TSR_ACCUM_FP32_ATOMIC;
 
-    DBuf<u32> counts(links), offsets(links), hitCount(L), segStart(L), segEnd(L);
+    DBuf<u32> hitCount(L);
     DBuf<double> confs(links);
-    DBuf<unsigned long long> kst(4);
-    auto readStats = [&](int n, unsigned long long *out) {
-        HIPCHECK(hipMemcpyAsync(out, kst.p, 8 * (size_t)n, hipMemcpyDeviceToHost, st));
-        HIPCHECK(hipStreamSynchronize(st));
-    };
     for (int pass = 0; pass < 2; ++pass) {
         const int sampleness = pass == 0 ? 1 : 0;
         const int type = sampleness;   // storage index: 0 = EMPTINESS, 1 = SAMPLENESS
@@ -1352,6 +1548,7 @@ void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *s
             tt.packed = packed;
         }
         if (atomicMode) {
+            DBuf<u32> counts(links), offsets(1);
             HIPCHECK(hipMemsetAsync(ksum, 0, (size_t)L * 4, st));
             HIPCHECK(hipMemsetAsync(clen, 0, (size_t)L * 4, st));
             HIPCHECK(hipMemsetAsync(hitCount.p, 0, (size_t)L * 4, st));
@@ -1366,191 +1563,8 @@ void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *s
             HIPCHECK(hipcub::DeviceReduce::Sum(sc.get(tmp), tmp, counts.p, offsets.p, (int)links, st));
             field->hits[type] = readScalar(offsets.p, st);
         } else {
-            // Links are processed in chunks of at most coneChunkLinks() (one chunk below that).
-            // Per chunk, one traversal counts every link's accepted hits and stages its first
-            // in.slots records; links with more are traversed again, writing at their final
-            // offsets; the records are sorted by leaf (stable: link order within a leaf) and
-            // reduced with Kahan sums.  Chunk results are folded in chunk order.
-            // Algorithmic bytes per link: sample SoA 37 B (position, normal, scale, confidence,
-            // camera link, depth) + 4 B count; per visited node 4 B child link (+4 B state in
-            // the E pass); 12 B per staged record.
-            const u64 nodeBytes = sampleness ? 4 : 8;
-            const u32 chunkLinks = coneChunkLinks();
-            const u32 chunks = (u32)((links + (u64)chunkLinks - 1) / chunkLinks);
-            DBuf<float> part, acc;
-            DBuf<u32> partHits;
-            bool merging = false;
-            auto startMerging = [&] {
-                merging = true;
-                part.alloc(4 * (size_t)L);
-                partHits.alloc(L);
-                acc.alloc(4 * (size_t)L);
-                HIPCHECK(hipMemsetAsync(acc.p, 0, 16 * (size_t)L, st));
-                HIPCHECK(hipMemsetAsync(hitCount.p, 0, 4 * (size_t)L, st));
-            };
-            if (chunks > 1) startMerging();
-            u64 totalHits = 0;
-            u32 totalOver = 0;
-            // link ranges in ascending order; the range length adapts to the records per link
-            // seen so far, and a range whose records would exceed what one sort handles is
-            // halved and traversed again
-            const bool trace = std::getenv("TSR_TRACE") != nullptr;
-            auto tmark = std::chrono::steady_clock::now();
-            auto lap = [&](const char *what) {   // TSR_TRACE: synchronised sub-step times
-                if (!trace) return;
-                HIPCHECK(hipStreamSynchronize(st));
-                const auto t = std::chrono::steady_clock::now();
-                std::fprintf(stderr, "[tsr occupancy]   %-28s %8.2f ms\n", what,
-                             std::chrono::duration<double, std::milli>(t - tmark).count());
-                tmark = t;
-            };
-            DBuf<u32> linkCursor(8);
-            u32 rangeLinks = chunkLinks;
-            if ((u64)links * 64 > CONE_MAX_RECORDS) {
-                // records per link estimated on every 64th link (counting traversal), so that the
-                // first range already fits one sort
-                LinkInputs ie = in;
-                ie.linkStride = 64;
-                const u32 sampled = (links + 63) / 64;
-                HIPCHECK(hipMemsetAsync(kst.p, 0, 32, st));
-                k_conePass<<<gridFor(sampled, BLOCK), BLOCK, 0, st>>>(tt, ie, links, sampleness, 0, counts.p, nullptr,
-                                                                      nullptr, nullptr, nullptr, nullptr, nullptr,
-                                                                      nullptr, kst.p);
-                unsigned long long es[2];
-                readStats(2, es);
-                const double perLink = std::max(1.0, 1.25 * (double)es[1] / sampled);
-                rangeLinks = (u32)std::max(1.0, std::min((double)chunkLinks, 0.75 * CONE_MAX_RECORDS / perLink));
-                if (trace)
-                    std::fprintf(stderr, "[tsr occupancy] %s pass: ~%.1f records per link (1/64 sample), ranges of %u links\n",
-                                 sampleness ? "sampleness" : "emptiness", (double)es[1] / sampled, rangeLinks);
-            }
-            for (u32 l0 = 0; l0 < links;) {
-                const u32 n = std::min<u32>(rangeLinks, links - l0);
-                lap("(chunk start)");
-                LinkInputs ic = in;
-                ic.linkBase = l0;
-                ic.order = nullptr;
-                DBuf<u32> chunkOrder;
-                if (!sampleness) {   // the chunk's links grouped by camera
-                    DBuf<u32> camKey(n), camKeyS(n), iota(n);
-                    k_camKeys<<<gridFor(n, BLOCK), BLOCK, 0, st>>>(tree->cams.p + l0, n, scene->camCount, l0,
-                                                                   camKey.p, iota.p);
-                    chunkOrder.alloc(n);
-                    int bits = 1;
-                    while (bits < 32 && (1ull << bits) <= scene->camCount) ++bits;
-                    sortPairs(sc, camKey.p, camKeyS.p, iota.p, chunkOrder.p, n, 0, bits, st);
-                    ic.order = chunkOrder.p;
-                }
-                ic.slots = (u32)std::min<size_t>(CONE_SLOTS_MAX, CONE_STAGING_BUDGET / (12 * (size_t)std::max(n, 1u)));
-                const size_t stageCount = ((size_t)n + STAGE_GROUP - 1) / STAGE_GROUP * STAGE_GROUP * ic.slots;
-                u32 *stLeaf = ar.stLeaf.ensure(stageCount);
-                uint2 *stVal = ar.stVal.ensure(stageCount);
-                lap("order + staging");
-                HIPCHECK(hipMemsetAsync(kst.p, 0, 32, st));
-                // persistent threads pay off for the long emptiness traversals only
-                const bool persistent = !sampleness;
-                if (persistent) HIPCHECK(hipMemsetAsync(linkCursor.p, 0, 32, st));
-                timedLaunch(ctx, sampleness ? KSTAT_S_COUNT : KSTAT_E_COUNT, [&] {
-                    if (persistent)
-                        k_conePassPT<<<std::min<u32>(gridFor(n, BLOCK), CONE_PT_BLOCKS), BLOCK, 0, st>>>(
-                            tt, ic, n, sampleness, counts.p, stLeaf, stVal, sampleness ? nullptr : confs.p,
-                            linkCursor.p, XCD_COUNT, kst.p);
-                    else
-                        k_conePass<<<xcdGrid(n, BLOCK), BLOCK, 0, st>>>(tt, ic, n, sampleness, 3, counts.p, nullptr,
-                                                                        stLeaf, stVal, nullptr, nullptr, nullptr,
-                                                                        sampleness ? nullptr : confs.p, kst.p);
-                });
-                unsigned long long cs[2];
-                readStats(2, cs);
-                {
-                    tsr_kernel_stat &k = ctx->kstat[sampleness ? KSTAT_S_COUNT : KSTAT_E_COUNT];
-                    k.bytes += (u64)n * 41 + cs[0] * nodeBytes + cs[1] * 12;
-                    k.units += n;
-                }
-                lap("traversal (mode 3)");
-                // cs[1] is the range's record total in 64 bits: a range over the cap is split
-                // before the 32-bit offsets could wrap
-                if (cs[1] >= CONE_MAX_RECORDS) {
-                    if (n < 2) throw tsr_error(TSR_ERR_OUT_OF_MEMORY, "too many kernel contributions for one link");
-                    if (!merging) startMerging();
-                    rangeLinks = n / 2;
-                    continue;
-                }
-                exclusiveScan(sc, counts.p + l0, offsets.p + l0, n, st);   // chunk-relative offsets
-                const u64 total = (u64)readScalar(offsets.p + l0 + n - 1, st) + readScalar(counts.p + l0 + n - 1, st);
-                if (total != cs[1]) throw tsr_error(TSR_ERR_HIP, "cone pass record counts disagree with the hit total");
-                totalHits += total;
-                u32 *recLeaf = ar.recLeaf.ensure(total), *recLeafS = ar.recLeafS.ensure(total);
-                uint2 *recVal = ar.recVal.ensure(total), *recValS = ar.recValS.ensure(total);
-                k_gatherSlots<<<gridFor(n, BLOCK), BLOCK, 0, st>>>(counts.p, offsets.p, ic.order, l0, n, ic.slots,
-                                                                  stLeaf, stVal, recLeaf, recVal);
-                lap("scan + gather");
-                // links beyond their slots: list them and traverse those again
-                DBuf<u32> ovFlag(n), ovList(n), ovCount(1);
-                k_overflowFlags<<<gridFor(n, BLOCK), BLOCK, 0, st>>>(counts.p + l0, n, ic.slots, ovFlag.p);
-                {
-                    DBuf<u32> iota(n);
-                    k_iotaU<<<gridFor(n, BLOCK), BLOCK, 0, st>>>(iota.p, n, l0);
-                    size_t tmp = 0;
-                    HIPCHECK(hipcub::DeviceSelect::Flagged(nullptr, tmp, iota.p, ovFlag.p, ovList.p, ovCount.p, (int)n, st));
-                    HIPCHECK(hipcub::DeviceSelect::Flagged(sc.get(tmp), tmp, iota.p, ovFlag.p, ovList.p, ovCount.p, (int)n, st));
-                }
-                const u32 nOver = readScalar(ovCount.p, st);
-                totalOver += nOver;
-                if (std::getenv("TSR_TRACE"))
-                    std::fprintf(stderr, "[tsr occupancy] %s pass, links %u..%u: %llu hits, %u links over %u slots\n",
-                                 sampleness ? "sampleness" : "emptiness", l0, l0 + n - 1,
-                                 (unsigned long long)total, nOver, ic.slots);
-                if (nOver) {
-                    LinkInputs ino = ic;
-                    ino.order = ovList.p;
-                    HIPCHECK(hipMemsetAsync(kst.p, 0, 32, st));
-                    // few, long traversals: one wavefront per link
-                    DBuf<u32> werr(1);
-                    HIPCHECK(hipMemsetAsync(werr.p, 0, 4, st));
-                    timedLaunch(ctx, sampleness ? KSTAT_S_WRITE : KSTAT_E_WRITE, [&] {
-                        k_coneLinkWave<<<std::min<u32>(nOver, 65536), 64, 0, st>>>(
-                            tt, ic, ovList.p, nOver, sampleness, tree->nodeLoc.p, offsets.p, counts.p, recLeaf,
-                            recVal, werr.p, kst.p);
-                    });
-                    if (const u32 we = readScalar(werr.p, st))
-                        throw tsr_error(TSR_ERR_HIP, we & 1u ? "cone traversal stack overflow"
-                                                             : "cone re-traversal disagrees with the counting pass");
-                    readStats(2, cs);
-                    tsr_kernel_stat &k = ctx->kstat[sampleness ? KSTAT_S_WRITE : KSTAT_E_WRITE];
-                    k.bytes += (u64)nOver * 45 + cs[0] * nodeBytes + cs[1] * 12;
-                    k.units += nOver;
-                }
-                lap("overflow list + re-traversal");
-                int bits = 1;
-                while (bits < 32 && (1ull << bits) < L) ++bits;
-                sortPairs(sc, recLeaf, recLeafS, recVal, recValS, total, 0, bits, st);
-                lap("sort by leaf");
-                k_fillU<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(segStart.p, L, INVALID);
-                if (total)
-                    k_segmentStarts<<<gridFor(total, BLOCK), BLOCK, 0, st>>>(recLeafS, total, segStart.p, segEnd.p);
-                if (!merging) {
-                    k_segmentReduce<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(recValS, segStart.p, segEnd.p, L,
-                                                                         ksum, clen, hitCount.p, nullptr, nullptr);
-                } else {
-                    float *pk = part.p, *pl = part.p + L, *pkc = part.p + 2 * (size_t)L, *plc = part.p + 3 * (size_t)L;
-                    k_segmentReduce<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(recValS, segStart.p, segEnd.p, L,
-                                                                         pk, pl, partHits.p, pkc, plc);
-                    k_mergeChunk<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(L, pk, pkc, pl, plc, partHits.p, acc.p,
-                                                                      acc.p + L, acc.p + 2 * (size_t)L,
-                                                                      acc.p + 3 * (size_t)L, hitCount.p);
-                }
-                {   // next range: about 3/4 of the record limit at this range's records per link
-                    const double perLink = std::max(1.0, (double)total / n);
-                    rangeLinks = (u32)std::max(1.0, std::min((double)chunkLinks, 0.75 * CONE_MAX_RECORDS / perLink));
-                }
-                l0 += n;
-            }
-            if (merging)
-                k_finishChunks<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(L, acc.p, acc.p + L, acc.p + 2 * (size_t)L,
-                                                                    acc.p + 3 * (size_t)L, ksum, clen);
-            field->hits[type] = totalHits;
-            field->overflowLinks[type] = totalOver;
+            conePassRecords(ctx, tree, tt, in, links, sampleness, ksum, clen, hitCount.p,
+                            sampleness ? nullptr : confs.p, field->hits[type], field->overflowLinks[type]);
         }
         if (!sampleness) {
             k_setLeafFlags<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(hitCount.p, tree->leafNode.p, L, FLAG_E, field->states.p);
@@ -1574,6 +1588,112 @@ void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *s
     k_occupancy<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(field->ksum.p, field->prior.p, L, field->coneCount,
                                                      (double)prm.confidence_threshold, field->occ.p);
     recordPhase(ctx, 4, false);
+    HIPCHECK(hipStreamSynchronize(st));
+}
+
+namespace {
+__global__ void k_gatherSamples(const u32 *__restrict__ idx, u32 n, u32 maxCams, const float *__restrict__ px,
+                                const float *__restrict__ py, const float *__restrict__ pz, const float *__restrict__ nx,
+                                const float *__restrict__ ny, const float *__restrict__ nz, const float *__restrict__ sc,
+                                const float *__restrict__ cf, const u32 *__restrict__ cams,
+                                const uint8_t *__restrict__ depth, float *__restrict__ out, u32 *__restrict__ camsOut,
+                                uint8_t *__restrict__ depthOut) {
+    const u32 j = blockIdx.x * BLOCK + threadIdx.x;
+    if (j >= n) return;
+    const u32 i = idx[j];
+    const float *src[8] = {px, py, pz, nx, ny, nz, sc, cf};
+    for (int a = 0; a < 8; ++a) out[(size_t)a * n + j] = src[a][i];
+    for (u32 k = 0; k < maxCams; ++k) camsOut[(size_t)j * maxCams + k] = cams[(size_t)i * maxCams + k];
+    depthOut[j] = depth[i];
+}
+// target += sign * part (per leaf)
+__global__ void k_signedAdd(float *__restrict__ ksum, float *__restrict__ clen, const float *__restrict__ pk,
+                            const float *__restrict__ pl, u32 leafCount, float sign) {
+    const u32 l = blockIdx.x * BLOCK + threadIdx.x;
+    if (l >= leafCount) return;
+    ksum[l] += sign * pk[l];
+    clen[l] += sign * pl[l];
+}
+__global__ void k_checkSampleIndices(const u32 *__restrict__ idx, u32 n, u32 count, u32 *err) {
+    const u32 j = blockIdx.x * BLOCK + threadIdx.x;
+    if (j < n && idx[j] >= count) atomicOr(err, 1u);
+}
+}  // namespace
+
+// Occupancy::addSamples / eraseSamples (Occupancy.h:167-175; Occupancy.cpp:129-136
+// onSamplesChanged, 138-248 updateKernels over the chosen samples, Occupancy.h:222-227
+// update): the S pass, then the S flags up the tree and the gated E pass, over the camera
+// links of the chosen samples only; their per-leaf kernel sums, cone lengths and M are
+// added to (or subtracted from) the field's.  Leaf flags are only ever set (additions), as
+// in the reference.  Then priors (with the field's priors mode) and the occupancy ratio.
+void updateOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *scene, const u32 *chosenHost, u32 n,
+                     bool removal, tsr_field *field) {
+    hipStream_t st = ctx->stream;
+    Scratch &sc = ctx->scratch;
+    const u32 L = tree->leafCount, N = tree->nodeCount;
+    const tsr_params prm = field->params;
+    const u32 maxCams = tree->maxCams;
+    const u64 links64 = (u64)n * maxCams;
+    if (links64 >= (u64)INVALID) throw tsr_error(TSR_ERR_LINK_OVERFLOW, "Number of traced rays exceeds maximum count. Supported maximum is 2^32-1.");
+    const u32 links = (u32)links64;
+    if (n) {
+        DBuf<u32> idx(n), err(1);
+        HIPCHECK(hipMemcpyAsync(idx.p, chosenHost, (size_t)n * 4, hipMemcpyHostToDevice, st));
+        HIPCHECK(hipMemsetAsync(err.p, 0, 4, st));
+        k_checkSampleIndices<<<gridFor(n, BLOCK), BLOCK, 0, st>>>(idx.p, n, tree->sampleCount, err.p);
+        if (readScalar(err.p, st)) throw tsr_error(TSR_ERR_INVALID_ARGUMENT, "sample index out of range");
+        DBuf<float> soa(8 * (size_t)n);
+        DBuf<u32> cams(links);
+        DBuf<uint8_t> depth(n);
+        k_gatherSamples<<<gridFor(n, BLOCK), BLOCK, 0, st>>>(idx.p, n, maxCams, tree->px.p, tree->py.p, tree->pz.p,
+                                                             tree->nx.p, tree->ny.p, tree->nz.p, tree->scale.p,
+                                                             tree->conf.p, tree->cams.p, tree->sampleDepth.p, soa.p,
+                                                             cams.p, depth.p);
+        const float *g = soa.p;
+        LinkInputs in{g, g + n, g + 2 * (size_t)n, g + 3 * (size_t)n, g + 4 * (size_t)n, g + 5 * (size_t)n,
+                      g + 6 * (size_t)n, g + 7 * (size_t)n, cams.p, scene->cx.p, scene->cy.p, scene->cz.p, depth.p,
+                      maxCams, scene->camCount, prm.bandwidth_factor, prm.relative_sample_cone_length};
+        in.linkStride = 1;
+        in.order = nullptr;
+        in.slots = 0;
+        in.linkBase = 0;
+        TraversalTree tt{nullptr, nullptr, field->states.p, tree->frame, (float)(tree->frame.rootSize * 1e-5), nullptr};
+        ConeArena &ar = ctx->cone;
+        traversalLayout(st, tree, ar.childPad, ar.blockParent, tt);
+        DBuf<float> pk(L), pl(L);
+        DBuf<u32> hits(L);
+        DBuf<double> confs(links);
+        const float sign = removal ? -1.f : 1.f;
+        u64 h = 0;
+        u32 over = 0;
+        // S pass
+        conePassRecords(ctx, tree, tt, in, links, 1, pk.p, pl.p, hits.p, nullptr, h, over);
+        k_signedAdd<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(field->ksum.p + L, field->clen.p + L, pk.p, pl.p, L, sign);
+        if (!removal) k_setLeafFlags<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(hits.p, tree->leafNode.p, L, FLAG_S, field->states.p);
+        field->hits[1] = removal ? field->hits[1] - std::min(field->hits[1], h) : field->hits[1] + h;
+        // E pass: S flags up the tree first (Occupancy.cpp:141-142)
+        k_propagateFlag<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(tree->leafNode.p, L, tree->parents.p, FLAG_S,
+                                                             field->states.p);
+        uint2 *packed = ar.packed.ensure((size_t)N + NODE_PAD) + NODE_PAD;
+        k_packChildState<<<gridFor(N, BLOCK), BLOCK, 0, st>>>(tree->child.p, field->states.p, N, packed);
+        tt.packed = packed;
+        conePassRecords(ctx, tree, tt, in, links, 0, pk.p, pl.p, hits.p, confs.p, h, over);
+        k_signedAdd<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(field->ksum.p, field->clen.p, pk.p, pl.p, L, sign);
+        if (!removal) k_setLeafFlags<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(hits.p, tree->leafNode.p, L, FLAG_E, field->states.p);
+        field->hits[0] = removal ? field->hits[0] - std::min(field->hits[0], h) : field->hits[0] + h;
+        DBuf<double> m(1);
+        size_t tmp = 0;
+        HIPCHECK(hipcub::DeviceReduce::Sum(nullptr, tmp, confs.p, m.p, (int)links, st));
+        HIPCHECK(hipcub::DeviceReduce::Sum(sc.get(tmp), tmp, confs.p, m.p, (int)links, st));
+        const double dm = readScalar(m.p, st);
+        field->coneCount += removal ? -dm : dm;
+    }
+    // priors and occupancy of the changed field (a learned field continues without the network)
+    int pf = field->flags & (TSR_PRIORS_ALL_LEAVES | TSR_PRIORS_CERTIFIED);
+    if (field->flags & TSR_CULL_LEARNED) pf |= TSR_PRIORS_CERTIFIED;
+    priors(ctx, tree, prm, pf, nullptr, field);
+    k_occupancy<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(field->ksum.p, field->prior.p, L, field->coneCount,
+                                                     (double)prm.confidence_threshold, field->occ.p);
     HIPCHECK(hipStreamSynchronize(st));
 }
 

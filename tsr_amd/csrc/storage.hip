@@ -540,4 +540,86 @@ void samplesFileRead(const char *path, float *px, float *py, float *pz, float *n
     f.end();
 }
 
+namespace {
+constexpr u32 ERASE_BLOCK = 256;
+// erased[i] = 1 for the samples to drop
+__global__ void k_erasedFlags(const uint8_t *__restrict__ inliers, u32 n, u32 *__restrict__ erased) {
+    const u32 i = blockIdx.x * ERASE_BLOCK + threadIdx.x;
+    if (i < n) erased[i] = inliers[i] ? 0u : 1u;
+}
+// Nodes::eraseSamples (Nodes.cpp:644-671): offsets[i] = erased samples before i (n + 1 entries)
+__global__ void k_eraseInNodes(u32 *__restrict__ counts, u32 *__restrict__ starts, u32 nodeCount,
+                               const u32 *__restrict__ offsets) {
+    const u32 nd = blockIdx.x * ERASE_BLOCK + threadIdx.x;
+    if (nd >= nodeCount) return;
+    const u32 start = starts[nd], end = start + counts[nd];
+    counts[nd] -= offsets[end] - offsets[start];
+    starts[nd] = start - offsets[start];
+}
+// Samples::compact: every kept sample moves offsets[i] places towards the front
+__global__ void k_compactSamples(const uint8_t *__restrict__ inliers, u32 n, const u32 *__restrict__ offsets,
+                                 u32 maxCams, const float *const *__restrict__ src, float *const *__restrict__ dst,
+                                 const u32 *__restrict__ cams, u32 *__restrict__ camsOut,
+                                 const uint8_t *__restrict__ depth, uint8_t *__restrict__ depthOut) {
+    const u32 i = blockIdx.x * ERASE_BLOCK + threadIdx.x;
+    if (i >= n || !inliers[i]) return;
+    const u32 j = i - offsets[i];
+    for (int a = 0; a < 8; ++a) dst[a][j] = src[a][i];
+    for (u32 k = 0; k < maxCams; ++k) camsOut[(size_t)j * maxCams + k] = cams[(size_t)i * maxCams + k];
+    depthOut[j] = depth[i];
+}
+// input index -> new node-order index (INVALID for erased samples)
+__global__ void k_remapTargets(u32 *__restrict__ target, u32 m, const uint8_t *__restrict__ inliers,
+                               const u32 *__restrict__ offsets) {
+    const u32 k = blockIdx.x * ERASE_BLOCK + threadIdx.x;
+    if (k >= m) return;
+    const u32 t = target[k];
+    if (t == INVALID) return;
+    target[k] = inliers[t] ? t - offsets[t] : INVALID;
+}
+}  // namespace
+
+// Tree::eraseSamples (Tree.cpp:50-53, Nodes.cpp:644-671) and Samples::compact, as called by
+// Scene::eraseSamples (Scene.cpp:270-296): the node sample counts and starts follow the
+// erased samples, and the tree's node-ordered sample arrays are compacted.
+void eraseTreeSamples(tsr_context *ctx, tsr_tree *tree, const uint8_t *inliersHost) {
+    hipStream_t st = ctx->stream;
+    const u32 n = tree->sampleCount, N = tree->nodeCount, mc = tree->maxCams;
+    if (n == 0) return;
+    DBuf<uint8_t> inl(n);
+    HIPCHECK(hipMemcpyAsync(inl.p, inliersHost, n, hipMemcpyHostToDevice, st));
+    DBuf<u32> erased(n), offsets((size_t)n + 1);
+    k_erasedFlags<<<gridFor(n, ERASE_BLOCK), ERASE_BLOCK, 0, st>>>(inl.p, n, erased.p);
+    exclusiveScan(ctx->scratch, erased.p, offsets.p, n, st);
+    const u32 removed = readScalar(offsets.p + n - 1, st) + readScalar(erased.p + n - 1, st);
+    HIPCHECK(hipMemcpyAsync(offsets.p + n, &removed, 4, hipMemcpyHostToDevice, st));
+    k_eraseInNodes<<<gridFor(N, ERASE_BLOCK), ERASE_BLOCK, 0, st>>>(tree->sampleCounts.p, tree->sampleStarts.p, N,
+                                                                    offsets.p);
+    const u32 m = n - removed;
+    DBuf<float> px(m), py(m), pz(m), nx(m), ny(m), nz(m), sc(m), cf(m);
+    DBuf<u32> cams((size_t)m * mc);
+    DBuf<uint8_t> depth(m);
+    const float *src[8] = {tree->px.p, tree->py.p, tree->pz.p, tree->nx.p, tree->ny.p, tree->nz.p, tree->scale.p,
+                           tree->conf.p};
+    float *dst[8] = {px.p, py.p, pz.p, nx.p, ny.p, nz.p, sc.p, cf.p};
+    DBuf<const float *> srcD(8);
+    DBuf<float *> dstD(8);
+    HIPCHECK(hipMemcpyAsync(srcD.p, src, sizeof(src), hipMemcpyHostToDevice, st));
+    HIPCHECK(hipMemcpyAsync(dstD.p, dst, sizeof(dst), hipMemcpyHostToDevice, st));
+    k_compactSamples<<<gridFor(n, ERASE_BLOCK), ERASE_BLOCK, 0, st>>>(inl.p, n, offsets.p, mc, srcD.p, dstD.p,
+                                                                      tree->cams.p, cams.p, tree->sampleDepth.p,
+                                                                      depth.p);
+    const u32 targets = (u32)tree->sampleTarget.n;
+    if (targets)
+        k_remapTargets<<<gridFor(targets, ERASE_BLOCK), ERASE_BLOCK, 0, st>>>(tree->sampleTarget.p, targets, inl.p,
+                                                                              offsets.p);
+    HIPCHECK(hipStreamSynchronize(st));
+    std::swap(tree->px, px); std::swap(tree->py, py); std::swap(tree->pz, pz);
+    std::swap(tree->nx, nx); std::swap(tree->ny, ny); std::swap(tree->nz, nz);
+    std::swap(tree->scale, sc); std::swap(tree->conf, cf);
+    std::swap(tree->cams, cams);
+    std::swap(tree->sampleDepth, depth);
+    tree->sampleCount = m;
+}
+
 }  // namespace tsr
