@@ -321,6 +321,44 @@ tsr_status tsr_mlp_train(tsr_context *ctx, float *weights, const float *features
                          const float *row_weights, uint32_t n, uint32_t epochs, uint32_t batch, float learning_rate,
                          uint64_t seed, float *loss_per_epoch);
 
+/* Sample preparation from depth maps (DepthImage.cpp:191-413 triangulate / triangulateBlock /
+ * createFlexibleMesh, Mesh.cpp:63-124 vertex scales, Scene.cpp:484-573 loadDepthMeshes /
+ * loadCameraIndices, Samples.cpp:224-273 addSamplesViaMeshes): every 2x2 pixel block with 3 or 4
+ * depths > 0 gets one triangle or the two along the diagonal of smaller depth difference, minus
+ * triangles with an edge across a depth discontinuity (depth difference above 5 pixel footprints
+ * of the closer corner, sqrt(2) times that on the diagonal); vertex ids follow the pixels' first
+ * use in block order.  Vertices get world positions (camera centre + unit ray * depth along the
+ * ray), angle-weighted normals, the mean distance to their mesh neighbours as scale, and parent
+ * cameras from a views image (views_channels per pixel) or, without one, camera_index alone.
+ * Each vertex is a sample of confidence 1 (Samples.cpp:258-269).  The pixel ray is
+ * ((x + 0.5 - ppx W) / fx, (y + 0.5 - ppy H) / fy, 1) in view space (x right, y down, z forward),
+ * fx = focal_length * max(W, H), fy = fx * aspect_ratio, rotated to world space by the transpose
+ * of `rotation` (world-to-view, row-major): BaseProject's PinholeCamera, whose mapping this
+ * replaces, is not part of the reference. */
+typedef struct tsr_depth_mesh tsr_depth_mesh;
+typedef struct tsr_depth_view {
+    uint32_t width, height;
+    const float *depths;          /* width * height, row-major; depth along the pixel ray, <= 0: none */
+    float position[3];            /* camera centre (world) */
+    float rotation[9];            /* world-to-view rotation, row-major */
+    float focal_length;           /* normalised by max(width, height) */
+    float principal_point[2];     /* normalised; (0.5, 0.5) = image centre */
+    float aspect_ratio;
+    uint32_t camera_index;        /* this view's camera: the parent link without a views image */
+    uint32_t views_channels;      /* parent cameras per pixel in `views`, 0 = no views image */
+    const uint32_t *views;        /* width * height * views_channels camera indices (0xFFFFFFFF: none) or NULL */
+} tsr_depth_view;
+tsr_status tsr_depth_mesh_create(tsr_context *ctx, const tsr_depth_view *view, tsr_depth_mesh **out);
+tsr_status tsr_depth_mesh_info(const tsr_depth_mesh *mesh, uint32_t *vertex_count, uint32_t *triangle_count,
+                               uint32_t *cameras_per_sample);
+/* positions / normals: 3 doubles per vertex; scales: 1; triangles: 3 vertex ids each;
+ * parent_cameras: cameras_per_sample per vertex; pixel_to_vertex: width * height (0xFFFFFFFF:
+ * pixel not triangulated).  NULL pointers are skipped. */
+tsr_status tsr_depth_mesh_download(tsr_context *ctx, const tsr_depth_mesh *mesh, double *positions, double *normals,
+                                   double *scales, uint32_t *triangles, uint32_t *parent_cameras,
+                                   uint32_t *pixel_to_vertex);
+void tsr_depth_mesh_destroy(tsr_depth_mesh *mesh);
+
 /* Seeded synthetic scenes (analytic stand-in for SyntheticScene.cpp): config 0..4,
  * resolution_scale multiplies the image size. Views stay valid until destroy. */
 tsr_status tsr_synthetic_create(int config, uint64_t seed, double resolution_scale, tsr_synthetic **out);

@@ -4,6 +4,6 @@ The compute path is libtsr_amd.so (HIP kernels behind the C ABI in include/tsr_g
 this package is its Python binding.
 """
 from ._lib import TsrError, default_params, lib  # noqa: F401
-from .api import (Context, Culler, DualMarchingCells, device_memory, read_samples_file, Occupancy, Scene, SyntheticScene, Tree,  # noqa: F401
+from .api import (Context, Culler, DepthMesh, DualMarchingCells, device_memory, read_samples_file, Occupancy, Scene, SyntheticScene, Tree,  # noqa: F401
                   init_mlp_weights, leaf_features, mlp_forward, mlp_gradient, mlp_train,
-                  reconstruct)
+                  reconstruct, samples_from_depth_maps)

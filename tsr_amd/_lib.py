@@ -142,6 +142,10 @@ SIGNATURES = [
     ("tsr_mlp_gradient", ctypes.c_int, [vp, f32p, f32p, f32p, f32p, ctypes.c_uint32, f32p, f64p]),
     ("tsr_mlp_train", ctypes.c_int, [vp, f32p, f32p, f32p, f32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                      ctypes.c_float, u64, f32p]),
+    ("tsr_depth_mesh_create", ctypes.c_int, [vp, ctypes.c_void_p, ctypes.POINTER(vp)]),
+    ("tsr_depth_mesh_info", ctypes.c_int, [vp, u32p, u32p, u32p]),
+    ("tsr_depth_mesh_download", ctypes.c_int, [vp, vp, f64p, f64p, f64p, u32p, u32p, u32p]),
+    ("tsr_depth_mesh_destroy", None, [vp]),
     ("tsr_synthetic_create", ctypes.c_int, [ctypes.c_int, u64, ctypes.c_double, ctypes.POINTER(vp)]),
     ("tsr_synthetic_views", ctypes.c_int, [vp, ctypes.POINTER(SamplesView), ctypes.POINTER(CamerasView)]),
     ("tsr_synthetic_destroy", None, [vp]),
@@ -184,3 +188,12 @@ def default_params():
     p = Params()
     lib().tsr_params_default(ctypes.byref(p))
     return p
+
+
+class DepthView(ctypes.Structure):
+    """tsr_depth_view"""
+    _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("depths", ctypes.POINTER(ctypes.c_float)),
+                ("position", ctypes.c_float * 3), ("rotation", ctypes.c_float * 9), ("focal_length", ctypes.c_float),
+                ("principal_point", ctypes.c_float * 2), ("aspect_ratio", ctypes.c_float),
+                ("camera_index", ctypes.c_uint32), ("views_channels", ctypes.c_uint32),
+                ("views", ctypes.POINTER(ctypes.c_uint32))]

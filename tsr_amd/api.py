@@ -517,3 +517,93 @@ def reconstruct(ctx, samples, cameras, params=None, flags=L.TSR_ACCUM_KAHAN_SEGM
     occ = Occupancy(ctx, tree, scene, params, flags, culler)
     mesh = DualMarchingCells(ctx, tree, occ)
     return scene, tree, occ, mesh
+
+
+class DepthMesh:
+    """DepthImage::triangulate + createFlexibleMesh with the view's parent cameras (tsr_depth_mesh).
+
+    view: dict with depths (H x W float32, depth along the pixel ray, <= 0: none), position (3),
+    rotation (3 x 3 world-to-view), focal_length (normalised by max(W, H)), principal_point (2,
+    normalised), aspect_ratio, camera_index, and optionally views (H x W x C camera indices,
+    0xFFFFFFFF: none)."""
+
+    def __init__(self, ctx, view):
+        self.ctx = ctx
+        d = np.ascontiguousarray(view["depths"], np.float32)
+        H, W = d.shape
+        v = L.DepthView()
+        v.width, v.height = W, H
+        v.depths = d.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+        v.position[:] = [float(x) for x in np.asarray(view["position"]).reshape(3)]
+        v.rotation[:] = [float(x) for x in np.asarray(view["rotation"]).reshape(9)]
+        v.focal_length = float(view["focal_length"])
+        v.principal_point[:] = [float(x) for x in view.get("principal_point", (0.5, 0.5))]
+        v.aspect_ratio = float(view.get("aspect_ratio", 1.0))
+        v.camera_index = int(view["camera_index"])
+        views = view.get("views")
+        if views is not None:
+            vw = np.ascontiguousarray(views, np.uint32).reshape(H, W, -1)
+            v.views_channels = vw.shape[2]
+            v.views = vw.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+        else:
+            vw = None
+            v.views_channels = 0
+            v.views = None
+        h = L.vp()
+        L.check(L.lib().tsr_depth_mesh_create(ctx.h, ctypes.byref(v), ctypes.byref(h)))
+        self.h = h
+        self.width, self.height = W, H
+        a, b, c = ctypes.c_uint32(0), ctypes.c_uint32(0), ctypes.c_uint32(0)
+        L.check(L.lib().tsr_depth_mesh_info(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        self.vertex_count, self.triangle_count, self.cameras_per_sample = a.value, b.value, c.value
+
+    def download(self):
+        V, T, C = self.vertex_count, self.triangle_count, self.cameras_per_sample
+        o = {"positions": np.zeros(3 * V), "normals": np.zeros(3 * V), "scales": np.zeros(V),
+             "triangles": np.zeros(3 * T, np.uint32), "parent_cameras": np.zeros(V * C, np.uint32),
+             "pixel_to_vertex": np.zeros(self.width * self.height, np.uint32)}
+        L.check(L.lib().tsr_depth_mesh_download(self.ctx.h, self.h, _dp(o["positions"]), _dp(o["normals"]),
+                                                _dp(o["scales"]), _up(o["triangles"]), _up(o["parent_cameras"]),
+                                                _up(o["pixel_to_vertex"])))
+        o["cameras_per_sample"] = C
+        return o
+
+    def close(self):
+        if self.h:
+            L.lib().tsr_depth_mesh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def samples_from_depth_maps(ctx, views):
+    """Scene::loadDepthMeshes + Samples::addSamplesViaMeshes (Scene.cpp:484-524, Samples.cpp:224-273):
+    one sample per depth-mesh vertex (confidence 1), parent cameras padded with 0xFFFFFFFF to the
+    largest cameras-per-sample; returns (samples dict, max cameras per sample)"""
+    meshes = []
+    for v in views:
+        m = DepthMesh(ctx, v)
+        meshes.append(m.download())
+        m.close()
+    mc = max([m["cameras_per_sample"] for m in meshes] + [1])
+    keys = ("px", "py", "pz", "nx", "ny", "nz", "scale", "confidence")
+    out = {k: [] for k in keys + ("parent_cameras",)}
+    for m in meshes:
+        P, N = m["positions"].reshape(-1, 3), m["normals"].reshape(-1, 3)
+        n = len(P)
+        for a, k in enumerate(("px", "py", "pz")):
+            out[k].append(P[:, a])
+        for a, k in enumerate(("nx", "ny", "nz")):
+            out[k].append(N[:, a])
+        out["scale"].append(m["scales"])
+        out["confidence"].append(np.ones(n))
+        cams = np.full((n, mc), 0xFFFFFFFF, np.uint32)
+        cams[:, :m["cameras_per_sample"]] = m["parent_cameras"].reshape(n, -1)
+        out["parent_cameras"].append(cams.reshape(-1))
+    res = {k: np.concatenate(out[k]).astype(np.float32) for k in keys}
+    res["parent_cameras"] = np.concatenate(out["parent_cameras"]).astype(np.uint32)
+    return res, mc
