@@ -196,6 +196,16 @@ tsr_status tsr_tree_neighbors_download(tsr_context *ctx, tsr_tree *tree, uint32_
  * library's FP64 priors). */
 tsr_status tsr_tree_save(tsr_context *ctx, tsr_tree *tree, const char *nodes_path, const char *leaves_path,
                          const char *cells_path);
+/* Cameras::saveToFile / loadFromFile (Cameras.cpp:116-177; header kind 6): the camera count
+ * and one CameraData record (CameraData.h:21-36) per camera, as the struct lays it out:
+ * 13 Reals -- orientation quaternion (BaseProject's Quaternion member order is not in the
+ * reference; stored as given), position, principal point, 2 distortion coefficients, focal
+ * length, image aspect ratio -- then the view ID (padded to 8 bytes when Real is double).
+ * Host only: data13 holds 13 doubles per camera. */
+tsr_status tsr_cameras_file_write(const char *path, uint32_t count, const double *data13, const uint32_t *view_ids,
+                                  int real_bytes);
+tsr_status tsr_cameras_file_info(const char *path, uint32_t *count);
+tsr_status tsr_cameras_file_read(const char *path, double *data13, uint32_t *view_ids);
 tsr_status tsr_tree_load(tsr_context *ctx, const tsr_scene *scene, const tsr_params *params, const char *nodes_path,
                          const char *leaves_path, const char *cells_path, tsr_tree **out);
 tsr_status tsr_tree_samples_save(tsr_context *ctx, const tsr_tree *tree, const char *path);

@@ -110,3 +110,31 @@ def test_samples_file_errors_without_device(tmp_path):
     buf = np.zeros(10, np.float32)
     assert lib.tsr_samples_file_read(str(short).encode(), *([buf.ctypes.data_as(L.f32p)] * 8), None) == 13
     assert b"too short" in lib.tsr_last_error()
+
+
+def test_cameras_file_round_trip_without_device(tmp_path):
+    """Cameras::saveToFile / loadFromFile (Cameras.cpp:116-177): count, then 13 Reals and the view
+    ID per camera as CameraData lays them out (CameraData.h:21-36)"""
+    from tsr_amd.api import read_cameras_file, write_cameras_file
+    rng = np.random.default_rng(0)
+    data = rng.normal(size=(3, 13))
+    ids = np.array([4, 9, 2], np.uint32)
+    p4, p8 = tmp_path / "a.cameras", tmp_path / "b.cameras"
+    write_cameras_file(p4, data, ids, real_bytes=4)
+    write_cameras_file(p8, data, ids, real_bytes=8)
+    raw = p4.read_bytes()
+    assert len(raw) == 16 + 4 + 3 * (13 * 4 + 4)
+    assert raw[:4] == b"TSRF" and np.frombuffer(raw[4:16], np.uint32).tolist() == [6, 0, 4]
+    rec = raw[20:20 + 56]
+    assert np.array_equal(np.frombuffer(rec[:52], np.float32), data[0].astype(np.float32))
+    assert np.frombuffer(rec[52:56], np.uint32)[0] == 4
+    assert len(p8.read_bytes()) == 16 + 4 + 3 * (13 * 8 + 8)
+    d4, i4 = read_cameras_file(p4)
+    d8, i8 = read_cameras_file(p8)
+    assert np.array_equal(d4, data.astype(np.float32).astype(np.float64)) and np.array_equal(i4, ids)
+    assert np.array_equal(d8, data) and np.array_equal(i8, ids)
+    bad = tmp_path / "c.cameras"
+    bad.write_bytes(raw[:-3])
+    with pytest.raises(L.TsrError) as e:
+        read_cameras_file(bad)
+    assert e.value.status_name == "TSR_ERR_FILE"

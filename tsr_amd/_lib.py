@@ -142,6 +142,9 @@ SIGNATURES = [
     ("tsr_mlp_gradient", ctypes.c_int, [vp, f32p, f32p, f32p, f32p, ctypes.c_uint32, f32p, f64p]),
     ("tsr_mlp_train", ctypes.c_int, [vp, f32p, f32p, f32p, f32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                      ctypes.c_float, u64, f32p]),
+    ("tsr_cameras_file_write", ctypes.c_int, [ctypes.c_char_p, ctypes.c_uint32, f64p, u32p, ctypes.c_int]),
+    ("tsr_cameras_file_info", ctypes.c_int, [ctypes.c_char_p, u32p]),
+    ("tsr_cameras_file_read", ctypes.c_int, [ctypes.c_char_p, f64p, u32p]),
     ("tsr_depth_mesh_create", ctypes.c_int, [vp, ctypes.c_void_p, ctypes.POINTER(vp)]),
     ("tsr_depth_mesh_info", ctypes.c_int, [vp, u32p, u32p, u32p]),
     ("tsr_depth_mesh_download", ctypes.c_int, [vp, vp, f64p, f64p, f64p, u32p, u32p, u32p]),

@@ -607,3 +607,21 @@ def samples_from_depth_maps(ctx, views):
     res = {k: np.concatenate(out[k]).astype(np.float32) for k in keys}
     res["parent_cameras"] = np.concatenate(out["parent_cameras"]).astype(np.uint32)
     return res, mc
+
+
+def write_cameras_file(path, data13, view_ids, real_bytes=4):
+    """Cameras::saveToFile: data13 (n x 13: quaternion, position, principal point, distortion 2,
+    focal length, aspect ratio) and view IDs"""
+    d = np.ascontiguousarray(data13, np.float64).reshape(-1, 13)
+    v = np.ascontiguousarray(view_ids, np.uint32)
+    L.check(L.lib().tsr_cameras_file_write(str(path).encode(), len(v), _dp(d), _up(v), int(real_bytes)))
+
+
+def read_cameras_file(path):
+    """Cameras::loadFromFile: (n x 13 data, view IDs)"""
+    n = ctypes.c_uint32(0)
+    L.check(L.lib().tsr_cameras_file_info(str(path).encode(), ctypes.byref(n)))
+    d = np.zeros((n.value, 13))
+    v = np.zeros(n.value, np.uint32)
+    L.check(L.lib().tsr_cameras_file_read(str(path).encode(), _dp(d), _up(v)))
+    return d, v

@@ -396,6 +396,23 @@ tsr_status tsr_samples_file_read(const char *path, float *px, float *py, float *
     return guarded([&] { samplesFileRead(path, px, py, pz, nx, ny, nz, scale, confidence, parent_cameras); });
 }
 
+tsr_status tsr_cameras_file_write(const char *path, uint32_t count, const double *data13, const uint32_t *view_ids,
+                                  int real_bytes) {
+    if (!path || (count && (!data13 || !view_ids))) return fail(TSR_ERR_INVALID_ARGUMENT, "null argument");
+    if (real_bytes != 4 && real_bytes != 8) return fail(TSR_ERR_INVALID_ARGUMENT, "real_bytes must be 4 or 8");
+    return guarded([&] { camerasFileWrite(path, count, data13, view_ids, real_bytes); });
+}
+
+tsr_status tsr_cameras_file_info(const char *path, uint32_t *count) {
+    if (!path || !count) return fail(TSR_ERR_INVALID_ARGUMENT, "null argument");
+    return guarded([&] { camerasFileInfo(path, *count); });
+}
+
+tsr_status tsr_cameras_file_read(const char *path, double *data13, uint32_t *view_ids) {
+    if (!path) return fail(TSR_ERR_INVALID_ARGUMENT, "null argument");
+    return guarded([&] { camerasFileRead(path, data13, view_ids); });
+}
+
 tsr_status tsr_field_save(tsr_context *ctx, const tsr_field *field, const char *path, int real_bytes) {
     if (!ctx || !field || !path) return fail(TSR_ERR_INVALID_ARGUMENT, "null argument");
     if (real_bytes != 4 && real_bytes != 8) return fail(TSR_ERR_INVALID_ARGUMENT, "real_bytes must be 4 or 8");

@@ -165,6 +165,9 @@ void saveField(tsr_context *ctx, const tsr_field *field, const char *path, int r
 void loadField(tsr_context *ctx, const tsr_tree *tree, const tsr_params &prm, const char *path, tsr_field *field);
 void saveTreeSamples(tsr_context *ctx, const tsr_tree *tree, const char *path);
 void samplesFileInfo(const char *path, u32 &count, u32 &maxCams);
+void camerasFileWrite(const char *path, u32 count, const double *data13, const u32 *viewIds, int realBytes);
+void camerasFileInfo(const char *path, u32 &count);
+void camerasFileRead(const char *path, double *data13, u32 *viewIds);
 void samplesFileRead(const char *path, float *px, float *py, float *pz, float *nx, float *ny, float *nz,
                      float *scale, float *conf, u32 *cams);
 void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *scene, const tsr_params &prm,

@@ -99,7 +99,7 @@ __global__ void k_insertNodes(HashView h, const u64 *__restrict__ loc, u32 n, u3
 }
 
 // ---- files -------------------------------------------------------------------------
-enum : u32 { KIND_NODES = 1, KIND_LEAVES = 2, KIND_DUAL_CELLS = 3, KIND_OCCUPANCY = 4, KIND_SAMPLES = 5 };
+enum : u32 { KIND_NODES = 1, KIND_LEAVES = 2, KIND_DUAL_CELLS = 3, KIND_OCCUPANCY = 4, KIND_SAMPLES = 5, KIND_CAMERAS = 6 };
 constexpr u32 FILE_VERSION = 0;   // Nodes/Leaves/DualCells/Occupancy/Samples::FILE_VERSION
 
 [[noreturn]] void fileError(const std::string &path, const std::string &what) {
@@ -500,6 +500,42 @@ void saveTreeSamples(tsr_context *ctx, const tsr_tree *tree, const char *path) {
     f.put(toHost(tree->conf, n, st).data(), 4, n);
     f.put(toHost(tree->scale, n, st).data(), 4, n);
     f.put(toHost(tree->cams, (size_t)n * mc, st).data(), 4, (size_t)n * mc);
+}
+
+// Cameras::saveToFile / loadFromFile (Scene/Camera/Cameras.cpp:116-177): the camera count,
+// then one CameraData record per camera (CameraData.h:21-36) as the struct lays it out: 13
+// Reals (orientation quaternion, position, principal point, 2 distortion coefficients, focal
+// length, image aspect ratio) and the view ID, padded to the Real size.
+void camerasFileWrite(const char *path, u32 count, const double *data13, const u32 *viewIds, int realBytes) {
+    File f(path, true, KIND_CAMERAS, (u32)realBytes);
+    f.put(&count, 4, 1);
+    const u32 pad = realBytes == 8 ? 4 : 0;
+    const u32 zero = 0;
+    for (u32 c = 0; c < count; ++c) {
+        f.putReals(std::vector<double>(data13 + 13 * (size_t)c, data13 + 13 * (size_t)c + 13));
+        f.put(&viewIds[c], 4, 1);
+        if (pad) f.put(&zero, 4, 1);
+    }
+}
+
+void camerasFileInfo(const char *path, u32 &count) {
+    File f(path, false, KIND_CAMERAS);
+    f.get(&count, 4, 1);
+}
+
+void camerasFileRead(const char *path, double *data13, u32 *viewIds) {
+    File f(path, false, KIND_CAMERAS);
+    u32 count = 0;
+    f.get(&count, 4, 1);
+    for (u32 c = 0; c < count; ++c) {
+        const std::vector<double> r = f.getReals(13);
+        if (data13) std::copy(r.begin(), r.end(), data13 + 13 * (size_t)c);
+        u32 id = 0;
+        f.get(&id, 4, 1);
+        if (viewIds) viewIds[c] = id;
+        if (f.realBytes == 8) f.get(&id, 4, 1);
+    }
+    f.end();
 }
 
 // header of a samples file: count and cameras per sample
