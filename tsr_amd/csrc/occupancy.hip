@@ -39,9 +39,9 @@ inline unsigned xcdGrid(size_t n, unsigned block) {
 
 // staging slots per link of the single-traversal record pass (as many as a memory
 // budget allows, at most CONE_SLOTS_MAX); links with more accepted hits are traversed
-// again (mode 1) into their final place
-constexpr u32 CONE_SLOTS_MAX = 256;
-constexpr size_t CONE_STAGING_BUDGET = (size_t)40 << 30;   // bytes
+// again (k_coneLinkWave) into their final place
+constexpr u32 CONE_SLOTS_MAX = 512;
+constexpr size_t CONE_STAGING_BUDGET = (size_t)64 << 30;   // bytes
 constexpr u64 CONE_MAX_RECORDS = 1536ull << 20;              // records of one link range (one sort)
 // links per chunk of a cone pass (TSR_CONE_CHUNK overrides, e.g. to exercise the
 // multi-chunk merge on small scenes)
@@ -51,8 +51,8 @@ inline u32 coneChunkLinks() {   // read at every occupancy computation
     return v > 0 ? (u32)v : (1u << 24);
 }
 
-// Staging layout of mode 3: the records of link position idx are contiguous, record j
-// at idx * slots + j, so that the gather into link order reads and writes whole runs.
+// Staging layout of the record pass: the records of link position idx are contiguous,
+// record j at idx * slots + j, so that the gather into link order reads and writes whole runs.
 constexpr u32 STAGE_GROUP = 256;   // positions per gather block (the staging area is padded to it)
 __device__ __forceinline__ size_t stagingIndex(u32 idx, u32 j, u32 slots) { return (size_t)idx * slots + j; }
 
@@ -106,7 +106,7 @@ struct LinkInputs {
     float bandwidth, relConeLength;
     u32 linkStride;            // process every linkStride-th link (1 = all)
     const u32 *order;          // thread index -> link (links grouped by camera), may be null
-    u32 slots;                 // mode 3: staging slots per link (layout: stagingIndex)
+    u32 slots;                 // record pass: staging slots per link (layout: stagingIndex)
     u32 linkBase;              // without `order`: thread index i -> link linkBase + i
 };
 
@@ -197,8 +197,7 @@ __device__ void traverseCone(const TraversalTree &t, const ConeCull &cull, u32 m
 
 __device__ void coneLink(const TraversalTree &t, const LinkInputs &in, u32 link, u32 idx, u32 linkCount,
                          int sampleness, int mode,
-                         u32 *__restrict__ counts, const u32 *__restrict__ offsets, u32 *__restrict__ recLeaf,
-                         uint2 *__restrict__ recVal, float *__restrict__ atomicSum, float *__restrict__ atomicLen,
+                         u32 *__restrict__ counts, float *__restrict__ atomicSum, float *__restrict__ atomicLen,
                          u32 *__restrict__ atomicStates, double *__restrict__ confOut, u32 &visits, u32 &count,
                          u32 *path);
 
@@ -214,22 +213,20 @@ __device__ void waveAccumulate(unsigned long long *acc, u32 v) {
     if ((threadIdx.x & 63) == 0) atomicAdd(acc, x);
 }
 
-// mode 0: count accepted hits per link; 1: write records at offsets[link]; 2: FP32 atomics
-// (precision study); 3: count, and write the first in.slots records into the link's
-// staging slots (stagingIndex: contiguous per link)
+// mode 0: count accepted hits per link (with the confidence of each link); 2: FP32 atomics
+// into the leaf sums (precision study)
 // stats[0] += node visits, stats[1] += accepted hits
 // Register use capped for 6 waves per SIMD (80 VGPRs, no spill).
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_conePass(TraversalTree t, LinkInputs in, u32 linkCount, int sampleness, int mode,
-                           u32 *__restrict__ counts, const u32 *__restrict__ offsets, u32 *__restrict__ recLeaf,
-                           uint2 *__restrict__ recVal, float *__restrict__ atomicSum, float *__restrict__ atomicLen,
+                           u32 *__restrict__ counts, float *__restrict__ atomicSum, float *__restrict__ atomicLen,
                            u32 *__restrict__ atomicStates, double *__restrict__ confOut,
                            unsigned long long *__restrict__ stats) {
     __shared__ u32 pathL[MAX_DEPTH + 2][BLOCK];   // ancestors per thread (launched with BLOCK threads)
     const u32 idx = (blockIdx.x * blockDim.x + threadIdx.x) * in.linkStride;
     u32 visits = 0, count = 0;
     const u32 link = idx < linkCount && in.order ? in.order[idx] : in.linkBase + idx;
-    if (idx < linkCount) coneLink(t, in, link, idx, linkCount, sampleness, mode, counts, offsets, recLeaf, recVal, atomicSum,
-                                   atomicLen, atomicStates, confOut, visits, count, &pathL[0][threadIdx.x]);
+    if (idx < linkCount) coneLink(t, in, link, idx, linkCount, sampleness, mode, counts, atomicSum, atomicLen,
+                                   atomicStates, confOut, visits, count, &pathL[0][threadIdx.x]);
     if (stats) {
         waveAccumulate(&stats[0], visits);
         waveAccumulate(&stats[1], count);
@@ -238,16 +235,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
 
 __device__ void coneLink(const TraversalTree &t, const LinkInputs &in, u32 link, u32 idx, u32 linkCount,
                          int sampleness, int mode,
-                         u32 *__restrict__ counts, const u32 *__restrict__ offsets, u32 *__restrict__ recLeaf,
-                         uint2 *__restrict__ recVal, float *__restrict__ atomicSum, float *__restrict__ atomicLen,
+                         u32 *__restrict__ counts, float *__restrict__ atomicSum, float *__restrict__ atomicLen,
                          u32 *__restrict__ atomicStates, double *__restrict__ confOut, u32 &visits, u32 &count,
                          u32 *path) {
     ConeD cone;
     u32 i;
     const bool ok = setupLink(in, link, sampleness != 0, cone, i);
-    if ((mode == 0 || mode == 3) && confOut) confOut[link] = ok ? (double)in.conf[i] : 0.0;
+    if (mode == 0 && confOut) confOut[link] = ok ? (double)in.conf[i] : 0.0;
     if (!ok) {
-        if (mode == 0 || mode == 3) counts[link] = 0;
+        if (mode == 0) counts[link] = 0;
         return;
     }
     ConeCull cull;
@@ -256,31 +252,26 @@ __device__ void coneLink(const TraversalTree &t, const LinkInputs &in, u32 link,
     const float weightedLength = (float)(conf * cone.length);
     const u32 maxDepth = (u32)in.sampleDepth[i] + MAX_DEPTH_DIFFERENCE;
     const u32 required = sampleness ? 0u : FLAG_S;
-    const size_t base = mode == 1 ? (size_t)offsets[link] : 0;
     traverseCone(t, cull, maxDepth, required, visits, path,
                  [&](u32 leaf, const double x[3]) {
         const double kv = coneKernel(cone, x, sampleness != 0, conf);
         if (kv == 0.0) return;
-        if (mode == 1 || (mode == 3 && count < in.slots)) {
-            const size_t at = mode == 1 ? base + count : stagingIndex(idx, count, in.slots);
-            recLeaf[at] = leaf;
-            recVal[at] = make_uint2(__float_as_uint((float)kv), __float_as_uint(weightedLength));
-        } else if (mode == 2) {
+        if (mode == 2) {
             atomicAdd(&atomicSum[leaf], (float)kv);
             atomicAdd(&atomicLen[leaf], weightedLength);
             atomicOr(&atomicStates[leaf], 1u);
         }
         ++count;
     });
-    if (mode == 0 || mode == 3) counts[link] = count;
+    if (mode == 0) counts[link] = count;
 }
 
-// Mode 3 with persistent threads (after Aila & Laine, "Understanding the efficiency of
-// ray traversal on GPUs", 2009): every lane runs the traversal of traverseCone as an
-// explicit state machine and, when its link is done, takes the next link index from a
-// shared counter (one atomic per wavefront for all idle lanes).  Lanes no longer idle
-// at the end of a wave's longest traversal.  Results, staging and counts are those of
-// k_conePass mode 3.
+// The record pass with persistent threads (after Aila & Laine, "Understanding the
+// efficiency of ray traversal on GPUs", 2009): every lane runs the traversal of
+// traverseCone as an explicit state machine and, when its link is done, takes the next
+// link index from a shared counter (one atomic per wavefront for all idle lanes).  Lanes
+// no longer idle at the end of a wave's longest traversal.  Per link it counts the
+// accepted hits and stages the first in.slots records (stagingIndex).
 //
 // With `slices` = 8 the link list is cut into 8 contiguous slices with a counter
 // each (nextIdx[0..7]); a wave starts on the slice of its XCD (blocks are dispatched
@@ -1360,8 +1351,7 @@ void conePassRecords(tsr_context *ctx, const tsr_tree *tree, TraversalTree &tt,
         const u32 sampled = (links + 63) / 64;
         HIPCHECK(hipMemsetAsync(kst.p, 0, 32, st));
         k_conePass<<<gridFor(sampled, BLOCK), BLOCK, 0, st>>>(tt, ie, links, sampleness, 0, counts.p, nullptr,
-                                                              nullptr, nullptr, nullptr, nullptr, nullptr,
-                                                              nullptr, kst.p);
+                                                              nullptr, nullptr, nullptr, kst.p);
         unsigned long long es[2];
         readStats(2, es);
         perLinkEstimate = std::max(1.0, 1.25 * (double)es[1] / sampled);
@@ -1395,18 +1385,14 @@ void conePassRecords(tsr_context *ctx, const tsr_tree *tree, TraversalTree &tt,
         uint2 *stVal = ar.stVal.ensure(stageCount);
         lap("order + staging");
         HIPCHECK(hipMemsetAsync(kst.p, 0, 32, st));
-        // persistent threads pay off for the long emptiness traversals only
-        const bool persistent = !sampleness;
-        if (persistent) HIPCHECK(hipMemsetAsync(linkCursor.p, 0, 32, st));
+        // persistent threads for both passes: a lane takes the next link as soon as its own
+        // is done (the sampleness traversals are short, but at scale their lengths vary enough
+        // to pay)
+        HIPCHECK(hipMemsetAsync(linkCursor.p, 0, 32, st));
         timedLaunch(ctx, sampleness ? KSTAT_S_COUNT : KSTAT_E_COUNT, [&] {
-            if (persistent)
-                k_conePassPT<<<std::min<u32>(gridFor(n, BLOCK), CONE_PT_BLOCKS), BLOCK, 0, st>>>(
-                    tt, ic, n, sampleness, counts.p, stLeaf, stVal, sampleness ? nullptr : confs,
-                    linkCursor.p, XCD_COUNT, kst.p);
-            else
-                k_conePass<<<xcdGrid(n, BLOCK), BLOCK, 0, st>>>(tt, ic, n, sampleness, 3, counts.p, nullptr,
-                                                                stLeaf, stVal, nullptr, nullptr, nullptr,
-                                                                sampleness ? nullptr : confs, kst.p);
+            k_conePassPT<<<std::min<u32>(gridFor(n, BLOCK), CONE_PT_BLOCKS), BLOCK, 0, st>>>(
+                tt, ic, n, sampleness, counts.p, stLeaf, stVal, sampleness ? nullptr : confs, linkCursor.p,
+                XCD_COUNT, kst.p);
         });
         unsigned long long cs[2];
         readStats(2, cs);
@@ -1415,7 +1401,7 @@ void conePassRecords(tsr_context *ctx, const tsr_tree *tree, TraversalTree &tt,
             k.bytes += (u64)n * 41 + cs[0] * nodeBytes + cs[1] * 12;
             k.units += n;
         }
-        lap("traversal (mode 3)");
+        lap("traversal (record pass)");
         // cs[1] is the range's record total in 64 bits: a range over the cap is split
         // before the 32-bit offsets could wrap
         bool impossible = false;
@@ -1553,11 +1539,10 @@ void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *s
             HIPCHECK(hipMemsetAsync(clen, 0, (size_t)L * 4, st));
             HIPCHECK(hipMemsetAsync(hitCount.p, 0, (size_t)L * 4, st));
             k_conePass<<<xcdGrid(links, BLOCK), BLOCK, 0, st>>>(tt, in, links, sampleness, 0, counts.p, nullptr,
-                                                                nullptr, nullptr, nullptr, nullptr, nullptr,
-                                                                sampleness ? nullptr : confs.p, nullptr);
-            k_conePass<<<xcdGrid(links, BLOCK), BLOCK, 0, st>>>(tt, in, links, sampleness, 2, counts.p, nullptr,
-                                                                nullptr, nullptr, ksum, clen, hitCount.p, nullptr,
+                                                                nullptr, nullptr, sampleness ? nullptr : confs.p,
                                                                 nullptr);
+            k_conePass<<<xcdGrid(links, BLOCK), BLOCK, 0, st>>>(tt, in, links, sampleness, 2, counts.p, ksum,
+                                                                clen, hitCount.p, nullptr, nullptr);
             size_t tmp = 0;
             HIPCHECK(hipcub::DeviceReduce::Sum(nullptr, tmp, counts.p, offsets.p, (int)links, st));
             HIPCHECK(hipcub::DeviceReduce::Sum(sc.get(tmp), tmp, counts.p, offsets.p, (int)links, st));
