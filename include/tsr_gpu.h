@@ -61,7 +61,10 @@ enum {
     TSR_PRIORS_ALL_LEAVES = 0x200,   /* priors for every leaf; by default only for the leaves whose
                                         occupancy reads them (E + S >= confidence threshold, the
                                         others get NaN)                                             */
-    TSR_PRIORS_CERTIFIED = 0x400     /* certified priors, every leaf from the first stage (below)   */
+    TSR_PRIORS_CERTIFIED = 0x400,    /* certified priors, every leaf from the first stage (below)   */
+    TSR_EMPTINESS_TRAVERSAL = 0x800  /* E pass as one cone traversal per link (the reference's loop
+                                        order); by default the same contributions are gathered per
+                                        sampleness-flagged leaf from a per-camera direction index  */
 };
 /* Certified priors (no reference counterpart).  The mesh reads of a leaf's occupancy only
  * its class: -EPSILON (pE >= pS), in (0, EPSILON], or above EPSILON (DualMarchingCell.cpp:
@@ -128,7 +131,8 @@ tsr_status tsr_context_phase_times(tsr_context *ctx, float *ms, int n);
  * output element counted once per launch) and work units (queries or links):
  * kernel 0 = priors sphere query, 1/2 = emptiness cone pass (one traversal per link,
  * staging its records) / re-traversal of the links with more records than staging
- * slots, 3/4 = the same for the sampleness cones. */
+ * slots, 3/4 = the same for the sampleness cones, 5 = emptiness pass gathered per
+ * sampleness-flagged leaf (the default E pass; units = leaves). */
 tsr_status tsr_context_kernel_stats(tsr_context *ctx, int kernel, double *ms, uint64_t *launches, uint64_t *bytes,
                                     uint64_t *units);
 tsr_status tsr_context_reset_kernel_stats(tsr_context *ctx);

@@ -82,6 +82,7 @@ TSR_ACCUM_FP32_ATOMIC = 1
 TSR_CULL_LEARNED = 0x100
 TSR_PRIORS_CERTIFIED = 0x400
 TSR_PRIORS_ALL_LEAVES = 0x200
+TSR_EMPTINESS_TRAVERSAL = 0x800
 MLP_PARAM_COUNT = 64 * 16 + 64 + 64 * 64 + 64 + 64 * 64 + 64 + 64 + 1
 
 # (name, restype, argtypes) of every exported entry point of include/tsr_gpu.h

@@ -53,7 +53,7 @@ class Context:
 
 
     KERNELS = {"priors": 0, "emptiness_pass": 1, "emptiness_overflow": 2, "sampleness_pass": 3,
-               "sampleness_overflow": 4}
+               "sampleness_overflow": 4, "emptiness_gather": 5}
 
     def kernel_stats(self, name):
         """(ms, launches, algorithmic bytes, units) accumulated for an instrumented kernel."""

@@ -451,7 +451,7 @@ tsr_status tsr_occupancy_compute(tsr_context *ctx, const tsr_tree *tree, const t
     *out = nullptr;
     if (tree->maxCams != scene->maxCams) return fail(TSR_ERR_INVALID_ARGUMENT, "scene and tree camera links differ");
     const int mode = flags & 0xff;
-    if (flags & ~(0xff | TSR_CULL_LEARNED | TSR_PRIORS_ALL_LEAVES | TSR_PRIORS_CERTIFIED))
+    if (flags & ~(0xff | TSR_CULL_LEARNED | TSR_PRIORS_ALL_LEAVES | TSR_PRIORS_CERTIFIED | TSR_EMPTINESS_TRAVERSAL))
         return fail(TSR_ERR_INVALID_ARGUMENT, "unknown flags");
     if (mode != TSR_ACCUM_KAHAN_SEGMENTED && mode != TSR_ACCUM_FP32_ATOMIC)
         return fail(TSR_ERR_INVALID_ARGUMENT, "unknown accumulation mode");

@@ -1,0 +1,804 @@
+// Emptiness pass gathered by leaves (Occupancy::updateKernels for EMPTINESS,
+// SR/SurfaceExtraction/Occupancy.cpp:138-248, with addKernel :279-346).
+//
+// The reference walks the tree once per camera->sample link (LeavesIterator with a
+// ConeNodesChecker gated by the sampleness flag and the link's depth limit) and adds the
+// cone kernel at every leaf it meets.  Node flags are ORed up the tree
+// (Occupancy.cpp:491-516) and the cone test only removes nodes that cannot hold a
+// contributing leaf centre, so the set of (link, leaf) contributions is fixed by the leaf
+// alone: a sampleness-flagged leaf, no deeper than the link's depth limit, whose centre
+// gets a non-zero kernel.  This pass enumerates that set from the leaves' side.
+//
+// A centre x gets a non-zero emptiness kernel only inside the oblique cone from the camera
+// centre a to the sample's end disc (centre e, radius R = bandwidth * scale): x = a + t (q - a)
+// with q in the disc and t in (0, 1), so x - a lies within asin(R / |e - a|) of the cone axis
+// and |x - a| < |e - a| + R.  Per camera the links are binned by the directions of their
+// cones on the six faces of a cube map (gnomonic coordinates w_j / |w_i| on the face of the
+// largest component i, a cell grid sized to the camera's mean cone angle).  A leaf looks up,
+// per camera, the one cell its direction falls into, tests that cell's links with an FP32
+// cone test whose margins exceed its rounding, and evaluates the reference's FP64 kernel
+// (coneKernel) for those that pass.  Links whose cones cover too many cells (cameras very
+// close to a sample) are listed per camera and tested by every leaf.
+//
+// Each leaf's contributions are summed by one wavefront in a fixed order (cameras in
+// batches of 64, a cell's links in link order, then the camera's wide links; Kahan per
+// lane, then a fixed two-sum tree over the lanes): deterministic, with no records and no
+// sort by leaf.
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+#include "handles.h"
+
+namespace tsr {
+
+namespace {
+
+constexpr u32 BLOCK = 256;
+constexpr u32 XCD_COUNT = 8;
+constexpr u32 FACES = 6;
+constexpr u32 WIDE_CELLS = 4096;        // a link whose cone covers more cells is listed per camera
+constexpr u32 MAX_FACE_CELLS = 1u << 15;  // cells along one side of a face grid
+constexpr u32 LEAVES_PER_WAVE = 4;      // consecutive leaves per wavefront of the leaf pass
+constexpr double REAL_MAX_D = 3.4028234663852886e38;
+
+// link classes
+constexpr uint8_t LINK_NONE = 0, LINK_GRID = 1, LINK_WIDE = 2;
+
+struct LinkSrc {
+    const float *px, *py, *pz, *nx, *ny, *nz, *scale, *conf;
+    const u32 *cams;
+    const float *cx, *cy, *cz;
+    const uint8_t *sampleDepth;
+    u32 maxCams, camCount;
+    float bandwidth;
+};
+
+// per-link data of the leaf pass, in link order
+struct LinkGeo {
+    float4 *axis;    // unit cone axis (e - a) / |e - a|, tan of the inflated cone angle (inf: wide)
+    float4 *aux;     // reach (|e - a| + R, inflated), inflated cone angle (radians), depth limit, 0
+    float4 *pos;     // sample position, scale
+    float4 *nrm;     // sample normal, confidence
+    double2 *cone;   // 1 / dot(e - a, n), |e - a| (ConeD::invDenom, ConeD::length)
+};
+
+struct FaceGrid {
+    int lo[2];       // first cell (u, v) of the bounding box of the face's links
+    u32 dim[2];      // cells of the box (0: no links on the face)
+    u32 base;        // index of the box's first cell in the cell arrays
+};
+
+struct CamGrid {
+    float a[3];      // camera centre
+    float inv;       // cells per unit of gnomonic coordinate (G / 2)
+    u32 G;           // cells along a face side
+    u32 wideBegin, wideEnd;
+    FaceGrid face[FACES];
+};
+
+// Cone of link l as the traversal sets it up (Occupancy.cpp:174-201, getCameraToSampleLinkData
+// :250-277): apex at the camera, end at the sample, end normal -n, R = bandwidth * scale,
+// uncut.  Writes the link's leaf-pass data and its confidence (0: not an emptiness cone).
+__global__ void k_gLinkSetup(LinkSrc s, u32 links, LinkGeo g, uint8_t *__restrict__ cls,
+                             double *__restrict__ confs, unsigned long long *__restrict__ camTheta,
+                             u32 *__restrict__ camKey, u32 *__restrict__ reachKey, u32 *__restrict__ iota) {
+    const u32 l = blockIdx.x * BLOCK + threadIdx.x;
+    if (l >= links) return;
+    cls[l] = LINK_NONE;
+    confs[l] = 0.0;
+    camKey[l] = s.camCount;   // sort keys: links without a cone last
+    reachKey[l] = 0xffffffffu;
+    iota[l] = l;
+    const u32 i = l / s.maxCams, cam = s.cams[l];
+    if (cam >= s.camCount) return;   // Cameras::isValid
+    const double apex[3] = {s.cx[cam], s.cy[cam], s.cz[cam]};
+    const double p[3] = {s.px[i], s.py[i], s.pz[i]};
+    const double nrm[3] = {-(double)s.nx[i], -(double)s.ny[i], -(double)s.nz[i]};
+    const double R = (double)s.bandwidth * (double)s.scale[i];
+    ConeD c;
+    coneInit(c, apex, nrm, p, R, REAL_MAX_D);
+    if (c.degenerate) return;
+    const double conf = s.conf[i];
+    confs[l] = conf;
+    const double A = c.length;
+    const bool wide = !(R < 0.5 * A);
+    // inflated cone angle: relative 1e-4 plus 1e-5 rad cover the FP32 rounding of the unit
+    // axis and of the face coordinate ranges
+    const double theta = wide ? 0.0 : asin(R / A);
+    const double thetaP = theta * (1.0 + 1e-4) + 1e-5;
+    const double tanT = wide ? (double)INFINITY : tan(theta) * (1.0 + 1e-4);
+    g.axis[l] = make_float4((float)(c.v[0] / A), (float)(c.v[1] / A), (float)(c.v[2] / A), (float)tanT);
+    g.aux[l] = make_float4((float)((A + R) * (1.0 + 1e-6)), (float)thetaP,
+                           (float)(s.sampleDepth[i] + MAX_DEPTH_DIFFERENCE), 0.f);
+    g.pos[l] = make_float4(s.px[i], s.py[i], s.pz[i], s.scale[i]);
+    g.nrm[l] = make_float4(s.nx[i], s.ny[i], s.nz[i], s.conf[i]);
+    g.cone[l] = make_double2(c.invDenom, A);
+    cls[l] = wide ? LINK_WIDE : LINK_GRID;
+    camKey[l] = cam;
+    reachKey[l] = ~__float_as_uint(g.aux[l].x);   // descending reach (positive floats order as bits)
+    if (!wide) {   // integer sums: the camera's mean angle does not depend on the order
+        atomicAdd(&camTheta[2 * cam], (unsigned long long)(thetaP * 4294967296.0));
+        atomicAdd(&camTheta[2 * cam + 1], 1ull);
+    }
+}
+
+__device__ __forceinline__ int cellOf(float u, float inv, int G) {
+    const int c = (int)floorf((u + 1.0f) * inv);
+    return c < 0 ? 0 : (c >= G ? G - 1 : c);
+}
+
+// Cells of face F (axis F / 2, sign + for even F) that a cone of unit axis h and inflated
+// angle tp can reach.  A direction w within tp of h differs from h by at most tp in every
+// component; on face F its axis component is the largest and >= 1/sqrt(3), so its face
+// coordinates w_j / w_i lie in the quotient intervals below (padded by 1e-5).
+__device__ bool faceRange(const float h[3], float tp, u32 F, float inv, int G, int r[4]) {
+    const u32 ax = F >> 1;
+    const float sg = (F & 1) ? -1.f : 1.f;
+    const u32 j = ax == 0 ? 1 : 0, k = ax == 2 ? 1 : 2;
+    const float hi = sg * h[ax];
+    const float m = fmaxf(fabsf(h[j]), fabsf(h[k]));
+    if (hi + 2.f * tp + 1e-5f < m) return false;
+    const float dlo = fmaxf(hi - tp, 0.57735027f - 1e-4f), dhi = hi + tp;
+    if (dhi < dlo) return false;
+    const float num[2] = {h[j], h[k]};
+    for (int c = 0; c < 2; ++c) {
+        const float nlo = num[c] - tp, nhi = num[c] + tp;
+        const float lo = fmaxf(fminf(nlo / dlo, nlo / dhi) - 1e-5f, -1.f);
+        const float up = fminf(fmaxf(nhi / dlo, nhi / dhi) + 1e-5f, 1.f);
+        if (lo > up) return false;
+        r[2 * c] = cellOf(lo, inv, G);
+        r[2 * c + 1] = cellOf(up, inv, G);
+    }
+    return true;
+}
+
+// wave minimum / maximum of v (every lane gets the result)
+__device__ __forceinline__ int waveMin(int v) {
+    for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off));
+    return v;
+}
+__device__ __forceinline__ int waveMax(int v) {
+    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off));
+    return v;
+}
+
+// Thread t takes link order[t].  mode 0 (links grouped by camera): cells per link (links over
+// WIDE_CELLS become wide) and the per-(camera, face) bounding boxes of the cells, one atomic
+// per wavefront and camera; mode 1 (links by descending reach): the (cell, link) entries at
+// offset[t], so that a stable sort by cell keeps each cell's links by descending reach.
+__global__ void k_gLinkCells(u32 links, int mode, const u32 *__restrict__ order, const u32 *__restrict__ cams,
+                             const float4 *__restrict__ axis, const float4 *__restrict__ aux,
+                             const CamGrid *__restrict__ grids, const u32 *__restrict__ camG,
+                             uint8_t *__restrict__ cls, u32 *__restrict__ count, int *__restrict__ box,
+                             const u32 *__restrict__ offset, u32 *__restrict__ keys, u32 *__restrict__ vals) {
+    const u32 t = blockIdx.x * BLOCK + threadIdx.x;
+    const u32 l = t < links ? order[t] : 0u;
+    bool valid = t < links;
+    if (valid && mode == 0) count[l] = 0;
+    if (valid && cls[l] == LINK_NONE) valid = false;
+    u32 cam = 0;
+    float h[3] = {0.f, 0.f, 0.f};
+    float tp = 0.f, inv = 1.f;
+    int G = 1;
+    bool isInfTan = false;
+    if (valid) {
+        cam = cams[l];
+        const float4 ax = axis[l];
+        h[0] = ax.x; h[1] = ax.y; h[2] = ax.z;
+        isInfTan = isinf(ax.w);
+        tp = aux[l].y;
+        G = (int)camG[cam];
+        inv = 0.5f * (float)G;
+    }
+    if (mode == 0) {
+        if (valid && cls[l] == LINK_WIDE && !isInfTan) cls[l] = LINK_GRID;   // re-planned with other cells
+        if (valid && cls[l] == LINK_WIDE) valid = false;
+        int r[4];
+        if (valid) {
+            u64 cells = 0;
+            for (u32 F = 0; F < FACES; ++F)
+                if (faceRange(h, tp, F, inv, G, r)) cells += (u64)(r[1] - r[0] + 1) * (u64)(r[3] - r[2] + 1);
+            if (cells > WIDE_CELLS) {
+                cls[l] = LINK_WIDE;
+                valid = false;
+            } else {
+                count[l] = (u32)cells;
+            }
+        }
+        for (u32 F = 0; F < FACES; ++F) {
+            bool has = valid && faceRange(h, tp, F, inv, G, r);
+            // lanes of one camera at a time: their box reduced over the wavefront first
+            for (u64 pending = __ballot(has); pending; pending = __ballot(has)) {
+                const int leader = __ffsll((long long)pending) - 1;
+                const u32 lc = __shfl(cam, leader);
+                const bool mine = has && cam == lc;
+                const int b0 = waveMin(mine ? r[0] : INT_MAX), b1 = waveMin(mine ? r[2] : INT_MAX);
+                const int b2 = waveMax(mine ? r[1] : INT_MIN), b3 = waveMax(mine ? r[3] : INT_MIN);
+                if ((int)(threadIdx.x & 63) == leader) {
+                    int *b = box + 4 * ((size_t)lc * FACES + F);
+                    atomicMin(&b[0], b0);
+                    atomicMin(&b[1], b1);
+                    atomicMax(&b[2], b2);
+                    atomicMax(&b[3], b3);
+                }
+                if (mine) has = false;
+            }
+        }
+        return;
+    }
+    if (!valid || cls[l] != LINK_GRID) return;
+    const CamGrid &cg = grids[cam];
+    u32 o = offset[t];
+    int r[4];
+    for (u32 F = 0; F < FACES; ++F) {
+        if (!faceRange(h, tp, F, inv, G, r)) continue;
+        const FaceGrid &fg = cg.face[F];
+        for (int v = r[2]; v <= r[3]; ++v)
+            for (int u = r[0]; u <= r[1]; ++u) {
+                keys[o] = fg.base + (u32)(v - fg.lo[1]) * fg.dim[0] + (u32)(u - fg.lo[0]);
+                vals[o] = l;
+                ++o;
+            }
+    }
+}
+
+__global__ void k_gatherU32(const u32 *__restrict__ src, const u32 *__restrict__ order, u32 n,
+                            u32 *__restrict__ dst) {
+    const u32 t = blockIdx.x * BLOCK + threadIdx.x;
+    if (t < n) dst[t] = src[order[t]];
+}
+
+// FP32 test data of each entry's link next to it (the leaf pass reads a cell's entries
+// contiguously): unit axis and tan of the inflated angle; reach, depth limit, link
+__global__ void k_entryRecords(const u32 *__restrict__ link, size_t n, const float4 *__restrict__ axis,
+                               const float4 *__restrict__ aux, float4 *__restrict__ rec,
+                               float *__restrict__ reach) {
+    const size_t e = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (e >= n) return;
+    const u32 l = link[e];
+    const float4 au = aux[l];
+    rec[2 * e] = axis[l];
+    rec[2 * e + 1] = make_float4(au.x, au.z, __uint_as_float(l), 0.f);
+    if (reach) reach[e] = au.x;
+}
+
+__global__ void k_fillBox(int *box, size_t n) {
+    const size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) box[i] = (i & 2) ? INT_MIN : INT_MAX;
+}
+
+__global__ void k_sumU32(const u32 *__restrict__ v, u32 n, unsigned long long *__restrict__ out) {
+    const u32 i = blockIdx.x * BLOCK + threadIdx.x;
+    unsigned long long x = i < n ? v[i] : 0u;
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+    if ((threadIdx.x & 63) == 0 && x) atomicAdd(out, x);
+}
+
+// per cell: its run [start, end) in the sorted entries and the largest reach of its links
+// (the first: entries of a cell are by descending reach)
+__global__ void k_cellRuns(const u32 *__restrict__ keys, size_t n, const float *__restrict__ entryReach,
+                           u32 *__restrict__ start, u32 *__restrict__ end, float *__restrict__ cellReach) {
+    const size_t j = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (j >= n) return;
+    const u32 c = keys[j];
+    if (j == 0 || keys[j - 1] != c) {
+        start[c] = (u32)j;
+        cellReach[c] = entryReach[j];
+    }
+    if (j + 1 == n || keys[j + 1] != c) end[c] = (u32)(j + 1);
+}
+
+__global__ void k_wideFlags(const uint8_t *__restrict__ cls, u32 links, u32 *__restrict__ flag) {
+    const u32 l = blockIdx.x * BLOCK + threadIdx.x;
+    if (l < links) flag[l] = cls[l] == LINK_WIDE ? 1u : 0u;
+}
+
+__global__ void k_iota(u32 *p, u32 n) {
+    const u32 i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) p[i] = i;
+}
+
+__global__ void k_wideCams(const u32 *__restrict__ wide, u32 n, const u32 *__restrict__ cams,
+                           u32 *__restrict__ key) {
+    const u32 i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) key[i] = cams[wide[i]];
+}
+
+__global__ void k_wideRuns(const u32 *__restrict__ camSorted, u32 n, u32 *__restrict__ begin,
+                           u32 *__restrict__ endOut) {
+    const u32 j = blockIdx.x * BLOCK + threadIdx.x;
+    if (j >= n) return;
+    const u32 c = camSorted[j];
+    if (j == 0 || camSorted[j - 1] != c) begin[c] = j;
+    if (j + 1 == n || camSorted[j + 1] != c) endOut[c] = j + 1;
+}
+
+__global__ void k_flaggedLeafFlags(const u32 *__restrict__ hitsS, u32 L, u32 *__restrict__ flag) {
+    const u32 l = blockIdx.x * BLOCK + threadIdx.x;
+    if (l < L) flag[l] = hitsS[l] ? 1u : 0u;
+}
+
+__global__ void k_listFlagged(const u32 *__restrict__ flag, const u32 *__restrict__ pos, u32 n,
+                              u32 *__restrict__ list) {
+    const u32 l = blockIdx.x * BLOCK + threadIdx.x;
+    if (l < n && flag[l]) list[pos[l]] = l;
+}
+
+struct GatherArgs {
+    Frame f;
+    const u64 *leafLoc;
+    const CamGrid *grids;
+    u32 camCount;
+    const u32 *cellStart, *cellEnd;
+    const float *cellReach;    // largest reach of the cell's links
+    const float *entryReach;   // per entry, descending within a cell
+    const float4 *rec;         // two per entry (k_entryRecords), sorted by cell
+    const float4 *wideRec;     // two per wide link, sorted by camera (link order within a camera)
+    LinkGeo g;
+    double bandwidth;
+};
+
+TSR_HD void twoSum(float &hi, float &lo, float x) {   // hi + lo += x (Kahan-Babuska)
+    const float s = hi + x;
+    const float bp = s - hi;
+    const float err = (hi - (s - bp)) + (x - bp);
+    hi = s;
+    lo += err;
+}
+
+// set bits of the wave-wide mask m below this lane
+__device__ __forceinline__ u32 lanesBelow(u64 m) {
+    return __builtin_amdgcn_mbcnt_hi((u32)(m >> 32), __builtin_amdgcn_mbcnt_lo((u32)m, 0u));
+}
+
+// The reference's FP64 kernel (coneKernel, Occupancy.cpp:279-346) of link `link` from camera
+// `cam` at leaf centre x, added to this lane's Kahan sums of kernel values and weighted cone
+// lengths when it is non-zero
+__device__ __forceinline__ void evalQueued(const GatherArgs &a, const double x[3], u32 link, u32 cam, float &sk,
+                                           float &ck, float &sl, float &cl, u32 &hits) {
+    const CamGrid &cg = a.grids[cam];
+    const float4 ps = a.g.pos[link];
+    const float4 nr = a.g.nrm[link];
+    const double2 cd = a.g.cone[link];
+    ConeD c;
+    c.a[0] = cg.a[0]; c.a[1] = cg.a[1]; c.a[2] = cg.a[2];
+    c.v[0] = (double)ps.x - c.a[0]; c.v[1] = (double)ps.y - c.a[1]; c.v[2] = (double)ps.z - c.a[2];
+    c.n[0] = -(double)nr.x; c.n[1] = -(double)nr.y; c.n[2] = -(double)nr.z;
+    c.R = a.bandwidth * (double)ps.w;
+    c.invDenom = cd.x;
+    c.tStart = 0.0;
+    c.invCut = 1.0;
+    c.length = cd.y;
+    c.degenerate = false;
+    const double conf = nr.w;
+    const double kv = coneKernel(c, x, false, conf);
+    if (kv == 0.0) return;
+    const float va = (float)kv - ck;
+    const float ta = sk + va;
+    ck = (ta - sk) - va;
+    sk = ta;
+    const float vb = (float)(conf * cd.y) - cl;
+    const float tb = sl + vb;
+    cl = (tb - sl) - vb;
+    sl = tb;
+    ++hits;
+}
+
+// One wavefront per leaf (LEAVES_PER_WAVE consecutive leaves of the list in turn; blocks are
+// one wavefront, so __syncthreads orders the wave's LDS segment table).  Per batch of 64
+// cameras, lane q looks up camera q's cell and keeps the cell's links whose reach is at least
+// the leaf's distance from the camera (a prefix: a cell's links are by descending reach); the
+// batch's candidates (each camera's prefix, then its wide links) are then spread over the 64
+// lanes in a fixed order.
+__global__ __launch_bounds__(64) void k_gLeaves(GatherArgs a, const u32 *__restrict__ leaves, u32 nLeaves,
+                                                u32 groupsPerXcd, float *__restrict__ ksum,
+                                                float *__restrict__ clen, u32 *__restrict__ hitCount,
+                                                unsigned long long *__restrict__ stats) {
+    __shared__ u32 segBeg[64], segLen[64], segWide[64], segEnd[64];
+    __shared__ float segA[3][64];   // camera centre of each lane's segment
+    __shared__ u32 qLink[128], qCam[128];   // candidates for the FP64 kernel
+    const u32 lane = threadIdx.x;
+    u32 qn = 0;   // queued candidates (wave-uniform)
+    // blocks go round-robin over the XCDs: give each XCD a contiguous run of leaf groups, so
+    // that one XCD's L2 holds the cells and links of one part of the scene
+    const u32 group = (blockIdx.x % XCD_COUNT) * groupsPerXcd + blockIdx.x / XCD_COUNT;
+    u32 lookups = 0, cand = 0, evals = 0;
+    const u32 K = a.camCount;
+    for (u32 t = 0; t < LEAVES_PER_WAVE; ++t) {
+        const u32 li = group * LEAVES_PER_WAVE + t;
+        if (li >= nLeaves) break;   // wave-uniform
+        const u32 leaf = leaves[li];
+        const u64 loc = a.leafLoc[leaf];
+        const u32 d = locDepth(loc);
+        u32 k[3];
+        locCoords(loc, k[0], k[1], k[2]);
+        double x[3];
+        for (int c = 0; c < 3; ++c) x[c] = a.f.B(c, 2u * k[c] + 1u, d + 1);   // the traversal's centre
+        const float xf[3] = {(float)x[0], (float)x[1], (float)x[2]};
+        const float xAbs = fabsf(xf[0]) + fabsf(xf[1]) + fabsf(xf[2]);
+        const float depthF = (float)d;
+        float sk = 0.f, ck = 0.f, sl = 0.f, cl = 0.f;
+        u32 hits = 0;
+        for (u32 cb = 0; cb < K; cb += 64) {
+            const u32 cam = cb + lane;
+            u32 beg = 0, len = 0, wb = 0, wl = 0;
+            if (cam < K) {
+                const CamGrid &cg = a.grids[cam];
+                const double w0 = x[0] - (double)cg.a[0], w1 = x[1] - (double)cg.a[1], w2 = x[2] - (double)cg.a[2];
+                const double b0 = fabs(w0), b1 = fabs(w1), b2 = fabs(w2);
+                u32 F;
+                double u, v, m;
+                if (b0 >= b1 && b0 >= b2) { F = w0 >= 0.0 ? 0 : 1; m = b0; u = w1; v = w2; }
+                else if (b1 >= b2) { F = w1 >= 0.0 ? 2 : 3; m = b1; u = w0; v = w2; }
+                else { F = w2 >= 0.0 ? 4 : 5; m = b2; u = w0; v = w1; }
+                const FaceGrid &fg = cg.face[F];
+                if (m > 0.0 && fg.dim[0]) {
+                    // face coordinates from the FP64 direction rounded to FP32 (relative error
+                    // ~1e-7, far inside the 1e-5 padding of the links' cell ranges)
+                    const int G = (int)cg.G;
+                    const float mf = (float)m;
+                    const int cu = cellOf((float)u / mf, cg.inv, G) - fg.lo[0];
+                    const int cv = cellOf((float)v / mf, cg.inv, G) - fg.lo[1];
+                    if (cu >= 0 && cv >= 0 && (u32)cu < fg.dim[0] && (u32)cv < fg.dim[1]) {
+                        const u32 cell = fg.base + (u32)cv * fg.dim[0] + (u32)cu;
+                        // a contributing link has |x - a| < |e - a| + R, and its reach is that
+                        // inflated by 1e-6: dist below |x - a| by more than FP32 rounding
+                        const float dist = sqrtf((float)(w0 * w0 + w1 * w1 + w2 * w2)) * (1.f - 4e-7f);
+                        ++lookups;
+                        if (a.cellReach[cell] >= dist) {
+                            u32 lo = a.cellStart[cell], hi = a.cellEnd[cell];
+                            beg = lo;
+                            while (lo < hi) {   // first entry with reach < dist
+                                const u32 mid = (lo + hi) >> 1;
+                                if (a.entryReach[mid] >= dist) lo = mid + 1;
+                                else hi = mid;
+                            }
+                            len = lo - beg;
+                        }
+                    }
+                }
+                wb = cg.wideBegin;
+                wl = cg.wideEnd - cg.wideBegin;
+                segA[0][lane] = cg.a[0];
+                segA[1][lane] = cg.a[1];
+                segA[2][lane] = cg.a[2];
+            }
+            u32 incl = len + wl;
+            for (u32 off = 1; off < 64; off <<= 1) {
+                const u32 y = __shfl_up(incl, off);
+                if (lane >= off) incl += y;
+            }
+            const u32 total = __shfl(incl, 63);
+            segBeg[lane] = beg;
+            segLen[lane] = len;
+            segWide[lane] = wb;
+            segEnd[lane] = incl;
+            __syncthreads();
+            // candidates passing the FP32 test are queued (link, camera) in LDS; the FP64 kernel
+            // runs on 64 queued candidates at a time, one per lane
+            u32 q = 0;
+            for (u32 base = 0; base < total; base += 64) {
+                const u32 j = base + lane;
+                bool pass = false;
+                u32 link = 0;
+                if (j < total) {
+                    while (segEnd[q] <= j) ++q;
+                    const u32 r = j - (q ? segEnd[q - 1] : 0u);
+                    const u32 sl0 = segLen[q];
+                    const float4 *rp = r < sl0 ? a.rec + 2 * (size_t)(segBeg[q] + r)
+                                               : a.wideRec + 2 * (size_t)(segWide[q] + (r - sl0));
+                    const float ca[3] = {segA[0][q], segA[1][q], segA[2][q]};
+                    ++cand;
+                    const float4 ax = rp[0];
+                    const float4 au = rp[1];   // reach, depth limit, link
+                    // FP32 cone test: along-axis distance s <= reach, distance from the axis
+                    // <= s tan(angle); the margin covers the rounding of x, a and the axis
+                    const float d0 = xf[0] - ca[0], d1 = xf[1] - ca[1], d2 = xf[2] - ca[2];
+                    const float s = d0 * ax.x + d1 * ax.y + d2 * ax.z;
+                    const float marg = 2e-6f * (xAbs + fabsf(ca[0]) + fabsf(ca[1]) + fabsf(ca[2]) + fabsf(s));
+                    const float q0 = d0 - s * ax.x, q1 = d1 - s * ax.y, q2 = d2 - s * ax.z;
+                    const float lim = fmaxf(s, 0.f) * ax.w + marg;
+                    // not deeper than the link's depth limit
+                    pass = depthF <= au.y && s >= -marg && s <= au.x + marg &&
+                           !(q0 * q0 + q1 * q1 + q2 * q2 > lim * lim);
+                    link = __float_as_uint(au.z);
+                }
+                const u64 m = __ballot(pass);
+                if (pass) {
+                    ++evals;
+                    const u32 at = qn + lanesBelow(m);
+                    qLink[at] = link;
+                    qCam[at] = cb + q;
+                }
+                qn += (u32)__popcll(m);
+                if (qn >= 64) {
+                    __syncthreads();
+                    evalQueued(a, x, qLink[lane], qCam[lane], sk, ck, sl, cl, hits);
+                    const u32 rest = qn - 64;
+                    u32 l2 = 0, c2 = 0;
+                    if (lane < rest) {
+                        l2 = qLink[64 + lane];
+                        c2 = qCam[64 + lane];
+                    }
+                    __syncthreads();
+                    if (lane < rest) {
+                        qLink[lane] = l2;
+                        qCam[lane] = c2;
+                    }
+                    qn = rest;
+                }
+            }
+            __syncthreads();
+        }
+        if (qn) {   // the queue's rest
+            __syncthreads();
+            if (lane < qn) evalQueued(a, x, qLink[lane], qCam[lane], sk, ck, sl, cl, hits);
+            qn = 0;
+            __syncthreads();
+        }
+        // the lanes' Kahan sums (sk - ck, sl - cl) folded by a fixed two-sum tree into lane 0
+        float hk = sk, lk = -ck, hl = sl, ll = -cl;
+        for (int off = 32; off > 0; off >>= 1) {
+            const float ohk = __shfl_down(hk, off), olk = __shfl_down(lk, off);
+            const float ohl = __shfl_down(hl, off), oll = __shfl_down(ll, off);
+            const u32 oh = __shfl_down(hits, off);
+            twoSum(hk, lk, ohk);
+            lk += olk;
+            twoSum(hl, ll, ohl);
+            ll += oll;
+            hits += oh;
+        }
+        if (lane == 0) {
+            ksum[leaf] = hk + lk;
+            clen[leaf] = hl + ll;
+            hitCount[leaf] = hits;
+        }
+    }
+    if (stats) {
+        unsigned long long v[3] = {lookups, cand, evals};
+        for (int i = 0; i < 3; ++i) {
+            unsigned long long y = v[i];
+            for (int off = 32; off > 0; off >>= 1) y += __shfl_xor(y, off);
+            if (lane == 0 && y) atomicAdd(&stats[i], y);
+        }
+    }
+}
+
+}  // namespace
+
+void emptinessByLeaves(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *scene, const tsr_params &prm,
+                       const u32 *hitsS, float *ksum, float *clen, u32 *hitCount, double *confs, u64 &hitsOut) {
+    hipStream_t st = ctx->stream;
+    Scratch &sc = ctx->scratch;
+    const u32 L = tree->leafCount, K = scene->camCount;
+    const u32 links = tree->sampleCount * tree->maxCams;
+    const bool trace = std::getenv("TSR_TRACE") != nullptr;
+    auto tmark = std::chrono::steady_clock::now();
+    auto lap = [&](const char *what) {   // TSR_TRACE: synchronised sub-step times
+        if (!trace) return;
+        HIPCHECK(hipStreamSynchronize(st));
+        const auto t = std::chrono::steady_clock::now();
+        std::fprintf(stderr, "[tsr occupancy]   %-28s %8.2f ms\n", what,
+                     std::chrono::duration<double, std::milli>(t - tmark).count());
+        tmark = t;
+    };
+    lap("(emptiness start)");
+    HIPCHECK(hipMemsetAsync(ksum, 0, (size_t)L * 4, st));
+    HIPCHECK(hipMemsetAsync(clen, 0, (size_t)L * 4, st));
+    HIPCHECK(hipMemsetAsync(hitCount, 0, (size_t)L * 4, st));
+    hitsOut = 0;
+    if (links == 0 || L == 0) return;
+
+    // ---- per-link cones; links grouped by camera and sorted by descending reach
+    DBuf<float4> axis(links), aux(links), pos(links), nrm(links);
+    DBuf<double2> cone(links);
+    DBuf<uint8_t> cls(links);
+    DBuf<unsigned long long> camTheta(2 * (size_t)std::max(K, 1u));
+    HIPCHECK(hipMemsetAsync(camTheta.p, 0, camTheta.bytes(), st));
+    const LinkSrc src{tree->px.p, tree->py.p, tree->pz.p, tree->nx.p, tree->ny.p, tree->nz.p, tree->scale.p,
+                      tree->conf.p, tree->cams.p, scene->cx.p, scene->cy.p, scene->cz.p, tree->sampleDepth.p,
+                      tree->maxCams, K, prm.bandwidth_factor};
+    const LinkGeo geo{axis.p, aux.p, pos.p, nrm.p, cone.p};
+    DBuf<u32> byCam(links), byReach(links);
+    {
+        DBuf<u32> camKey(links), reachKey(links), iota(links), keyS(links);
+        k_gLinkSetup<<<gridFor(links, BLOCK), BLOCK, 0, st>>>(src, links, geo, cls.p, confs, camTheta.p,
+                                                              camKey.p, reachKey.p, iota.p);
+        int bits = 1;
+        while (bits < 32 && (1ull << bits) <= K) ++bits;
+        sortPairs(sc, camKey.p, keyS.p, iota.p, byCam.p, links, 0, bits, st);
+        sortPairs(sc, reachKey.p, keyS.p, iota.p, byReach.p, links, 0, 32, st);
+    }
+    std::vector<unsigned long long> hTheta(2 * (size_t)K);
+    std::vector<float> hcx(K), hcy(K), hcz(K);
+    if (K) {
+        HIPCHECK(hipMemcpyAsync(hTheta.data(), camTheta.p, 16 * (size_t)K, hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipMemcpyAsync(hcx.data(), scene->cx.p, 4 * (size_t)K, hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipMemcpyAsync(hcy.data(), scene->cy.p, 4 * (size_t)K, hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipMemcpyAsync(hcz.data(), scene->cz.p, 4 * (size_t)K, hipMemcpyDeviceToHost, st));
+    }
+    HIPCHECK(hipStreamSynchronize(st));
+    // cells along a face side per camera: a cell about the camera's mean cone angle
+    std::vector<u32> camG(std::max(K, 1u), 1);
+    for (u32 c = 0; c < K; ++c) {
+        if (!hTheta[2 * c + 1]) continue;
+        const double mean = (double)hTheta[2 * c] / 4294967296.0 / (double)hTheta[2 * c + 1];
+        camG[c] = (u32)std::min<double>(MAX_FACE_CELLS, std::max(1.0, std::ceil(2.0 / mean)));
+    }
+    lap("link cones");
+
+    // ---- cell grids: bounding boxes per (camera, face), coarsened until they fit the budget
+    const u64 cellBudget = std::min<u64>(1ull << 31, std::max<u64>(1ull << 22, 8ull * links));
+    const u64 entryBudget = 1ull << 31;
+    DBuf<u32> camGd(std::max(K, 1u)), count(links);
+    DBuf<int> box(4 * (size_t)FACES * std::max(K, 1u));
+    DBuf<unsigned long long> sum1(1);
+    std::vector<int> hbox(4 * (size_t)FACES * K);
+    std::vector<CamGrid> grids(std::max(K, 1u));
+    u64 totalCells = 0, totalEntries = 0;
+    for (int attempt = 0;; ++attempt) {
+        if (attempt > 40) throw tsr_error(TSR_ERR_OUT_OF_MEMORY, "emptiness cell grid does not fit its budget");
+        HIPCHECK(hipMemcpyAsync(camGd.p, camG.data(), 4 * (size_t)std::max(K, 1u), hipMemcpyHostToDevice, st));
+        k_fillBox<<<gridFor(box.n, BLOCK), BLOCK, 0, st>>>(box.p, box.n);
+        k_gLinkCells<<<gridFor(links, BLOCK), BLOCK, 0, st>>>(links, 0, byCam.p, tree->cams.p, axis.p, aux.p,
+                                                              nullptr, camGd.p, cls.p, count.p, box.p, nullptr,
+                                                              nullptr, nullptr);
+        HIPCHECK(hipMemsetAsync(sum1.p, 0, 8, st));
+        k_sumU32<<<gridFor(links, BLOCK), BLOCK, 0, st>>>(count.p, links, sum1.p);
+        if (K) HIPCHECK(hipMemcpyAsync(hbox.data(), box.p, 4 * hbox.size(), hipMemcpyDeviceToHost, st));
+        totalEntries = readScalar(sum1.p, st);
+        totalCells = 0;
+        for (u32 c = 0; c < K; ++c) {
+            CamGrid &g = grids[c];
+            g.a[0] = hcx[c]; g.a[1] = hcy[c]; g.a[2] = hcz[c];
+            g.G = camG[c];
+            g.inv = 0.5f * (float)camG[c];
+            g.wideBegin = g.wideEnd = 0;
+            for (u32 F = 0; F < FACES; ++F) {
+                const int *b = &hbox[4 * ((size_t)c * FACES + F)];
+                FaceGrid &fg = g.face[F];
+                fg.base = (u32)std::min<u64>(totalCells, 0xffffffffull);
+                if (b[0] > b[2]) {
+                    fg.lo[0] = fg.lo[1] = 0;
+                    fg.dim[0] = fg.dim[1] = 0;
+                    continue;
+                }
+                fg.lo[0] = b[0];
+                fg.lo[1] = b[1];
+                fg.dim[0] = (u32)(b[2] - b[0] + 1);
+                fg.dim[1] = (u32)(b[3] - b[1] + 1);
+                totalCells += (u64)fg.dim[0] * fg.dim[1];
+            }
+        }
+        if (totalCells <= cellBudget && totalEntries < entryBudget) break;
+        bool coarser = false;
+        for (u32 c = 0; c < K; ++c)
+            if (camG[c] > 1) {
+                camG[c] = (camG[c] + 1) / 2;
+                coarser = true;
+            }
+        if (!coarser) throw tsr_error(TSR_ERR_OUT_OF_MEMORY, "emptiness cell grid does not fit its budget");
+    }
+    lap("cell grids");
+
+    // ---- wide links per camera (link order within a camera) with their test records
+    DBuf<float4> wideRec;
+    u32 nWide = 0;
+    {
+        DBuf<u32> flag(links), iota(links), cnt(1), wl(links);
+        k_wideFlags<<<gridFor(links, BLOCK), BLOCK, 0, st>>>(cls.p, links, flag.p);
+        k_iota<<<gridFor(links, BLOCK), BLOCK, 0, st>>>(iota.p, links);
+        size_t tmp = 0;
+        HIPCHECK(hipcub::DeviceSelect::Flagged(nullptr, tmp, iota.p, flag.p, wl.p, cnt.p, (int)links, st));
+        HIPCHECK(hipcub::DeviceSelect::Flagged(sc.get(tmp), tmp, iota.p, flag.p, wl.p, cnt.p, (int)links, st));
+        nWide = readScalar(cnt.p, st);
+        if (nWide) {
+            DBuf<u32> key(nWide), keyS(nWide), wideSorted(nWide), wb(K), we(K);
+            k_wideCams<<<gridFor(nWide, BLOCK), BLOCK, 0, st>>>(wl.p, nWide, tree->cams.p, key.p);
+            int bits = 1;
+            while (bits < 32 && (1ull << bits) < K) ++bits;
+            sortPairs(sc, key.p, keyS.p, wl.p, wideSorted.p, nWide, 0, bits, st);
+            wideRec.alloc(2 * (size_t)nWide);
+            k_entryRecords<<<gridFor(nWide, BLOCK), BLOCK, 0, st>>>(wideSorted.p, nWide, axis.p, aux.p, wideRec.p,
+                                                                    nullptr);
+            HIPCHECK(hipMemsetAsync(wb.p, 0, 4 * (size_t)K, st));
+            HIPCHECK(hipMemsetAsync(we.p, 0, 4 * (size_t)K, st));
+            k_wideRuns<<<gridFor(nWide, BLOCK), BLOCK, 0, st>>>(keyS.p, nWide, wb.p, we.p);
+            std::vector<u32> hb(K), he(K);
+            HIPCHECK(hipMemcpyAsync(hb.data(), wb.p, 4 * (size_t)K, hipMemcpyDeviceToHost, st));
+            HIPCHECK(hipMemcpyAsync(he.data(), we.p, 4 * (size_t)K, hipMemcpyDeviceToHost, st));
+            HIPCHECK(hipStreamSynchronize(st));
+            for (u32 c = 0; c < K; ++c) {
+                grids[c].wideBegin = hb[c];
+                grids[c].wideEnd = he[c];
+            }
+        }
+    }
+    DBuf<CamGrid> gridsD(std::max(K, 1u));
+    HIPCHECK(hipMemcpyAsync(gridsD.p, grids.data(), sizeof(CamGrid) * std::max(K, 1u), hipMemcpyHostToDevice, st));
+
+    // ---- (cell, link) entries sorted by cell, descending reach within a cell, with the
+    // links' test records alongside
+    const size_t nE = (size_t)totalEntries;
+    DBuf<float4> rec(2 * std::max<size_t>(nE, 1));
+    DBuf<float> entryReach(std::max<size_t>(nE, 1));
+    DBuf<u32> keysS(std::max<size_t>(nE, 1));
+    {
+        DBuf<u32> keys(std::max<size_t>(nE, 1)), vals(std::max<size_t>(nE, 1)), entries(std::max<size_t>(nE, 1));
+        {
+            DBuf<u32> countR(links), offR(links);
+            k_gatherU32<<<gridFor(links, BLOCK), BLOCK, 0, st>>>(count.p, byReach.p, links, countR.p);
+            exclusiveScan(sc, countR.p, offR.p, links, st);
+            k_gLinkCells<<<gridFor(links, BLOCK), BLOCK, 0, st>>>(links, 1, byReach.p, tree->cams.p, axis.p, aux.p,
+                                                                  gridsD.p, camGd.p, cls.p, nullptr, nullptr,
+                                                                  offR.p, keys.p, vals.p);
+        }
+        int bits = 1;
+        while (bits < 32 && (1ull << bits) < totalCells) ++bits;
+        sortPairs(sc, keys.p, keysS.p, vals.p, entries.p, nE, 0, bits, st);
+        if (nE) k_entryRecords<<<gridFor(nE, BLOCK), BLOCK, 0, st>>>(entries.p, nE, axis.p, aux.p, rec.p,
+                                                                     entryReach.p);
+    }
+    const size_t nC = std::max<u64>(totalCells, 1);
+    DBuf<u32> cellStart(nC), cellEnd(nC);
+    DBuf<float> cellReach(nC);
+    HIPCHECK(hipMemsetAsync(cellStart.p, 0, 4 * nC, st));
+    HIPCHECK(hipMemsetAsync(cellEnd.p, 0, 4 * nC, st));
+    HIPCHECK(hipMemsetAsync(cellReach.p, 0, 4 * nC, st));
+    if (nE) k_cellRuns<<<gridFor(nE, BLOCK), BLOCK, 0, st>>>(keysS.p, nE, entryReach.p, cellStart.p, cellEnd.p,
+                                                             cellReach.p);
+    keysS.release();
+    lap("cell index");
+    if (trace)
+        std::fprintf(stderr, "[tsr occupancy] emptiness index: %u links, %llu cells, %llu entries, %u wide links\n",
+                     links, (unsigned long long)totalCells, (unsigned long long)totalEntries, nWide);
+
+    // ---- the sampleness-flagged leaves, in leaf order
+    DBuf<u32> lflag(L), lpos(L), leafList(L);
+    k_flaggedLeafFlags<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(hitsS, L, lflag.p);
+    exclusiveScan(sc, lflag.p, lpos.p, L, st);
+    k_listFlagged<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(lflag.p, lpos.p, L, leafList.p);
+    const u32 nLeaves = readScalar(lpos.p + L - 1, st) + readScalar(lflag.p + L - 1, st);
+    lflag.release();
+    lpos.release();
+
+    // ---- the leaf pass
+    GatherArgs ga{tree->frame, tree->leafLoc.p, gridsD.p, K, cellStart.p, cellEnd.p, cellReach.p,
+                  entryReach.p, rec.p, nWide ? wideRec.p : rec.p, geo, (double)prm.bandwidth_factor};
+    DBuf<unsigned long long> kst(4);
+    HIPCHECK(hipMemsetAsync(kst.p, 0, 32, st));
+    const u32 groups = (nLeaves + LEAVES_PER_WAVE - 1) / LEAVES_PER_WAVE;
+    const u32 perXcd = (groups + XCD_COUNT - 1) / XCD_COUNT;
+    if (nLeaves) {
+        timedLaunch(ctx, KSTAT_E_GATHER, [&] {
+            k_gLeaves<<<perXcd * XCD_COUNT, 64, 0, st>>>(ga, leafList.p, nLeaves, perXcd, ksum, clen, hitCount,
+                                                          kst.p);
+        });
+    }
+    unsigned long long ks[3] = {0, 0, 0};
+    HIPCHECK(hipMemcpyAsync(ks, kst.p, 24, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemsetAsync(sum1.p, 0, 8, st));
+    k_sumU32<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(hitCount, L, sum1.p);
+    hitsOut = readScalar(sum1.p, st);
+    {
+        // algorithmic bytes: per leaf its list entry, code and the three outputs (24 B); per
+        // (leaf, camera) cell lookup the cell's run and reach (12 B); per candidate its test
+        // record (32 B); per FP64 evaluation the sample and cone data (48 B)
+        tsr_kernel_stat &k = ctx->kstat[KSTAT_E_GATHER];
+        k.bytes += (u64)nLeaves * 24 + ks[0] * 12 + ks[1] * 32 + ks[2] * 48;
+        k.units += nLeaves;
+    }
+    lap("leaf pass");
+    if (trace)
+        std::fprintf(stderr, "[tsr occupancy] emptiness by leaves: %u flagged leaves, %llu cell lookups, "
+                     "%llu candidates, %llu kernel evaluations, %llu hits\n", nLeaves,
+                     (unsigned long long)ks[0], (unsigned long long)ks[1], (unsigned long long)ks[2],
+                     (unsigned long long)hitsOut);
+}
+
+}  // namespace tsr

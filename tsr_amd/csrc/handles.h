@@ -129,7 +129,7 @@ void *pinnedHost(tsr_context *ctx, size_t bytes, int slot = 0);
 // runs fn(begin, end) over [0, n) on up to 16 host threads (OMP_NUM_THREADS caps it),
 // at least `grain` items per thread
 void parallelFor(size_t n, const std::function<void(size_t, size_t)> &fn, size_t grain = 65536);
-enum { KSTAT_PRIORS = 0, KSTAT_E_COUNT = 1, KSTAT_E_WRITE = 2, KSTAT_S_COUNT = 3, KSTAT_S_WRITE = 4 };
+enum { KSTAT_PRIORS = 0, KSTAT_E_COUNT = 1, KSTAT_E_WRITE = 2, KSTAT_S_COUNT = 3, KSTAT_S_WRITE = 4, KSTAT_E_GATHER = 5 };
 // time one kernel launch on the context stream with HIP events (synchronises)
 template <typename Launch>
 void timedLaunch(tsr_context *ctx, int stat, Launch launch) {
@@ -172,6 +172,11 @@ void samplesFileRead(const char *path, float *px, float *py, float *pz, float *n
                      float *scale, float *conf, u32 *cams);
 void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *scene, const tsr_params &prm,
                       int flags, const tsr_culler *culler, tsr_field *field);
+// E pass gathered by the sampleness-flagged leaves (hitsS[leaf] > 0): per-leaf kernel sums,
+// cone length sums and hit counts (L each, device), the confidence of every link
+// (0: no emptiness cone; device, links) and the total hit count
+void emptinessByLeaves(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *scene, const tsr_params &prm,
+                       const u32 *hitsS, float *ksum, float *clen, u32 *hitCount, double *confs, u64 &hitsOut);
 // Occupancy::addSamples / eraseSamples: the field follows a change of samples (indices into
 // the tree's node-ordered samples, host array of n)
 void updateOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *scene, const u32 *chosenHost, u32 n,

@@ -1533,7 +1533,11 @@ void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *s
             k_packChildState<<<gridFor(N, BLOCK), BLOCK, 0, st>>>(tree->child.p, field->states.p, N, packed);
             tt.packed = packed;
         }
-        if (atomicMode) {
+        if (!sampleness && !atomicMode && !(flags & TSR_EMPTINESS_TRAVERSAL)) {
+            emptinessByLeaves(ctx, tree, scene, prm, field->hitsS.p, ksum, clen, hitCount.p, confs.p,
+                              field->hits[type]);
+            field->overflowLinks[type] = 0;
+        } else if (atomicMode) {
             DBuf<u32> counts(links), offsets(1);
             HIPCHECK(hipMemsetAsync(ksum, 0, (size_t)L * 4, st));
             HIPCHECK(hipMemsetAsync(clen, 0, (size_t)L * 4, st));
