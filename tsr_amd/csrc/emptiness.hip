@@ -44,6 +44,9 @@ constexpr u32 FACES = 6;
 constexpr u32 WIDE_CELLS = 4096;        // a link whose cone covers more cells is listed per camera
 constexpr u32 MAX_FACE_CELLS = 1u << 15;  // cells along one side of a face grid
 constexpr u32 LEAVES_PER_WAVE = 4;      // consecutive leaves per wavefront of the leaf pass
+constexpr u32 CAND_UNROLL = 4;          // candidates per lane and step of the leaf pass
+constexpr u32 GATHER_WAVES = 8;         // wavefronts per block of the leaf pass
+constexpr u32 CAM_LDS = 256;            // cameras whose grids the leaf pass keeps in LDS
 constexpr double REAL_MAX_D = 3.4028234663852886e38;
 
 // link classes
@@ -254,16 +257,17 @@ __global__ void k_gatherU32(const u32 *__restrict__ src, const u32 *__restrict__
 }
 
 // FP32 test data of each entry's link next to it (the leaf pass reads a cell's entries
-// contiguously): unit axis and tan of the inflated angle; reach, depth limit, link
+// contiguously), as two arrays: unit axis and tan of the inflated angle; reach, depth limit,
+// link
 __global__ void k_entryRecords(const u32 *__restrict__ link, size_t n, const float4 *__restrict__ axis,
-                               const float4 *__restrict__ aux, float4 *__restrict__ rec,
-                               float *__restrict__ reach) {
+                               const float4 *__restrict__ aux, float4 *__restrict__ recAxis,
+                               float4 *__restrict__ recAux, float *__restrict__ reach) {
     const size_t e = (size_t)blockIdx.x * BLOCK + threadIdx.x;
     if (e >= n) return;
     const u32 l = link[e];
     const float4 au = aux[l];
-    rec[2 * e] = axis[l];
-    rec[2 * e + 1] = make_float4(au.x, au.z, __uint_as_float(l), 0.f);
+    recAxis[e] = axis[l];
+    recAux[e] = make_float4(au.x, au.z, __uint_as_float(l), 0.f);
     if (reach) reach[e] = au.x;
 }
 
@@ -337,8 +341,8 @@ struct GatherArgs {
     const u32 *cellStart, *cellEnd;
     const float *cellReach;    // largest reach of the cell's links
     const float *entryReach;   // per entry, descending within a cell
-    const float4 *rec;         // two per entry (k_entryRecords), sorted by cell
-    const float4 *wideRec;     // two per wide link, sorted by camera (link order within a camera)
+    const float4 *recAxis, *recAux;     // per entry (k_entryRecords), sorted by cell
+    const float4 *wideAxis, *wideAux;   // per wide link, sorted by camera (link order within a camera)
     LinkGeo g;
     double bandwidth;
 };
@@ -359,9 +363,9 @@ __device__ __forceinline__ u32 lanesBelow(u64 m) {
 // The reference's FP64 kernel (coneKernel, Occupancy.cpp:279-346) of link `link` from camera
 // `cam` at leaf centre x, added to this lane's Kahan sums of kernel values and weighted cone
 // lengths when it is non-zero
-__device__ __forceinline__ void evalQueued(const GatherArgs &a, const double x[3], u32 link, u32 cam, float &sk,
-                                           float &ck, float &sl, float &cl, u32 &hits) {
-    const CamGrid &cg = a.grids[cam];
+__device__ __forceinline__ void evalQueued(const GatherArgs &a, const CamGrid *camL, const double x[3], u32 link,
+                                           u32 cam, float &sk, float &ck, float &sl, float &cl, u32 &hits) {
+    const CamGrid &cg = cam < CAM_LDS ? camL[cam] : a.grids[cam];
     const float4 ps = a.g.pos[link];
     const float4 nr = a.g.nrm[link];
     const double2 cd = a.g.cone[link];
@@ -389,24 +393,49 @@ __device__ __forceinline__ void evalQueued(const GatherArgs &a, const double x[3
     ++hits;
 }
 
-// One wavefront per leaf (LEAVES_PER_WAVE consecutive leaves of the list in turn; blocks are
-// one wavefront, so __syncthreads orders the wave's LDS segment table).  Per batch of 64
-// cameras, lane q looks up camera q's cell and keeps the cell's links whose reach is at least
-// the leaf's distance from the camera (a prefix: a cell's links are by descending reach); the
-// batch's candidates (each camera's prefix, then its wide links) are then spread over the 64
-// lanes in a fixed order.
-__global__ __launch_bounds__(64) void k_gLeaves(GatherArgs a, const u32 *__restrict__ leaves, u32 nLeaves,
-                                                u32 groupsPerXcd, float *__restrict__ ksum,
-                                                float *__restrict__ clen, u32 *__restrict__ hitCount,
-                                                unsigned long long *__restrict__ stats) {
-    __shared__ u32 segBeg[64], segLen[64], segWide[64], segEnd[64];
-    __shared__ float segA[3][64];   // camera centre of each lane's segment
-    __shared__ u32 qLink[128], qCam[128];   // candidates for the FP64 kernel
-    const u32 lane = threadIdx.x;
+// LDS written by some lanes of a wavefront and read by others after this point: the
+// wavefront's LDS operations run in order, so a compiler fence at wavefront scope suffices
+// (no s_barrier; each wavefront of a block works on its own LDS slice)
+__device__ __forceinline__ void waveSync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+
+// One wavefront per leaf (LEAVES_PER_WAVE consecutive leaves of the list in turn).  Per batch
+// of 64 cameras, lane q looks up camera q's cell and keeps the cell's links whose reach is at
+// least the leaf's distance from the camera (a prefix: a cell's links are by descending
+// reach); the batch's candidates (each camera's prefix, then its wide links) are then spread
+// over the 64 lanes in a fixed order.
+__global__ __launch_bounds__(64 * GATHER_WAVES) void k_gLeaves(GatherArgs a, const u32 *__restrict__ leaves,
+                                                               u32 nLeaves, u32 blocksPerXcd,
+                                                               float *__restrict__ ksum, float *__restrict__ clen,
+                                                               u32 *__restrict__ hitCount,
+                                                               unsigned long long *__restrict__ stats) {
+    __shared__ u32 segStartS[GATHER_WAVES][64], segBegS[GATHER_WAVES][64], segLenS[GATHER_WAVES][64],
+        segWideS[GATHER_WAVES][64], segCamS[GATHER_WAVES][64];
+    __shared__ float segAS[GATHER_WAVES][3][64];   // camera centre of each segment
+    __shared__ u32 qLinkS[GATHER_WAVES][64 * (CAND_UNROLL + 1)], qCamS[GATHER_WAVES][64 * (CAND_UNROLL + 1)];
+    const u32 lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    u32 *segStart = segStartS[wv], *segBeg = segBegS[wv], *segLen = segLenS[wv], *segWide = segWideS[wv],
+        *segCam = segCamS[wv];
+    float (*segA)[64] = segAS[wv];
+    u32 *qLink = qLinkS[wv], *qCam = qCamS[wv];   // candidates for the FP64 kernel
     u32 qn = 0;   // queued candidates (wave-uniform)
+    // the first CAM_LDS cameras' grids, staged once per block (a lane reads its own camera's
+    // grid: from LDS, not 64 scattered cache lines)
+    __shared__ CamGrid camL[CAM_LDS];
+    {
+        const u32 words = (u32)(min(a.camCount, CAM_LDS) * (sizeof(CamGrid) / 4));
+        const u32 *src = reinterpret_cast<const u32 *>(a.grids);
+        u32 *dst = reinterpret_cast<u32 *>(camL);
+        for (u32 i = threadIdx.x; i < words; i += blockDim.x) dst[i] = src[i];
+        __syncthreads();
+    }
     // blocks go round-robin over the XCDs: give each XCD a contiguous run of leaf groups, so
     // that one XCD's L2 holds the cells and links of one part of the scene
-    const u32 group = (blockIdx.x % XCD_COUNT) * groupsPerXcd + blockIdx.x / XCD_COUNT;
+    const u32 group = ((blockIdx.x % XCD_COUNT) * blocksPerXcd + blockIdx.x / XCD_COUNT) * GATHER_WAVES + wv;
     u32 lookups = 0, cand = 0, evals = 0;
     const u32 K = a.camCount;
     for (u32 t = 0; t < LEAVES_PER_WAVE; ++t) {
@@ -428,7 +457,7 @@ __global__ __launch_bounds__(64) void k_gLeaves(GatherArgs a, const u32 *__restr
             const u32 cam = cb + lane;
             u32 beg = 0, len = 0, wb = 0, wl = 0;
             if (cam < K) {
-                const CamGrid &cg = a.grids[cam];
+                const CamGrid &cg = cam < CAM_LDS ? camL[cam] : a.grids[cam];
                 const double w0 = x[0] - (double)cg.a[0], w1 = x[1] - (double)cg.a[1], w2 = x[2] - (double)cg.a[2];
                 const double b0 = fabs(w0), b1 = fabs(w1), b2 = fabs(w2);
                 u32 F;
@@ -464,82 +493,118 @@ __global__ __launch_bounds__(64) void k_gLeaves(GatherArgs a, const u32 *__restr
                 }
                 wb = cg.wideBegin;
                 wl = cg.wideEnd - cg.wideBegin;
-                segA[0][lane] = cg.a[0];
-                segA[1][lane] = cg.a[1];
-                segA[2][lane] = cg.a[2];
             }
-            u32 incl = len + wl;
+            const u32 n = len + wl;
+            u32 incl = n;
             for (u32 off = 1; off < 64; off <<= 1) {
                 const u32 y = __shfl_up(incl, off);
                 if (lane >= off) incl += y;
             }
             const u32 total = __shfl(incl, 63);
-            segBeg[lane] = beg;
-            segLen[lane] = len;
-            segWide[lane] = wb;
-            segEnd[lane] = incl;
-            __syncthreads();
+            // the non-empty segments, compacted in camera order
+            const u64 nonEmpty = __ballot(n > 0);
+            const u32 nseg = (u32)__popcll(nonEmpty);
+            if (n > 0) {
+                const u32 at = lanesBelow(nonEmpty);
+                const CamGrid &cg = cam < CAM_LDS ? camL[cam] : a.grids[cam];
+                segStart[at] = incl - n;
+                segBeg[at] = beg;
+                segLen[at] = len;
+                segWide[at] = wb;
+                segCam[at] = cam;
+                segA[0][at] = cg.a[0];
+                segA[1][at] = cg.a[1];
+                segA[2][at] = cg.a[2];
+            }
+            waveSync();
             // candidates passing the FP32 test are queued (link, camera) in LDS; the FP64 kernel
-            // runs on 64 queued candidates at a time, one per lane
-            u32 q = 0;
-            for (u32 base = 0; base < total; base += 64) {
-                const u32 j = base + lane;
-                bool pass = false;
-                u32 link = 0;
-                if (j < total) {
-                    while (segEnd[q] <= j) ++q;
-                    const u32 r = j - (q ? segEnd[q - 1] : 0u);
-                    const u32 sl0 = segLen[q];
-                    const float4 *rp = r < sl0 ? a.rec + 2 * (size_t)(segBeg[q] + r)
-                                               : a.wideRec + 2 * (size_t)(segWide[q] + (r - sl0));
-                    const float ca[3] = {segA[0][q], segA[1][q], segA[2][q]};
-                    ++cand;
-                    const float4 ax = rp[0];
-                    const float4 au = rp[1];   // reach, depth limit, link
-                    // FP32 cone test: along-axis distance s <= reach, distance from the axis
-                    // <= s tan(angle); the margin covers the rounding of x, a and the axis
-                    const float d0 = xf[0] - ca[0], d1 = xf[1] - ca[1], d2 = xf[2] - ca[2];
-                    const float s = d0 * ax.x + d1 * ax.y + d2 * ax.z;
-                    const float marg = 2e-6f * (xAbs + fabsf(ca[0]) + fabsf(ca[1]) + fabsf(ca[2]) + fabsf(s));
-                    const float q0 = d0 - s * ax.x, q1 = d1 - s * ax.y, q2 = d2 - s * ax.z;
-                    const float lim = fmaxf(s, 0.f) * ax.w + marg;
-                    // not deeper than the link's depth limit
-                    pass = depthF <= au.y && s >= -marg && s <= au.x + marg &&
-                           !(q0 * q0 + q1 * q1 + q2 * q2 > lim * lim);
-                    link = __float_as_uint(au.z);
-                }
-                const u64 m = __ballot(pass);
-                if (pass) {
-                    ++evals;
-                    const u32 at = qn + lanesBelow(m);
-                    qLink[at] = link;
-                    qCam[at] = cb + q;
-                }
-                qn += (u32)__popcll(m);
-                if (qn >= 64) {
-                    __syncthreads();
-                    evalQueued(a, x, qLink[lane], qCam[lane], sk, ck, sl, cl, hits);
-                    const u32 rest = qn - 64;
-                    u32 l2 = 0, c2 = 0;
-                    if (lane < rest) {
-                        l2 = qLink[64 + lane];
-                        c2 = qCam[64 + lane];
+            // runs on 64 queued candidates at a time, one per lane.  A lane takes CAND_UNROLL
+            // candidates per step (j, j + 64, ...) with their record loads issued together.
+            for (u32 base = 0; base < total; base += 64 * CAND_UNROLL) {
+                float4 ax[CAND_UNROLL], au[CAND_UNROLL];
+                u32 qs[CAND_UNROLL];
+#pragma unroll
+                for (u32 uu = 0; uu < CAND_UNROLL; ++uu) {
+                    const u32 j = base + 64 * uu + lane;
+                    qs[uu] = 64;
+                    if (j < total) {
+                        u32 lo = 0, hi = nseg - 1;   // last segment starting at or before j
+                        while (lo < hi) {
+                            const u32 mid = (lo + hi + 1) >> 1;
+                            if (segStart[mid] <= j) lo = mid;
+                            else hi = mid - 1;
+                        }
+                        const u32 q = lo;
+                        const u32 r = j - segStart[q];
+                        const u32 sl0 = segLen[q];
+                        if (r < sl0) {
+                            ax[uu] = a.recAxis[segBeg[q] + r];
+                            au[uu] = a.recAux[segBeg[q] + r];   // reach, depth limit, link
+                        } else {
+                            ax[uu] = a.wideAxis[segWide[q] + (r - sl0)];
+                            au[uu] = a.wideAux[segWide[q] + (r - sl0)];
+                        }
+                        qs[uu] = q;
                     }
-                    __syncthreads();
-                    if (lane < rest) {
-                        qLink[lane] = l2;
-                        qCam[lane] = c2;
+                }
+#pragma unroll
+                for (u32 uu = 0; uu < CAND_UNROLL; ++uu) {
+                    bool pass = false;
+                    if (qs[uu] < 64) {
+                        ++cand;
+                        const u32 qq = qs[uu];
+                        const float ca[3] = {segA[0][qq], segA[1][qq], segA[2][qq]};
+                        // FP32 cone test: along-axis distance s <= reach, distance from the axis
+                        // <= s tan(angle); the margin covers the rounding of x, a and the axis
+                        const float d0 = xf[0] - ca[0], d1 = xf[1] - ca[1], d2 = xf[2] - ca[2];
+                        const float s = d0 * ax[uu].x + d1 * ax[uu].y + d2 * ax[uu].z;
+                        const float marg = 2e-6f * (xAbs + fabsf(ca[0]) + fabsf(ca[1]) + fabsf(ca[2]) + fabsf(s));
+                        const float q0 = d0 - s * ax[uu].x, q1 = d1 - s * ax[uu].y, q2 = d2 - s * ax[uu].z;
+                        const float lim = fmaxf(s, 0.f) * ax[uu].w + marg;
+                        // not deeper than the link's depth limit
+                        pass = depthF <= au[uu].y && s >= -marg && s <= au[uu].x + marg &&
+                               !(q0 * q0 + q1 * q1 + q2 * q2 > lim * lim);
+                    }
+                    const u64 m = __ballot(pass);
+                    if (pass) {
+                        ++evals;
+                        const u32 at = qn + lanesBelow(m);
+                        qLink[at] = __float_as_uint(au[uu].z);
+                        qCam[at] = segCam[qs[uu]];
+                    }
+                    qn += (u32)__popcll(m);
+                }
+                while (qn >= 64) {
+                    waveSync();
+                    evalQueued(a, camL, x, qLink[lane], qCam[lane], sk, ck, sl, cl, hits);
+                    // the rest moves to the front (read all, then write)
+                    const u32 rest = qn - 64;
+                    u32 l2[CAND_UNROLL], c2[CAND_UNROLL];
+#pragma unroll
+                    for (u32 uu = 0; uu < CAND_UNROLL; ++uu) {
+                        const u32 i = lane + 64 * uu;
+                        l2[uu] = i < rest ? qLink[64 + i] : 0u;
+                        c2[uu] = i < rest ? qCam[64 + i] : 0u;
+                    }
+                    waveSync();
+#pragma unroll
+                    for (u32 uu = 0; uu < CAND_UNROLL; ++uu) {
+                        const u32 i = lane + 64 * uu;
+                        if (i < rest) {
+                            qLink[i] = l2[uu];
+                            qCam[i] = c2[uu];
+                        }
                     }
                     qn = rest;
                 }
             }
-            __syncthreads();
+            waveSync();
         }
         if (qn) {   // the queue's rest
-            __syncthreads();
-            if (lane < qn) evalQueued(a, x, qLink[lane], qCam[lane], sk, ck, sl, cl, hits);
+            waveSync();
+            if (lane < qn) evalQueued(a, camL, x, qLink[lane], qCam[lane], sk, ck, sl, cl, hits);
             qn = 0;
-            __syncthreads();
+            waveSync();
         }
         // the lanes' Kahan sums (sk - ck, sl - cl) folded by a fixed two-sum tree into lane 0
         float hk = sk, lk = -ck, hl = sl, ll = -cl;
@@ -687,7 +752,7 @@ void emptinessByLeaves(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *
     lap("cell grids");
 
     // ---- wide links per camera (link order within a camera) with their test records
-    DBuf<float4> wideRec;
+    DBuf<float4> wideAxis, wideAux;
     u32 nWide = 0;
     {
         DBuf<u32> flag(links), iota(links), cnt(1), wl(links);
@@ -703,9 +768,10 @@ void emptinessByLeaves(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *
             int bits = 1;
             while (bits < 32 && (1ull << bits) < K) ++bits;
             sortPairs(sc, key.p, keyS.p, wl.p, wideSorted.p, nWide, 0, bits, st);
-            wideRec.alloc(2 * (size_t)nWide);
-            k_entryRecords<<<gridFor(nWide, BLOCK), BLOCK, 0, st>>>(wideSorted.p, nWide, axis.p, aux.p, wideRec.p,
-                                                                    nullptr);
+            wideAxis.alloc(nWide);
+            wideAux.alloc(nWide);
+            k_entryRecords<<<gridFor(nWide, BLOCK), BLOCK, 0, st>>>(wideSorted.p, nWide, axis.p, aux.p, wideAxis.p,
+                                                                    wideAux.p, nullptr);
             HIPCHECK(hipMemsetAsync(wb.p, 0, 4 * (size_t)K, st));
             HIPCHECK(hipMemsetAsync(we.p, 0, 4 * (size_t)K, st));
             k_wideRuns<<<gridFor(nWide, BLOCK), BLOCK, 0, st>>>(keyS.p, nWide, wb.p, we.p);
@@ -725,7 +791,7 @@ void emptinessByLeaves(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *
     // ---- (cell, link) entries sorted by cell, descending reach within a cell, with the
     // links' test records alongside
     const size_t nE = (size_t)totalEntries;
-    DBuf<float4> rec(2 * std::max<size_t>(nE, 1));
+    DBuf<float4> recAxis(std::max<size_t>(nE, 1)), recAux(std::max<size_t>(nE, 1));
     DBuf<float> entryReach(std::max<size_t>(nE, 1));
     DBuf<u32> keysS(std::max<size_t>(nE, 1));
     {
@@ -741,8 +807,8 @@ void emptinessByLeaves(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *
         int bits = 1;
         while (bits < 32 && (1ull << bits) < totalCells) ++bits;
         sortPairs(sc, keys.p, keysS.p, vals.p, entries.p, nE, 0, bits, st);
-        if (nE) k_entryRecords<<<gridFor(nE, BLOCK), BLOCK, 0, st>>>(entries.p, nE, axis.p, aux.p, rec.p,
-                                                                     entryReach.p);
+        if (nE) k_entryRecords<<<gridFor(nE, BLOCK), BLOCK, 0, st>>>(entries.p, nE, axis.p, aux.p, recAxis.p,
+                                                                     recAux.p, entryReach.p);
     }
     const size_t nC = std::max<u64>(totalCells, 1);
     DBuf<u32> cellStart(nC), cellEnd(nC);
@@ -769,15 +835,17 @@ void emptinessByLeaves(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *
 
     // ---- the leaf pass
     GatherArgs ga{tree->frame, tree->leafLoc.p, gridsD.p, K, cellStart.p, cellEnd.p, cellReach.p,
-                  entryReach.p, rec.p, nWide ? wideRec.p : rec.p, geo, (double)prm.bandwidth_factor};
+                  entryReach.p, recAxis.p, recAux.p, nWide ? wideAxis.p : recAxis.p, nWide ? wideAux.p : recAux.p,
+                  geo, (double)prm.bandwidth_factor};
     DBuf<unsigned long long> kst(4);
     HIPCHECK(hipMemsetAsync(kst.p, 0, 32, st));
     const u32 groups = (nLeaves + LEAVES_PER_WAVE - 1) / LEAVES_PER_WAVE;
-    const u32 perXcd = (groups + XCD_COUNT - 1) / XCD_COUNT;
+    const u32 blocks = (groups + GATHER_WAVES - 1) / GATHER_WAVES;
+    const u32 perXcd = (blocks + XCD_COUNT - 1) / XCD_COUNT;
     if (nLeaves) {
         timedLaunch(ctx, KSTAT_E_GATHER, [&] {
-            k_gLeaves<<<perXcd * XCD_COUNT, 64, 0, st>>>(ga, leafList.p, nLeaves, perXcd, ksum, clen, hitCount,
-                                                          kst.p);
+            k_gLeaves<<<perXcd * XCD_COUNT, 64 * GATHER_WAVES, 0, st>>>(ga, leafList.p, nLeaves, perXcd, ksum, clen,
+                                                                         hitCount, kst.p);
         });
     }
     unsigned long long ks[3] = {0, 0, 0};
