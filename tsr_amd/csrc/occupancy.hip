@@ -201,6 +201,45 @@ __device__ void coneLink(const TraversalTree &t, const LinkInputs &in, u32 link,
                          u32 *__restrict__ atomicStates, double *__restrict__ confOut, u32 &visits, u32 &count,
                          u32 *path);
 
+// Deepest node whose subtree holds every leaf centre the cone can reach: a centre with a
+// non-zero kernel lies in a ball B(a + v t, R t) with t in [tStart, 1], so inside the box
+// of the two end balls; the descent follows the child whose half contains that box on
+// every axis (a lower child's centres are < the splitting plane, an upper child's >) and
+// stops where the box straddles a plane or at a leaf.  Any node above it that the
+// traversal would accept contains the same contributing centres, and the culling never
+// rejects a node holding one, so a traversal started here meets the same leaves.
+__device__ void coneStart(const TraversalTree &t, const ConeD &c, u32 &node, u32 &d, u32 &k0, u32 &k1, u32 &k2) {
+    double lo[3], hi[3];
+    const double m = t.f.rootSize * 1e-9;   // far above the FP64 rounding of the kernel's tests
+    for (int a = 0; a < 3; ++a) {
+        const double c1 = c.a[a] + c.v[a] * c.tStart, r1 = c.R * c.tStart;
+        const double c2 = c.a[a] + c.v[a], r2 = c.R;
+        lo[a] = fmin(c1 - r1, c2 - r2) - m;
+        hi[a] = fmax(c1 + r1, c2 + r2) + m;
+    }
+    node = 0;
+    d = 0;
+    u32 k[3] = {0, 0, 0};
+    for (;;) {
+        const u32 ch = t.packed ? t.packed[node].x : t.child[node];
+        if ((ch & LEAF_BIT) || ch == INVALID) break;
+        u32 slot = 0;
+        bool straddles = false;
+        for (int a = 0; a < 3; ++a) {
+            const double mid = t.f.B(a, 2u * k[a] + 1u, d + 1);
+            if (lo[a] >= mid) slot |= 1u << a;
+            else if (!(hi[a] < mid)) straddles = true;
+        }
+        if (straddles) break;
+        node = ch + slot;
+        ++d;
+        for (int a = 0; a < 3; ++a) k[a] = 2 * k[a] + ((slot >> a) & 1u);
+    }
+    k0 = k[0];
+    k1 = k[1];
+    k2 = k[2];
+}
+
 // set bits of the wave-wide mask m below this lane (v_mbcnt: no per-lane mask register)
 __device__ __forceinline__ u32 lanesBelow(u64 m) {
     return __builtin_amdgcn_mbcnt_hi((u32)(m >> 32), __builtin_amdgcn_mbcnt_lo((u32)m, 0u));
@@ -268,7 +307,8 @@ __device__ void coneLink(const TraversalTree &t, const LinkInputs &in, u32 link,
 
 // The record pass with persistent threads (after Aila & Laine, "Understanding the
 // efficiency of ray traversal on GPUs", 2009): every lane runs the traversal of
-// traverseCone as an explicit state machine and, when its link is done, takes the next
+// traverseCone as an explicit state machine, from the deepest node enclosing its cone
+// (coneStart) instead of the root, and, when its link is done, takes the next
 // link index from a shared counter (one atomic per wavefront for all idle lanes).  Lanes
 // no longer idle at the end of a wave's longest traversal.  Per link it counts the
 // accepted hits and stages the first in.slots records (stagingIndex).
@@ -278,7 +318,7 @@ __device__ void coneLink(const TraversalTree &t, const LinkInputs &in, u32 link,
 // round-robin over the 8 XCDs) and moves on to the next slice when its own is used
 // up.  Each XCD's L2 then serves cones of a few cameras towards one part of the
 // scene instead of the whole GPU's window of links.
-__global__ __launch_bounds__(256) void k_conePassPT(TraversalTree t, LinkInputs in, u32 linkCount, int sampleness,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_conePassPT(TraversalTree t, LinkInputs in, u32 linkCount, int sampleness,
                                                     u32 *__restrict__ counts, u32 *__restrict__ stLeaf,
                                                     uint2 *__restrict__ stVal, double *__restrict__ confOut,
                                                     u32 *__restrict__ nextIdx, u32 slices,
@@ -291,7 +331,7 @@ __global__ __launch_bounds__(256) void k_conePassPT(TraversalTree t, LinkInputs 
     u32 cur = blockIdx.x % slices, tried = 0;   // wave-uniform
     const u32 required = sampleness ? 0u : FLAG_S;
     bool active = false, more = true;
-    u32 idx = 0, link = 0, count = 0, maxDepth = 0, node = 0, d = 0, k0 = 0, k1 = 0, k2 = 0;
+    u32 idx = 0, link = 0, count = 0, maxDepth = 0, node = 0, d = 0, k0 = 0, k1 = 0, k2 = 0, start = 0;
     bool descend = true;
     ConeD cone;
     ConeCull cull;
@@ -332,9 +372,8 @@ __global__ __launch_bounds__(256) void k_conePassPT(TraversalTree t, LinkInputs 
                         wl = (float)(conf * cone.length);
                         maxDepth = (u32)in.sampleDepth[i] + MAX_DEPTH_DIFFERENCE;
                         count = 0;
-                        node = 0;
-                        d = 0;
-                        k0 = k1 = k2 = 0;
+                        coneStart(t, cone, node, d, k0, k1, k2);
+                        start = node;
                         descend = true;
                         active = true;
                     }
@@ -388,9 +427,9 @@ __global__ __launch_bounds__(256) void k_conePassPT(TraversalTree t, LinkInputs 
                 }
             }
         }
-        if (advance) {   // next sibling, or climb; done when back at the root
+        if (advance) {   // next sibling, or climb; done when back at the start node
             for (;;) {
-                if (node == 0) {
+                if (node == start) {
                     counts[link] = count;
                     hits += count;
                     active = false;
@@ -1431,9 +1470,9 @@ void conePassRecords(tsr_context *ctx, const tsr_tree *tree, TraversalTree &tt,
         const u32 nOver = readScalar(ovCount.p, st);
         totalOver += nOver;
         if (std::getenv("TSR_TRACE"))
-            std::fprintf(stderr, "[tsr occupancy] %s pass, links %u..%u: %llu hits, %u links over %u slots\n",
-                         sampleness ? "sampleness" : "emptiness", l0, l0 + n - 1,
-                         (unsigned long long)total, nOver, ic.slots);
+            std::fprintf(stderr, "[tsr occupancy] %s pass, links %u..%u: %llu node visits, %llu hits, %u links over "
+                         "%u slots\n", sampleness ? "sampleness" : "emptiness", l0, l0 + n - 1,
+                         (unsigned long long)cs[0], (unsigned long long)total, nOver, ic.slots);
         if (nOver) {
             LinkInputs ino = ic;
             ino.order = ovList.p;
