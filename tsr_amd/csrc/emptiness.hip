@@ -88,8 +88,8 @@ struct CamGrid {
 // :250-277): apex at the camera, end at the sample, end normal -n, R = bandwidth * scale,
 // uncut.  Writes the link's leaf-pass data and its confidence (0: not an emptiness cone).
 __global__ void k_gLinkSetup(LinkSrc s, u32 links, LinkGeo g, uint8_t *__restrict__ cls,
-                             double *__restrict__ confs, unsigned long long *__restrict__ camTheta,
-                             u32 *__restrict__ camKey, u32 *__restrict__ reachKey, u32 *__restrict__ iota) {
+                             double *__restrict__ confs, u32 *__restrict__ camKey, u32 *__restrict__ reachKey,
+                             u32 *__restrict__ iota) {
     const u32 l = blockIdx.x * BLOCK + threadIdx.x;
     if (l >= links) return;
     cls[l] = LINK_NONE;
@@ -110,12 +110,14 @@ __global__ void k_gLinkSetup(LinkSrc s, u32 links, LinkGeo g, uint8_t *__restric
     confs[l] = conf;
     const double A = c.length;
     const bool wide = !(R < 0.5 * A);
-    // inflated cone angle: relative 1e-4 plus 1e-5 rad cover the FP32 rounding of the unit
-    // axis and of the face coordinate ranges
-    const double theta = wide ? 0.0 : asin(R / A);
-    const double thetaP = theta * (1.0 + 1e-4) + 1e-5;
-    const double tanT = wide ? (double)INFINITY : tan(theta) * (1.0 + 1e-4);
-    g.axis[l] = make_float4((float)(c.v[0] / A), (float)(c.v[1] / A), (float)(c.v[2] / A), (float)tanT);
+    // tan of the cone angle asin(R / A), which also bounds the angle from above; inflated:
+    // relative 1e-4 plus 1e-5 rad cover the FP32 rounding of the unit axis and of the face
+    // coordinate ranges
+    const double tanA = wide ? 0.0 : R / sqrt(A * A - R * R);
+    const double thetaP = tanA * (1.0 + 1e-4) + 1e-5;
+    const double tanT = wide ? (double)INFINITY : tanA * (1.0 + 1e-4);
+    const double invA = 1.0 / A;
+    g.axis[l] = make_float4((float)(c.v[0] * invA), (float)(c.v[1] * invA), (float)(c.v[2] * invA), (float)tanT);
     g.aux[l] = make_float4((float)((A + R) * (1.0 + 1e-6)), (float)thetaP,
                            (float)(s.sampleDepth[i] + MAX_DEPTH_DIFFERENCE), 0.f);
     g.pos[l] = make_float4(s.px[i], s.py[i], s.pz[i], s.scale[i]);
@@ -124,9 +126,33 @@ __global__ void k_gLinkSetup(LinkSrc s, u32 links, LinkGeo g, uint8_t *__restric
     cls[l] = wide ? LINK_WIDE : LINK_GRID;
     camKey[l] = cam;
     reachKey[l] = ~__float_as_uint(g.aux[l].x);   // descending reach (positive floats order as bits)
-    if (!wide) {   // integer sums: the camera's mean angle does not depend on the order
-        atomicAdd(&camTheta[2 * cam], (unsigned long long)(thetaP * 4294967296.0));
-        atomicAdd(&camTheta[2 * cam + 1], 1ull);
+}
+
+// per camera, the sum of the grid links' inflated angles (in units of 2^-32 rad, integer:
+// the sum does not depend on the order) and their count; links taken in camera order, one
+// atomic per wavefront and camera
+__global__ void k_camTheta(const u32 *__restrict__ byCam, u32 links, const uint8_t *__restrict__ cls,
+                           const u32 *__restrict__ cams, const float4 *__restrict__ aux,
+                           unsigned long long *__restrict__ camTheta) {
+    const u32 t = blockIdx.x * BLOCK + threadIdx.x;
+    const u32 l = t < links ? byCam[t] : 0u;
+    bool has = t < links && cls[l] == LINK_GRID;
+    const u32 cam = has ? cams[l] : 0u;
+    const unsigned long long th = has ? (unsigned long long)((double)aux[l].y * 4294967296.0) : 0ull;
+    for (u64 pending = __ballot(has); pending; pending = __ballot(has)) {
+        const int leader = __ffsll((long long)pending) - 1;
+        const u32 lc = __shfl(cam, leader);
+        const bool mine = has && cam == lc;
+        unsigned long long sum = mine ? th : 0ull, cnt = mine ? 1ull : 0ull;
+        for (int off = 32; off > 0; off >>= 1) {
+            sum += __shfl_xor(sum, off);
+            cnt += __shfl_xor(cnt, off);
+        }
+        if ((int)(threadIdx.x & 63) == leader) {
+            atomicAdd(&camTheta[2 * lc], sum);
+            atomicAdd(&camTheta[2 * lc + 1], cnt);
+        }
+        if (mine) has = false;
     }
 }
 
@@ -158,6 +184,41 @@ __device__ bool faceRange(const float h[3], float tp, u32 F, float inv, int G, i
         r[2 * c + 1] = cellOf(up, inv, G);
     }
     return true;
+}
+
+// On the face of the cone axis h itself (the largest component of h, at angle phi0 from the
+// face axis, cos phi0 = |h_i|), the face coordinates of a direction within tp of h lie within
+// rho = tp / cos^2(phi0 + tp) of those of h: the gnomonic map stretches lengths by at most
+// 1 / cos^2 of the angle from the face axis along a path inside the cap.  Cells of the box
+// farther than rho from (u0, v0) are then skipped.  Valid while phi0 + tp stays below
+// 0.95 rad (then the box alone is used).
+struct CapDisc {
+    u32 face;
+    float u0, v0, rho;
+    bool valid;
+};
+__device__ CapDisc capDisc(const float h[3], float tp) {
+    CapDisc c;
+    const float b0 = fabsf(h[0]), b1 = fabsf(h[1]), b2 = fabsf(h[2]);
+    float m, u, v;
+    if (b0 >= b1 && b0 >= b2) { c.face = h[0] >= 0.f ? 0 : 1; m = b0; u = h[1]; v = h[2]; }
+    else if (b1 >= b2) { c.face = h[1] >= 0.f ? 2 : 3; m = b1; u = h[0]; v = h[2]; }
+    else { c.face = h[2] >= 0.f ? 4 : 5; m = b2; u = h[0]; v = h[1]; }
+    const float phi = acosf(fminf(m, 1.f)) + tp;
+    c.valid = phi < 0.95f;
+    const float cp = cosf(phi);
+    c.u0 = u / m;
+    c.v0 = v / m;
+    c.rho = tp / (cp * cp) * (1.f + 1e-4f) + 1e-5f;
+    return c;
+}
+// whether cell (cu, cv) of a grid with `inv` cells per unit may meet the disc
+__device__ __forceinline__ bool cellMeetsDisc(const CapDisc &c, int cu, int cv, float inv) {
+    const float cs = 1.f / inv;
+    const float lo0 = (float)cu * cs - 1.f, lo1 = (float)cv * cs - 1.f;
+    const float du = fmaxf(fmaxf(lo0 - c.u0, c.u0 - (lo0 + cs)), 0.f);
+    const float dv = fmaxf(fmaxf(lo1 - c.v0, c.v0 - (lo1 + cs)), 0.f);
+    return du * du + dv * dv <= c.rho * c.rho * (1.f + 1e-5f) + 1e-12f;
 }
 
 // wave minimum / maximum of v (every lane gets the result)
@@ -203,9 +264,18 @@ __global__ void k_gLinkCells(u32 links, int mode, const u32 *__restrict__ order,
         if (valid && cls[l] == LINK_WIDE) valid = false;
         int r[4];
         if (valid) {
+            const CapDisc disc = capDisc(h, tp);
             u64 cells = 0;
-            for (u32 F = 0; F < FACES; ++F)
-                if (faceRange(h, tp, F, inv, G, r)) cells += (u64)(r[1] - r[0] + 1) * (u64)(r[3] - r[2] + 1);
+            for (u32 F = 0; F < FACES; ++F) {
+                if (!faceRange(h, tp, F, inv, G, r)) continue;
+                const u64 box = (u64)(r[1] - r[0] + 1) * (u64)(r[3] - r[2] + 1);
+                if (F == disc.face && disc.valid && box <= WIDE_CELLS) {
+                    for (int v = r[2]; v <= r[3]; ++v)
+                        for (int u = r[0]; u <= r[1]; ++u) cells += cellMeetsDisc(disc, u, v, inv) ? 1u : 0u;
+                } else {
+                    cells += box;
+                }
+            }
             if (cells > WIDE_CELLS) {
                 cls[l] = LINK_WIDE;
                 valid = false;
@@ -236,13 +306,16 @@ __global__ void k_gLinkCells(u32 links, int mode, const u32 *__restrict__ order,
     }
     if (!valid || cls[l] != LINK_GRID) return;
     const CamGrid &cg = grids[cam];
+    const CapDisc disc = capDisc(h, tp);
     u32 o = offset[t];
     int r[4];
     for (u32 F = 0; F < FACES; ++F) {
         if (!faceRange(h, tp, F, inv, G, r)) continue;
         const FaceGrid &fg = cg.face[F];
+        const bool useDisc = F == disc.face && disc.valid;   // the box has <= WIDE_CELLS cells here
         for (int v = r[2]; v <= r[3]; ++v)
             for (int u = r[0]; u <= r[1]; ++u) {
+                if (useDisc && !cellMeetsDisc(disc, u, v, inv)) continue;
                 keys[o] = fg.base + (u32)(v - fg.lo[1]) * fg.dim[0] + (u32)(u - fg.lo[0]);
                 vals[o] = l;
                 ++o;
@@ -276,9 +349,12 @@ __global__ void k_fillBox(int *box, size_t n) {
     if (i < n) box[i] = (i & 2) ? INT_MIN : INT_MAX;
 }
 
+// sum of n values into *out: a fixed grid of SUM_BLOCKS blocks strides over the array, one
+// atomic per wavefront
+constexpr u32 SUM_BLOCKS = 1024;
 __global__ void k_sumU32(const u32 *__restrict__ v, u32 n, unsigned long long *__restrict__ out) {
-    const u32 i = blockIdx.x * BLOCK + threadIdx.x;
-    unsigned long long x = i < n ? v[i] : 0u;
+    unsigned long long x = 0;
+    for (u32 i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += SUM_BLOCKS * BLOCK) x += v[i];
     for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
     if ((threadIdx.x & 63) == 0 && x) atomicAdd(out, x);
 }
@@ -672,12 +748,14 @@ void emptinessByLeaves(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *
     DBuf<u32> byCam(links), byReach(links);
     {
         DBuf<u32> camKey(links), reachKey(links), iota(links), keyS(links);
-        k_gLinkSetup<<<gridFor(links, BLOCK), BLOCK, 0, st>>>(src, links, geo, cls.p, confs, camTheta.p,
-                                                              camKey.p, reachKey.p, iota.p);
+        k_gLinkSetup<<<gridFor(links, BLOCK), BLOCK, 0, st>>>(src, links, geo, cls.p, confs, camKey.p, reachKey.p,
+                                                              iota.p);
         int bits = 1;
         while (bits < 32 && (1ull << bits) <= K) ++bits;
         sortPairs(sc, camKey.p, keyS.p, iota.p, byCam.p, links, 0, bits, st);
         sortPairs(sc, reachKey.p, keyS.p, iota.p, byReach.p, links, 0, 32, st);
+        k_camTheta<<<gridFor(links, BLOCK), BLOCK, 0, st>>>(byCam.p, links, cls.p, tree->cams.p, aux.p,
+                                                            camTheta.p);
     }
     std::vector<unsigned long long> hTheta(2 * (size_t)K);
     std::vector<float> hcx(K), hcy(K), hcz(K);
@@ -714,7 +792,7 @@ void emptinessByLeaves(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *
                                                               nullptr, camGd.p, cls.p, count.p, box.p, nullptr,
                                                               nullptr, nullptr);
         HIPCHECK(hipMemsetAsync(sum1.p, 0, 8, st));
-        k_sumU32<<<gridFor(links, BLOCK), BLOCK, 0, st>>>(count.p, links, sum1.p);
+        k_sumU32<<<SUM_BLOCKS, BLOCK, 0, st>>>(count.p, links, sum1.p);
         if (K) HIPCHECK(hipMemcpyAsync(hbox.data(), box.p, 4 * hbox.size(), hipMemcpyDeviceToHost, st));
         totalEntries = readScalar(sum1.p, st);
         totalCells = 0;
@@ -851,7 +929,7 @@ void emptinessByLeaves(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *
     unsigned long long ks[3] = {0, 0, 0};
     HIPCHECK(hipMemcpyAsync(ks, kst.p, 24, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipMemsetAsync(sum1.p, 0, 8, st));
-    k_sumU32<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(hitCount, L, sum1.p);
+    k_sumU32<<<SUM_BLOCKS, BLOCK, 0, st>>>(hitCount, L, sum1.p);
     hitsOut = readScalar(sum1.p, st);
     {
         // algorithmic bytes: per leaf its list entry, code and the three outputs (24 B); per
