@@ -51,6 +51,9 @@ constexpr double REAL_MAX_D = 3.4028234663852886e38;
 
 // link classes
 constexpr uint8_t LINK_NONE = 0, LINK_GRID = 1, LINK_WIDE = 2;
+constexpr u32 ENTRY_OWN_FACE = 0x80000000u;   // entry value bit: the cell is on the cone axis's face
+constexpr float NO_DISC = 1e30f;               // disc radius of entries without a face-coordinate test
+constexpr u32 EVAL_WORDS = 4;                  // float4s per link of LinkGeo::eval
 
 struct LinkSrc {
     const float *px, *py, *pz, *nx, *ny, *nz, *scale, *conf;
@@ -65,9 +68,10 @@ struct LinkSrc {
 struct LinkGeo {
     float4 *axis;    // unit cone axis (e - a) / |e - a|, tan of the inflated cone angle (inf: wide)
     float4 *aux;     // reach (|e - a| + R, inflated), inflated cone angle (radians), depth limit, 0
-    float4 *pos;     // sample position, scale
-    float4 *nrm;     // sample normal, confidence
-    double2 *cone;   // 1 / dot(e - a, n), |e - a| (ConeD::invDenom, ConeD::length)
+    // the FP64 kernel's data, one 64-byte line per link (EVAL_WORDS float4s): sample position
+    // and scale; sample normal and confidence; 1 / dot(e - a, n) and |e - a| (ConeD::invDenom,
+    // ConeD::length) as two doubles; padding
+    float4 *eval;
 };
 
 struct FaceGrid {
@@ -120,9 +124,10 @@ __global__ void k_gLinkSetup(LinkSrc s, u32 links, LinkGeo g, uint8_t *__restric
     g.axis[l] = make_float4((float)(c.v[0] * invA), (float)(c.v[1] * invA), (float)(c.v[2] * invA), (float)tanT);
     g.aux[l] = make_float4((float)((A + R) * (1.0 + 1e-6)), (float)thetaP,
                            (float)(s.sampleDepth[i] + MAX_DEPTH_DIFFERENCE), 0.f);
-    g.pos[l] = make_float4(s.px[i], s.py[i], s.pz[i], s.scale[i]);
-    g.nrm[l] = make_float4(s.nx[i], s.ny[i], s.nz[i], s.conf[i]);
-    g.cone[l] = make_double2(c.invDenom, A);
+    float4 *ev = g.eval + EVAL_WORDS * (size_t)l;
+    ev[0] = make_float4(s.px[i], s.py[i], s.pz[i], s.scale[i]);
+    ev[1] = make_float4(s.nx[i], s.ny[i], s.nz[i], s.conf[i]);
+    reinterpret_cast<double2 *>(ev)[2] = make_double2(c.invDenom, A);
     cls[l] = wide ? LINK_WIDE : LINK_GRID;
     camKey[l] = cam;
     reachKey[l] = ~__float_as_uint(g.aux[l].x);   // descending reach (positive floats order as bits)
@@ -317,7 +322,7 @@ __global__ void k_gLinkCells(u32 links, int mode, const u32 *__restrict__ order,
             for (int u = r[0]; u <= r[1]; ++u) {
                 if (useDisc && !cellMeetsDisc(disc, u, v, inv)) continue;
                 keys[o] = fg.base + (u32)(v - fg.lo[1]) * fg.dim[0] + (u32)(u - fg.lo[0]);
-                vals[o] = l;
+                vals[o] = l | (useDisc ? ENTRY_OWN_FACE : 0u);   // links < 2^31
                 ++o;
             }
     }
@@ -329,18 +334,29 @@ __global__ void k_gatherU32(const u32 *__restrict__ src, const u32 *__restrict__
     if (t < n) dst[t] = src[order[t]];
 }
 
-// FP32 test data of each entry's link next to it (the leaf pass reads a cell's entries
-// contiguously), as two arrays: unit axis and tan of the inflated angle; reach, depth limit,
-// link
-__global__ void k_entryRecords(const u32 *__restrict__ link, size_t n, const float4 *__restrict__ axis,
-                               const float4 *__restrict__ aux, float4 *__restrict__ recAxis,
-                               float4 *__restrict__ recAux, float *__restrict__ reach) {
+// The leaf pass's test record of each entry (a cell's entries are read contiguously): on the
+// face of the link's own cone axis, the axis's face coordinates (u0, v0) and the disc radius
+// rho (capDisc); the leaf's face coordinates must lie within rho.  Elsewhere (a cone crossing
+// to another face, or a wide link) rho is 1e30: no test.  The low 5 bits of rho hold the
+// link's depth limit, after rho is rounded up to a multiple of 32 ulps.  Then the link.
+__global__ void k_entryRecords(const u32 *__restrict__ val, size_t n, const float4 *__restrict__ axis,
+                               const float4 *__restrict__ aux, float4 *__restrict__ rec, float *__restrict__ reach) {
     const size_t e = (size_t)blockIdx.x * BLOCK + threadIdx.x;
     if (e >= n) return;
-    const u32 l = link[e];
+    const u32 v = val[e];
+    const u32 l = v & ~ENTRY_OWN_FACE;
     const float4 au = aux[l];
-    recAxis[e] = axis[l];
-    recAux[e] = make_float4(au.x, au.z, __uint_as_float(l), 0.f);
+    float u0 = 0.f, v0 = 0.f, rho = NO_DISC;
+    if (v & ENTRY_OWN_FACE) {
+        const float4 ax = axis[l];
+        const float h[3] = {ax.x, ax.y, ax.z};
+        const CapDisc d = capDisc(h, au.y);
+        u0 = d.u0;
+        v0 = d.v0;
+        rho = d.rho;
+    }
+    const u32 rb = ((__float_as_uint(rho) + 32u) & ~31u) | min((u32)au.z, 31u);
+    rec[e] = make_float4(u0, v0, __uint_as_float(rb), __uint_as_float(l));
     if (reach) reach[e] = au.x;
 }
 
@@ -417,8 +433,8 @@ struct GatherArgs {
     const u32 *cellStart, *cellEnd;
     const float *cellReach;    // largest reach of the cell's links
     const float *entryReach;   // per entry, descending within a cell
-    const float4 *recAxis, *recAux;     // per entry (k_entryRecords), sorted by cell
-    const float4 *wideAxis, *wideAux;   // per wide link, sorted by camera (link order within a camera)
+    const float4 *rec;       // per entry (k_entryRecords), sorted by cell
+    const float4 *wideRec;   // per wide link, sorted by camera (link order within a camera)
     LinkGeo g;
     double bandwidth;
 };
@@ -442,9 +458,10 @@ __device__ __forceinline__ u32 lanesBelow(u64 m) {
 __device__ __forceinline__ void evalQueued(const GatherArgs &a, const CamGrid *camL, const double x[3], u32 link,
                                            u32 cam, float &sk, float &ck, float &sl, float &cl, u32 &hits) {
     const CamGrid &cg = cam < CAM_LDS ? camL[cam] : a.grids[cam];
-    const float4 ps = a.g.pos[link];
-    const float4 nr = a.g.nrm[link];
-    const double2 cd = a.g.cone[link];
+    const float4 *ev = a.g.eval + EVAL_WORDS * (size_t)link;
+    const float4 ps = ev[0];
+    const float4 nr = ev[1];
+    const double2 cd = reinterpret_cast<const double2 *>(ev)[2];
     ConeD c;
     c.a[0] = cg.a[0]; c.a[1] = cg.a[1]; c.a[2] = cg.a[2];
     c.v[0] = (double)ps.x - c.a[0]; c.v[1] = (double)ps.y - c.a[1]; c.v[2] = (double)ps.z - c.a[2];
@@ -491,12 +508,12 @@ __global__ __launch_bounds__(64 * GATHER_WAVES) void k_gLeaves(GatherArgs a, con
                                                                unsigned long long *__restrict__ stats) {
     __shared__ u32 segStartS[GATHER_WAVES][64], segBegS[GATHER_WAVES][64], segLenS[GATHER_WAVES][64],
         segWideS[GATHER_WAVES][64], segCamS[GATHER_WAVES][64];
-    __shared__ float segAS[GATHER_WAVES][3][64];   // camera centre of each segment
+    __shared__ float segUS[GATHER_WAVES][64], segVS[GATHER_WAVES][64];   // the leaf's face coordinates per segment
     __shared__ u32 qLinkS[GATHER_WAVES][64 * (CAND_UNROLL + 1)], qCamS[GATHER_WAVES][64 * (CAND_UNROLL + 1)];
     const u32 lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     u32 *segStart = segStartS[wv], *segBeg = segBegS[wv], *segLen = segLenS[wv], *segWide = segWideS[wv],
         *segCam = segCamS[wv];
-    float (*segA)[64] = segAS[wv];
+    float *segU = segUS[wv], *segV = segVS[wv];
     u32 *qLink = qLinkS[wv], *qCam = qCamS[wv];   // candidates for the FP64 kernel
     u32 qn = 0;   // queued candidates (wave-uniform)
     // the first CAM_LDS cameras' grids, staged once per block (a lane reads its own camera's
@@ -532,6 +549,7 @@ __global__ __launch_bounds__(64 * GATHER_WAVES) void k_gLeaves(GatherArgs a, con
         for (u32 cb = 0; cb < K; cb += 64) {
             const u32 cam = cb + lane;
             u32 beg = 0, len = 0, wb = 0, wl = 0;
+            float uf = 0.f, vf = 0.f;
             if (cam < K) {
                 const CamGrid &cg = cam < CAM_LDS ? camL[cam] : a.grids[cam];
                 const double w0 = x[0] - (double)cg.a[0], w1 = x[1] - (double)cg.a[1], w2 = x[2] - (double)cg.a[2];
@@ -547,8 +565,10 @@ __global__ __launch_bounds__(64 * GATHER_WAVES) void k_gLeaves(GatherArgs a, con
                     // ~1e-7, far inside the 1e-5 padding of the links' cell ranges)
                     const int G = (int)cg.G;
                     const float mf = (float)m;
-                    const int cu = cellOf((float)u / mf, cg.inv, G) - fg.lo[0];
-                    const int cv = cellOf((float)v / mf, cg.inv, G) - fg.lo[1];
+                    uf = (float)u / mf;
+                    vf = (float)v / mf;
+                    const int cu = cellOf(uf, cg.inv, G) - fg.lo[0];
+                    const int cv = cellOf(vf, cg.inv, G) - fg.lo[1];
                     if (cu >= 0 && cv >= 0 && (u32)cu < fg.dim[0] && (u32)cv < fg.dim[1]) {
                         const u32 cell = fg.base + (u32)cv * fg.dim[0] + (u32)cu;
                         // a contributing link has |x - a| < |e - a| + R, and its reach is that
@@ -582,22 +602,20 @@ __global__ __launch_bounds__(64 * GATHER_WAVES) void k_gLeaves(GatherArgs a, con
             const u32 nseg = (u32)__popcll(nonEmpty);
             if (n > 0) {
                 const u32 at = lanesBelow(nonEmpty);
-                const CamGrid &cg = cam < CAM_LDS ? camL[cam] : a.grids[cam];
                 segStart[at] = incl - n;
                 segBeg[at] = beg;
                 segLen[at] = len;
                 segWide[at] = wb;
                 segCam[at] = cam;
-                segA[0][at] = cg.a[0];
-                segA[1][at] = cg.a[1];
-                segA[2][at] = cg.a[2];
+                segU[at] = uf;
+                segV[at] = vf;
             }
             waveSync();
             // candidates passing the FP32 test are queued (link, camera) in LDS; the FP64 kernel
             // runs on 64 queued candidates at a time, one per lane.  A lane takes CAND_UNROLL
             // candidates per step (j, j + 64, ...) with their record loads issued together.
             for (u32 base = 0; base < total; base += 64 * CAND_UNROLL) {
-                float4 ax[CAND_UNROLL], au[CAND_UNROLL];
+                float4 rc[CAND_UNROLL];
                 u32 qs[CAND_UNROLL];
 #pragma unroll
                 for (u32 uu = 0; uu < CAND_UNROLL; ++uu) {
@@ -613,13 +631,7 @@ __global__ __launch_bounds__(64 * GATHER_WAVES) void k_gLeaves(GatherArgs a, con
                         const u32 q = lo;
                         const u32 r = j - segStart[q];
                         const u32 sl0 = segLen[q];
-                        if (r < sl0) {
-                            ax[uu] = a.recAxis[segBeg[q] + r];
-                            au[uu] = a.recAux[segBeg[q] + r];   // reach, depth limit, link
-                        } else {
-                            ax[uu] = a.wideAxis[segWide[q] + (r - sl0)];
-                            au[uu] = a.wideAux[segWide[q] + (r - sl0)];
-                        }
+                        rc[uu] = r < sl0 ? a.rec[segBeg[q] + r] : a.wideRec[segWide[q] + (r - sl0)];
                         qs[uu] = q;
                     }
                 }
@@ -628,24 +640,18 @@ __global__ __launch_bounds__(64 * GATHER_WAVES) void k_gLeaves(GatherArgs a, con
                     bool pass = false;
                     if (qs[uu] < 64) {
                         ++cand;
-                        const u32 qq = qs[uu];
-                        const float ca[3] = {segA[0][qq], segA[1][qq], segA[2][qq]};
-                        // FP32 cone test: along-axis distance s <= reach, distance from the axis
-                        // <= s tan(angle); the margin covers the rounding of x, a and the axis
-                        const float d0 = xf[0] - ca[0], d1 = xf[1] - ca[1], d2 = xf[2] - ca[2];
-                        const float s = d0 * ax[uu].x + d1 * ax[uu].y + d2 * ax[uu].z;
-                        const float marg = 2e-6f * (xAbs + fabsf(ca[0]) + fabsf(ca[1]) + fabsf(ca[2]) + fabsf(s));
-                        const float q0 = d0 - s * ax[uu].x, q1 = d1 - s * ax[uu].y, q2 = d2 - s * ax[uu].z;
-                        const float lim = fmaxf(s, 0.f) * ax[uu].w + marg;
-                        // not deeper than the link's depth limit
-                        pass = depthF <= au[uu].y && s >= -marg && s <= au[uu].x + marg &&
-                               !(q0 * q0 + q1 * q1 + q2 * q2 > lim * lim);
+                        // not deeper than the link's depth limit, and the leaf's face coordinates
+                        // within the cone's disc (every entry of the prefix reaches the leaf's
+                        // distance from the camera)
+                        const u32 rb = __float_as_uint(rc[uu].z);
+                        const float du = segU[qs[uu]] - rc[uu].x, dv = segV[qs[uu]] - rc[uu].y;
+                        pass = depthF <= (float)(rb & 31u) && du * du + dv * dv <= rc[uu].z * rc[uu].z;
                     }
                     const u64 m = __ballot(pass);
                     if (pass) {
                         ++evals;
                         const u32 at = qn + lanesBelow(m);
-                        qLink[at] = __float_as_uint(au[uu].z);
+                        qLink[at] = __float_as_uint(rc[uu].w);
                         qCam[at] = segCam[qs[uu]];
                     }
                     qn += (u32)__popcll(m);
@@ -736,15 +742,14 @@ void emptinessByLeaves(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *
     if (links == 0 || L == 0) return;
 
     // ---- per-link cones; links grouped by camera and sorted by descending reach
-    DBuf<float4> axis(links), aux(links), pos(links), nrm(links);
-    DBuf<double2> cone(links);
+    DBuf<float4> axis(links), aux(links), eval(EVAL_WORDS * (size_t)links);
     DBuf<uint8_t> cls(links);
     DBuf<unsigned long long> camTheta(2 * (size_t)std::max(K, 1u));
     HIPCHECK(hipMemsetAsync(camTheta.p, 0, camTheta.bytes(), st));
     const LinkSrc src{tree->px.p, tree->py.p, tree->pz.p, tree->nx.p, tree->ny.p, tree->nz.p, tree->scale.p,
                       tree->conf.p, tree->cams.p, scene->cx.p, scene->cy.p, scene->cz.p, tree->sampleDepth.p,
                       tree->maxCams, K, prm.bandwidth_factor};
-    const LinkGeo geo{axis.p, aux.p, pos.p, nrm.p, cone.p};
+    const LinkGeo geo{axis.p, aux.p, eval.p};
     DBuf<u32> byCam(links), byReach(links);
     {
         DBuf<u32> camKey(links), reachKey(links), iota(links), keyS(links);
@@ -830,7 +835,7 @@ void emptinessByLeaves(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *
     lap("cell grids");
 
     // ---- wide links per camera (link order within a camera) with their test records
-    DBuf<float4> wideAxis, wideAux;
+    DBuf<float4> wideRec;
     u32 nWide = 0;
     {
         DBuf<u32> flag(links), iota(links), cnt(1), wl(links);
@@ -846,10 +851,9 @@ void emptinessByLeaves(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *
             int bits = 1;
             while (bits < 32 && (1ull << bits) < K) ++bits;
             sortPairs(sc, key.p, keyS.p, wl.p, wideSorted.p, nWide, 0, bits, st);
-            wideAxis.alloc(nWide);
-            wideAux.alloc(nWide);
-            k_entryRecords<<<gridFor(nWide, BLOCK), BLOCK, 0, st>>>(wideSorted.p, nWide, axis.p, aux.p, wideAxis.p,
-                                                                    wideAux.p, nullptr);
+            wideRec.alloc(nWide);
+            k_entryRecords<<<gridFor(nWide, BLOCK), BLOCK, 0, st>>>(wideSorted.p, nWide, axis.p, aux.p, wideRec.p,
+                                                                    nullptr);
             HIPCHECK(hipMemsetAsync(wb.p, 0, 4 * (size_t)K, st));
             HIPCHECK(hipMemsetAsync(we.p, 0, 4 * (size_t)K, st));
             k_wideRuns<<<gridFor(nWide, BLOCK), BLOCK, 0, st>>>(keyS.p, nWide, wb.p, we.p);
@@ -869,7 +873,7 @@ void emptinessByLeaves(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *
     // ---- (cell, link) entries sorted by cell, descending reach within a cell, with the
     // links' test records alongside
     const size_t nE = (size_t)totalEntries;
-    DBuf<float4> recAxis(std::max<size_t>(nE, 1)), recAux(std::max<size_t>(nE, 1));
+    DBuf<float4> rec(std::max<size_t>(nE, 1));
     DBuf<float> entryReach(std::max<size_t>(nE, 1));
     DBuf<u32> keysS(std::max<size_t>(nE, 1));
     {
@@ -885,8 +889,8 @@ void emptinessByLeaves(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *
         int bits = 1;
         while (bits < 32 && (1ull << bits) < totalCells) ++bits;
         sortPairs(sc, keys.p, keysS.p, vals.p, entries.p, nE, 0, bits, st);
-        if (nE) k_entryRecords<<<gridFor(nE, BLOCK), BLOCK, 0, st>>>(entries.p, nE, axis.p, aux.p, recAxis.p,
-                                                                     recAux.p, entryReach.p);
+        if (nE) k_entryRecords<<<gridFor(nE, BLOCK), BLOCK, 0, st>>>(entries.p, nE, axis.p, aux.p, rec.p,
+                                                                     entryReach.p);
     }
     const size_t nC = std::max<u64>(totalCells, 1);
     DBuf<u32> cellStart(nC), cellEnd(nC);
@@ -913,8 +917,7 @@ void emptinessByLeaves(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *
 
     // ---- the leaf pass
     GatherArgs ga{tree->frame, tree->leafLoc.p, gridsD.p, K, cellStart.p, cellEnd.p, cellReach.p,
-                  entryReach.p, recAxis.p, recAux.p, nWide ? wideAxis.p : recAxis.p, nWide ? wideAux.p : recAux.p,
-                  geo, (double)prm.bandwidth_factor};
+                  entryReach.p, rec.p, nWide ? wideRec.p : rec.p, geo, (double)prm.bandwidth_factor};
     DBuf<unsigned long long> kst(4);
     HIPCHECK(hipMemsetAsync(kst.p, 0, 32, st));
     const u32 groups = (nLeaves + LEAVES_PER_WAVE - 1) / LEAVES_PER_WAVE;

@@ -1572,7 +1572,8 @@ void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *s
             k_packChildState<<<gridFor(N, BLOCK), BLOCK, 0, st>>>(tree->child.p, field->states.p, N, packed);
             tt.packed = packed;
         }
-        if (!sampleness && !atomicMode && !(flags & TSR_EMPTINESS_TRAVERSAL)) {
+        // the gathered E pass marks entries in bit 31 of the link index: links < 2^31
+        if (!sampleness && !atomicMode && !(flags & TSR_EMPTINESS_TRAVERSAL) && links < 0x80000000u) {
             emptinessByLeaves(ctx, tree, scene, prm, field->hitsS.p, ksum, clen, hitCount.p, confs.p,
                               field->hits[type]);
             field->overflowLinks[type] = 0;
