@@ -270,12 +270,8 @@ void depthMesh(tsr_context *ctx, const tsr_depth_view &view, tsr_depth_mesh *m) 
     k_firstUse<<<gridFor(B, BLOCK), BLOCK, 0, st>>>(tri.p, W, H, firstKey.p);
     // used pixels ordered by their first use = vertex ids
     k_usedPixels<<<gridFor(P, BLOCK), BLOCK, 0, st>>>(firstKey.p, P, flag.p, pixel.p);
-    size_t tmp = 0;
-    HIPCHECK(hipcub::DeviceSelect::Flagged(nullptr, tmp, firstKey.p, flag.p, keysSel.p, counter.p, (int)P, st));
-    HIPCHECK(hipcub::DeviceSelect::Flagged(sc.get(tmp), tmp, firstKey.p, flag.p, keysSel.p, counter.p, (int)P, st));
-    tmp = 0;
-    HIPCHECK(hipcub::DeviceSelect::Flagged(nullptr, tmp, pixel.p, flag.p, pixSel.p, counter.p, (int)P, st));
-    HIPCHECK(hipcub::DeviceSelect::Flagged(sc.get(tmp), tmp, pixel.p, flag.p, pixSel.p, counter.p, (int)P, st));
+    selectFlagged(sc, firstKey.p, flag.p, keysSel.p, counter.p, P, st);
+    selectFlagged(sc, pixel.p, flag.p, pixSel.p, counter.p, P, st);
     const u32 V = readScalar(counter.p, st);
     DBuf<u32> keysSorted(std::max(V, 1u)), pixSorted(std::max(V, 1u));
     if (V) sortPairs(sc, keysSel.p, keysSorted.p, pixSel.p, pixSorted.p, V, 0, 32, st);

@@ -1,11 +1,18 @@
 // Device-side helpers: HIP error checking, a 64-bit open-addressing hash map
-// keyed by octree location codes, device buffers, and rocPRIM-backed radix
-// sort / scan wrappers used by the tree build, occupancy and DMC stages.
+// keyed by octree location codes, device buffers, and radix sort / scan / select /
+// reduce wrappers over rocPRIM used by the tree build, occupancy and DMC stages.
 #pragma once
 
 #include <hip/hip_runtime.h>
 #include <algorithm>
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_reduce.hpp>
+#include <rocprim/device/device_run_length_encode.hpp>
+#include <rocprim/device/device_scan.hpp>
+#include <rocprim/device/device_select.hpp>
+
+#include <cstring>
+#include <limits>
 
 #include <atomic>
 #include <map>
@@ -274,8 +281,8 @@ struct DHash {
 };
 
 // ---------------------------------------------------------------------------
-// Sort / scan wrappers (hipCUB on rocPRIM).  Temporary storage is kept in a
-// growing scratch buffer owned by the caller.
+// Sort / scan / select / reduce wrappers over rocPRIM's device primitives.  Temporary
+// storage is kept in a growing scratch buffer owned by the caller.
 // ---------------------------------------------------------------------------
 struct Scratch {
     DBuf<unsigned char> buf;
@@ -285,49 +292,87 @@ struct Scratch {
     }
 };
 
+// stable LSD radix sort of (key, value) pairs on bits [beginBit, endBit)
 template <typename K, typename V>
 void sortPairs(Scratch &sc, const K *kin, K *kout, const V *vin, V *vout, size_t n, int beginBit, int endBit,
                hipStream_t st) {
     if (n == 0) return;
     size_t tmp = 0;
-    HIPCHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, kin, kout, vin, vout, (int)n, beginBit, endBit, st));
-    void *d = sc.get(tmp);
-    HIPCHECK(hipcub::DeviceRadixSort::SortPairs(d, tmp, kin, kout, vin, vout, (int)n, beginBit, endBit, st));
+    HIPCHECK(rocprim::radix_sort_pairs(nullptr, tmp, kin, kout, vin, vout, n, (unsigned)beginBit, (unsigned)endBit,
+                                       st));
+    HIPCHECK(rocprim::radix_sort_pairs(sc.get(tmp), tmp, kin, kout, vin, vout, n, (unsigned)beginBit,
+                                       (unsigned)endBit, st));
 }
 
 template <typename K>
 void sortKeys(Scratch &sc, const K *kin, K *kout, size_t n, int beginBit, int endBit, hipStream_t st) {
     if (n == 0) return;
     size_t tmp = 0;
-    HIPCHECK(hipcub::DeviceRadixSort::SortKeys(nullptr, tmp, kin, kout, (int)n, beginBit, endBit, st));
-    void *d = sc.get(tmp);
-    HIPCHECK(hipcub::DeviceRadixSort::SortKeys(d, tmp, kin, kout, (int)n, beginBit, endBit, st));
+    HIPCHECK(rocprim::radix_sort_keys(nullptr, tmp, kin, kout, n, (unsigned)beginBit, (unsigned)endBit, st));
+    HIPCHECK(rocprim::radix_sort_keys(sc.get(tmp), tmp, kin, kout, n, (unsigned)beginBit, (unsigned)endBit, st));
 }
 
 template <typename T>
 void exclusiveScan(Scratch &sc, const T *in, T *out, size_t n, hipStream_t st) {
     if (n == 0) return;
     size_t tmp = 0;
-    HIPCHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, in, out, (int)n, st));
-    void *d = sc.get(tmp);
-    HIPCHECK(hipcub::DeviceScan::ExclusiveSum(d, tmp, in, out, (int)n, st));
+    HIPCHECK(rocprim::exclusive_scan(nullptr, tmp, in, out, T(0), n, rocprim::plus<T>(), st));
+    HIPCHECK(rocprim::exclusive_scan(sc.get(tmp), tmp, in, out, T(0), n, rocprim::plus<T>(), st));
 }
 
 template <typename T>
 void inclusiveScan(Scratch &sc, const T *in, T *out, size_t n, hipStream_t st) {
     if (n == 0) return;
     size_t tmp = 0;
-    HIPCHECK(hipcub::DeviceScan::InclusiveSum(nullptr, tmp, in, out, (int)n, st));
-    void *d = sc.get(tmp);
-    HIPCHECK(hipcub::DeviceScan::InclusiveSum(d, tmp, in, out, (int)n, st));
+    HIPCHECK(rocprim::inclusive_scan(nullptr, tmp, in, out, n, rocprim::plus<T>(), st));
+    HIPCHECK(rocprim::inclusive_scan(sc.get(tmp), tmp, in, out, n, rocprim::plus<T>(), st));
 }
 
+// distinct consecutive values of in (n) into out, their number into *numOut (device)
 template <typename T>
 void uniqueKeys(Scratch &sc, const T *in, T *out, u32 *numOut, size_t n, hipStream_t st) {
     size_t tmp = 0;
-    HIPCHECK(hipcub::DeviceSelect::Unique(nullptr, tmp, in, out, numOut, (int)n, st));
-    void *d = sc.get(tmp);
-    HIPCHECK(hipcub::DeviceSelect::Unique(d, tmp, in, out, numOut, (int)n, st));
+    HIPCHECK(rocprim::unique(nullptr, tmp, in, out, numOut, n, rocprim::equal_to<T>(), st));
+    HIPCHECK(rocprim::unique(sc.get(tmp), tmp, in, out, numOut, n, rocprim::equal_to<T>(), st));
+}
+
+// in[i] with flags[i] != 0, in order, into out; their number into *numOut (device)
+template <typename T, typename F>
+void selectFlagged(Scratch &sc, const T *in, const F *flags, T *out, u32 *numOut, size_t n, hipStream_t st) {
+    size_t tmp = 0;
+    HIPCHECK(rocprim::select(nullptr, tmp, in, flags, out, numOut, n, st));
+    HIPCHECK(rocprim::select(sc.get(tmp), tmp, in, flags, out, numOut, n, st));
+}
+
+// in[i] with pred(in[i]), in order, into out; their number into *numOut (device)
+template <typename T, typename Pred>
+void selectIf(Scratch &sc, const T *in, T *out, u32 *numOut, size_t n, Pred pred, hipStream_t st) {
+    size_t tmp = 0;
+    HIPCHECK(rocprim::select(nullptr, tmp, in, out, numOut, n, pred, st));
+    HIPCHECK(rocprim::select(sc.get(tmp), tmp, in, out, numOut, n, pred, st));
+}
+
+// sum (fixed reduction tree: deterministic) / maximum of in (n) into *out (device)
+template <typename T, typename R>
+void reduceSum(Scratch &sc, const T *in, R *out, size_t n, hipStream_t st) {
+    size_t tmp = 0;
+    HIPCHECK(rocprim::reduce(nullptr, tmp, in, out, R(0), n, rocprim::plus<R>(), st));
+    HIPCHECK(rocprim::reduce(sc.get(tmp), tmp, in, out, R(0), n, rocprim::plus<R>(), st));
+}
+template <typename T>
+void reduceMax(Scratch &sc, const T *in, T *out, size_t n, hipStream_t st) {
+    size_t tmp = 0;
+    HIPCHECK(rocprim::reduce(nullptr, tmp, in, out, std::numeric_limits<T>::lowest(), n, rocprim::maximum<T>(), st));
+    HIPCHECK(rocprim::reduce(sc.get(tmp), tmp, in, out, std::numeric_limits<T>::lowest(), n, rocprim::maximum<T>(),
+                             st));
+}
+
+// runs of equal consecutive values: their values, lengths and number (device)
+template <typename T>
+void runLengthEncode(Scratch &sc, const T *in, T *unique, u32 *counts, u32 *runs, size_t n, hipStream_t st) {
+    size_t tmp = 0;
+    HIPCHECK(rocprim::run_length_encode(nullptr, tmp, in, (unsigned)n, unique, counts, runs, st));
+    HIPCHECK(rocprim::run_length_encode(sc.get(tmp), tmp, in, (unsigned)n, unique, counts, runs, st));
 }
 
 template <typename T>

@@ -305,29 +305,29 @@ template <typename K, typename V>
 void sortPairsIn(DmcArena &ar, const K *kin, K *kout, const V *vin, V *vout, u32 n, int endBit, hipStream_t st) {
     if (n == 0) return;
     size_t tmp = 0;
-    HIPCHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, kin, kout, vin, vout, (int)n, 0, endBit, st));
+    HIPCHECK(rocprim::radix_sort_pairs(nullptr, tmp, kin, kout, vin, vout, n, 0u, (unsigned)endBit, st));
     void *d = ar.get<unsigned char>(tmp);
-    HIPCHECK(hipcub::DeviceRadixSort::SortPairs(d, tmp, kin, kout, vin, vout, (int)n, 0, endBit, st));
+    HIPCHECK(rocprim::radix_sort_pairs(d, tmp, kin, kout, vin, vout, n, 0u, (unsigned)endBit, st));
 }
 
 template <typename K>
 void sortKeysIn(DmcArena &ar, const K *kin, K *kout, u32 n, int endBit, hipStream_t st) {
     if (n == 0) return;
     size_t tmp = 0;
-    HIPCHECK(hipcub::DeviceRadixSort::SortKeys(nullptr, tmp, kin, kout, (int)n, 0, endBit, st));
+    HIPCHECK(rocprim::radix_sort_keys(nullptr, tmp, kin, kout, n, 0u, (unsigned)endBit, st));
     void *d = ar.get<unsigned char>(tmp);
-    HIPCHECK(hipcub::DeviceRadixSort::SortKeys(d, tmp, kin, kout, (int)n, 0, endBit, st));
+    HIPCHECK(rocprim::radix_sort_keys(d, tmp, kin, kout, n, 0u, (unsigned)endBit, st));
 }
 
 template <bool Inclusive>
 void scanIn(DmcArena &ar, const u32 *in, u32 *out, u32 n, hipStream_t st) {
     if (n == 0) return;
     size_t tmp = 0;
-    if (Inclusive) HIPCHECK(hipcub::DeviceScan::InclusiveSum(nullptr, tmp, in, out, (int)n, st));
-    else HIPCHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, in, out, (int)n, st));
+    if (Inclusive) HIPCHECK(rocprim::inclusive_scan(nullptr, tmp, in, out, (size_t)n, rocprim::plus<u32>(), st));
+    else HIPCHECK(rocprim::exclusive_scan(nullptr, tmp, in, out, 0u, (size_t)n, rocprim::plus<u32>(), st));
     void *d = ar.get<unsigned char>(tmp);
-    if (Inclusive) HIPCHECK(hipcub::DeviceScan::InclusiveSum(d, tmp, in, out, (int)n, st));
-    else HIPCHECK(hipcub::DeviceScan::ExclusiveSum(d, tmp, in, out, (int)n, st));
+    if (Inclusive) HIPCHECK(rocprim::inclusive_scan(d, tmp, in, out, (size_t)n, rocprim::plus<u32>(), st));
+    else HIPCHECK(rocprim::exclusive_scan(d, tmp, in, out, 0u, (size_t)n, rocprim::plus<u32>(), st));
 }
 
 // Where the mesh goes: the mesh handle's own buffers (allocated once the counts are

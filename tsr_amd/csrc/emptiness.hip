@@ -841,9 +841,7 @@ void emptinessByLeaves(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *
         DBuf<u32> flag(links), iota(links), cnt(1), wl(links);
         k_wideFlags<<<gridFor(links, BLOCK), BLOCK, 0, st>>>(cls.p, links, flag.p);
         k_iota<<<gridFor(links, BLOCK), BLOCK, 0, st>>>(iota.p, links);
-        size_t tmp = 0;
-        HIPCHECK(hipcub::DeviceSelect::Flagged(nullptr, tmp, iota.p, flag.p, wl.p, cnt.p, (int)links, st));
-        HIPCHECK(hipcub::DeviceSelect::Flagged(sc.get(tmp), tmp, iota.p, flag.p, wl.p, cnt.p, (int)links, st));
+        selectFlagged(sc, iota.p, flag.p, wl.p, cnt.p, links, st);
         nWide = readScalar(cnt.p, st);
         if (nWide) {
             DBuf<u32> key(nWide), keyS(nWide), wideSorted(nWide), wb(K), we(K);

@@ -1463,9 +1463,7 @@ void conePassRecords(tsr_context *ctx, const tsr_tree *tree, TraversalTree &tt,
         {
             DBuf<u32> iota(n);
             k_iotaU<<<gridFor(n, BLOCK), BLOCK, 0, st>>>(iota.p, n, l0);
-            size_t tmp = 0;
-            HIPCHECK(hipcub::DeviceSelect::Flagged(nullptr, tmp, iota.p, ovFlag.p, ovList.p, ovCount.p, (int)n, st));
-            HIPCHECK(hipcub::DeviceSelect::Flagged(sc.get(tmp), tmp, iota.p, ovFlag.p, ovList.p, ovCount.p, (int)n, st));
+            selectFlagged(sc, iota.p, ovFlag.p, ovList.p, ovCount.p, n, st);
         }
         const u32 nOver = readScalar(ovCount.p, st);
         totalOver += nOver;
@@ -1587,9 +1585,7 @@ void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *s
                                                                 nullptr);
             k_conePass<<<xcdGrid(links, BLOCK), BLOCK, 0, st>>>(tt, in, links, sampleness, 2, counts.p, ksum,
                                                                 clen, hitCount.p, nullptr, nullptr);
-            size_t tmp = 0;
-            HIPCHECK(hipcub::DeviceReduce::Sum(nullptr, tmp, counts.p, offsets.p, (int)links, st));
-            HIPCHECK(hipcub::DeviceReduce::Sum(sc.get(tmp), tmp, counts.p, offsets.p, (int)links, st));
+            reduceSum(sc, counts.p, offsets.p, links, st);
             field->hits[type] = readScalar(offsets.p, st);
         } else {
             conePassRecords(ctx, tree, tt, in, links, sampleness, ksum, clen, hitCount.p,
@@ -1599,9 +1595,7 @@ void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *s
             k_setLeafFlags<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(hitCount.p, tree->leafNode.p, L, FLAG_E, field->states.p);
             // M: FP64 sum of confidences of non-degenerate emptiness cones (Occupancy.cpp:205-213,239-241)
             DBuf<double> m(1);
-            size_t tmp = 0;
-            HIPCHECK(hipcub::DeviceReduce::Sum(nullptr, tmp, confs.p, m.p, (int)links, st));
-            HIPCHECK(hipcub::DeviceReduce::Sum(sc.get(tmp), tmp, confs.p, m.p, (int)links, st));
+            reduceSum(sc, confs.p, m.p, links, st);
             field->coneCount = readScalar(m.p, st);
         }
         if (sampleness) {
@@ -1711,9 +1705,7 @@ void updateOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *sc
         if (!removal) k_setLeafFlags<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(hits.p, tree->leafNode.p, L, FLAG_E, field->states.p);
         field->hits[0] = removal ? field->hits[0] - std::min(field->hits[0], h) : field->hits[0] + h;
         DBuf<double> m(1);
-        size_t tmp = 0;
-        HIPCHECK(hipcub::DeviceReduce::Sum(nullptr, tmp, confs.p, m.p, (int)links, st));
-        HIPCHECK(hipcub::DeviceReduce::Sum(sc.get(tmp), tmp, confs.p, m.p, (int)links, st));
+        reduceSum(sc, confs.p, m.p, links, st);
         const double dm = readScalar(m.p, st);
         field->coneCount += removal ? -dm : dm;
     }

@@ -359,9 +359,7 @@ struct NonZeroKey {
     __host__ __device__ bool operator()(const u64 &k) const { return k != 0ull; }
 };
 void compactKeys(Scratch &sc, const DHash &h, u64 *out, u32 *count, hipStream_t st) {
-    size_t tmp = 0;
-    HIPCHECK(hipcub::DeviceSelect::If(nullptr, tmp, h.keys.p, out, count, (int)h.capacity, NonZeroKey(), st));
-    HIPCHECK(hipcub::DeviceSelect::If(sc.get(tmp), tmp, h.keys.p, out, count, (int)h.capacity, NonZeroKey(), st));
+    selectIf(sc, h.keys.p, out, count, h.capacity, NonZeroKey(), st);
 }
 
 __global__ void k_insertList(HashView h, const u64 *__restrict__ locs, u32 n, u32 *err) {
@@ -1477,9 +1475,7 @@ void buildTree(tsr_context *ctx, const tsr_scene *scene, const tsr_params &prm, 
     u32 maxContainerDepth = 0;
     {
         DBuf<uint8_t> md(1);
-        size_t tmp = 0;
-        HIPCHECK(hipcub::DeviceReduce::Max(nullptr, tmp, cdepth.p, md.p, (int)n, st));
-        HIPCHECK(hipcub::DeviceReduce::Max(sc.get(tmp), tmp, cdepth.p, md.p, (int)n, st));
+        reduceMax(sc, cdepth.p, md.p, n, st);
         maxContainerDepth = readScalar(md.p, st);
     }
     mark("containment");
@@ -1536,10 +1532,8 @@ void buildTree(tsr_context *ctx, const tsr_scene *scene, const tsr_params &prm, 
     }
     size_t pairs = 0;   // all visits of settled units (statistics, traced runs only)
     if (trace) {
-        size_t tmp = 0;
         DBuf<u32> tot(1);
-        HIPCHECK(hipcub::DeviceReduce::Sum(nullptr, tmp, visitCount.p, tot.p, (int)units, st));
-        HIPCHECK(hipcub::DeviceReduce::Sum(sc.get(tmp), tmp, visitCount.p, tot.p, (int)units, st));
+        reduceSum(sc, visitCount.p, tot.p, units, st);
         pairs = readScalar(tot.p, st);
     }
     checkErr(readScalar(err.p, st));
@@ -1560,11 +1554,7 @@ void buildTree(tsr_context *ctx, const tsr_scene *scene, const tsr_params &prm, 
         DBuf<u64> runLeaf(std::max<size_t>(hotPairs, 1));
         DBuf<u32> runLen(std::max<size_t>(hotPairs, 1)), runOff(std::max<size_t>(hotPairs, 1) + 1);
         if (hotPairs) {
-            size_t tmp = 0;
-            HIPCHECK(hipcub::DeviceRunLengthEncode::Encode(nullptr, tmp, hotLeafSorted.p, runLeaf.p, runLen.p,
-                                                           counter.p, (int)hotPairs, st));
-            HIPCHECK(hipcub::DeviceRunLengthEncode::Encode(sc.get(tmp), tmp, hotLeafSorted.p, runLeaf.p, runLen.p,
-                                                           counter.p, (int)hotPairs, st));
+            runLengthEncode(sc, hotLeafSorted.p, runLeaf.p, runLen.p, counter.p, hotPairs, st);
             runs = readScalar(counter.p, st);
             exclusiveScan(sc, runLen.p, runOff.p, runs, st);
         }
@@ -1613,9 +1603,7 @@ void buildTree(tsr_context *ctx, const tsr_scene *scene, const tsr_params &prm, 
         k_matchStaged<<<gridFor(units, BLOCK), BLOCK, 0, st>>>(staged.p, visitCount.p, isEvent.p, units, loc,
                                                                unitFlag.p);
         DBuf<u32> all(units);
-        size_t tmp = 0;
-        HIPCHECK(hipcub::DeviceSelect::Flagged(nullptr, tmp, unitIota.p, unitFlag.p, all.p, counter.p, (int)units, st));
-        HIPCHECK(hipcub::DeviceSelect::Flagged(sc.get(tmp), tmp, unitIota.p, unitFlag.p, all.p, counter.p, (int)units, st));
+        selectFlagged(sc, unitIota.p, unitFlag.p, all.p, counter.p, units, st);
         const u32 m = readScalar(counter.p, st);
         out.resize(m);
         if (m) HIPCHECK(hipMemcpyAsync(out.data(), all.p, (size_t)m * 4, hipMemcpyDeviceToHost, st));
