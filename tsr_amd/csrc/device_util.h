@@ -253,6 +253,17 @@ struct HashView {
         }
         return INVALID;
     }
+    // the value slot of key, or null
+    __device__ u32 *slot(u64 key) const {
+        u64 s = hashMix(key) & mask;
+        for (u64 probe = 0; probe <= mask; ++probe) {
+            const u64 k = keys[s];
+            if (k == key) return &vals[s];
+            if (k == 0ull) return nullptr;
+            s = (s + 1) & mask;
+        }
+        return nullptr;
+    }
     __device__ bool contains(u64 key) const {
         u64 s = hashMix(key) & mask;
         for (u64 probe = 0; probe <= mask; ++probe) {

@@ -111,6 +111,36 @@ __global__ void k_containers(Frame f, const float *__restrict__ px, const float 
     depthOut[i] = (uint8_t)d;
 }
 
+// Inner-node set of the containment tree with, as each node's value, the mask of its
+// inner children (k_zeroVals, then k_childMasks over the set's keys).  A DFS that tests the
+// 8 children of an open node then looks the set up once per node it opens, through a
+// one-entry cache of the last parent's mask, instead of once per child.
+__global__ void k_zeroVals(HashView h, const u64 *__restrict__ keys, u32 n) {
+    const u32 i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    if (u32 *v = h.slot(keys[i])) *v = 0u;
+}
+__global__ void k_childMasks(HashView h, const u64 *__restrict__ keys, u32 n) {
+    const u32 i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n || keys[i] <= 1ull) return;
+    if (u32 *v = h.slot(keys[i] >> 3)) atomicOr(v, 1u << (keys[i] & 7));   // the parent is inner too
+}
+struct InnerMasks {
+    HashView h;
+    u64 cacheKey = 0;
+    u32 cacheMask = 0;
+    __device__ bool operator()(u64 loc) {
+        if (loc == 1ull) return h.contains(1ull);
+        const u64 parent = loc >> 3;
+        if (parent != cacheKey) {
+            const u32 *v = h.slot(parent);
+            cacheKey = parent;
+            cacheMask = v ? *v : 0u;
+        }
+        return (cacheMask >> (loc & 7)) & 1u;
+    }
+};
+
 __global__ void k_insertAncestors(HashView h, const u64 *__restrict__ containerLoc, u32 n, u32 *err) {
     const u32 i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n) return;
@@ -335,7 +365,8 @@ __global__ void k_samplingEval(Frame f, HashView inner, const float *__restrict_
     const SamplingUnit u = loadUnit(px, py, pz, nx, ny, nz, scale, maxRel, i);
     u32 count = 0;
     const u32 base = mode == 1 ? offsets[unit] : 0;
-    auto isInner = [&](u64 loc) { return inner.contains(loc); };
+    InnerMasks masks{inner};
+    auto isInner = [&](u64 loc) { return masks(loc); };
     auto onLeaf = [&](u64 loc) {
         if (mode == 1 && count < VISIT_CAP) {
             pairLeaf[base + count] = loc;
@@ -1181,6 +1212,7 @@ __global__ void k_markHot(Frame f, HashView inner, HashView hot, const float *__
         hot.insert(1ull, 0, err);
         return;
     }
+    InnerMasks isInner{inner};
     u64 loc = 1ull;
     u32 next = 0;
     for (;;) {
@@ -1192,7 +1224,7 @@ __global__ void k_markHot(Frame f, HashView inner, HashView hot, const float *__
         }
         const u64 ch = (loc << 3) | next;
         if (overlaps(ch)) {
-            if (inner.contains(ch)) {
+            if (isInner(ch)) {
                 loc = ch;
                 next = 0;
                 continue;
@@ -1485,6 +1517,10 @@ void buildTree(tsr_context *ctx, const tsr_scene *scene, const tsr_params &prm, 
     HIPCHECK(hipMemsetAsync(t0Counter.p, 0, 4, st));
     compactKeys(sc, inner, innerList.p, t0Counter.p, st);
     const u32 t0Count = readScalar(t0Counter.p, st);
+    if (t0Count) {   // each inner node's value: the mask of its inner children
+        k_zeroVals<<<gridFor(t0Count, BLOCK), BLOCK, 0, st>>>(inner.view(), innerList.p, t0Count);
+        k_childMasks<<<gridFor(t0Count, BLOCK), BLOCK, 0, st>>>(inner.view(), innerList.p, t0Count);
+    }
     // the inner codes in pinned host memory (any order: the host sets insert every key
     // before they record children in the parents' masks)
     u64 *hInner = (u64 *)pinnedHost(ctx, std::max<size_t>((size_t)t0Count * 8, 8), 1);
