@@ -122,3 +122,27 @@ def test_gather_matches_oracle(fields):
         r = rel_err(g[k], of[k], floor=1e-30)
         assert np.max(r) <= 1e-5, f"{k}: max rel err {np.max(r):.3e} vs the oracle"
     assert abs(g["cone_count"] - of["cone_count"]) <= 1e-9 * of["cone_count"]
+
+
+def test_index_rebuilt_for_other_parameters(gpu_ctx):
+    """The tree keeps the direction index built from its scene with its parameters; a field
+    computed with another bandwidth factor builds its own (over the tree's node-order links)
+    and must still equal the per-link traversal with that bandwidth."""
+    from tsr_amd import Occupancy, Scene, Tree
+    from tsr_amd._lib import TSR_EMPTINESS_TRAVERSAL, TSR_PRIORS_ALL_LEAVES, default_params
+    samples, cameras = scene(1, 0.0625)
+    sc = Scene(gpu_ctx, samples, cameras)
+    tree = Tree(gpu_ctx, sc)
+    params = default_params()
+    params.bandwidth_factor = 3.0   # instead of 2
+    g = Occupancy(gpu_ctx, tree, sc, params=params, flags=TSR_PRIORS_ALL_LEAVES)
+    t = Occupancy(gpu_ctx, tree, sc, params=params, flags=TSR_PRIORS_ALL_LEAVES | TSR_EMPTINESS_TRAVERSAL)
+    fg, ft = g.download(), t.download()
+    L = len(fg["occupancy"])
+    assert np.array_equal(fg["node_states"], ft["node_states"])
+    assert np.array_equal(g.hit_counts(), t.hit_counts())
+    assert fg["cone_count"] == ft["cone_count"]
+    for k in ("kernel_sums", "cone_lengths"):
+        assert np.max(rel_err(fg[k][:L], ft[k][:L], floor=1e-30)) <= 1e-6, k
+    for x in (t, g, tree, sc):
+        x.close()

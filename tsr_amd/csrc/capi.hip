@@ -1,5 +1,6 @@
 // C ABI entry points (include/tsr_gpu.h): argument checks, handle lifetime,
 // exception -> status translation, host <-> device copies.
+#include <atomic>
 #include <cstring>
 #include <string>
 
@@ -74,6 +75,7 @@ tsr_status tsr_context_create(int device, tsr_context **out) {
         c->pool = std::make_shared<tsr::DevicePool>(device);
         (void)tsr::noAllocScratch(device);   // this GPU's copy of the no-allocation scratch array
         HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        HIPCHECK(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
         for (hipEvent_t &e : c->ev) HIPCHECK(hipEventCreate(&e));
         for (hipEvent_t &e : c->kev) HIPCHECK(hipEventCreate(&e));
         *out = c;
@@ -92,7 +94,9 @@ void tsr_context_destroy(tsr_context *ctx) {
         if (p) (void)hipHostFree(p);
     for (hipEvent_t &e : ctx->ev) (void)hipEventDestroy(e);
     for (hipEvent_t &e : ctx->kev) (void)hipEventDestroy(e);
+    (void)hipStreamSynchronize(ctx->stream2);
     (void)hipStreamDestroy(ctx->stream);
+    (void)hipStreamDestroy(ctx->stream2);
     delete ctx;
 }
 
@@ -164,6 +168,8 @@ tsr_status tsr_scene_upload(tsr_context *ctx, const tsr_samples_view *s, const t
         activate(ctx);
         tsr_scene *sc = new tsr_scene();
         try {
+            static std::atomic<unsigned long long> sceneIds{0};
+            sc->id = ++sceneIds;
             hipStream_t st = ctx->stream;
             const u32 n = s->count;
             sc->ctx = ctx;

@@ -36,43 +36,7 @@
 
 namespace tsr {
 
-namespace {
-
-constexpr u32 BLOCK = 256;
-constexpr u32 XCD_COUNT = 8;
-constexpr u32 FACES = 6;
-constexpr u32 WIDE_CELLS = 4096;        // a link whose cone covers more cells is listed per camera
-constexpr u32 MAX_FACE_CELLS = 1u << 15;  // cells along one side of a face grid
-constexpr u32 LEAVES_PER_WAVE = 4;      // consecutive leaves per wavefront of the leaf pass
-constexpr u32 CAND_UNROLL = 4;          // candidates per lane and step of the leaf pass
-constexpr u32 GATHER_WAVES = 8;         // wavefronts per block of the leaf pass
-constexpr u32 CAM_LDS = 256;            // cameras whose grids the leaf pass keeps in LDS
-constexpr double REAL_MAX_D = 3.4028234663852886e38;
-
-// link classes
-constexpr uint8_t LINK_NONE = 0, LINK_GRID = 1, LINK_WIDE = 2;
-constexpr u32 ENTRY_OWN_FACE = 0x80000000u;   // entry value bit: the cell is on the cone axis's face
-constexpr float NO_DISC = 1e30f;               // disc radius of entries without a face-coordinate test
-constexpr u32 EVAL_WORDS = 4;                  // float4s per link of LinkGeo::eval
-
-struct LinkSrc {
-    const float *px, *py, *pz, *nx, *ny, *nz, *scale, *conf;
-    const u32 *cams;
-    const float *cx, *cy, *cz;
-    const uint8_t *sampleDepth;
-    u32 maxCams, camCount;
-    float bandwidth;
-};
-
-// per-link data of the leaf pass, in link order
-struct LinkGeo {
-    float4 *axis;    // unit cone axis (e - a) / |e - a|, tan of the inflated cone angle (inf: wide)
-    float4 *aux;     // reach (|e - a| + R, inflated), inflated cone angle (radians), depth limit, 0
-    // the FP64 kernel's data, one 64-byte line per link (EVAL_WORDS float4s): sample position
-    // and scale; sample normal and confidence; 1 / dot(e - a, n) and |e - a| (ConeD::invDenom,
-    // ConeD::length) as two doubles; padding
-    float4 *eval;
-};
+constexpr u32 FACES = 6;   // cube-map faces per camera
 
 struct FaceGrid {
     int lo[2];       // first cell (u, v) of the bounding box of the face's links
@@ -88,16 +52,79 @@ struct CamGrid {
     FaceGrid face[FACES];
 };
 
+// The direction index of the gathered E pass over one set of links (emptinessByLeaves; built
+// by buildEmptinessIndex): per-link FP64-stage records, per-camera cube-map grids, entries
+// sorted by cell and by descending reach within a cell with their test records, the wide links
+// per camera.
+struct EmptinessIndex {
+    u32 links = 0, camCount = 0, nWide = 0;
+    float bandwidth = 0.f;
+    unsigned long long sceneId = 0;
+    u64 totalCells = 0, totalEntries = 0;
+    DBuf<float4> eval;                 // EVAL_WORDS per link
+    DBuf<CamGrid> grids;               // per camera
+    DBuf<u32> cellStart, cellEnd;      // per cell: run of its entries
+    DBuf<float> cellReach;             // per cell: largest reach
+    DBuf<float> entryReach;            // per entry
+    DBuf<float4> rec, wideRec;         // test records per entry / per wide link
+};
+
+namespace {
+
+constexpr u32 BLOCK = 256;
+constexpr u32 XCD_COUNT = 8;
+constexpr u32 WIDE_CELLS = 4096;        // a link whose cone covers more cells is listed per camera
+constexpr u32 MAX_FACE_CELLS = 1u << 15;  // cells along one side of a face grid
+constexpr u32 LEAVES_PER_WAVE = 4;      // consecutive leaves per wavefront of the leaf pass
+constexpr u32 CAND_UNROLL = 4;          // candidates per lane and step of the leaf pass
+constexpr u32 GATHER_WAVES = 8;         // wavefronts per block of the leaf pass
+constexpr u32 CAM_LDS = 256;            // cameras whose grids the leaf pass keeps in LDS
+constexpr double REAL_MAX_D = 3.4028234663852886e38;
+
+// link classes
+constexpr uint8_t LINK_NONE = 0, LINK_GRID = 1, LINK_WIDE = 2;
+constexpr u32 ENTRY_OWN_FACE = 0x80000000u;   // entry value bit: the cell is on the cone axis's face
+constexpr float NO_DISC = 1e30f;               // disc radius of entries without a face-coordinate test
+constexpr u32 EVAL_WORDS = 4;                  // float4s per link of LinkGeo::eval
+
+
+// per-link data of the leaf pass, in link order
+struct LinkGeo {
+    float4 *axis;    // unit cone axis (e - a) / |e - a|, tan of the inflated cone angle (inf: wide)
+    float4 *aux;     // reach (|e - a| + R, inflated), inflated cone angle (radians), depth limit, 0
+    // the FP64 kernel's data, one 64-byte line per link (EVAL_WORDS float4s): sample position
+    // and scale; sample normal and confidence; 1 / dot(e - a, n) and |e - a| (ConeD::invDenom,
+    // ConeD::length) as two doubles; padding
+    float4 *eval;
+};
+
+
+// confidence of each link's emptiness cone, 0 when it has none (invalid camera or degenerate
+// cone): the terms of the cone count M (Occupancy.cpp:205-213,239-241)
+__global__ void k_linkConfs(EmptinessLinks s, u32 links, double *__restrict__ confs) {
+    const u32 l = blockIdx.x * BLOCK + threadIdx.x;
+    if (l >= links) return;
+    const u32 i = l / s.maxCams, cam = s.cams[l];
+    double conf = 0.0;
+    if (cam < s.camCount) {
+        const double apex[3] = {s.cx[cam], s.cy[cam], s.cz[cam]};
+        const double p[3] = {s.px[i], s.py[i], s.pz[i]};
+        const double nrm[3] = {-(double)s.nx[i], -(double)s.ny[i], -(double)s.nz[i]};
+        ConeD c;
+        coneInit(c, apex, nrm, p, (double)s.bandwidth * (double)s.scale[i], REAL_MAX_D);
+        if (!c.degenerate) conf = s.conf[i];
+    }
+    confs[l] = conf;
+}
+
 // Cone of link l as the traversal sets it up (Occupancy.cpp:174-201, getCameraToSampleLinkData
 // :250-277): apex at the camera, end at the sample, end normal -n, R = bandwidth * scale,
 // uncut.  Writes the link's leaf-pass data and its confidence (0: not an emptiness cone).
-__global__ void k_gLinkSetup(LinkSrc s, u32 links, LinkGeo g, uint8_t *__restrict__ cls,
-                             double *__restrict__ confs, u32 *__restrict__ camKey, u32 *__restrict__ reachKey,
-                             u32 *__restrict__ iota) {
+__global__ void k_gLinkSetup(EmptinessLinks s, u32 links, LinkGeo g, uint8_t *__restrict__ cls,
+                             u32 *__restrict__ camKey, u32 *__restrict__ reachKey, u32 *__restrict__ iota) {
     const u32 l = blockIdx.x * BLOCK + threadIdx.x;
     if (l >= links) return;
     cls[l] = LINK_NONE;
-    confs[l] = 0.0;
     camKey[l] = s.camCount;   // sort keys: links without a cone last
     reachKey[l] = 0xffffffffu;
     iota[l] = l;
@@ -110,8 +137,6 @@ __global__ void k_gLinkSetup(LinkSrc s, u32 links, LinkGeo g, uint8_t *__restric
     ConeD c;
     coneInit(c, apex, nrm, p, R, REAL_MAX_D);
     if (c.degenerate) return;
-    const double conf = s.conf[i];
-    confs[l] = conf;
     const double A = c.length;
     const bool wide = !(R < 0.5 * A);
     // tan of the cone angle asin(R / A), which also bounds the angle from above; inflated:
@@ -718,57 +743,56 @@ __global__ __launch_bounds__(64 * GATHER_WAVES) void k_gLeaves(GatherArgs a, con
 
 }  // namespace
 
-void emptinessByLeaves(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *scene, const tsr_params &prm,
-                       const u32 *hitsS, float *ksum, float *clen, u32 *hitCount, double *confs, u64 &hitsOut) {
-    hipStream_t st = ctx->stream;
-    Scratch &sc = ctx->scratch;
-    const u32 L = tree->leafCount, K = scene->camCount;
-    const u32 links = tree->sampleCount * tree->maxCams;
+std::shared_ptr<EmptinessIndex> buildEmptinessIndex(hipStream_t st, Scratch &sc, const EmptinessLinks &src) {
+    const u32 K = src.camCount;
+    const u32 links = src.samples * src.maxCams;
+    auto ix = std::make_shared<EmptinessIndex>();
+    ix->links = links;
+    ix->camCount = K;
+    ix->bandwidth = src.bandwidth;
+    ix->sceneId = src.sceneId;
     const bool trace = std::getenv("TSR_TRACE") != nullptr;
     auto tmark = std::chrono::steady_clock::now();
     auto lap = [&](const char *what) {   // TSR_TRACE: synchronised sub-step times
         if (!trace) return;
         HIPCHECK(hipStreamSynchronize(st));
         const auto t = std::chrono::steady_clock::now();
-        std::fprintf(stderr, "[tsr occupancy]   %-28s %8.2f ms\n", what,
+        std::fprintf(stderr, "[tsr emptiness index] %-22s %8.2f ms\n", what,
                      std::chrono::duration<double, std::milli>(t - tmark).count());
         tmark = t;
     };
-    lap("(emptiness start)");
-    HIPCHECK(hipMemsetAsync(ksum, 0, (size_t)L * 4, st));
-    HIPCHECK(hipMemsetAsync(clen, 0, (size_t)L * 4, st));
-    HIPCHECK(hipMemsetAsync(hitCount, 0, (size_t)L * 4, st));
-    hitsOut = 0;
-    if (links == 0 || L == 0) return;
+    ix->grids.alloc(std::max(K, 1u));
+    if (links == 0) {
+        ix->cellStart.alloc(1);
+        ix->cellEnd.alloc(1);
+        ix->cellReach.alloc(1);
+        return ix;
+    }
 
     // ---- per-link cones; links grouped by camera and sorted by descending reach
-    DBuf<float4> axis(links), aux(links), eval(EVAL_WORDS * (size_t)links);
+    DBuf<float4> axis(links), aux(links);
+    ix->eval.alloc(EVAL_WORDS * (size_t)links);
     DBuf<uint8_t> cls(links);
     DBuf<unsigned long long> camTheta(2 * (size_t)std::max(K, 1u));
     HIPCHECK(hipMemsetAsync(camTheta.p, 0, camTheta.bytes(), st));
-    const LinkSrc src{tree->px.p, tree->py.p, tree->pz.p, tree->nx.p, tree->ny.p, tree->nz.p, tree->scale.p,
-                      tree->conf.p, tree->cams.p, scene->cx.p, scene->cy.p, scene->cz.p, tree->sampleDepth.p,
-                      tree->maxCams, K, prm.bandwidth_factor};
-    const LinkGeo geo{axis.p, aux.p, eval.p};
+    const LinkGeo geo{axis.p, aux.p, ix->eval.p};
     DBuf<u32> byCam(links), byReach(links);
     {
         DBuf<u32> camKey(links), reachKey(links), iota(links), keyS(links);
-        k_gLinkSetup<<<gridFor(links, BLOCK), BLOCK, 0, st>>>(src, links, geo, cls.p, confs, camKey.p, reachKey.p,
-                                                              iota.p);
+        k_gLinkSetup<<<gridFor(links, BLOCK), BLOCK, 0, st>>>(src, links, geo, cls.p, camKey.p, reachKey.p, iota.p);
         int bits = 1;
         while (bits < 32 && (1ull << bits) <= K) ++bits;
         sortPairs(sc, camKey.p, keyS.p, iota.p, byCam.p, links, 0, bits, st);
         sortPairs(sc, reachKey.p, keyS.p, iota.p, byReach.p, links, 0, 32, st);
-        k_camTheta<<<gridFor(links, BLOCK), BLOCK, 0, st>>>(byCam.p, links, cls.p, tree->cams.p, aux.p,
-                                                            camTheta.p);
+        k_camTheta<<<gridFor(links, BLOCK), BLOCK, 0, st>>>(byCam.p, links, cls.p, src.cams, aux.p, camTheta.p);
     }
     std::vector<unsigned long long> hTheta(2 * (size_t)K);
     std::vector<float> hcx(K), hcy(K), hcz(K);
     if (K) {
         HIPCHECK(hipMemcpyAsync(hTheta.data(), camTheta.p, 16 * (size_t)K, hipMemcpyDeviceToHost, st));
-        HIPCHECK(hipMemcpyAsync(hcx.data(), scene->cx.p, 4 * (size_t)K, hipMemcpyDeviceToHost, st));
-        HIPCHECK(hipMemcpyAsync(hcy.data(), scene->cy.p, 4 * (size_t)K, hipMemcpyDeviceToHost, st));
-        HIPCHECK(hipMemcpyAsync(hcz.data(), scene->cz.p, 4 * (size_t)K, hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipMemcpyAsync(hcx.data(), src.cx, 4 * (size_t)K, hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipMemcpyAsync(hcy.data(), src.cy, 4 * (size_t)K, hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipMemcpyAsync(hcz.data(), src.cz, 4 * (size_t)K, hipMemcpyDeviceToHost, st));
     }
     HIPCHECK(hipStreamSynchronize(st));
     // cells along a face side per camera: a cell about the camera's mean cone angle
@@ -790,12 +814,12 @@ void emptinessByLeaves(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *
     std::vector<CamGrid> grids(std::max(K, 1u));
     u64 totalCells = 0, totalEntries = 0;
     for (int attempt = 0;; ++attempt) {
-        if (attempt > 40) throw tsr_error(TSR_ERR_OUT_OF_MEMORY, "emptiness cell grid does not fit its budget");
+        if (attempt > 40) return nullptr;
         HIPCHECK(hipMemcpyAsync(camGd.p, camG.data(), 4 * (size_t)std::max(K, 1u), hipMemcpyHostToDevice, st));
         k_fillBox<<<gridFor(box.n, BLOCK), BLOCK, 0, st>>>(box.p, box.n);
-        k_gLinkCells<<<gridFor(links, BLOCK), BLOCK, 0, st>>>(links, 0, byCam.p, tree->cams.p, axis.p, aux.p,
-                                                              nullptr, camGd.p, cls.p, count.p, box.p, nullptr,
-                                                              nullptr, nullptr);
+        k_gLinkCells<<<gridFor(links, BLOCK), BLOCK, 0, st>>>(links, 0, byCam.p, src.cams, axis.p, aux.p, nullptr,
+                                                              camGd.p, cls.p, count.p, box.p, nullptr, nullptr,
+                                                              nullptr);
         HIPCHECK(hipMemsetAsync(sum1.p, 0, 8, st));
         k_sumU32<<<SUM_BLOCKS, BLOCK, 0, st>>>(count.p, links, sum1.p);
         if (K) HIPCHECK(hipMemcpyAsync(hbox.data(), box.p, 4 * hbox.size(), hipMemcpyDeviceToHost, st));
@@ -830,28 +854,27 @@ void emptinessByLeaves(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *
                 camG[c] = (camG[c] + 1) / 2;
                 coarser = true;
             }
-        if (!coarser) throw tsr_error(TSR_ERR_OUT_OF_MEMORY, "emptiness cell grid does not fit its budget");
+        if (!coarser) return nullptr;
     }
     lap("cell grids");
 
     // ---- wide links per camera (link order within a camera) with their test records
-    DBuf<float4> wideRec;
-    u32 nWide = 0;
     {
         DBuf<u32> flag(links), iota(links), cnt(1), wl(links);
         k_wideFlags<<<gridFor(links, BLOCK), BLOCK, 0, st>>>(cls.p, links, flag.p);
         k_iota<<<gridFor(links, BLOCK), BLOCK, 0, st>>>(iota.p, links);
         selectFlagged(sc, iota.p, flag.p, wl.p, cnt.p, links, st);
-        nWide = readScalar(cnt.p, st);
+        ix->nWide = readScalar(cnt.p, st);
+        const u32 nWide = ix->nWide;
         if (nWide) {
             DBuf<u32> key(nWide), keyS(nWide), wideSorted(nWide), wb(K), we(K);
-            k_wideCams<<<gridFor(nWide, BLOCK), BLOCK, 0, st>>>(wl.p, nWide, tree->cams.p, key.p);
+            k_wideCams<<<gridFor(nWide, BLOCK), BLOCK, 0, st>>>(wl.p, nWide, src.cams, key.p);
             int bits = 1;
             while (bits < 32 && (1ull << bits) < K) ++bits;
             sortPairs(sc, key.p, keyS.p, wl.p, wideSorted.p, nWide, 0, bits, st);
-            wideRec.alloc(nWide);
-            k_entryRecords<<<gridFor(nWide, BLOCK), BLOCK, 0, st>>>(wideSorted.p, nWide, axis.p, aux.p, wideRec.p,
-                                                                    nullptr);
+            ix->wideRec.alloc(nWide);
+            k_entryRecords<<<gridFor(nWide, BLOCK), BLOCK, 0, st>>>(wideSorted.p, nWide, axis.p, aux.p,
+                                                                    ix->wideRec.p, nullptr);
             HIPCHECK(hipMemsetAsync(wb.p, 0, 4 * (size_t)K, st));
             HIPCHECK(hipMemsetAsync(we.p, 0, 4 * (size_t)K, st));
             k_wideRuns<<<gridFor(nWide, BLOCK), BLOCK, 0, st>>>(keyS.p, nWide, wb.p, we.p);
@@ -865,14 +888,16 @@ void emptinessByLeaves(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *
             }
         }
     }
-    DBuf<CamGrid> gridsD(std::max(K, 1u));
-    HIPCHECK(hipMemcpyAsync(gridsD.p, grids.data(), sizeof(CamGrid) * std::max(K, 1u), hipMemcpyHostToDevice, st));
+    HIPCHECK(hipMemcpyAsync(ix->grids.p, grids.data(), sizeof(CamGrid) * std::max(K, 1u), hipMemcpyHostToDevice,
+                            st));
 
     // ---- (cell, link) entries sorted by cell, descending reach within a cell, with the
     // links' test records alongside
     const size_t nE = (size_t)totalEntries;
-    DBuf<float4> rec(std::max<size_t>(nE, 1));
-    DBuf<float> entryReach(std::max<size_t>(nE, 1));
+    ix->totalCells = totalCells;
+    ix->totalEntries = totalEntries;
+    ix->rec.alloc(std::max<size_t>(nE, 1));
+    ix->entryReach.alloc(std::max<size_t>(nE, 1));
     DBuf<u32> keysS(std::max<size_t>(nE, 1));
     {
         DBuf<u32> keys(std::max<size_t>(nE, 1)), vals(std::max<size_t>(nE, 1)), entries(std::max<size_t>(nE, 1));
@@ -880,29 +905,67 @@ void emptinessByLeaves(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *
             DBuf<u32> countR(links), offR(links);
             k_gatherU32<<<gridFor(links, BLOCK), BLOCK, 0, st>>>(count.p, byReach.p, links, countR.p);
             exclusiveScan(sc, countR.p, offR.p, links, st);
-            k_gLinkCells<<<gridFor(links, BLOCK), BLOCK, 0, st>>>(links, 1, byReach.p, tree->cams.p, axis.p, aux.p,
-                                                                  gridsD.p, camGd.p, cls.p, nullptr, nullptr,
+            k_gLinkCells<<<gridFor(links, BLOCK), BLOCK, 0, st>>>(links, 1, byReach.p, src.cams, axis.p, aux.p,
+                                                                  ix->grids.p, camGd.p, cls.p, nullptr, nullptr,
                                                                   offR.p, keys.p, vals.p);
         }
         int bits = 1;
         while (bits < 32 && (1ull << bits) < totalCells) ++bits;
         sortPairs(sc, keys.p, keysS.p, vals.p, entries.p, nE, 0, bits, st);
-        if (nE) k_entryRecords<<<gridFor(nE, BLOCK), BLOCK, 0, st>>>(entries.p, nE, axis.p, aux.p, rec.p,
-                                                                     entryReach.p);
+        if (nE) k_entryRecords<<<gridFor(nE, BLOCK), BLOCK, 0, st>>>(entries.p, nE, axis.p, aux.p, ix->rec.p,
+                                                                     ix->entryReach.p);
     }
     const size_t nC = std::max<u64>(totalCells, 1);
-    DBuf<u32> cellStart(nC), cellEnd(nC);
-    DBuf<float> cellReach(nC);
-    HIPCHECK(hipMemsetAsync(cellStart.p, 0, 4 * nC, st));
-    HIPCHECK(hipMemsetAsync(cellEnd.p, 0, 4 * nC, st));
-    HIPCHECK(hipMemsetAsync(cellReach.p, 0, 4 * nC, st));
-    if (nE) k_cellRuns<<<gridFor(nE, BLOCK), BLOCK, 0, st>>>(keysS.p, nE, entryReach.p, cellStart.p, cellEnd.p,
-                                                             cellReach.p);
-    keysS.release();
+    ix->cellStart.alloc(nC);
+    ix->cellEnd.alloc(nC);
+    ix->cellReach.alloc(nC);
+    HIPCHECK(hipMemsetAsync(ix->cellStart.p, 0, 4 * nC, st));
+    HIPCHECK(hipMemsetAsync(ix->cellEnd.p, 0, 4 * nC, st));
+    HIPCHECK(hipMemsetAsync(ix->cellReach.p, 0, 4 * nC, st));
+    if (nE) k_cellRuns<<<gridFor(nE, BLOCK), BLOCK, 0, st>>>(keysS.p, nE, ix->entryReach.p, ix->cellStart.p,
+                                                             ix->cellEnd.p, ix->cellReach.p);
     lap("cell index");
     if (trace)
-        std::fprintf(stderr, "[tsr occupancy] emptiness index: %u links, %llu cells, %llu entries, %u wide links\n",
-                     links, (unsigned long long)totalCells, (unsigned long long)totalEntries, nWide);
+        std::fprintf(stderr, "[tsr emptiness index] %u links, %llu cells, %llu entries, %u wide links\n", links,
+                     (unsigned long long)totalCells, (unsigned long long)totalEntries, ix->nWide);
+    return ix;
+}
+
+void emptinessByLeaves(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *scene, const tsr_params &prm,
+                       const u32 *hitsS, float *ksum, float *clen, u32 *hitCount, double *confs, u64 &hitsOut) {
+    hipStream_t st = ctx->stream;
+    Scratch &sc = ctx->scratch;
+    const u32 L = tree->leafCount, K = scene->camCount;
+    const u32 links = tree->sampleCount * tree->maxCams;
+    const bool trace = std::getenv("TSR_TRACE") != nullptr;
+    auto tmark = std::chrono::steady_clock::now();
+    auto lap = [&](const char *what) {   // TSR_TRACE: synchronised sub-step times
+        if (!trace) return;
+        HIPCHECK(hipStreamSynchronize(st));
+        const auto t = std::chrono::steady_clock::now();
+        std::fprintf(stderr, "[tsr occupancy]   %-28s %8.2f ms\n", what,
+                     std::chrono::duration<double, std::milli>(t - tmark).count());
+        tmark = t;
+    };
+    lap("(emptiness start)");
+    HIPCHECK(hipMemsetAsync(ksum, 0, (size_t)L * 4, st));
+    HIPCHECK(hipMemsetAsync(clen, 0, (size_t)L * 4, st));
+    HIPCHECK(hipMemsetAsync(hitCount, 0, (size_t)L * 4, st));
+    hitsOut = 0;
+    // the tree's links (node order): the cone count's terms, and the index when the tree has none
+    const EmptinessLinks treeLinks{tree->px.p, tree->py.p, tree->pz.p, tree->nx.p, tree->ny.p, tree->nz.p,
+                                   tree->scale.p, tree->conf.p, tree->cams.p, scene->cx.p, scene->cy.p,
+                                   scene->cz.p, tree->sampleDepth.p, tree->sampleCount, tree->maxCams, K,
+                                   prm.bandwidth_factor, scene->id};
+    if (links) k_linkConfs<<<gridFor(links, BLOCK), BLOCK, 0, st>>>(treeLinks, links, confs);
+    if (links == 0 || L == 0) return;
+    std::shared_ptr<EmptinessIndex> ix = tree->eIndex;
+    if (!ix || ix->sceneId != scene->id || ix->bandwidth != prm.bandwidth_factor || ix->links != links ||
+        ix->camCount != K) {
+        ix = buildEmptinessIndex(st, sc, treeLinks);
+        if (!ix) throw tsr_error(TSR_ERR_OUT_OF_MEMORY, "emptiness cell grid does not fit its budget");
+    }
+    lap("index");
 
     // ---- the sampleness-flagged leaves, in leaf order
     DBuf<u32> lflag(L), lpos(L), leafList(L);
@@ -914,8 +977,10 @@ void emptinessByLeaves(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *
     lpos.release();
 
     // ---- the leaf pass
-    GatherArgs ga{tree->frame, tree->leafLoc.p, gridsD.p, K, cellStart.p, cellEnd.p, cellReach.p,
-                  entryReach.p, rec.p, nWide ? wideRec.p : rec.p, geo, (double)prm.bandwidth_factor};
+    const LinkGeo geo{nullptr, nullptr, ix->eval.p};
+    GatherArgs ga{tree->frame, tree->leafLoc.p, ix->grids.p, K, ix->cellStart.p, ix->cellEnd.p, ix->cellReach.p,
+                  ix->entryReach.p, ix->rec.p, ix->nWide ? ix->wideRec.p : ix->rec.p, geo,
+                  (double)prm.bandwidth_factor};
     DBuf<unsigned long long> kst(4);
     HIPCHECK(hipMemsetAsync(kst.p, 0, 32, st));
     const u32 groups = (nLeaves + LEAVES_PER_WAVE - 1) / LEAVES_PER_WAVE;
@@ -929,15 +994,16 @@ void emptinessByLeaves(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *
     }
     unsigned long long ks[3] = {0, 0, 0};
     HIPCHECK(hipMemcpyAsync(ks, kst.p, 24, hipMemcpyDeviceToHost, st));
+    DBuf<unsigned long long> sum1(1);
     HIPCHECK(hipMemsetAsync(sum1.p, 0, 8, st));
     k_sumU32<<<SUM_BLOCKS, BLOCK, 0, st>>>(hitCount, L, sum1.p);
     hitsOut = readScalar(sum1.p, st);
     {
         // algorithmic bytes: per leaf its list entry, code and the three outputs (24 B); per
         // (leaf, camera) cell lookup the cell's run and reach (12 B); per candidate its test
-        // record (32 B); per FP64 evaluation the sample and cone data (48 B)
+        // record (16 B); per FP64 evaluation the link's 64-byte record
         tsr_kernel_stat &k = ctx->kstat[KSTAT_E_GATHER];
-        k.bytes += (u64)nLeaves * 24 + ks[0] * 12 + ks[1] * 32 + ks[2] * 48;
+        k.bytes += (u64)nLeaves * 24 + ks[0] * 12 + ks[1] * 16 + ks[2] * 64;
         k.units += nLeaves;
     }
     lap("leaf pass");

@@ -13,6 +13,7 @@ struct tsr_kernel_stat {
 };
 
 namespace tsr {
+struct EmptinessIndex;   // direction index of the gathered emptiness pass (emptiness.hip)
 // buffers of the cone passes, kept by the context between passes and calls (grow-only):
 // repeated allocation and release of tens of GB costs more than the passes themselves
 struct ConeArena {
@@ -31,6 +32,9 @@ struct ConeArena {
 struct tsr_context {
     int device = 0;
     hipStream_t stream = nullptr;
+    // second stream (default priority): the emptiness index is built on it, from a worker
+    // thread, while the tree build replays the sampling refinement on the host
+    hipStream_t stream2 = nullptr;
     hipEvent_t ev[12] = {};   // start/end pair per phase
     hipEvent_t kev[2] = {};   // start/end of an instrumented kernel
     float phaseMs[6] = {0, 0, 0, 0, 0, 0};
@@ -48,6 +52,7 @@ struct tsr_context {
 struct tsr_scene {
     tsr_context *ctx = nullptr;
     int device = 0;   // destroy needs only the device, not the context
+    unsigned long long id = 0;   // unique per scene of the process (indexes built from it name it)
     tsr::u32 n = 0, maxCams = 1, camCount = 0;
     tsr::DBuf<float> px, py, pz, nx, ny, nz, scale, conf;
     tsr::DBuf<tsr::u32> cams;
@@ -82,6 +87,9 @@ struct tsr_tree {
     tsr::DHash nodeHash;               // location code -> node id (all nodes)
     tsr::u64 samplingEvents = 0, samplingCreated = 0, samplingUnits = 0;
     tsr::u64 samplingMisses = 0;   // visit-index lookups answered by the device (leaves outside the hot part)
+    // the emptiness pass's direction index over the scene's links, built during the tree build
+    // (null after an erase, for a loaded tree, or when it did not fit)
+    std::shared_ptr<tsr::EmptinessIndex> eIndex;
     // leaf face-neighbour graph (Leaves.cpp:82-180), CSR built on request or loaded
     bool hasNeighbours = false;
     tsr::DBuf<tsr::u32> nbrOffsets, nbrs;
@@ -173,10 +181,25 @@ void samplesFileRead(const char *path, float *px, float *py, float *pz, float *n
 void computeOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *scene, const tsr_params &prm,
                       int flags, const tsr_culler *culler, tsr_field *field);
 // E pass gathered by the sampleness-flagged leaves (hitsS[leaf] > 0): per-leaf kernel sums,
-// cone length sums and hit counts (L each, device), the confidence of every link
-// (0: no emptiness cone; device, links) and the total hit count
+// cone length sums and hit counts (L each, device), the confidence of every link in the
+// tree's order (0: no emptiness cone; device, links) and the total hit count.  Uses the
+// tree's direction index when it was built from this scene with this bandwidth.
 void emptinessByLeaves(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *scene, const tsr_params &prm,
                        const u32 *hitsS, float *ksum, float *clen, u32 *hitCount, double *confs, u64 &hitsOut);
+// the links an emptiness index is built over: samples (SoA, device) with their camera links
+// and container depths, in one order; cameras; the kernel bandwidth factor
+struct EmptinessLinks {
+    const float *px, *py, *pz, *nx, *ny, *nz, *scale, *conf;
+    const u32 *cams;
+    const float *cx, *cy, *cz;
+    const uint8_t *sampleDepth;
+    u32 samples, maxCams, camCount;
+    float bandwidth;
+    unsigned long long sceneId;
+};
+// direction index of the gathered E pass, built on stream st with scratch sc (the caller's
+// current block pool); null when its cell grid cannot fit its budget
+std::shared_ptr<EmptinessIndex> buildEmptinessIndex(hipStream_t st, Scratch &sc, const EmptinessLinks &src);
 // Occupancy::addSamples / eraseSamples: the field follows a change of samples (indices into
 // the tree's node-ordered samples, host array of n)
 void updateOccupancy(tsr_context *ctx, const tsr_tree *tree, const tsr_scene *scene, const u32 *chosenHost, u32 n,

@@ -620,6 +620,7 @@ __global__ void k_remapTargets(u32 *__restrict__ target, u32 m, const uint8_t *_
 // erased samples, and the tree's node-ordered sample arrays are compacted.
 void eraseTreeSamples(tsr_context *ctx, tsr_tree *tree, const uint8_t *inliersHost) {
     hipStream_t st = ctx->stream;
+    tree->eIndex.reset();   // built over the samples before the erase
     const u32 n = tree->sampleCount, N = tree->nodeCount, mc = tree->maxCams;
     if (n == 0) return;
     DBuf<uint8_t> inl(n);

@@ -1646,11 +1646,49 @@ void buildTree(tsr_context *ctx, const tsr_scene *scene, const tsr_params &prm, 
         HIPCHECK(hipStreamSynchronize(st));
     };
     mark("sampling speculation");
+    // The emptiness pass's direction index needs only the samples, their container depths and
+    // the cameras: a worker thread builds it on the context's second stream while this thread
+    // replays the sampling refinement (the GPU is otherwise idle then).  Its temporaries come
+    // from a block pool of its own, so no block is shared between the two streams.
+    struct EventGuard {   // destroyed after the index future below (declared before it)
+        hipEvent_t e = nullptr;
+        ~EventGuard() {
+            if (e) (void)hipEventDestroy(e);
+        }
+    } readyGuard;
+    HIPCHECK(hipEventCreateWithFlags(&readyGuard.e, hipEventDisableTiming));
+    const hipEvent_t samplesReady = readyGuard.e;
+    HIPCHECK(hipEventRecord(samplesReady, st));
+    const EmptinessLinks inputLinks{scene->px.p, scene->py.p, scene->pz.p, scene->nx.p, scene->ny.p, scene->nz.p,
+                                    scene->scale.p, scene->conf.p, scene->cams.p, scene->cx.p, scene->cy.p,
+                                    scene->cz.p, cdepth.p, n, scene->maxCams, scene->camCount,
+                                    prm.bandwidth_factor, scene->id};
+    std::future<std::shared_ptr<EmptinessIndex>> indexFuture =
+        std::async(std::launch::async, [ctx, inputLinks, samplesReady]() -> std::shared_ptr<EmptinessIndex> {
+            HIPCHECK(hipSetDevice(ctx->device));
+            auto pool = std::make_shared<DevicePool>(ctx->device);
+            PoolScope scope(pool);
+            HIPCHECK(hipStreamWaitEvent(ctx->stream2, samplesReady, 0));
+            std::shared_ptr<EmptinessIndex> ix;
+            {
+                Scratch sc2;
+                try {
+                    ix = buildEmptinessIndex(ctx->stream2, sc2, inputLinks);
+                } catch (const tsr_error &e) {
+                    if (e.status != TSR_ERR_OUT_OF_MEMORY) throw;
+                    ix.reset();   // the occupancy builds its own
+                }
+                HIPCHECK(hipStreamSynchronize(ctx->stream2));
+            }
+            pool->trim();   // the temporaries' blocks (their work is done)
+            return ix;
+        });
     std::vector<u64> created;
     u64 events = 0;
     ReplaySets sets = setsFuture.get();
     mark("replay sets (host, overlapped)");
     replaySampling(f, scene, maxRel, sets, hEventBits, units, index, created, events);
+    tree->eIndex = indexFuture.get();
     if (trace)
         std::fprintf(stderr, "[tsr tree] units %u pairs %zu hot pairs %zu t0-inner %u events %llu created nodes %zu "
                      "index misses %llu\n", units, pairs, hotPairs, t0Count, (unsigned long long)events,
