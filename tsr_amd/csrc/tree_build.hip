@@ -643,6 +643,12 @@ TSR_HD void swKey(u32 vx, u32 vy, u32 vz, u64 &hi, u64 &lo) {
 __device__ u32 leafOfCell(HashView inner, HashView nodes, const u32 *__restrict__ nodeToLeaf, u32 cx, u32 cy, u32 cz,
                           int &guess) {
     const u64 finest = makeLoc(cx, cy, cz, MAX_DEPTH);
+    if (guess >= 0 && guess < (int)MAX_DEPTH) {
+        // the node at the guessed leaf depth: when it exists and is a leaf, it is the one
+        // leaf holding the point
+        const u32 id = nodes.find(finest >> (3 * (MAX_DEPTH - (u32)(guess + 1))));
+        if (id != INVALID && nodeToLeaf[id] != INVALID) return nodeToLeaf[id];
+    }
     auto innerAt = [&](int m) { return m < (int)MAX_DEPTH && inner.contains(finest >> (3 * (MAX_DEPTH - m))); };
     // largest m with the depth-m ancestor inner (monotone in m)
     int lo = 0, hi = (int)MAX_DEPTH;   // inner(lo) true (root), inner(hi) false
