@@ -259,9 +259,14 @@ TSR_HD double coneKernel(const ConeD &c, const double x[3], bool sampleness, dou
 // ObliqueCircularCone::haveSeparatingAxis, ConeNodesChecker.cpp:65): the cut cone
 // lies inside the union of balls B(a + v t, R t), t in [tStart, 1]; a box is
 // culled only if its bounding ball misses that union by a safety margin.
+// Besides, a point with a non-zero kernel has tComplete in [tStart, 1), so dot(x - a, n)
+// lies in the slab [tStart D, D] with D = dot(e - a, n) > 0: a cube whose projection on n
+// (centre +- half (|n_x| + |n_y| + |n_z|)) misses the slab holds no such point.  For a
+// sampleness cone the slab is a few sample scales thick around the sample's tangent plane.
 struct ConeCull {
     float a[3], vhat[3];
     float A, R, tStart, slope;   // slope = R / sqrt(A^2 - R^2), < 0 when R >= A
+    float n[3], slabLo, slabHi;  // end normal and the slab of dot(x - a, n)
 };
 
 TSR_HD void cullInit(ConeCull &k, const ConeD &c) {
@@ -269,11 +274,15 @@ TSR_HD void cullInit(ConeCull &k, const ConeD &c) {
     for (int i = 0; i < 3; ++i) {
         k.a[i] = (float)c.a[i];
         k.vhat[i] = (float)(c.v[i] / A);
+        k.n[i] = (float)c.n[i];
     }
     k.A = (float)A;
     k.R = (float)c.R;
     k.tStart = (float)c.tStart;
     k.slope = c.R < A ? (float)(c.R / sqrt(A * A - c.R * c.R)) : -1.0f;
+    const double D = c.v[0] * c.n[0] + c.v[1] * c.n[1] + c.v[2] * c.n[2];
+    k.slabLo = (float)(c.tStart * D);
+    k.slabHi = (float)D;
 }
 
 // true if the cube [lo, lo + size]^3 cannot meet the cone.  The square roots and the
@@ -284,6 +293,12 @@ __device__ __forceinline__ bool cullSeparated(const ConeCull &k, float lx, float
                                               float absMargin) {
     const float half = 0.5f * size;
     const float b0 = lx + half - k.a[0], b1 = ly + half - k.a[1], b2 = lz + half - k.a[2];
+    // the slab along the end normal (margin: absolute, plus the FP32 rounding of the
+    // projections, relative 1e-6)
+    const float cn = b0 * k.n[0] + b1 * k.n[1] + b2 * k.n[2];
+    const float ext = half * (fabsf(k.n[0]) + fabsf(k.n[1]) + fabsf(k.n[2]));
+    const float sm = absMargin + 1e-6f * (fabsf(cn) + ext + fabsf(k.slabHi));
+    if (cn + ext < k.slabLo - sm || cn - ext > k.slabHi + sm) return true;
     const float u = b0 * k.vhat[0] + b1 * k.vhat[1] + b2 * k.vhat[2];
     const float p0 = b0 - u * k.vhat[0], p1 = b1 - u * k.vhat[1], p2 = b2 - u * k.vhat[2];
     const float w2 = p0 * p0 + p1 * p1 + p2 * p2;
