@@ -1199,6 +1199,23 @@ void priorRecords(tsr_context *ctx, const tsr_tree *tree, const tsr_field *field
 namespace {
 
 // leaves with flag 1, in leaf order, into list; returns their count
+__global__ void k_queryDepth(const u32 *__restrict__ q, u32 n, const u64 *__restrict__ leafLoc, u32 *__restrict__ key) {
+    const u32 i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) key[i] = locDepth(leafLoc[q[i]]);
+}
+
+// The queries of a priors pass grouped by depth (stable: leaf order within a depth), so that a
+// group of consecutive queries shares one sphere radius and their stack entries' query masks
+// are denser (priors 531 -> 518 ms at 36.5M samples).  The prior of each leaf does not depend
+// on the order.
+void queriesByDepth(Scratch &sc, const tsr_tree *tree, u32 *queries, u32 n, hipStream_t st) {
+    if (n < 2) return;
+    DBuf<u32> keys(n), keysS(n), sorted(n);
+    k_queryDepth<<<gridFor(n, BLOCK), BLOCK, 0, st>>>(queries, n, tree->leafLoc.p, keys.p);
+    sortPairs(sc, keys.p, keysS.p, queries, sorted.p, n, 0, 5, st);
+    HIPCHECK(hipMemcpyAsync(queries, sorted.p, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
+}
+
 u32 flaggedLeaves(Scratch &sc, const u32 *flag, u32 L, u32 *list, hipStream_t st) {
     if (L == 0) return 0;
     DBuf<u32> pos(L);
@@ -1207,7 +1224,7 @@ u32 flaggedLeaves(Scratch &sc, const u32 *flag, u32 L, u32 *list, hipStream_t st
     return readScalar(pos.p + L - 1, st) + readScalar(flag + L - 1, st);
 }
 
-// One priors traversal over `queries` (nq leaves in leaf order): the short-stack launch,
+// One priors traversal over `queries` (nq leaves, grouped by depth): the short-stack launch,
 // then the groups that overflowed it with the long stack.  capLevels 0: exact priors into
 // prior[leaf]; otherwise depth-capped intervals into interval[query index].
 void priorsPass(tsr_context *ctx, const tsr_tree *tree, const PriorRecords &rec, const u32 *queries, u32 nq,
@@ -1291,6 +1308,7 @@ void priors(tsr_context *ctx, const tsr_tree *tree, const tsr_params &prm, int f
     field->priorStages[0] = nq;
     const float rr = prm.relative_radius_for_priors;
     if (!certified) {
+        queriesByDepth(sc, tree, queries.p, nq, st);
         priorsPass(ctx, tree, rec, queries.p, nq, rr, 0, field->prior.p, nullptr, field);
         field->priorStages[3] = nq;
         return;
@@ -1314,6 +1332,7 @@ void priors(tsr_context *ctx, const tsr_tree *tree, const tsr_params &prm, int f
         k_stageFlags<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(stage.p, L, (uint8_t)s, qflag.p);
         const u32 n = flaggedLeaves(sc, qflag.p, L, queries.p, st);
         field->priorStages[1 + s] = n;
+        queriesByDepth(sc, tree, queries.p, n, st);
         if (s == 2) {
             priorsPass(ctx, tree, rec, queries.p, n, rr, 0, field->prior.p, nullptr, field);
             k_setLevel<<<gridFor(std::max(n, 1u), BLOCK), BLOCK, 0, st>>>(queries.p, n, 2, field->levels.p);
