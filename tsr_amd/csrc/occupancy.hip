@@ -802,7 +802,7 @@ __global__ void k_packPriorLinks(const u32 *__restrict__ child, const u32 *__res
 // The priors traversal keeps an LDS stack of inner nodes still to open.  The first
 // pass uses a short stack; a group that would overflow it is abandoned and listed,
 // and a second pass redoes the listed groups with a stack of the proven bound.  The
-// short stack (14 B per entry) leaves room in the 160 KB of LDS per CU for 7 waves per
+// short stack (14 B per entry) leaves room in the 160 KB of LDS per CU for 8 waves per
 // SIMD of one-wave blocks.
 constexpr u32 PRIOR_STACK_SHORT = 320;
 constexpr u32 XCDS = 8;   // MI355X: blocks are dispatched round-robin over 8 XCDs, each with its own L2
@@ -827,8 +827,7 @@ constexpr u32 PRIOR_STACK = 1344;   // >= 64 * (MAX_DEPTH + 1): one step pushes 
 // traversal sums is a sub-sum of the child's subtree sums (all terms are >= 0), so those
 // go to upper-bound accumulators (Eu, Su) instead, and the query gets the interval
 // [E / (E + S + Su), (E + Eu) / (E + Eu + S)] (interval[query index]), which holds the exact
-// prior.  The capped form serves 4 queries per wave, so that its four FP64 sums per query
-// take the registers of the exact form's two.
+// prior.  The capped form keeps four FP64 sums per query, the exact form two.
 #define PRIORS_GROUP_PARAMS                                                                                \
     Frame f, const uint2 *__restrict__ links, const double2 *__restrict__ sums, const u64 *__restrict__ leafLoc, \
         const u32 *__restrict__ queries, u32 queryCount, float relRadius, u32 capLevels,                       \
@@ -1079,15 +1078,21 @@ __device__ __forceinline__ void priorsGroup(PRIORS_GROUP_PARAMS) {
     }
 }
 
-// register use capped for 7 waves per SIMD (72 VGPRs; a little spill of set-up values and
-// one query's sums): the step's memory and LDS round trips are hidden better than with 5
-// or 6 waves; 8 waves spill more and run slower.  The long-stack instantiation is held to
-// few waves by its LDS stack anyway and is not capped.
+// register use capped for 8 waves per SIMD (64 VGPRs) in the exact form: the step's memory
+// and LDS round trips are hidden better than with fewer waves (7: 488 ms, 8: 474 ms on 36.5M
+// samples); 9 waves spill and run slower.  The capped form, with twice the FP64 sums, is
+// capped at 7 waves (72 VGPRs; at 8 its spill takes the certified priors 207 -> 254 ms).
+// The long-stack instantiation is held to few waves by its LDS stack anyway and is not capped.
 template <u32 GROUP_Q, u32 STACK, bool CAPPED>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(STACK == PRIOR_STACK_SHORT ? 7 : 1))) void k_priorsGroup(PRIORS_GROUP_PARAMS) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(STACK == PRIOR_STACK_SHORT ? (CAPPED ? 7 : 8) : 1))) void k_priorsGroup(PRIORS_GROUP_PARAMS) {
     priorsGroup<GROUP_Q, STACK, CAPPED>(PRIORS_GROUP_ARGS);
 }
-constexpr u32 PRIOR_GROUP = 8, PRIOR_GROUP_CAPPED = 4;   // queries per wave, exact / capped
+// Queries per wave, exact / capped.  Each step runs the per-query loop once per query of
+// the group whose bit some popped entry carries, while a child lane is busy only for the
+// queries of its own entry (about 3 of 8): 4 queries per group run fewer loop iterations in
+// all than 8 (6.7G against 8.1G on 36.5M samples) although they pop more entries (13.4G
+// against 8.1G): 474 against 516 ms; 2 queries per group pop 22.3G entries (500 ms).
+constexpr u32 PRIOR_GROUP = 4, PRIOR_GROUP_CAPPED = 4;
 
 // Occupancy.cpp:570-587, FP64
 __global__ void k_occupancy(const float *__restrict__ ksum, const double *__restrict__ prior, u32 leafCount,
@@ -1198,7 +1203,6 @@ void priorRecords(tsr_context *ctx, const tsr_tree *tree, const tsr_field *field
 
 namespace {
 
-// leaves with flag 1, in leaf order, into list; returns their count
 __global__ void k_queryDepth(const u32 *__restrict__ q, u32 n, const u64 *__restrict__ leafLoc, u32 *__restrict__ key) {
     const u32 i = blockIdx.x * BLOCK + threadIdx.x;
     if (i < n) key[i] = locDepth(leafLoc[q[i]]);
@@ -1216,6 +1220,7 @@ void queriesByDepth(Scratch &sc, const tsr_tree *tree, u32 *queries, u32 n, hipS
     HIPCHECK(hipMemcpyAsync(queries, sorted.p, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
 }
 
+// leaves with flag 1, in leaf order, into list; returns their count
 u32 flaggedLeaves(Scratch &sc, const u32 *flag, u32 L, u32 *list, hipStream_t st) {
     if (L == 0) return 0;
     DBuf<u32> pos(L);
