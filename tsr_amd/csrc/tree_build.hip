@@ -925,6 +925,89 @@ ReplaySets buildReplaySets(const u64 *t0Inner, size_t n) {
 
 unsigned hostThreads();   // worker threads of parallelFor (defined with its pool below)
 
+// Outcome of the refinement of one unit whose reliable-leaf search failed.
+enum class Refine { DONE, TOO_DEEP, ZERO_SIZE, LEFT_NODE };
+
+// Nodes.cpp:284-336: subdivide along the child with the largest overlap of the segment
+// [p, p +- n msd] until a reliable leaf exists.  `create(n)` turns leaf n into an inner node
+// (isInner(n) answers true afterwards).  The descent from the root passes, without creating
+// anything, through the inner nodes that strictly contain the segment (in each, the one
+// child holding the whole segment has overlap 1 and every other child misses it), so it
+// starts at the deepest of them.  The result is a function of the isInner answers alone.
+template <typename IsInner, typename Create>
+Refine refineUnit(const Frame &f, const SamplingUnit &u, bool positive, IsInner isInner, Create create) {
+    const double sgn = positive ? u.msd : -u.msd;
+    const double en[3] = {u.p[0] + u.n[0] * sgn, u.p[1] + u.n[1] * sgn, u.p[2] + u.n[2] * sgn};
+    double s0[3], s1[3];
+    for (int a = 0; a < 3; ++a) {
+        const double m = 1e-9 * std::fabs(en[a] - u.p[a]) + 1e-12 * f.rootSize;
+        s0[a] = std::min(u.p[a], en[a]) - m;
+        s1[a] = std::max(u.p[a], en[a]) + m;
+    }
+    u64 n = deepestContaining(f, s0, s1, true, isInner);
+    auto noLeaf = [](u64) {};
+    for (;;) {
+        if (!isInner(n)) {
+            if (locDepth(n) + 1 > MAX_DEPTH) return Refine::TOO_DEEP;
+            create(n);
+            if (findReliableHost(f, n, positive, u, isInner, noLeaf)) return Refine::DONE;
+        }
+        const u32 d = locDepth(n) + 1;
+        if (f.h(d) == 0.0) return Refine::ZERO_SIZE;
+        double score = -1.0e300;
+        u32 best = INVALID;
+        // segmentBox for the 8 children: per axis the children's bounds are the three
+        // planes B(2k), B(2k+1), B(2k+2) at depth d, so their slab parameters
+        // (plane - p) / dir are formed once (the same quotients segmentBox forms per child)
+        u32 kp[3];
+        locCoords(n, kp[0], kp[1], kp[2]);
+        double plane[3][3], q[3][3], dir[3];
+        for (int a = 0; a < 3; ++a) {
+            dir[a] = en[a] - u.p[a];
+            for (u32 j = 0; j < 3; ++j) {
+                plane[a][j] = f.B(a, 2 * kp[a] + j, d);
+                if (dir[a] != 0.0) q[a][j] = (plane[a][j] - u.p[a]) / dir[a];
+            }
+        }
+        auto childSegment = [&](u32 c, double &tIn, double &tOut) {
+            tIn = 0.0;
+            tOut = 1.0;
+            for (int a = 0; a < 3; ++a) {
+                const u32 b = (c >> a) & 1u;
+                if (dir[a] == 0.0) {
+                    if (u.p[a] < plane[a][b] || u.p[a] > plane[a][b + 1]) return false;
+                    continue;
+                }
+                double t1 = q[a][b], t2 = q[a][b + 1];
+                if (t1 > t2) { const double tmp = t1; t1 = t2; t2 = tmp; }
+                if (t1 > tIn) tIn = t1;
+                if (t2 < tOut) tOut = t2;
+            }
+            return tIn <= tOut;
+        };
+        for (u32 c = 0; c < 8; ++c) {
+            double tIn, tOut;
+            if (!childSegment(c, tIn, tOut)) continue;
+            const double overlap = tOut - tIn;
+            if (overlap <= score) continue;
+            score = overlap;
+            best = c;
+            if (overlap >= 1.0 - EPS_D) break;
+        }
+        if (best == INVALID) return Refine::LEFT_NODE;
+        n = (n << 3) | best;
+    }
+}
+
+void throwRefine(Refine r) {
+    switch (r) {
+    case Refine::TOO_DEEP: throw tsr_error(TSR_ERR_TREE_TOO_DEEP, "octree deeper than 19 levels");
+    case Refine::ZERO_SIZE: throw tsr_error(TSR_ERR_ZERO_SIZE_NODE, "Reached a tree node of size 0");
+    case Refine::LEFT_NODE: throw tsr_error(TSR_ERR_INVALID_ARGUMENT, "sampling refinement: segment leaves the node");
+    case Refine::DONE: break;
+    }
+}
+
 void replaySampling(const Frame &f, const tsr_scene *scene, float maxRel, ReplaySets &sets,
                     u64 *queued, u32 units, VisitIndex &index, std::vector<u64> &created,
                     u64 &eventCount) {
@@ -969,13 +1052,14 @@ void replaySampling(const Frame &f, const tsr_scene *scene, float maxRel, Replay
     double taintMs = 0, fullMs = 0, specMs = 0, createMs = 0;
     using clock = std::chrono::steady_clock;
     auto msSince = [](clock::time_point t) { return std::chrono::duration<double, std::milli>(clock::now() - t).count(); };
-    auto taint = [&](u64 subdivided, u32 writer) {
-        const auto tt = clock::now();
-        // the T0 leaf containing the subdivided node
+    // the T0 leaf containing a subdivided node
+    auto t0LeafOf = [&](u64 subdivided) {
         u64 l = subdivided;
         while (l > 1 && !t0.contains(l >> 3)) l >>= 3;
-        const u32 *first, *last;
-        index.units(l, first, last, tmp);
+        return l;
+    };
+    // the units of [first, last) after `writer` become pending
+    auto taintRange = [&](const u32 *first, const u32 *last, u32 writer) {
         const u32 *from = std::upper_bound(first, last, writer);
         candidates += (u64)(last - from);
         for (const u32 *it = from; it != last; ++it) {
@@ -986,6 +1070,12 @@ void replaySampling(const Frame &f, const tsr_scene *scene, float maxRel, Replay
             word |= bit;
             enqueue(*it);
         }
+    };
+    auto taint = [&](u64 subdivided, u32 writer) {
+        const auto tt = clock::now();
+        const u32 *first, *last;
+        index.units(t0LeafOf(subdivided), first, last, tmp);
+        taintRange(first, last, writer);
         taintMs += msSince(tt);
     };
     eventCount = 0;
@@ -993,6 +1083,26 @@ void replaySampling(const Frame &f, const tsr_scene *scene, float maxRel, Replay
     FlatSet createdSince;
     std::vector<u64> createdList;
     createdSince.init(1u << 10);   // a batch's creations; grown when needed
+    // leaf n becomes an inner node of the evolving tree
+    auto createNode = [&](u64 n) {
+        if (inner.capacity() < 2 * (inner.count + 16)) {   // keep the load factor <= 1/2
+            InnerSet grown;
+            grown.init(2 * inner.count + 1024);
+            for (const FlatMap::Slot &sl : inner.map.slots)
+                if (sl.key) grown.map.put(sl.key, sl.val);
+            grown.count = inner.count;
+            inner = std::move(grown);
+        }
+        inner.insert(n);
+        created.push_back(n);
+        if (createdList.size() * 2 + 2 > createdSince.keys.size()) {
+            FlatSet grown;
+            grown.init(createdList.size() * 2 + 16);
+            for (u64 k : createdList) grown.insert(k);
+            createdSince = std::move(grown);
+        }
+        if (createdSince.insert(n)) createdList.push_back(n);
+    };
     // the sequential semantics: evaluate `unit` on the current tree, refine if needed
     auto processFull = [&](u32 unit, bool knownToFail) {
         ++eventCount;
@@ -1013,110 +1123,47 @@ void replaySampling(const Frame &f, const tsr_scene *scene, float maxRel, Replay
             clock::time_point t;
             ~AccC() { ms += std::chrono::duration<double, std::milli>(clock::now() - t).count(); }
         } accC{createMs, tc};
-        // Nodes.cpp:284-336.  The descent from the root passes, without creating anything,
-        // through the inner nodes that strictly contain the segment [p, p +- n msd] (in
-        // each, the one child holding the whole segment has overlap 1 and every other
-        // child misses it), so it starts at the deepest of them.
-        const double sgn = positive ? u.msd : -u.msd;
-        const double en[3] = {u.p[0] + u.n[0] * sgn, u.p[1] + u.n[1] * sgn, u.p[2] + u.n[2] * sgn};
-        double s0[3], s1[3];
-        for (int a = 0; a < 3; ++a) {
-            const double m = 1e-9 * std::fabs(en[a] - u.p[a]) + 1e-12 * f.rootSize;
-            s0[a] = std::min(u.p[a], en[a]) - m;
-            s1[a] = std::max(u.p[a], en[a]) + m;
-        }
-        u64 n = deepestContaining(f, s0, s1, true, isInner);
         // the nodes one descent creates form a chain below the first, so they all lie in
         // the same containment-tree leaf: after the first taint the others add no unit
         bool taintDone = false;
-        for (;;) {
-            if (!inner.contains(n)) {
-                if (locDepth(n) + 1 > MAX_DEPTH) throw tsr_error(TSR_ERR_TREE_TOO_DEEP, "octree deeper than 19 levels");
-                if (inner.capacity() < 2 * (inner.count + 16)) {   // keep the load factor <= 1/2
-                    InnerSet grown;
-                    grown.init(2 * inner.count + 1024);
-                    for (const FlatMap::Slot &sl : inner.map.slots)
-                        if (sl.key) grown.map.put(sl.key, sl.val);
-                    grown.count = inner.count;
-                    inner = std::move(grown);
-                }
-                inner.insert(n);
-                created.push_back(n);
-                if (createdList.size() * 2 + 2 > createdSince.keys.size()) {
-                    FlatSet grown;
-                    grown.init(createdList.size() * 2 + 16);
-                    for (u64 k : createdList) grown.insert(k);
-                    createdSince = std::move(grown);
-                }
-                if (createdSince.insert(n)) createdList.push_back(n);
-                if (!taintDone) {
-                    taint(n, unit);
-                    taintDone = true;
-                }
-                if (findReliableHost(f, n, positive, u, isInner, noLeaf)) break;
+        throwRefine(refineUnit(f, u, positive, isInner, [&](u64 n) {
+            createNode(n);
+            if (!taintDone) {
+                taint(n, unit);
+                taintDone = true;
             }
-            const u32 d = locDepth(n) + 1;
-            if (f.h(d) == 0.0) throw tsr_error(TSR_ERR_ZERO_SIZE_NODE, "Reached a tree node of size 0");
-            double score = -1.0e300;
-            u32 best = INVALID;
-            // segmentBox for the 8 children: per axis the children's bounds are the three
-            // planes B(2k), B(2k+1), B(2k+2) at depth d, so their slab parameters
-            // (plane - p) / dir are formed once (the same quotients segmentBox forms per child)
-            u32 kp[3];
-            locCoords(n, kp[0], kp[1], kp[2]);
-            double plane[3][3], q[3][3], dir[3];
-            for (int a = 0; a < 3; ++a) {
-                dir[a] = en[a] - u.p[a];
-                for (u32 j = 0; j < 3; ++j) {
-                    plane[a][j] = f.B(a, 2 * kp[a] + j, d);
-                    if (dir[a] != 0.0) q[a][j] = (plane[a][j] - u.p[a]) / dir[a];
-                }
-            }
-            auto childSegment = [&](u32 c, double &tIn, double &tOut) {
-                tIn = 0.0;
-                tOut = 1.0;
-                for (int a = 0; a < 3; ++a) {
-                    const u32 b = (c >> a) & 1u;
-                    if (dir[a] == 0.0) {
-                        if (u.p[a] < plane[a][b] || u.p[a] > plane[a][b + 1]) return false;
-                        continue;
-                    }
-                    double t1 = q[a][b], t2 = q[a][b + 1];
-                    if (t1 > t2) { const double tmp = t1; t1 = t2; t2 = tmp; }
-                    if (t1 > tIn) tIn = t1;
-                    if (t2 < tOut) tOut = t2;
-                }
-                return tIn <= tOut;
-            };
-            for (u32 c = 0; c < 8; ++c) {
-                double tIn, tOut;
-                if (!childSegment(c, tIn, tOut)) continue;
-                const double overlap = tOut - tIn;
-                if (overlap <= score) continue;
-                score = overlap;
-                best = c;
-                if (overlap >= 1.0 - EPS_D) break;
-            }
-            if (best == INVALID) throw tsr_error(TSR_ERR_INVALID_ARGUMENT, "sampling refinement: segment leaves the node");
-            n = (n << 3) | best;
-        }
+        }));
     };
     // Batches of the next pending units (in order) are first evaluated in parallel on
-    // the tree as it is when the batch starts, recording the leaves each DFS visits.
-    // Committing in order, a unit whose parallel result was "reliable leaf found" and
-    // none of whose visited leaves has been subdivided since is settled: a DFS only
-    // changes when a leaf it visits becomes an inner node.  Every other unit, and
-    // units made pending meanwhile that precede the rest of the batch, run the
-    // sequential step above.
+    // the tree as it is when the batch starts: the reliable-leaf search, recording the
+    // leaves it visits, and for a unit whose search fails, the refinement too, with the
+    // nodes it creates kept in a per-unit overlay and every "not inner" answer of the
+    // batch's tree recorded.  Committing in order, a unit none of whose recorded nodes has
+    // been made inner since is settled by its speculation: a search only changes when a
+    // leaf it visits becomes an inner node, and the refinement is a function of its
+    // inner-ness answers, of which only "not inner" ones can change.  A settled refinement
+    // inserts its overlay and taints as the sequential step would.  Every other unit, and
+    // units made pending meanwhile that precede the rest of the batch, run the sequential
+    // step above.
     constexpr u32 BATCH = 4096, SPEC_CAP = 48;
     struct Spec {
-        u32 state;   // 1 = reliable leaf found, 2 = none (refinement needed), 0 = visit cap exceeded
-        u32 count;
+        // 0 = visit cap exceeded, 1 = reliable leaf found, 2 = none and the refinement
+        // speculated, 3 = none and the refinement not speculated
+        u32 state;
+        u32 count;      // visited[0, count): the leaves the search visited
+        u32 total;      // visited[count, total): the refinement's "not inner" answers
+        u32 chainLen;   // the nodes the refinement creates, in order
         u64 visited[SPEC_CAP];
+        u64 chain[MAX_DEPTH + 1];
+        const u32 *taintFirst, *taintLast;   // the first creation's units, when in the hot index
+        bool taintKnown;
     };
     std::vector<u32> batch;
     std::vector<Spec> spec(BATCH);
-    u64 speculated = 0;
+    u64 speculated = 0, specRefined = 0;
+    // sequential steps by cause: made pending within the batch's range, search over the visit
+    // cap, search invalidated, refinement not speculated, refinement invalidated
+    u64 seqPending = 0, seqCap = 0, seqInvalid = 0, seqUnspeculated = 0, seqInvalidRefine = 0;
     const auto tLoop = std::chrono::steady_clock::now();
     u32 cursor = 0;   // every queued unit below it has run
     for (;;) {
@@ -1150,6 +1197,40 @@ void replaySampling(const Frame &f, const tsr_scene *scene, float maxRel, Replay
                 const bool positive = (batch[j] & 1) == 0;
                 const bool ok = findReliableHost(f, searchStart(f, su, inView), positive, su, inView, record);
                 sp.state = over ? 0u : ok ? 1u : 2u;
+                sp.total = sp.count;
+                if (sp.state != 2u) continue;
+                sp.chainLen = 0;
+                auto specInner = [&](u64 loc) {
+                    for (u32 k = 0; k < sp.chainLen; ++k)
+                        if (sp.chain[k] == loc) return true;
+                    if (sp.chainLen) {   // below the first created node nothing else is inner
+                        const u32 d0 = locDepth(sp.chain[0]), d = locDepth(loc);
+                        if (d > d0 && (loc >> (3 * (d - d0))) == sp.chain[0]) return false;
+                    }
+                    if (view.contains(loc)) return true;
+                    if (sp.total < SPEC_CAP) sp.visited[sp.total++] = loc;
+                    else over = true;
+                    return false;
+                };
+                const Refine r = refineUnit(f, su, positive, specInner, [&](u64 n) {
+                    if (sp.chainLen <= MAX_DEPTH) sp.chain[sp.chainLen++] = n;
+                    else over = true;
+                });
+                if (over || r != Refine::DONE || sp.chainLen == 0) {
+                    sp.state = 3u;
+                    continue;
+                }
+                // the first creation's taint target (a leaf outside the hot part of the
+                // index is looked up on the device at commit)
+                sp.taintKnown = false;
+                if (const RunSlot *rs = index.run.find(t0LeafOf(sp.chain[0]))) {
+                    sp.taintFirst = sp.taintLast = nullptr;
+                    if (rs->count) {
+                        sp.taintFirst = index.unit + rs->first;
+                        sp.taintLast = sp.taintFirst + rs->count;
+                    }
+                    sp.taintKnown = true;
+                }
             }
         }, 1);
         specMs += msSince(ts);
@@ -1157,12 +1238,24 @@ void replaySampling(const Frame &f, const tsr_scene *scene, float maxRel, Replay
             std::fill(createdSince.keys.begin(), createdSince.keys.end(), 0ull);
             createdList.clear();
         }
+        // the map slots a settled refinement's inserts touch, requested ahead of its commit
+        auto prefetchChain = [&](const Spec &sp) {
+            if (sp.state != 2u) return;
+            const FlatMap::Slot *slots = inner.map.slots.data();
+            const u64 mask = inner.map.mask;
+            __builtin_prefetch(&slots[hashMix(sp.chain[0] >> 3) & mask], 1);
+            for (u32 k = 0; k < sp.chainLen; ++k) __builtin_prefetch(&slots[hashMix(sp.chain[k]) & mask], 1);
+        };
+        constexpr size_t AHEAD = 8;
+        for (size_t j = 0; j < std::min(AHEAD, batch.size()); ++j) prefetchChain(spec[j]);
         u32 done = cursor;   // units below `done` have run
         for (size_t j = 0; j < batch.size(); ++j) {
             const u32 u = batch[j];
+            if (j + AHEAD < batch.size()) prefetchChain(spec[j + AHEAD]);
             for (;;) {   // made pending after the batch was formed, before u
                 const u32 v = takeFrom(done, u);
                 if (v == INVALID) break;
+                ++seqPending;
                 processFull(v, false);
                 done = v + 1;
             }
@@ -1176,20 +1269,46 @@ void replaySampling(const Frame &f, const tsr_scene *scene, float maxRel, Replay
                 ++speculated;
                 continue;
             }
-            processFull(u, valid);   // valid here: the DFS is known to find no reliable leaf
+            if (valid && sp.state == 2u) {
+                bool refined = true;
+                for (u32 i = sp.count; refined && i < sp.total && !createdList.empty(); ++i)
+                    if (createdSince.contains(sp.visited[i])) refined = false;
+                if (refined) {
+                    ++eventCount;
+                    ++creating;
+                    ++specRefined;
+                    for (u32 k = 0; k < sp.chainLen; ++k) createNode(sp.chain[k]);
+                    if (sp.taintKnown) {
+                        const auto tt = clock::now();
+                        if (sp.taintFirst) taintRange(sp.taintFirst, sp.taintLast, u);
+                        taintMs += msSince(tt);
+                    } else {
+                        taint(sp.chain[0], u);
+                    }
+                    continue;
+                }
+            }
+            ++(sp.state == 0u ? seqCap : !valid ? seqInvalid : sp.state == 3u ? seqUnspeculated : seqInvalidRefine);
+            processFull(u, valid);   // valid here: the search is known to find no reliable leaf
         }
         cursor = batch.back() + 1;   // units made pending from here on come after the batch
     }
     if (std::getenv("TSR_TRACE"))
         std::fprintf(stderr, "[tsr tree]   replay set-up %.2f ms, in-order loop %.2f ms; initial events %zu, "
-                     "creating %llu, tainted %llu, settled by parallel speculation %llu\n"
+                     "creating %llu (refinement settled by speculation %llu), tainted %llu, searches settled by speculation %llu\n"
                      "[tsr tree]   parallel speculation %.1f ms, sequential steps %.1f ms (refinement %.1f ms, "
                      "taints %.1f ms over %llu candidates)\n",
                      std::chrono::duration<double, std::milli>(tLoop - tInit).count(),
                      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tLoop).count(),
-                     initialEvents, (unsigned long long)creating, (unsigned long long)tainted,
+                     initialEvents, (unsigned long long)creating, (unsigned long long)specRefined,
+                     (unsigned long long)tainted,
                      (unsigned long long)speculated, specMs, fullMs, createMs, taintMs,
                      (unsigned long long)candidates);
+    if (std::getenv("TSR_TRACE"))
+        std::fprintf(stderr, "[tsr tree]   sequential steps: pending in the batch range %llu, over the visit cap %llu, "
+                     "search invalidated %llu, refinement not speculated %llu, refinement invalidated %llu\n",
+                     (unsigned long long)seqPending, (unsigned long long)seqCap, (unsigned long long)seqInvalid,
+                     (unsigned long long)seqUnspeculated, (unsigned long long)seqInvalidRefine);
 }
 
 // T0 leaves within HOT_RADIUS x max sampling distance of a unit the speculation could not
@@ -1777,18 +1896,10 @@ void buildTree(tsr_context *ctx, const tsr_scene *scene, const tsr_params &prm, 
     tree->leafNode.alloc(L);
     k_leafArrays<<<gridFor(L, BLOCK), BLOCK, 0, st>>>(leafIdsSorted.p, L, tree->nodeLoc.p, tree->child.p,
                                                       tree->nodeToLeaf.p, tree->leafLoc.p, tree->leafNode.p);
-    {
-        std::vector<u64> lastKey(1);
-        HIPCHECK(hipMemcpyAsync(lastKey.data(), leafKeysSorted.p, 8, hipMemcpyDeviceToHost, st));
-    }
-    // max depth
-    {
-        std::vector<uint8_t> hd(N);
-        HIPCHECK(hipMemcpyAsync(hd.data(), tree->nodeDepth.p, N, hipMemcpyDeviceToHost, st));
-        HIPCHECK(hipStreamSynchronize(st));
-        u32 md = 0;
-        for (uint8_t d : hd) md = std::max<u32>(md, d);
-        tree->maxDepth = md;
+    {   // max depth
+        DBuf<uint8_t> md(1);
+        reduceMax(sc, tree->nodeDepth.p, md.p, N, st);
+        tree->maxDepth = readScalar(md.p, st);
     }
 
     mark("node/leaf tables");
