@@ -318,7 +318,8 @@ __device__ void coneLink(const TraversalTree &t, const LinkInputs &in, u32 link,
 // round-robin over the 8 XCDs) and moves on to the next slice when its own is used
 // up.  Each XCD's L2 then serves cones of a few cameras towards one part of the
 // scene instead of the whole GPU's window of links.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_conePassPT(TraversalTree t, LinkInputs in, u32 linkCount, int sampleness,
+template <bool SAMPLENESS>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_conePassPT(TraversalTree t, LinkInputs in, u32 linkCount,
                                                     u32 *__restrict__ counts, u32 *__restrict__ stLeaf,
                                                     uint2 *__restrict__ stVal, double *__restrict__ confOut,
                                                     u32 *__restrict__ nextIdx, u32 slices,
@@ -329,7 +330,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     const u32 lane = threadIdx.x & 63;
     const u32 sliceLen = (linkCount + slices - 1) / slices;
     u32 cur = blockIdx.x % slices, tried = 0;   // wave-uniform
-    const u32 required = sampleness ? 0u : FLAG_S;
+    constexpr u32 required = SAMPLENESS ? 0u : FLAG_S;
     bool active = false, more = true;
     u32 idx = 0, link = 0, count = 0, maxDepth = 0, node = 0, d = 0, k0 = 0, k1 = 0, k2 = 0, start = 0;
     bool descend = true;
@@ -362,8 +363,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                 } else {
                     link = in.order ? in.order[idx] : in.linkBase + idx;
                     u32 i;
-                    const bool ok = setupLink(in, link, sampleness != 0, cone, i);
-                    if (confOut) confOut[link] = ok ? (double)in.conf[i] : 0.0;
+                    const bool ok = setupLink(in, link, SAMPLENESS, cone, i);
+                    if (!SAMPLENESS) confOut[link] = ok ? (double)in.conf[i] : 0.0;
                     if (!ok) {
                         counts[link] = 0;
                     } else {
@@ -404,7 +405,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                     const u32 k[3] = {k0, k1, k2};
                     double x[3];
                     for (int a = 0; a < 3; ++a) x[a] = t.f.B(a, 2ull * k[a] + 1, d + 1);
-                    const double kv = coneKernel(cone, x, sampleness != 0, conf);
+                    const double kv = coneKernel(cone, x, SAMPLENESS, conf);
                     if (kv != 0.0) {
                         if (count < in.slots) {
                             const size_t at = stagingIndex(idx, count, in.slots);
@@ -1453,9 +1454,12 @@ void conePassRecords(tsr_context *ctx, const tsr_tree *tree, TraversalTree &tt,
         // to pay)
         HIPCHECK(hipMemsetAsync(linkCursor.p, 0, 32, st));
         timedLaunch(ctx, sampleness ? KSTAT_S_COUNT : KSTAT_E_COUNT, [&] {
-            k_conePassPT<<<std::min<u32>(gridFor(n, BLOCK), CONE_PT_BLOCKS), BLOCK, 0, st>>>(
-                tt, ic, n, sampleness, counts.p, stLeaf, stVal, sampleness ? nullptr : confs, linkCursor.p,
-                XCD_COUNT, kst.p);
+            if (sampleness)
+                k_conePassPT<true><<<std::min<u32>(gridFor(n, BLOCK), CONE_PT_BLOCKS), BLOCK, 0, st>>>(
+                    tt, ic, n, counts.p, stLeaf, stVal, nullptr, linkCursor.p, XCD_COUNT, kst.p);
+            else
+                k_conePassPT<false><<<std::min<u32>(gridFor(n, BLOCK), CONE_PT_BLOCKS), BLOCK, 0, st>>>(
+                    tt, ic, n, counts.p, stLeaf, stVal, confs, linkCursor.p, XCD_COUNT, kst.p);
         });
         unsigned long long cs[2];
         readStats(2, cs);
