@@ -159,10 +159,7 @@ __global__ void k_insertAncestors(HashView h, const u64 *__restrict__ containerL
 // A leaf is reliable when its centre fits and so does one of its 8 children's
 // (fittingCentre at depth d + 1, with the two centre offsets per axis, their squares and
 // plane terms formed once: the same operations in the same order).
-TSR_HD bool reliableLeafAt(const Frame &f, u64 loc, bool positive, const SamplingUnit &u) {
-    u32 k[3];
-    locCoords(loc, k[0], k[1], k[2]);
-    const u32 d = locDepth(loc);
+TSR_HD bool reliableLeafAtK(const Frame &f, const u32 k[3], u32 d, bool positive, const SamplingUnit &u) {
     if (!fittingCentre(f, k[0], k[1], k[2], d, positive, u)) return false;
     double sq[3][2], pl[3][2];
     for (int a = 0; a < 3; ++a)
@@ -179,105 +176,147 @@ TSR_HD bool reliableLeafAt(const Frame &f, u64 loc, bool positive, const Samplin
     }
     return false;
 }
+TSR_HD bool reliableLeafAt(const Frame &f, u64 loc, bool positive, const SamplingUnit &u) {
+    u32 k[3];
+    locCoords(loc, k[0], k[1], k[2]);
+    return reliableLeafAtK(f, k, locDepth(loc), positive, u);
+}
 
+// The terms an opened node's 8 children share in intersectNode: per axis and child bit b,
+// over the child's two boundary planes, the extreme of (plane - p) n (the largest for the
+// positive side, the smallest for the negative one), and the squared distance from p to
+// the child's slab.  FP64 rounding is monotone, so the extremes added in intersectNode's
+// order give exactly the extreme of its eight computed corner sums: some corner is on the
+// wanted side iff that sum is.  The distance sum adds intersectNode's terms in its order
+// (a term it skips is 0.0 here, and adding 0.0 changes nothing).
+struct OpenTerms {
+    double ext[3][2], dist[3][2];
+};
+TSR_HD void openTerms(const Frame &f, const u32 k[3], u32 d, bool positive, const SamplingUnit &u, OpenTerms &o) {
+    for (int a = 0; a < 3; ++a) {
+        double pl[3], t[3];
+        for (u32 j = 0; j < 3; ++j) {
+            pl[j] = f.B(a, 2 * k[a] + j, d + 1);
+            t[j] = (pl[j] - u.p[a]) * u.n[a];
+        }
+        for (u32 b = 0; b < 2; ++b) {
+            o.ext[a][b] = positive ? fmax(t[b], t[b + 1]) : fmin(t[b], t[b + 1]);
+            const double lo = pl[b], hi = pl[b + 1];
+            o.dist[a][b] = u.p[a] < lo ? (lo - u.p[a]) * (lo - u.p[a])
+                                       : (u.p[a] > hi ? (u.p[a] - hi) * (u.p[a] - hi) : 0.0);
+        }
+    }
+}
+// intersectNode for child c of the opened node
+// (selects, not indexing by the runtime slot: that would put the terms in scratch memory)
+TSR_HD bool intersectChild(const OpenTerms &o, u32 c, bool positive, double msd2) {
+    const bool bx = c & 1u, by = (c >> 1) & 1u, bz = (c >> 2) & 1u;
+    const double v = ((bx ? o.ext[0][1] : o.ext[0][0]) + (by ? o.ext[1][1] : o.ext[1][0])) +
+                     (bz ? o.ext[2][1] : o.ext[2][0]);
+    if ((v > 0.0) != positive) return false;
+    const double d2 = ((bx ? o.dist[0][1] : o.dist[0][0]) + (by ? o.dist[1][1] : o.dist[1][0])) +
+                      (bz ? o.dist[2][1] : o.dist[2][0]);
+    return d2 <= msd2;
+}
+
+// Depth-first search of findReliableSamplingNode from `start` (Nodes.cpp:433-469): children
+// in slot order, inner ones opened as met, each leaf met reported to onLeaf; true at the
+// first reliable leaf.  The opened node is kept as grid coordinates and depth, with its
+// children's shared terms formed on opening (and again on returning to it).
 template <typename IsInner, typename OnLeaf>
 TSR_HD bool findReliable(const Frame &f, u64 start, bool positive, const SamplingUnit &u, IsInner isInner,
                          OnLeaf onLeaf) {
-    auto reliableLeaf = [&](u64 loc) { return reliableLeafAt(f, loc, positive, u); };
+    u32 k[3];
+    locCoords(start, k[0], k[1], k[2]);
+    u32 d = locDepth(start);
     if (!isInner(start)) {
         onLeaf(start);
-        return reliableLeaf(start);
+        return reliableLeafAtK(f, k, d, positive, u);
     }
+    const double msd2 = u.msd * u.msd;
     u64 loc = start;
+    OpenTerms o;
+    openTerms(f, k, d, positive, u, o);
     u32 next = 0;
     for (;;) {
         if (next == 8) {
             if (loc == start) return false;
             next = (u32)(loc & 7) + 1;
             loc >>= 3;
+            for (int a = 0; a < 3; ++a) k[a] >>= 1;
+            --d;
+            openTerms(f, k, d, positive, u, o);
             continue;
         }
-        const u64 ch = (loc << 3) | next;
-        u32 kx, ky, kz;
-        locCoords(ch, kx, ky, kz);
-        if (intersectNode(f, kx, ky, kz, locDepth(ch), positive, u)) {
+        const u32 c = next;
+        if (intersectChild(o, c, positive, msd2)) {
+            const u64 ch = (loc << 3) | c;
+            const u32 kc[3] = {2 * k[0] + (c & 1u), 2 * k[1] + ((c >> 1) & 1u), 2 * k[2] + (c >> 2)};
             if (isInner(ch)) {
                 loc = ch;
+                for (int a = 0; a < 3; ++a) k[a] = kc[a];
+                ++d;
+                openTerms(f, k, d, positive, u, o);
                 next = 0;
                 continue;
             }
             onLeaf(ch);
-            if (reliableLeaf(ch)) return true;
+            if (reliableLeafAtK(f, kc, d + 1, positive, u)) return true;
         }
         ++next;
     }
 }
 
 // Host form of findReliable for the in-order replay: the same depth-first search and the
-// same predicates, with each opened node's child planes B(2k + j) (j = 0, 1, 2 per axis)
-// and their plane-distance products formed once for its 8 children (intersectNode forms
-// them per child: the same operations in the same order) and kept on a stack of open nodes.
+// same predicates, with each open node's shared child terms kept on a stack of open nodes
+// (no recomputation on returning to a node).
 template <typename IsInner, typename OnLeaf>
 bool findReliableHost(const Frame &f, u64 start, bool positive, const SamplingUnit &u, IsInner isInner,
                       OnLeaf onLeaf) {
-    if (!isInner(start)) {
-        onLeaf(start);
-        return reliableLeafAt(f, start, positive, u);
-    }
     struct Open {
         u64 loc;
-        double plane[3][3], prod[3][3];
+        u32 k[3], d;
+        OpenTerms o;
     };
     Open stack[MAX_DEPTH + 2];
     int top = 0;
-    auto open = [&](u64 loc) {
-        Open &o = stack[top++];
-        o.loc = loc;
-        u32 k[3];
-        locCoords(loc, k[0], k[1], k[2]);
-        const u32 d = locDepth(loc) + 1;
-        for (int a = 0; a < 3; ++a)
-            for (u32 j = 0; j < 3; ++j) {
-                o.plane[a][j] = f.B(a, 2 * k[a] + j, d);
-                o.prod[a][j] = (o.plane[a][j] - u.p[a]) * u.n[a];
-            }
-    };
-    // intersectNode for child c of the open node o
-    auto intersectChild = [&](const Open &o, u32 c) {
-        const u32 b[3] = {c & 1u, (c >> 1) & 1u, c >> 2};
-        bool anyRight = false;
-        for (u32 q = 0; q < 8 && !anyRight; ++q) {
-            const double v = (o.prod[0][b[0] + (q & 1)] + o.prod[1][b[1] + ((q >> 1) & 1)]) + o.prod[2][b[2] + (q >> 2)];
-            if ((v > 0.0) == positive) anyRight = true;
+    {
+        Open &s0 = stack[0];
+        s0.loc = start;
+        locCoords(start, s0.k[0], s0.k[1], s0.k[2]);
+        s0.d = locDepth(start);
+        if (!isInner(start)) {
+            onLeaf(start);
+            return reliableLeafAtK(f, s0.k, s0.d, positive, u);
         }
-        if (!anyRight) return false;
-        double d2 = 0.0;
-        for (int a = 0; a < 3; ++a) {
-            const double lo = o.plane[a][b[a]], hi = o.plane[a][b[a] + 1];
-            if (u.p[a] < lo) d2 += (lo - u.p[a]) * (lo - u.p[a]);
-            else if (u.p[a] > hi) d2 += (u.p[a] - hi) * (u.p[a] - hi);
-        }
-        return d2 <= u.msd * u.msd;
-    };
-    open(start);
+        openTerms(f, s0.k, s0.d, positive, u, s0.o);
+        top = 1;
+    }
+    const double msd2 = u.msd * u.msd;
     u32 next = 0;
     for (;;) {
-        Open &o = stack[top - 1];
+        const Open &o = stack[top - 1];
         if (next == 8) {
             if (o.loc == start) return false;
             next = (u32)(o.loc & 7) + 1;
             --top;
             continue;
         }
-        const u64 ch = (o.loc << 3) | next;
-        if (intersectChild(o, next)) {
+        const u32 c = next;
+        if (intersectChild(o.o, c, positive, msd2)) {
+            const u64 ch = (o.loc << 3) | c;
+            const u32 kc[3] = {2 * o.k[0] + (c & 1u), 2 * o.k[1] + ((c >> 1) & 1u), 2 * o.k[2] + (c >> 2)};
             if (isInner(ch)) {
-                open(ch);
+                Open &n = stack[top++];
+                n.loc = ch;
+                n.k[0] = kc[0]; n.k[1] = kc[1]; n.k[2] = kc[2];
+                n.d = o.d + 1;
+                openTerms(f, n.k, n.d, positive, u, n.o);
                 next = 0;
                 continue;
             }
             onLeaf(ch);
-            if (reliableLeafAt(f, ch, positive, u)) return true;
+            if (reliableLeafAtK(f, kc, o.d + 1, positive, u)) return true;
         }
         ++next;
     }
