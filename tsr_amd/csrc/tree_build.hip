@@ -1699,6 +1699,7 @@ void buildTree(tsr_context *ctx, const tsr_scene *scene, const tsr_params &prm, 
     DBuf<uint8_t> isEvent(units);
     // one traversal per unit: visit counts, creating units, and the visits staged
     DBuf<u64> staged(((size_t)units + 255) / 256 * 256 * VISIT_CAP);
+    mark("speculation set-up");
     k_samplingEval<<<gridFor(units, BLOCK), BLOCK, 0, st>>>(f, inner.view(), scene->px.p, scene->py.p, scene->pz.p,
                                                             scene->nx.p, scene->ny.p, scene->nz.p, scene->scale.p,
                                                             maxRel, units, 2, visitCount.p, isEvent.p, nullptr,
