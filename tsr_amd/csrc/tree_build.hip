@@ -1400,24 +1400,31 @@ __global__ void k_markHot(Frame f, HashView inner, HashView hot, const float *__
 }
 
 // hot visits per unit (non-event units only) from the staged visits; mode 1 writes them
+// The visits of settled units to hot leaves: mode 0 counts them per unit and records which
+// of the unit's visits they are (bit j: visit j, VISIT_CAP <= 64); mode 1 writes them as
+// (leaf, unit) pairs at the unit's offset, reading only those visits.
+static_assert(VISIT_CAP <= 64, "hot-visit masks hold one bit per staged visit");
 __global__ void k_hotVisits(HashView hot, const u64 *__restrict__ staged, const u32 *__restrict__ visitCount,
                             const uint8_t *__restrict__ isEvent, u32 units, int mode, u32 *__restrict__ hotCount,
-                            const u32 *__restrict__ hotOff, u64 *__restrict__ outLeaf, u32 *__restrict__ outUnit) {
+                            u64 *__restrict__ hotMask, const u32 *__restrict__ hotOff, u64 *__restrict__ outLeaf,
+                            u32 *__restrict__ outUnit) {
     const u32 u = blockIdx.x * BLOCK + threadIdx.x;
     if (u >= units) return;
-    const u32 c = isEvent[u] ? 0u : visitCount[u];
-    u32 k = 0;
-    const size_t base = mode == 1 ? hotOff[u] : 0;
-    for (u32 j = 0; j < c; ++j) {
-        const u64 leaf = staged[visitSlot(u, j)];
-        if (!hot.contains(leaf)) continue;
-        if (mode == 1) {
-            outLeaf[base + k] = leaf;
-            outUnit[base + k] = u;
-        }
-        ++k;
+    if (mode == 0) {
+        const u32 c = isEvent[u] ? 0u : visitCount[u];
+        u64 m = 0;
+        for (u32 j = 0; j < c; ++j)
+            if (hot.contains(staged[visitSlot(u, j)])) m |= 1ull << j;
+        hotMask[u] = m;
+        hotCount[u] = (u32)__popcll(m);
+        return;
     }
-    if (mode == 0) hotCount[u] = k;
+    const size_t base = hotOff[u];
+    u32 k = 0;
+    for (u64 m = hotMask[u]; m; m &= m - 1, ++k) {
+        outLeaf[base + k] = staged[visitSlot(u, (u32)__ffsll((long long)m) - 1)];
+        outUnit[base + k] = u;
+    }
 }
 
 // the hot-leaf table as RunSlots: each occupied slot's key with its run of units (a run
@@ -1717,15 +1724,16 @@ void buildTree(tsr_context *ctx, const tsr_scene *scene, const tsr_params &prm, 
     DBuf<u32> hotUnitSorted;
     {   // hot (leaf, unit) pairs in unit order, then a stable sort by leaf code keeps units
         // ascending per leaf.  Visited leaves are T0 leaves, no deeper than the deepest container.
+        DBuf<u64> hotMask(units);
         k_hotVisits<<<gridFor(units, BLOCK), BLOCK, 0, st>>>(hot.view(), staged.p, visitCount.p, isEvent.p, units, 0,
-                                                             offsets.p, nullptr, nullptr, nullptr);
+                                                             offsets.p, hotMask.p, nullptr, nullptr, nullptr);
         DBuf<u32> hotOff(units);
         exclusiveScan(sc, offsets.p, hotOff.p, units, st);
         hotPairs = (size_t)readScalar(hotOff.p + units - 1, st) + readScalar(offsets.p + units - 1, st);
         DBuf<u64> hotLeaf(std::max<size_t>(hotPairs, 1));
         DBuf<u32> hotUnit(std::max<size_t>(hotPairs, 1));
         k_hotVisits<<<gridFor(units, BLOCK), BLOCK, 0, st>>>(hot.view(), staged.p, visitCount.p, isEvent.p, units, 1,
-                                                             nullptr, hotOff.p, hotLeaf.p, hotUnit.p);
+                                                             nullptr, hotMask.p, hotOff.p, hotLeaf.p, hotUnit.p);
         hotLeafSorted.alloc(std::max<size_t>(hotPairs, 1));
         hotUnitSorted.alloc(std::max<size_t>(hotPairs, 1));
         if (hotPairs) sortPairs(sc, hotLeaf.p, hotLeafSorted.p, hotUnit.p, hotUnitSorted.p, hotPairs, 0,
