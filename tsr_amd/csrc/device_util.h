@@ -15,6 +15,9 @@
 #include <limits>
 
 #include <atomic>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -180,12 +183,22 @@ struct DBuf {
         const size_t want = DevicePool::roundUp(sizeof(T) * count);
         p = (T *)pool->take(want, blockBytes);
         if (!p) {
+            // TSR_TRACE: fresh blocks of 1 GiB or more, and out-of-memory retries, on stderr
+            static const bool trace = std::getenv("TSR_TRACE") != nullptr;
+            const auto t0 = std::chrono::steady_clock::now();
             hipError_t e = trackedMalloc((void **)&p, want);
+            bool retried = false;
             if (e == hipErrorOutOfMemory) {   // give the cached blocks back and try once more
                 (void)hipGetLastError();
                 pool->trim();
                 e = trackedMalloc((void **)&p, want);
+                retried = true;
             }
+            if (trace && (want >= (size_t(1) << 30) || retried))
+                std::fprintf(stderr, "[tsr memory] fresh block %.2f GiB%s: %.2f ms (held %.1f GiB)\n",
+                             (double)want / (1u << 30), retried ? " after an out-of-memory trim" : "",
+                             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count(),
+                             (double)memCounter().current.load() / (1u << 30));
             if (e != hipSuccess) p = nullptr;
             HIPCHECK(e);
             blockBytes = want;

@@ -689,17 +689,33 @@ __device__ u32 leafOfCell(HashView inner, HashView nodes, const u32 *__restrict_
         if (id != INVALID && nodeToLeaf[id] != INVALID) return nodeToLeaf[id];
     }
     auto innerAt = [&](int m) { return m < (int)MAX_DEPTH && inner.contains(finest >> (3 * (MAX_DEPTH - m))); };
-    // largest m with the depth-m ancestor inner (monotone in m)
+    // largest m with the depth-m ancestor inner (monotone in m): bracketed by steps of 1, 2,
+    // 4, ... away from the guess (neighbouring leaves differ by a level or two), then bisected
     int lo = 0, hi = (int)MAX_DEPTH;   // inner(lo) true (root), inner(hi) false
     if (guess < 0) {
         if (!inner.contains(1ull)) return nodeToLeaf[0];
     } else if (innerAt(guess)) {
         lo = guess;
-        if (!innerAt(guess + 1)) hi = guess + 1;
-        else lo = guess + 1;
+        for (int step = 1;; step *= 2) {
+            const int m = lo + step;
+            if (m >= (int)MAX_DEPTH) break;
+            if (!innerAt(m)) {
+                hi = m;
+                break;
+            }
+            lo = m;
+        }
     } else {
         hi = guess;
-        if (guess > 0 && innerAt(guess - 1)) lo = guess - 1;
+        for (int step = 1;; step *= 2) {
+            const int m = hi - step;
+            if (m <= 0) break;   // the root is inner
+            if (innerAt(m)) {
+                lo = m;
+                break;
+            }
+            hi = m;
+        }
     }
     while (hi - lo > 1) {
         const int mid = (lo + hi) / 2;
