@@ -4,7 +4,8 @@
 * Reduced resolution (1/16 of the image side, ~142k samples): parity with the FP64
   oracle -- tree tables bit for bit, kernel sums within 1e-5 relative, occupancy
   within 1e-5 of (pS + pE) with no sign flip, the same mesh.
-* Full size: properties that do not need the oracle -- a rerun gives bit-identical
+* Full size (and the 9.2M-sample thin wedge + sphere scene at full size too): properties
+  that do not need the oracle -- a rerun gives bit-identical
   tables, field and mesh; node ids are in DFS child-block pre-order (Nodes.cpp:536-554);
   the container rule (Nodes.cpp:86-99) and 2:1 face balance
   (Nodes.cpp:339-395) on a sample of samples and leaves; every mesh edge is shared by
@@ -113,11 +114,13 @@ def _leaf_depth_at(q, d, children, depth):
     return depth[node]
 
 
-def test_config4_full_size_properties(gpu_ctx):
+@pytest.mark.parametrize("config", [4, 1])
+def test_full_size_properties(gpu_ctx, config):
+    """config 4 (36.5M samples) and the thin wedge + sphere scene (config 1, 9.2M samples)"""
     from tsr_amd import DualMarchingCells, Occupancy, Scene, Tree
-    samples, cameras = scene(4, 1.0)
+    samples, cameras = scene(config, 1.0)
     n = len(samples["px"])
-    print(f"\n[config4 full] samples={n}", flush=True)
+    print(f"\n[config{config} full] samples={n}", flush=True)
     sc = Scene(gpu_ctx, samples, cameras)
     digests = []
     keep = None
@@ -132,7 +135,7 @@ def test_config4_full_size_properties(gpu_ctx):
         digests.append((_digest(t["children"], t["cells"], t["sample_target"]),
                         _digest(f["kernel_sums"], f["cone_lengths"], f["occupancy"], f["node_states"]),
                         _digest(m["triangles"], m["positions"], m["edge_keys"])))
-        print(f"[config4 full] run {run}: nodes={tree.node_count} leaves={tree.leaf_count} "
+        print(f"[config{config} full] run {run}: nodes={tree.node_count} leaves={tree.leaf_count} "
               f"cells={tree.cell_count} triangles={mesh.triangle_count}", flush=True)
         if run == 0:
             keep = (t, m, tree.root_min.astype(np.float64), float(tree.root_size))
