@@ -110,33 +110,6 @@ TSR_HD bool fittingCentre(const Frame &f, u32 kx, u32 ky, u32 kz, u32 d, bool po
     return onPositive == positive;
 }
 
-// Nodes.cpp:490-517 intersect: a corner on the wanted side of the sample plane
-// and box/sphere overlap (squared distance <= msd^2).
-// The corner plane distances are planeDistance's sums ((x - px) nx + (y - py) ny) + (z - pz) nz
-// with the six per-axis products formed once (the same operations in the same order).
-TSR_HD bool intersectNode(const Frame &f, u32 kx, u32 ky, u32 kz, u32 d, bool positive, const SamplingUnit &u) {
-    const u32 k[3] = {kx, ky, kz};
-    double lo[3], hi[3], t[3][2];
-    for (int a = 0; a < 3; ++a) {
-        lo[a] = f.B(a, k[a], d);
-        hi[a] = f.B(a, k[a] + 1, d);
-        t[a][0] = (lo[a] - u.p[a]) * u.n[a];
-        t[a][1] = (hi[a] - u.p[a]) * u.n[a];
-    }
-    bool anyRight = false;
-    for (u32 c = 0; c < 8 && !anyRight; ++c) {
-        const bool inPositive = (t[0][c & 1] + t[1][(c >> 1) & 1]) + t[2][c >> 2] > 0.0;
-        if (inPositive == positive) anyRight = true;
-    }
-    if (!anyRight) return false;
-    double d2 = 0.0;
-    for (int a = 0; a < 3; ++a) {
-        if (u.p[a] < lo[a]) d2 += (lo[a] - u.p[a]) * (lo[a] - u.p[a]);
-        else if (u.p[a] > hi[a]) d2 += (u.p[a] - hi[a]) * (u.p[a] - hi[a]);
-    }
-    return d2 <= u.msd * u.msd;
-}
-
 // Slab clip of segment st->en against a box, parameters in [0,1]
 // (CollisionDetection::intersectAABBWithLineSegment stand-in, Nodes.cpp:319).
 TSR_HD bool segmentBox(double &tIn, double &tOut, const double st[3], const double en[3], const double lo[3],

@@ -176,19 +176,17 @@ TSR_HD bool reliableLeafAtK(const Frame &f, const u32 k[3], u32 d, bool positive
     }
     return false;
 }
-TSR_HD bool reliableLeafAt(const Frame &f, u64 loc, bool positive, const SamplingUnit &u) {
-    u32 k[3];
-    locCoords(loc, k[0], k[1], k[2]);
-    return reliableLeafAtK(f, k, locDepth(loc), positive, u);
-}
 
-// The terms an opened node's 8 children share in intersectNode: per axis and child bit b,
-// over the child's two boundary planes, the extreme of (plane - p) n (the largest for the
+// Nodes.cpp:490-517 intersect, per child box: some corner on the wanted side of the sample
+// plane (its plane distance ((x - px) nx + (y - py) ny) + (z - pz) nz, with x, y, z the
+// box's planes, > 0 for the positive side) and box/ball overlap (the squared distances to
+// the box's slabs, summed over the axes in order, <= msd^2).
+// The terms an opened node's 8 children share in that test: per axis and child bit b, over
+// the child's two boundary planes, the extreme of (plane - p) n (the largest for the
 // positive side, the smallest for the negative one), and the squared distance from p to
-// the child's slab.  FP64 rounding is monotone, so the extremes added in intersectNode's
-// order give exactly the extreme of its eight computed corner sums: some corner is on the
-// wanted side iff that sum is.  The distance sum adds intersectNode's terms in its order
-// (a term it skips is 0.0 here, and adding 0.0 changes nothing).
+// the child's slab.  FP64 rounding is monotone, so the extremes added in the corner sums'
+// order give exactly the extreme of the eight computed corner sums: some corner is on the
+// wanted side iff that sum is.  The distance sum adds the same terms in the same order.
 struct OpenTerms {
     double ext[3][2], dist[3][2];
 };
@@ -207,7 +205,7 @@ TSR_HD void openTerms(const Frame &f, const u32 k[3], u32 d, bool positive, cons
         }
     }
 }
-// intersectNode for child c of the opened node
+// the test for child c of the opened node
 // (selects, not indexing by the runtime slot: that would put the terms in scratch memory)
 TSR_HD bool intersectChild(const OpenTerms &o, u32 c, bool positive, double msd2) {
     const bool bx = c & 1u, by = (c >> 1) & 1u, bz = (c >> 2) & 1u;
@@ -324,8 +322,8 @@ bool findReliableHost(const Frame &f, u64 start, bool positive, const SamplingUn
 
 // Where the DFS of findReliable can start without changing its result: the
 // deepest existing node whose box strictly contains the unit's search ball
-// (centre p, radius msd) -- every node that intersectNode accepts lies inside
-// it, all its ancestors pass intersectNode (the sample plane crosses their
+// (centre p, radius msd) -- every node that the intersect test accepts lies inside
+// it, all its ancestors pass that test (the sample plane crosses their
 // interior), and their other children are rejected, so a DFS from the root
 // reaches this node first and yields the same leaves in the same order.  The
 // ball is inflated far above FP64 rounding before the strict-containment test.
@@ -358,8 +356,8 @@ TSR_HD u64 deepestContaining(const Frame &f, const double a0[3], const double a1
 
 // Where the DFS of findReliable can start without changing its result: the
 // deepest existing node whose box strictly contains the unit's search ball
-// (centre p, radius msd) -- every node that intersectNode accepts lies inside
-// it, all its ancestors pass intersectNode (the sample plane crosses their
+// (centre p, radius msd) -- every node that the intersect test accepts lies inside
+// it, all its ancestors pass that test (the sample plane crosses their
 // interior), and their other children are rejected, so a DFS from the root
 // reaches this node first and yields the same leaves in the same order.  The
 // ball is inflated far above FP64 rounding before the strict-containment test.
