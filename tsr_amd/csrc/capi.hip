@@ -72,7 +72,8 @@ tsr_status tsr_context_create(int device, tsr_context **out) {
         HIPCHECK(hipSetDevice(device));
         tsr_context *c = new tsr_context();
         c->device = device;
-        c->pool = std::make_shared<tsr::DevicePool>(device);
+        c->pool = tsr::makePool(device);
+        c->pool2 = tsr::makePool(device);
         (void)tsr::noAllocScratch(device);   // this GPU's copy of the no-allocation scratch array
         HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         HIPCHECK(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
@@ -90,6 +91,9 @@ void tsr_context_destroy(tsr_context *ctx) {
     ctx->cone.release();
     if (ctx->pool) ctx->pool->trim();   // blocks still held by live handles are freed by their owners
     ctx->pool.reset();
+    (void)hipStreamSynchronize(ctx->stream2);
+    if (ctx->pool2) ctx->pool2->trim();
+    ctx->pool2.reset();
     for (void *p : ctx->pinned)
         if (p) (void)hipHostFree(p);
     for (hipEvent_t &e : ctx->ev) (void)hipEventDestroy(e);

@@ -23,6 +23,7 @@
 #include <mutex>
 #include <string>
 #include <utility>
+#include <vector>
 
 #include "common.h"
 #include "tsr_capi_internal.h"
@@ -126,6 +127,35 @@ class DevicePool {
     size_t cachedBytes_ = 0;
 };
 
+// Every pool of the process, so that an out-of-memory retry can give back the cached blocks
+// of all of them (a context's second pool caches the blocks of its second stream's work).
+inline std::mutex &poolRegistryMutex() {
+    static std::mutex m;
+    return m;
+}
+inline std::vector<std::weak_ptr<DevicePool>> &poolRegistry() {
+    static std::vector<std::weak_ptr<DevicePool>> r;
+    return r;
+}
+inline std::shared_ptr<DevicePool> makePool(int device) {
+    auto p = std::make_shared<DevicePool>(device);
+    std::lock_guard<std::mutex> g(poolRegistryMutex());
+    auto &r = poolRegistry();
+    r.erase(std::remove_if(r.begin(), r.end(), [](const std::weak_ptr<DevicePool> &w) { return w.expired(); }),
+            r.end());
+    r.push_back(p);
+    return p;
+}
+inline void trimAllPools() {
+    std::vector<std::shared_ptr<DevicePool>> live;
+    {
+        std::lock_guard<std::mutex> g(poolRegistryMutex());
+        for (const auto &w : poolRegistry())
+            if (auto p = w.lock()) live.push_back(std::move(p));
+    }
+    for (auto &p : live) p->trim();
+}
+
 // the pool of the context whose entry point runs on this thread (null outside one)
 inline std::shared_ptr<DevicePool> &currentPool() {
     thread_local std::shared_ptr<DevicePool> pool;
@@ -188,9 +218,9 @@ struct DBuf {
             const auto t0 = std::chrono::steady_clock::now();
             hipError_t e = trackedMalloc((void **)&p, want);
             bool retried = false;
-            if (e == hipErrorOutOfMemory) {   // give the cached blocks back and try once more
+            if (e == hipErrorOutOfMemory) {   // give every pool's cached blocks back and try once more
                 (void)hipGetLastError();
-                pool->trim();
+                trimAllPools();
                 e = trackedMalloc((void **)&p, want);
                 retried = true;
             }

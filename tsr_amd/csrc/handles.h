@@ -40,6 +40,9 @@ struct tsr_context {
     float phaseMs[6] = {0, 0, 0, 0, 0, 0};
     tsr_kernel_stat kstat[8];
     std::shared_ptr<tsr::DevicePool> pool;   // device blocks reused across the stages' temporaries
+    // blocks of the work on stream2 (the emptiness index built during the tree build), kept
+    // for the next build instead of being mapped and unmapped each time
+    std::shared_ptr<tsr::DevicePool> pool2;
     tsr::Scratch scratch;
     tsr::ConeArena cone;
     // pinned host staging for large device->host transfers, one grow-only buffer per use

@@ -501,6 +501,19 @@ __global__ void k_nodeHash(HashView nodes, const u64 *__restrict__ innerSorted, 
     for (u32 c = 0; c < 8; ++c) nodes.insert((loc << 3) | c, 1u + 8u * r + c, err);
 }
 
+// each inner node's value: its rank in DFS block order, so that the id of a node is
+// 1 + 8 rank(parent) + slot (the root: 0) with one lookup in the (smaller) inner set
+__global__ void k_innerRanks(HashView inner, const u64 *__restrict__ innerSorted, u32 innerCount) {
+    const u32 r = blockIdx.x * BLOCK + threadIdx.x;
+    if (r >= innerCount) return;
+    if (u32 *v = inner.slot(innerSorted[r])) *v = r;
+}
+__device__ __forceinline__ u32 nodeIdByRank(HashView innerRank, u64 loc) {
+    if (loc == 1ull) return 0u;
+    const u32 r = innerRank.find(loc >> 3);
+    return r == INVALID ? INVALID : 1u + 8u * r + (u32)(loc & 7);
+}
+
 __global__ void k_nodeArrays(HashView nodes, HashView inner, const u64 *__restrict__ innerSorted, u32 innerCount,
                              u32 *__restrict__ child, u32 *__restrict__ parents, uint8_t *__restrict__ depth,
                              u64 *__restrict__ nodeLoc, u64 *__restrict__ leafKeys, u32 *__restrict__ leafIds,
@@ -677,13 +690,13 @@ TSR_HD void swKey(u32 vx, u32 vy, u32 vz, u64 &hi, u64 &lo) {
 // neighbouring cell (-1: none): the depth of the deepest inner ancestor is first tested at
 // guess and guess +- 1 (2:1 balanced neighbours), then found by bisection; the result is
 // the same either way and is returned in `guess`.
-__device__ u32 leafOfCell(HashView inner, HashView nodes, const u32 *__restrict__ nodeToLeaf, u32 cx, u32 cy, u32 cz,
+__device__ u32 leafOfCell(HashView inner, const u32 *__restrict__ nodeToLeaf, u32 cx, u32 cy, u32 cz,
                           int &guess) {
     const u64 finest = makeLoc(cx, cy, cz, MAX_DEPTH);
     if (guess >= 0 && guess < (int)MAX_DEPTH) {
         // the node at the guessed leaf depth: when it exists and is a leaf, it is the one
         // leaf holding the point
-        const u32 id = nodes.find(finest >> (3 * (MAX_DEPTH - (u32)(guess + 1))));
+        const u32 id = nodeIdByRank(inner, finest >> (3 * (MAX_DEPTH - (u32)(guess + 1))));
         if (id != INVALID && nodeToLeaf[id] != INVALID) return nodeToLeaf[id];
     }
     auto innerAt = [&](int m) { return m < (int)MAX_DEPTH && inner.contains(finest >> (3 * (MAX_DEPTH - m))); };
@@ -722,7 +735,7 @@ __device__ u32 leafOfCell(HashView inner, HashView nodes, const u32 *__restrict_
     }
     guess = lo;
     const u64 leafLoc = finest >> (3 * (MAX_DEPTH - (lo + 1)));
-    const u32 id = nodes.find(leafLoc);
+    const u32 id = nodeIdByRank(inner, leafLoc);
     return id == INVALID ? INVALID : nodeToLeaf[id];
 }
 
@@ -733,7 +746,7 @@ __global__ void k_checkCells(const u32 *__restrict__ cells, size_t n, u32 *err) 
 
 // keyHi / keyLo: the path key halves shifted right by hiShift / loShift (bits below
 // that no level of this tree sets), so the sorts start at bit 0
-__global__ void k_cells(HashView inner, HashView nodes, const u32 *__restrict__ nodeToLeaf,
+__global__ void k_cells(HashView inner, const u32 *__restrict__ nodeToLeaf,
                         const u64 *__restrict__ vcodes, u32 count, u32 D, u32 hiShift, u32 loShift,
                         u32 *__restrict__ cellsUnsorted, u64 *__restrict__ keyHi, u64 *__restrict__ keyLo,
                         u32 *__restrict__ idx) {
@@ -746,7 +759,7 @@ __global__ void k_cells(HashView inner, HashView nodes, const u32 *__restrict__ 
     int guess = -1;
     for (u32 s = 0; s < 8; ++s) {
         const u32 x = v[0] - 1 + (s & 1), y = v[1] - 1 + ((s >> 1) & 1), z = v[2] - 1 + ((s >> 2) & 1);
-        cellsUnsorted[8 * (size_t)c + s] = leafOfCell(inner, nodes, nodeToLeaf, x, y, z, guess);
+        cellsUnsorted[8 * (size_t)c + s] = leafOfCell(inner, nodeToLeaf, x, y, z, guess);
     }
     u64 h, l;
     swKey(v[0], v[1], v[2], h, l);
@@ -1836,7 +1849,8 @@ void buildTree(tsr_context *ctx, const tsr_scene *scene, const tsr_params &prm, 
     // The emptiness pass's direction index needs only the samples, their container depths and
     // the cameras: a worker thread builds it on the context's second stream while this thread
     // replays the sampling refinement (the GPU is otherwise idle then).  Its temporaries come
-    // from a block pool of its own, so no block is shared between the two streams.
+    // from the context's second block pool, which only work on the second stream uses, so no
+    // block is shared between the two streams; the pool keeps them for the next build.
     struct EventGuard {   // destroyed after the index future below (declared before it)
         hipEvent_t e = nullptr;
         ~EventGuard() {
@@ -1853,8 +1867,8 @@ void buildTree(tsr_context *ctx, const tsr_scene *scene, const tsr_params &prm, 
     std::future<std::shared_ptr<EmptinessIndex>> indexFuture =
         std::async(std::launch::async, [ctx, inputLinks, samplesReady]() -> std::shared_ptr<EmptinessIndex> {
             HIPCHECK(hipSetDevice(ctx->device));
-            auto pool = std::make_shared<DevicePool>(ctx->device);
-            PoolScope scope(pool);
+            // the context's second pool: only stream2 work takes and returns its blocks
+            PoolScope scope(ctx->pool2);
             HIPCHECK(hipStreamWaitEvent(ctx->stream2, samplesReady, 0));
             std::shared_ptr<EmptinessIndex> ix;
             {
@@ -1867,7 +1881,6 @@ void buildTree(tsr_context *ctx, const tsr_scene *scene, const tsr_params &prm, 
                 }
                 HIPCHECK(hipStreamSynchronize(ctx->stream2));
             }
-            pool->trim();   // the temporaries' blocks (their work is done)
             return ix;
         });
     std::vector<u64> created;
@@ -1935,6 +1948,7 @@ void buildTree(tsr_context *ctx, const tsr_scene *scene, const tsr_params &prm, 
     const u32 N = 1 + 8 * I;
     tree->innerCount = I;
     tree->nodeCount = N;
+    if (I) k_innerRanks<<<gridFor(I, BLOCK), BLOCK, 0, st>>>(inner.view(), innerSorted.p, I);
     tree->nodeHash.alloc(N, st);
     k_nodeHash<<<gridFor(std::max<u32>(I, 1), BLOCK), BLOCK, 0, st>>>(tree->nodeHash.view(), innerSorted.p, I, err.p);
     tree->child.alloc(N);
@@ -2018,7 +2032,7 @@ void buildTree(tsr_context *ctx, const tsr_scene *scene, const tsr_params &prm, 
             // 4 + 5 (11 - lvl), levels 12-18 at lo bits 4 + 5 (18 - lvl); the key halves are
             // shifted down past the bits no level < D sets, and the sorts cover the rest.
             const u32 hiShift = D > 12 ? 4u : 4u + 5u * (12u - D), loShift = D > 12 ? 4u + 5u * (19u - D) : 39u;
-            k_cells<<<gridFor(V, BLOCK), BLOCK, 0, st>>>(inner.view(), tree->nodeHash.view(), tree->nodeToLeaf.p,
+            k_cells<<<gridFor(V, BLOCK), BLOCK, 0, st>>>(inner.view(), tree->nodeToLeaf.p,
                                                          uniq.p, V, D, hiShift, loShift, cellsUnsorted.p, kh.p,
                                                          kl.p, idx.p);
             if (D > 12) {
