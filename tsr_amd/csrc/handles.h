@@ -87,7 +87,7 @@ struct tsr_tree {
     tsr::u32 maxCams = 1;
     tsr::DBuf<float> px, py, pz, nx, ny, nz, scale, conf;
     tsr::DBuf<tsr::u32> cams;
-    tsr::DHash nodeHash;               // location code -> node id (all nodes)
+    tsr::DHash nodeHash;               // location code -> node id (all nodes), built when first needed
     tsr::u64 samplingEvents = 0, samplingCreated = 0, samplingUnits = 0;
     tsr::u64 samplingMisses = 0;   // visit-index lookups answered by the device (leaves outside the hot part)
     // the emptiness pass's direction index over the scene's links, built during the tree build

@@ -192,6 +192,14 @@ void leafNeighbours(tsr_context *ctx, tsr_tree *tree) {
     const u64 slots = 6ull * L;
     if (slots >= 0xffffffffull) throw tsr_error(TSR_ERR_NODE_OVERFLOW, "too many leaves for the neighbour graph");
     DBuf<u32> other(slots), valid(slots), pos(slots);
+    if (tree->nodeHash.capacity == 0) {   // location code -> node id, built on first need
+        DBuf<u32> err(1);
+        HIPCHECK(hipMemsetAsync(err.p, 0, 4, st));
+        tree->nodeHash.alloc(tree->nodeCount, st);
+        k_insertNodes<<<gridFor(tree->nodeCount, BLOCK), BLOCK, 0, st>>>(tree->nodeHash.view(), tree->nodeLoc.p,
+                                                                       tree->nodeCount, err.p);
+        if (readScalar(err.p, st)) throw tsr_error(TSR_ERR_HIP, "node hash full");
+    }
     k_neighbourSlots<<<gridFor(slots, BLOCK), BLOCK, 0, st>>>(tree->nodeHash.view(), tree->child.p, tree->leafLoc.p,
                                                               tree->leafNode.p, L, other.p);
     k_slotValid<<<gridFor(slots, BLOCK), BLOCK, 0, st>>>(other.p, slots, valid.p);

@@ -493,14 +493,6 @@ __global__ void k_dfsKeys(const u64 *__restrict__ locs, u32 n, u64 *__restrict__
 }
 
 // children of the inner node with pre-order rank r get ids 1 + 8r + c (Nodes.cpp:536-554)
-__global__ void k_nodeHash(HashView nodes, const u64 *__restrict__ innerSorted, u32 innerCount, u32 *err) {
-    const u32 r = blockIdx.x * BLOCK + threadIdx.x;
-    if (r == 0) nodes.insert(1ull, 0u, err);
-    if (r >= innerCount) return;
-    const u64 loc = innerSorted[r];
-    for (u32 c = 0; c < 8; ++c) nodes.insert((loc << 3) | c, 1u + 8u * r + c, err);
-}
-
 // each inner node's value: its rank in DFS block order, so that the id of a node is
 // 1 + 8 rank(parent) + slot (the root: 0) with one lookup in the (smaller) inner set
 __global__ void k_innerRanks(HashView inner, const u64 *__restrict__ innerSorted, u32 innerCount) {
@@ -514,7 +506,7 @@ __device__ __forceinline__ u32 nodeIdByRank(HashView innerRank, u64 loc) {
     return r == INVALID ? INVALID : 1u + 8u * r + (u32)(loc & 7);
 }
 
-__global__ void k_nodeArrays(HashView nodes, HashView inner, const u64 *__restrict__ innerSorted, u32 innerCount,
+__global__ void k_nodeArrays(HashView inner, const u64 *__restrict__ innerSorted, u32 innerCount,
                              u32 *__restrict__ child, u32 *__restrict__ parents, uint8_t *__restrict__ depth,
                              u64 *__restrict__ nodeLoc, u64 *__restrict__ leafKeys, u32 *__restrict__ leafIds,
                              u32 *leafCounter) {
@@ -532,7 +524,7 @@ __global__ void k_nodeArrays(HashView nodes, HashView inner, const u64 *__restri
     }
     if (r >= innerCount) return;
     const u64 loc = innerSorted[r];
-    const u32 id = nodes.find(loc);
+    const u32 id = nodeIdByRank(inner, loc);
     child[id] = 1u + 8u * r;
     const u32 d = locDepth(loc) + 1;
     for (u32 c = 0; c < 8; ++c) {
@@ -566,11 +558,11 @@ __global__ void k_fillU32(u32 *p, size_t n, u32 v) {
     if (i < n) p[i] = v;
 }
 
-__global__ void k_sampleNodes(HashView nodes, const u64 *__restrict__ containerLoc, u32 n, u32 *__restrict__ nodeOf,
-                              u32 *__restrict__ counts, u32 *err) {
+__global__ void k_sampleNodes(HashView innerRank, const u64 *__restrict__ containerLoc, u32 n,
+                              u32 *__restrict__ nodeOf, u32 *__restrict__ counts, u32 *err) {
     const u32 i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n) return;
-    const u32 id = nodes.find(containerLoc[i]);
+    const u32 id = nodeIdByRank(innerRank, containerLoc[i]);
     if (id == INVALID) {
         atomicOr(err, (u32)ERR_NOT_INSERTED);
         nodeOf[i] = 0;
@@ -1949,8 +1941,6 @@ void buildTree(tsr_context *ctx, const tsr_scene *scene, const tsr_params &prm, 
     tree->innerCount = I;
     tree->nodeCount = N;
     if (I) k_innerRanks<<<gridFor(I, BLOCK), BLOCK, 0, st>>>(inner.view(), innerSorted.p, I);
-    tree->nodeHash.alloc(N, st);
-    k_nodeHash<<<gridFor(std::max<u32>(I, 1), BLOCK), BLOCK, 0, st>>>(tree->nodeHash.view(), innerSorted.p, I, err.p);
     tree->child.alloc(N);
     tree->parents.alloc(N);
     tree->nodeDepth.alloc(N);
@@ -1962,7 +1952,7 @@ void buildTree(tsr_context *ctx, const tsr_scene *scene, const tsr_params &prm, 
     DBuf<u32> leafIds(Lmax), leafIdsSorted(Lmax);
     HIPCHECK(hipMemsetAsync(counter.p, 0, 4, st));
     k_nodeArrays<<<gridFor(std::max<u32>(I, 1), BLOCK), BLOCK, 0, st>>>(
-        tree->nodeHash.view(), inner.view(), innerSorted.p, I, tree->child.p, tree->parents.p, tree->nodeDepth.p,
+        inner.view(), innerSorted.p, I, tree->child.p, tree->parents.p, tree->nodeDepth.p,
         tree->nodeLoc.p, leafKeys.p, leafIds.p, counter.p);
     const u32 L = readScalar(counter.p, st);
     checkErr(readScalar(err.p, st));
@@ -1984,7 +1974,7 @@ void buildTree(tsr_context *ctx, const tsr_scene *scene, const tsr_params &prm, 
     tree->sampleCounts.alloc(N);
     tree->sampleStarts.alloc(N);
     HIPCHECK(hipMemsetAsync(tree->sampleCounts.p, 0, (size_t)N * 4, st));
-    k_sampleNodes<<<gridFor(n, BLOCK), BLOCK, 0, st>>>(tree->nodeHash.view(), containerLoc.p, n, nodeOf.p,
+    k_sampleNodes<<<gridFor(n, BLOCK), BLOCK, 0, st>>>(inner.view(), containerLoc.p, n, nodeOf.p,
                                                        tree->sampleCounts.p, err.p);
     checkErr(readScalar(err.p, st));
     exclusiveScan(sc, tree->sampleCounts.p, tree->sampleStarts.p, N, st);
